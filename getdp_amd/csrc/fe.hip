@@ -1,0 +1,988 @@
+// Batched Galerkin assembly on the device.
+//
+// Replaces, for the canonical terms listed in gdp_hip.h, the reference's
+// per-element chain Treatment_FemFormulation (Treatment_Formulation.cpp:413)
+// -> Cal_GalerkinTermOfFemEquation (Cal_GalerkinTermOfFemEquation.cpp:531)
+// -> Cal_AssembleTerm_NoDt -> Dof_AssembleInMat -> LinAlg_AddDoubleInMatrix.
+//
+// Design: the reference sums every matrix entry in ascending element order
+// (elements sorted by number, GeoData.cpp:1018; each element adds its Ek
+// before the next element), and within one element term by term.  Here one
+// thread owns one matrix row (one unknown DOF): it walks the elements
+// incident to the row's entity in ascending element order and, for each
+// element and term, computes the row of Ek that belongs to it and adds it into
+// the row's CSR slots.  Each entry thus receives the same additions in the
+// same order as in the reference, without atomics, and every Ek entry is
+// computed with the reference's order of operations (this file is compiled
+// without floating-point contraction).  Per-element Jacobians are computed
+// once, by a first kernel.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+#include "gdp_device.h"
+
+namespace gdp {
+
+namespace {
+
+constexpr int kGeoStride = 20;  // doubles of per-element geometry
+constexpr int kMaxTerms = 16;
+constexpr int kMaxSpaces = 3;
+
+enum GeoKind { GEO_NONE = 0, GEO_TRI2D = 1, GEO_TET = 2, GEO_TRISUR = 3 };
+enum Family { FAM_NODAL = 0, FAM_EDGE = 1 };
+
+struct DevTerm {
+  int kind, equ, dof, npoints, ncoef, pad;
+  const double *coef;           // device: coef[region * ncoef + c]
+  const unsigned char *active;  // device: per region id, term applies
+};
+
+struct DevPlan {
+  int family;
+  int nspaces, nterms, N, nreg;
+  const int *etype, *region, *conn, *edges, *geokind;
+  const double *geo;            // nelem * kGeoStride
+  const int *dofnum[kMaxSpaces];
+  const double *fixval[kMaxSpaces];
+  const int *inc_ptr, *inc_item;
+  const int *row_space, *row_ent;
+  const int *ia, *ja;
+  DevTerm term[kMaxTerms];
+};
+
+// ---- reference element data ---------------------------------------------------
+// Gauss tables with the reference's truncated literals (Gauss_Triangle.h:8-22,
+// Gauss_Tetrahedron.h:8-23).
+__device__ inline void gauss_pt(int tri, int np, int q, double &u, double &v, double &w,
+                                double &wt)
+{
+  w = 0.;
+  if(tri) {
+    if(np == 1) {
+      u = 0.333333333333333;
+      v = 0.333333333333333;
+      wt = 0.5;
+    }
+    else if(np == 3) {
+      u = (q == 1) ? 0.66666666666666 : 0.16666666666666;
+      v = (q == 2) ? 0.66666666666666 : 0.16666666666666;
+      wt = 0.16666666666666;
+    }
+    else {
+      u = q == 0 ? 0.333333333333333 : (q == 1 ? 0.6 : 0.2);
+      v = q == 0 ? 0.3333333333333333 : (q == 2 ? 0.6 : 0.2);
+      wt = q == 0 ? -0.28125 : .260416666666;
+    }
+  }
+  else {
+    if(np == 1) {
+      u = v = w = 0.25;
+      wt = 0.166666666667;
+    }
+    else {
+      const double a = 0.138196601125, b = 0.585410196625;
+      u = (q == 3) ? b : a;
+      v = (q == 2) ? b : a;
+      w = (q == 1) ? b : a;
+      wt = 0.0416666666667;
+    }
+  }
+}
+
+// BF_Node (BF_Node.cpp:49-76)
+__device__ inline double bf_node(int tri, int i, double u, double v, double w)
+{
+  if(tri) return i == 0 ? 1. - u - v : (i == 1 ? u : v);
+  return i == 0 ? 1. - u - v - w : (i == 1 ? u : (i == 2 ? v : w));
+}
+
+// BF_GradNode (BF_Node.cpp:245-320)
+__device__ inline void bf_gradnode(int tri, int i, double s[3])
+{
+  s[0] = s[1] = s[2] = 0.;
+  if(i == 0) {
+    s[0] = -1.;
+    s[1] = -1.;
+    s[2] = tri ? 0. : -1.;
+  }
+  else
+    s[i - 1] = 1.;
+}
+
+// BF_Edge / BF_CurlEdge on tetrahedra (BF_Edge.cpp:95-131, 452-486), with the
+// sign flip of negative edges (BF_Edge.cpp:366-372, 720-726).
+__device__ inline void bf_edge(int k, double u, double v, double w, int neg, double s[3])
+{
+  switch(k) {
+  case 0: s[0] = 1. - v - w; s[1] = u; s[2] = u; break;
+  case 1: s[0] = v; s[1] = 1. - u - w; s[2] = v; break;
+  case 2: s[0] = w; s[1] = w; s[2] = 1. - u - v; break;
+  case 3: s[0] = -v; s[1] = u; s[2] = 0.; break;
+  case 4: s[0] = -w; s[1] = 0.; s[2] = u; break;
+  default: s[0] = 0.; s[1] = -w; s[2] = v; break;
+  }
+  if(neg) {
+    s[0] = -s[0];
+    s[1] = -s[1];
+    s[2] = -s[2];
+  }
+}
+
+__device__ inline void bf_curledge(int k, int neg, double s[3])
+{
+  switch(k) {
+  case 0: s[0] = 0.; s[1] = -2.; s[2] = 2.; break;
+  case 1: s[0] = 2.; s[1] = 0.; s[2] = -2.; break;
+  case 2: s[0] = -2.; s[1] = 2.; s[2] = 0.; break;
+  case 3: s[0] = 0.; s[1] = 0.; s[2] = 2.; break;
+  case 4: s[0] = 0.; s[1] = -2.; s[2] = 0.; break;
+  default: s[0] = 2.; s[1] = 0.; s[2] = 0.; break;
+  }
+  if(neg) {
+    s[0] = -s[0];
+    s[1] = -s[1];
+    s[2] = -s[2];
+  }
+}
+
+// Geometry layout per element (kGeoStride doubles):
+//   [0..8]   Jac  c11 c12 c13 c21 c22 c23 c31 c32 c33
+//   [9]      DetJac
+//   [10..18] InvJac, same order
+// ChangeOfCoord_Form1 (Get_Geometry.cpp:1895-1904)
+__device__ inline void form1(const double *I, const double u[3], double x[3])
+{
+  x[0] = u[0] * I[0] + u[1] * I[1] + u[2] * I[2];
+  x[1] = u[0] * I[3] + u[1] * I[4] + u[2] * I[5];
+  x[2] = u[0] * I[6] + u[1] * I[7] + u[2] * I[8];
+}
+
+// ChangeOfCoord_Form2 (Get_Geometry.cpp:1906-1921)
+__device__ inline void form2(const double *J, double d, const double u[3], double x[3])
+{
+  x[0] = (u[0] * J[0] + u[1] * J[3] + u[2] * J[6]) / d;
+  x[1] = (u[0] * J[1] + u[1] * J[4] + u[2] * J[7]) / d;
+  x[2] = (u[0] * J[2] + u[1] * J[5] + u[2] * J[8]) / d;
+}
+
+// Cal_ProductValue (Cal_Value.cpp:641-660, 887-897): coefficient x DOF value.
+__device__ inline void coef_times(int ncoef, const double *c, const double v[3], double r[3])
+{
+  if(ncoef == 1) {
+    double a0 = c[0];
+    r[0] = a0 * v[0];
+    r[1] = a0 * v[1];
+    r[2] = a0 * v[2];
+  }
+  else {
+    double t0 = c[0] * v[0] + c[1] * v[1] + c[2] * v[2];
+    double t1 = c[3] * v[0] + c[4] * v[1] + c[5] * v[2];
+    double t2 = c[6] * v[0] + c[7] * v[1] + c[8] * v[2];
+    r[0] = t0;
+    r[1] = t1;
+    r[2] = t2;
+  }
+}
+
+// Cal_Product123 (Get_Geometry.cpp:1965-1968)
+__device__ inline double dot3(const double a[3], const double b[3])
+{
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// ---- kernel 1: per-element Jacobian, determinant, inverse ----------------------
+// JacobianVol2D / JacobianVol3D / JacobianSur3D and Get_InverseMatrix
+// (Get_Geometry.cpp:1099-1124, 1435-1469, 1629-1680, 1755-1808).
+__global__ void __launch_bounds__(256) k_geometry(int nelem, const int *__restrict__ geokind,
+                                                  const int *__restrict__ conn,
+                                                  const double *__restrict__ xyz,
+                                                  double *__restrict__ geo)
+{
+  int e = blockIdx.x * 256 + threadIdx.x;
+  if(e >= nelem) return;
+  int gk = geokind[e];
+  if(gk == GEO_NONE) return;
+  int tri = (gk != GEO_TET);
+  int nn = tri ? 3 : 4;
+  double x[4], y[4], z[4];
+  for(int i = 0; i < nn; i++) {
+    long nd = conn[(long)e * 4 + i] - 1;
+    x[i] = xyz[3 * nd];
+    y[i] = xyz[3 * nd + 1];
+    z[i] = xyz[3 * nd + 2];
+  }
+  double J[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  double I[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  double det;
+  if(gk == GEO_TRI2D) {
+    J[8] = 1.;
+    for(int i = 0; i < nn; i++) {
+      double g[3];
+      bf_gradnode(1, i, g);
+      J[0] += x[i] * g[0];
+      J[3] += x[i] * g[1];
+      J[1] += y[i] * g[0];
+      J[4] += y[i] * g[1];
+    }
+    det = J[0] * J[4] - J[1] * J[3];
+    I[0] = J[4] * J[8] / det;
+    I[3] = -J[3] * J[8] / det;
+    I[1] = -J[1] * J[8] / det;
+    I[4] = J[0] * J[8] / det;
+    I[8] = 1. / J[8];
+  }
+  else if(gk == GEO_TRISUR) {
+    for(int i = 0; i < nn; i++) {
+      double g[3];
+      bf_gradnode(1, i, g);
+      J[0] += x[i] * g[0];
+      J[3] += x[i] * g[1];
+      J[1] += y[i] * g[0];
+      J[4] += y[i] * g[1];
+      J[2] += z[i] * g[0];
+      J[5] += z[i] * g[1];
+    }
+    double a1 = J[0] * J[4] - J[1] * J[3];
+    double a2 = J[2] * J[3] - J[0] * J[5];
+    double a3 = J[1] * J[5] - J[2] * J[4];
+    det = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+  }
+  else {
+    for(int i = 0; i < nn; i++) {
+      double g[3];
+      bf_gradnode(0, i, g);
+      J[0] += x[i] * g[0];
+      J[3] += x[i] * g[1];
+      J[6] += x[i] * g[2];
+      J[1] += y[i] * g[0];
+      J[4] += y[i] * g[1];
+      J[7] += y[i] * g[2];
+      J[2] += z[i] * g[0];
+      J[5] += z[i] * g[1];
+      J[8] += z[i] * g[2];
+    }
+    det = J[0] * J[4] * J[8] + J[2] * J[3] * J[7] + J[1] * J[5] * J[6] -
+          J[2] * J[4] * J[6] - J[0] * J[5] * J[7] - J[1] * J[3] * J[8];
+    I[0] = (J[4] * J[8] - J[5] * J[7]) / det;
+    I[3] = -(J[3] * J[8] - J[5] * J[6]) / det;
+    I[6] = (J[3] * J[7] - J[4] * J[6]) / det;
+    I[1] = -(J[1] * J[8] - J[2] * J[7]) / det;
+    I[4] = (J[0] * J[8] - J[2] * J[6]) / det;
+    I[7] = -(J[0] * J[7] - J[1] * J[6]) / det;
+    I[2] = (J[1] * J[5] - J[2] * J[4]) / det;
+    I[5] = -(J[0] * J[5] - J[2] * J[3]) / det;
+    I[8] = (J[0] * J[4] - J[1] * J[3]) / det;
+  }
+  double *G = geo + (long)e * kGeoStride;
+  for(int k = 0; k < 9; k++) G[k] = J[k];
+  G[9] = det;
+  for(int k = 0; k < 9; k++) G[10 + k] = I[k];
+}
+
+__device__ inline int find_slot(const int *__restrict__ ja, int lo, int hi, int col)
+{
+  while(lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if(ja[mid] < col) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Add one Ek entry of the row (Dof_AssembleInMat, DofData.cpp:1464-1500):
+// unknown column -> matrix slot; fixed column -> b -= value * Ek (Val != 0).
+__device__ inline void scatter(const DevPlan &P, int dsp, int dofent, double val,
+                               double *__restrict__ a, int rs, int re, double &bacc)
+{
+  int c = P.dofnum[dsp][dofent];
+  if(c > 0) {
+    int s = find_slot(P.ja, rs, re, c - 1);
+    a[s] += val;
+  }
+  else if(c < 0 && val != 0.) {
+    bacc += (P.fixval[dsp][-c - 1] * 1.) * (-val);
+  }
+}
+
+// ---- kernel 2, nodal spaces: one thread per unknown row -----------------------
+__global__ void __launch_bounds__(128) k_gather_nodal(DevPlan P, double *__restrict__ a,
+                                                      double *__restrict__ b)
+{
+  int r = blockIdx.x * 128 + threadIdx.x;
+  if(r >= P.N) return;
+  const int s = P.row_space[r], ent = P.row_ent[r];
+  const int rs = P.ia[r], re = P.ia[r + 1];
+  double bacc = 0.;
+  for(int p = P.inc_ptr[ent]; p < P.inc_ptr[ent + 1]; p++) {
+    const int item = P.inc_item[p];
+    const int e = item >> 3, k = item & 7;
+    const int reg = P.region[e];
+    const int gk = P.geokind[e];
+    const int tri = (gk != GEO_TET);
+    const int nn = tri ? 3 : 4;
+    const int *cn = P.conn + (long)e * 4;
+    const double *G = P.geo + (long)e * kGeoStride;
+    const double adet = fabs(G[9]);
+    // physical gradients of the element's nodal functions (FORM1 change of
+    // coordinates of BF_GradNode; constant on P1 elements)
+    double g[4][3];
+    if(gk != GEO_TRISUR)
+      for(int j = 0; j < nn; j++) {
+        double u[3];
+        bf_gradnode(tri, j, u);
+        form1(G + 10, u, g[j]);
+      }
+    for(int ti = 0; ti < P.nterms; ti++) {
+      const DevTerm &t = P.term[ti];
+      if(t.equ != s || !t.active[reg]) continue;
+      const double *cf = t.coef + (long)reg * t.ncoef;
+      if(t.kind == GDP_T_GRAD_GRAD) {
+        const int sym = (t.dof == t.equ);
+        for(int j = 0; j < nn; j++) {
+          // symmetric terms compute Ek[i][j] for j <= i and mirror it
+          // (Cal_GalerkinTermOfFemEquation.cpp:838-841, 921-924)
+          int ii = k, jj = j;
+          if(sym && j > k) {
+            ii = j;
+            jj = k;
+          }
+          double cg[3];
+          coef_times(t.ncoef, cf, g[jj], cg);
+          double pr = dot3(g[ii], cg);
+          double acc = 0.;
+          for(int q = 0; q < t.npoints; q++) {
+            double u, v, w, wt;
+            gauss_pt(tri, t.npoints, q, u, v, w, wt);
+            acc += (wt * adet) * pr;
+          }
+          scatter(P, t.dof, cn[j], acc, a, rs, re, bacc);
+        }
+      }
+      else if(t.kind == GDP_T_NODE_MASS) {
+        for(int j = 0; j < nn; j++) {
+          int ii = k, jj = j;
+          if(j > k) {
+            ii = j;
+            jj = k;
+          }
+          double acc = 0.;
+          for(int q = 0; q < t.npoints; q++) {
+            double u, v, w, wt;
+            gauss_pt(tri, t.npoints, q, u, v, w, wt);
+            double pi = bf_node(tri, ii, u, v, w), pj = bf_node(tri, jj, u, v, w);
+            acc += (wt * adet) * (pi * (cf[0] * pj));
+          }
+          scatter(P, t.dof, cn[j], acc, a, rs, re, bacc);
+        }
+      }
+      else if(t.kind == GDP_T_SRC_NODE) {
+        // [ f, {u} ]: Ek[i][0] with the fixed pseudo-DOF of value 1
+        // (Dof_InitDofForNoDof, DofData.cpp:1970-1981)
+        double acc = 0.;
+        for(int q = 0; q < t.npoints; q++) {
+          double u, v, w, wt;
+          gauss_pt(tri, t.npoints, q, u, v, w, wt);
+          acc += (wt * adet) * (bf_node(tri, k, u, v, w) * cf[0]);
+        }
+        if(acc != 0.) bacc += (1. * 1.) * (-acc);
+      }
+    }
+  }
+  b[r] = bacc;
+}
+
+// ---- kernel 2, Whitney edge space on tetrahedra: one thread per row -------------
+__global__ void __launch_bounds__(128) k_gather_edge(DevPlan P, double *__restrict__ a,
+                                                     double *__restrict__ b)
+{
+  int r = blockIdx.x * 128 + threadIdx.x;
+  if(r >= P.N) return;
+  const int ent = P.row_ent[r];
+  const int rs = P.ia[r], re = P.ia[r + 1];
+  double bacc = 0.;
+  for(int p = P.inc_ptr[ent]; p < P.inc_ptr[ent + 1]; p++) {
+    const int item = P.inc_item[p];
+    const int e = item >> 3, k = item & 7;
+    const int reg = P.region[e];
+    const int *ed = P.edges + (long)e * 6;
+    const double *G = P.geo + (long)e * kGeoStride;
+    const double det = G[9], adet = fabs(det);
+    for(int ti = 0; ti < P.nterms; ti++) {
+      const DevTerm &t = P.term[ti];
+      if(!t.active[reg]) continue;
+      const double *cf = t.coef + (long)reg * t.ncoef;
+      if(t.kind == GDP_T_CURL_CURL) {
+        for(int j = 0; j < 6; j++) {
+          int ii = k, jj = j;
+          if(j > k) {
+            ii = j;
+            jj = k;
+          }
+          double ci[3], cj[3], xi[3], xj[3], cx[3];
+          bf_curledge(ii, ed[ii] < 0, ci);
+          bf_curledge(jj, ed[jj] < 0, cj);
+          form2(G, det, ci, xi);
+          form2(G, det, cj, xj);
+          coef_times(t.ncoef, cf, xj, cx);
+          double pr = dot3(xi, cx);
+          double acc = 0.;
+          for(int q = 0; q < t.npoints; q++) {
+            double u, v, w, wt;
+            gauss_pt(0, t.npoints, q, u, v, w, wt);
+            acc += (wt * adet) * pr;
+          }
+          scatter(P, 0, ed[j] < 0 ? -ed[j] : ed[j], acc, a, rs, re, bacc);
+        }
+      }
+      else if(t.kind == GDP_T_EDGE_MASS) {
+        for(int j = 0; j < 6; j++) {
+          int ii = k, jj = j;
+          if(j > k) {
+            ii = j;
+            jj = k;
+          }
+          double acc = 0.;
+          for(int q = 0; q < t.npoints; q++) {
+            double u, v, w, wt;
+            gauss_pt(0, t.npoints, q, u, v, w, wt);
+            double si[3], sj[3], xi[3], xj[3], cx[3];
+            bf_edge(ii, u, v, w, ed[ii] < 0, si);
+            bf_edge(jj, u, v, w, ed[jj] < 0, sj);
+            form1(G + 10, si, xi);
+            form1(G + 10, sj, xj);
+            coef_times(t.ncoef, cf, xj, cx);
+            acc += (wt * adet) * dot3(xi, cx);
+          }
+          scatter(P, 0, ed[j] < 0 ? -ed[j] : ed[j], acc, a, rs, re, bacc);
+        }
+      }
+      else if(t.kind == GDP_T_SRC_EDGE) {
+        double acc = 0.;
+        for(int q = 0; q < t.npoints; q++) {
+          double u, v, w, wt;
+          gauss_pt(0, t.npoints, q, u, v, w, wt);
+          double si[3], xi[3];
+          bf_edge(k, u, v, w, ed[k] < 0, si);
+          form1(G + 10, si, xi);
+          acc += (wt * adet) * dot3(xi, cf);
+        }
+        if(acc != 0.) bacc += (1. * 1.) * (-acc);
+      }
+    }
+  }
+  b[r] = bacc;
+}
+
+int nodes_of(int et) { return et == GDP_TET ? 4 : et == GDP_TRI ? 3 : et == GDP_LINE ? 2 : 0; }
+int edges_of(int et) { return et == GDP_TET ? 6 : et == GDP_TRI ? 3 : et == GDP_LINE ? 1 : 0; }
+
+}  // namespace
+}  // namespace gdp
+
+using namespace gdp;
+
+// =============================================================================
+// FE system object
+// =============================================================================
+struct gdp_fe {
+  int family = FAM_NODAL;
+  int nelem = 0, nnodes = 0, nedges = 0, nmax = 0, nreg = 0;
+  int nspaces = 0, nterms = 0, N = 0;
+  long nnz = 0;
+  // host copies
+  std::vector<int> h_etype, h_region, h_conn, h_edges, h_geokind;
+  std::vector<std::vector<int>> h_dofnum;  // per space, nmax+1
+  std::vector<int> h_ia, h_ja;
+  std::vector<int> h_inc_ptr, h_inc_item, h_row_space, h_row_ent;
+  std::vector<double> h_xyz;
+  std::vector<std::vector<double>> h_fixval, h_coef;
+  std::vector<std::vector<unsigned char>> h_active;
+  std::vector<gdp_term_desc> h_terms;
+  bool uploaded = false;
+  // device
+  int *d_etype = nullptr, *d_region = nullptr, *d_conn = nullptr, *d_edges = nullptr,
+      *d_geokind = nullptr;
+  double *d_xyz = nullptr, *d_geo = nullptr;
+  int *d_dofnum[kMaxSpaces] = {nullptr, nullptr, nullptr};
+  double *d_fixval[kMaxSpaces] = {nullptr, nullptr, nullptr};
+  int *d_inc_ptr = nullptr, *d_inc_item = nullptr, *d_row_space = nullptr, *d_row_ent = nullptr;
+  int *d_ia = nullptr, *d_ja = nullptr;
+  double *d_coef[kMaxTerms] = {};
+  unsigned char *d_active[kMaxTerms] = {};
+  DevPlan plan{};
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  double ms_total = 0, ms_geo = 0, ms_gather = 0;
+
+  ~gdp_fe()
+  {
+    dfree(d_etype);
+    dfree(d_region);
+    dfree(d_conn);
+    dfree(d_edges);
+    dfree(d_geokind);
+    dfree(d_xyz);
+    dfree(d_geo);
+    for(int s = 0; s < kMaxSpaces; s++) {
+      dfree(d_dofnum[s]);
+      dfree(d_fixval[s]);
+    }
+    dfree(d_inc_ptr);
+    dfree(d_inc_item);
+    dfree(d_row_space);
+    dfree(d_row_ent);
+    dfree(d_ia);
+    dfree(d_ja);
+    for(int t = 0; t < kMaxTerms; t++) {
+      dfree(d_coef[t]);
+      dfree(d_active[t]);
+    }
+    for(auto &e : ev)
+      if(e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+bool contains(int n, const int *l, int v)
+{
+  for(int i = 0; i < n; i++)
+    if(l[i] == v) return true;
+  return false;
+}
+
+// Entity k of element e in a space: node number or |edge number|.
+inline int entity_of(const gdp_fe &F, int bf, int e, int k)
+{
+  if(bf == GDP_BF_NODE) return F.h_conn[(size_t)e * 4 + k];
+  int v = F.h_edges[(size_t)e * 6 + k];
+  return v < 0 ? -v : v;
+}
+
+template <typename Fn> void parallel_for(int n, Fn fn)
+{
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if(n < 4096 || nt == 1) {
+    fn(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  int chunk = (n + (int)nt - 1) / (int)nt;
+  for(unsigned t = 0; t < nt; t++) {
+    int lo = (int)t * chunk, hi = std::min(n, lo + chunk);
+    if(lo >= hi) break;
+    th.emplace_back([=]() { fn(lo, hi); });
+  }
+  for(auto &x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
+                             const gdp_space_desc *spaces, int nterms,
+                             const gdp_term_desc *terms, gdp_fe **out)
+{
+  if(!mesh || !spaces || !terms || !out || nspaces < 1 || nterms < 1)
+    return set_error(GDP_E_ARG, "gdp_fe_create: bad arguments");
+  if(nspaces > kMaxSpaces || nterms > kMaxTerms)
+    return set_error(GDP_E_UNSUPPORTED, "at most %d spaces and %d terms", kMaxSpaces, kMaxTerms);
+  std::unique_ptr<gdp_fe> F(new gdp_fe);
+  F->nelem = mesh->nelem;
+  F->nnodes = mesh->nnodes;
+  F->nedges = mesh->edges ? mesh->nedges : 0;
+  F->nmax = std::max(F->nnodes, F->nedges);
+  F->nspaces = nspaces;
+  F->nterms = nterms;
+  const int E = F->nelem;
+  F->h_etype.assign(mesh->etype, mesh->etype + E);
+  F->h_region.assign(mesh->region, mesh->region + E);
+  F->h_conn.assign(mesh->conn, mesh->conn + (size_t)E * 4);
+  if(mesh->edges) F->h_edges.assign(mesh->edges, mesh->edges + (size_t)E * 6);
+  int maxreg = 0;
+  for(int e = 0; e < E; e++) {
+    if(F->h_region[e] < 0) return set_error(GDP_E_ARG, "negative region id");
+    maxreg = std::max(maxreg, F->h_region[e]);
+  }
+  F->nreg = maxreg + 1;
+
+  // ---- classification (Cal_InitGalerkinTermOfFemEquation analogue) ----
+  int nedge_sp = 0;
+  for(int s = 0; s < nspaces; s++) {
+    if(spaces[s].bf == GDP_BF_EDGE) nedge_sp++;
+    else if(spaces[s].bf != GDP_BF_NODE)
+      return set_error(GDP_E_UNSUPPORTED, "space %d: unknown basis function", s);
+  }
+  if(nedge_sp && (nspaces != 1 || !mesh->edges))
+    return set_error(GDP_E_UNSUPPORTED, "edge spaces: exactly one space and mesh edges needed");
+  F->family = nedge_sp ? FAM_EDGE : FAM_NODAL;
+  F->h_geokind.assign(E, GEO_NONE);
+  for(int ti = 0; ti < nterms; ti++) {
+    const gdp_term_desc &t = terms[ti];
+    if(t.equ < 0 || t.equ >= nspaces || t.dof < -1 || t.dof >= nspaces)
+      return set_error(GDP_E_ARG, "term %d: bad space index", ti);
+    bool src = (t.kind == GDP_T_SRC_NODE || t.kind == GDP_T_SRC_EDGE);
+    if(src != (t.dof < 0)) return set_error(GDP_E_ARG, "term %d: source terms have no Dof", ti);
+    bool edgek = (t.kind == GDP_T_CURL_CURL || t.kind == GDP_T_EDGE_MASS || t.kind == GDP_T_SRC_EDGE);
+    bool nodek = (t.kind == GDP_T_GRAD_GRAD || t.kind == GDP_T_NODE_MASS || t.kind == GDP_T_SRC_NODE);
+    if(!edgek && !nodek) return set_error(GDP_E_UNSUPPORTED, "term %d: kind %d", ti, t.kind);
+    if(edgek != (F->family == FAM_EDGE))
+      return set_error(GDP_E_UNSUPPORTED, "term %d: mixes edge and nodal spaces", ti);
+    int need_coef = (t.kind == GDP_T_GRAD_GRAD) ? -1 : (t.kind == GDP_T_SRC_EDGE ? 3 : 1);
+    if(need_coef == -1 ? (t.ncoef != 1 && t.ncoef != 9) : t.ncoef != need_coef)
+      return set_error(GDP_E_UNSUPPORTED, "term %d: %d coefficient values", ti, t.ncoef);
+    if(t.coef_nreg < F->nreg) return set_error(GDP_E_ARG, "term %d: coefficient table too short", ti);
+    if(!t.coef || !t.regions) return set_error(GDP_E_ARG, "term %d: null table", ti);
+    for(int e = 0; e < E; e++) {
+      if(!contains(t.nregions, t.regions, F->h_region[e])) continue;
+      int et = F->h_etype[e], gk;
+      if(et == GDP_TET) {
+        if(t.jac_sur) return set_error(GDP_E_UNSUPPORTED, "term %d: Jacobian Sur on a tetrahedron", ti);
+        gk = GEO_TET;
+        if(t.npoints != 1 && t.npoints != 4)
+          return set_error(GDP_E_UNSUPPORTED, "term %d: %d Gauss points on tetrahedra", ti, t.npoints);
+      }
+      else if(et == GDP_TRI) {
+        if(edgek) return set_error(GDP_E_UNSUPPORTED, "term %d: edge elements on triangles", ti);
+        gk = t.jac_sur ? GEO_TRISUR : GEO_TRI2D;
+        if(t.jac_sur && t.kind != GDP_T_SRC_NODE)
+          return set_error(GDP_E_UNSUPPORTED, "term %d: Jacobian Sur only for sources", ti);
+        if(t.npoints != 1 && t.npoints != 3 && t.npoints != 4)
+          return set_error(GDP_E_UNSUPPORTED, "term %d: %d Gauss points on triangles", ti, t.npoints);
+      }
+      else
+        return set_error(GDP_E_UNSUPPORTED, "term %d: element type %d", ti, et);
+      if(F->h_geokind[e] != GEO_NONE && F->h_geokind[e] != gk)
+        return set_error(GDP_E_UNSUPPORTED, "element %d used with two Jacobian types", e + 1);
+      F->h_geokind[e] = gk;
+    }
+  }
+  for(int e = 0; e < E; e++)
+    for(int k = 0; k < nodes_of(F->h_etype[e]); k++) {
+      int nd = F->h_conn[(size_t)e * 4 + k];
+      if(F->h_geokind[e] != GEO_NONE && (nd < 1 || nd > F->nnodes))
+        return set_error(GDP_E_ARG, "element %d: node %d out of range", e + 1, nd);
+    }
+
+  // term activity per region: the element's region is in the term's regions
+  // and in the supports of the spaces the term uses (Get_DofOfElement.cpp:102-110)
+  std::vector<std::vector<unsigned char>> active(nterms, std::vector<unsigned char>(F->nreg, 0));
+  for(int ti = 0; ti < nterms; ti++) {
+    const gdp_term_desc &t = terms[ti];
+    for(int r = 0; r < F->nreg; r++) {
+      bool on = contains(t.nregions, t.regions, r) &&
+                contains(spaces[t.equ].nsupport, spaces[t.equ].support, r) &&
+                (t.dof < 0 || contains(spaces[t.dof].nsupport, spaces[t.dof].support, r));
+      active[ti][r] = on;
+    }
+  }
+
+  // ---- DOF numbering (DofData.cpp:36-45, 1279-1323) ----
+  const int nmax = F->nmax;
+  F->h_dofnum.assign(nspaces, std::vector<int>((size_t)nmax + 1, 0));
+  for(int e = 0; e < E; e++) {
+    int reg = F->h_region[e];
+    for(int ti = 0; ti < nterms; ti++) {
+      if(!active[ti][reg]) continue;
+      int sps[2] = {terms[ti].equ, terms[ti].dof};
+      for(int q = 0; q < 2; q++) {
+        int s = sps[q];
+        if(s < 0) continue;
+        int nb = spaces[s].bf == GDP_BF_NODE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+        for(int k = 0; k < nb; k++) {
+          int ent = entity_of(*F, spaces[s].bf, e, k);
+          if(ent < 1 || ent > nmax) return set_error(GDP_E_ARG, "entity %d out of range", ent);
+          F->h_dofnum[s][ent] = 1;
+        }
+      }
+    }
+  }
+  for(int s = 0; s < nspaces; s++)
+    for(int f = 0; f < spaces[s].nfixed; f++) {
+      int ent = spaces[s].fixed_entity[f];
+      if(ent >= 1 && ent <= nmax && F->h_dofnum[s][ent]) F->h_dofnum[s][ent] = -(f + 1);
+    }
+  std::vector<int> order(nspaces);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return spaces[x].numtype < spaces[y].numtype; });
+  std::vector<int> row_space, row_ent;
+  int N = 0;
+  for(int s : order)
+    for(int ent = 1; ent <= nmax; ent++)
+      if(F->h_dofnum[s][ent] > 0) {
+        F->h_dofnum[s][ent] = ++N;
+        row_space.push_back(s);
+        row_ent.push_back(ent);
+      }
+  F->N = N;
+
+  // ---- incidence: entity -> (element, local index), elements ascending ----
+  int space_bf = spaces[0].bf;
+  std::vector<int> inc_ptr((size_t)nmax + 2, 0), inc_item;
+  auto elem_used = [&](int e) {
+    int reg = F->h_region[e];
+    for(int ti = 0; ti < nterms; ti++)
+      if(active[ti][reg]) return true;
+    return false;
+  };
+  for(int e = 0; e < E; e++) {
+    if(!elem_used(e)) continue;
+    int nb = space_bf == GDP_BF_NODE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+    for(int k = 0; k < nb; k++) inc_ptr[entity_of(*F, space_bf, e, k) + 1]++;
+  }
+  for(int i = 0; i <= nmax; i++) inc_ptr[i + 1] += inc_ptr[i];
+  inc_item.resize(inc_ptr[nmax + 1]);
+  {
+    std::vector<int> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+    for(int e = 0; e < E; e++) {
+      if(!elem_used(e)) continue;
+      int nb = space_bf == GDP_BF_NODE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+      for(int k = 0; k < nb; k++) inc_item[fill[entity_of(*F, space_bf, e, k)]++] = e * 8 + k;
+    }
+  }
+
+  // ---- CSR pattern: all unknown DOF pairs sharing an element in a bilinear
+  // term, stored zeros included (F6; csr_format sorts columns) ----
+  std::vector<int> rowlen(N, 0);
+  std::vector<std::vector<int>> rowcols(N);
+  parallel_for(N, [&](int lo, int hi) {
+    std::vector<int> cand;
+    for(int r = lo; r < hi; r++) {
+      int s = row_space[r], ent = row_ent[r];
+      cand.clear();
+      for(int p = inc_ptr[ent]; p < inc_ptr[ent + 1]; p++) {
+        int e = inc_item[p] >> 3;
+        int reg = F->h_region[e];
+        for(int ti = 0; ti < nterms; ti++) {
+          const gdp_term_desc &t = terms[ti];
+          if(t.dof < 0 || t.equ != s || !active[ti][reg]) continue;
+          int nb = spaces[t.dof].bf == GDP_BF_NODE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+          for(int j = 0; j < nb; j++) {
+            int c = F->h_dofnum[t.dof][entity_of(*F, spaces[t.dof].bf, e, j)];
+            if(c > 0) cand.push_back(c - 1);
+          }
+        }
+      }
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      rowcols[r] = cand;
+      rowlen[r] = (int)cand.size();
+    }
+  });
+  F->h_ia.assign((size_t)N + 1, 0);
+  for(int r = 0; r < N; r++) {
+    if(!rowlen[r]) return set_error(GDP_E_ZERO_ROW, "Zero row in matrix (DOF %d)", r + 1);
+    F->h_ia[r + 1] = F->h_ia[r] + rowlen[r];
+  }
+  F->nnz = F->h_ia[N];
+  if(F->nnz > 2147483647L) return set_error(GDP_E_UNSUPPORTED, "nnz exceeds int32");
+  F->h_ja.resize((size_t)F->nnz);
+  parallel_for(N, [&](int lo, int hi) {
+    for(int r = lo; r < hi; r++)
+      std::copy(rowcols[r].begin(), rowcols[r].end(), F->h_ja.begin() + F->h_ia[r]);
+  });
+  rowcols.clear();
+  rowcols.shrink_to_fit();
+
+  // ---- keep what the device upload needs (done on first Generate) ----
+  F->h_inc_ptr.swap(inc_ptr);
+  F->h_inc_item.swap(inc_item);
+  F->h_row_space.swap(row_space);
+  F->h_row_ent.swap(row_ent);
+  F->h_xyz.assign(mesh->xyz, mesh->xyz + (size_t)F->nnodes * 3);
+  F->h_fixval.resize(nspaces);
+  for(int s = 0; s < nspaces; s++) {
+    F->h_fixval[s].assign(std::max(1, spaces[s].nfixed), 0.);
+    for(int f = 0; f < spaces[s].nfixed; f++) F->h_fixval[s][f] = spaces[s].fixed_value[f];
+  }
+  F->h_active.swap(active);
+  F->h_coef.resize(nterms);
+  F->h_terms.assign(terms, terms + nterms);
+  for(int ti = 0; ti < nterms; ti++) {
+    F->h_coef[ti].assign(terms[ti].coef, terms[ti].coef + (size_t)F->nreg * terms[ti].ncoef);
+    F->h_terms[ti].coef = nullptr;
+    F->h_terms[ti].regions = nullptr;
+  }
+  *out = F.release();
+  return GDP_OK;
+}
+
+static int fe_upload(gdp_fe *F)
+{
+  if(F->uploaded) return GDP_OK;
+  GDP_TRY(ensure_ready());
+  const int E = F->nelem, N = F->N, nmax = F->nmax, nspaces = F->nspaces, nterms = F->nterms;
+  GDP_TRY(dalloc(&F->d_etype, E));
+  GDP_TRY(dalloc(&F->d_region, E));
+  GDP_TRY(dalloc(&F->d_conn, (size_t)E * 4));
+  GDP_TRY(dalloc(&F->d_geokind, E));
+  GDP_TRY(dalloc(&F->d_xyz, (size_t)F->nnodes * 3));
+  GDP_TRY(dalloc(&F->d_geo, (size_t)E * kGeoStride));
+  GDP_TRY(h2d(F->d_etype, F->h_etype.data(), E));
+  GDP_TRY(h2d(F->d_region, F->h_region.data(), E));
+  GDP_TRY(h2d(F->d_conn, F->h_conn.data(), (size_t)E * 4));
+  GDP_TRY(h2d(F->d_geokind, F->h_geokind.data(), E));
+  GDP_TRY(h2d(F->d_xyz, F->h_xyz.data(), (size_t)F->nnodes * 3));
+  if(F->family == FAM_EDGE) {
+    GDP_TRY(dalloc(&F->d_edges, (size_t)E * 6));
+    GDP_TRY(h2d(F->d_edges, F->h_edges.data(), (size_t)E * 6));
+  }
+  for(int s = 0; s < nspaces; s++) {
+    GDP_TRY(dalloc(&F->d_dofnum[s], (size_t)nmax + 1));
+    GDP_TRY(h2d(F->d_dofnum[s], F->h_dofnum[s].data(), (size_t)nmax + 1));
+    GDP_TRY(dalloc(&F->d_fixval[s], F->h_fixval[s].size()));
+    GDP_TRY(h2d(F->d_fixval[s], F->h_fixval[s].data(), F->h_fixval[s].size()));
+  }
+  GDP_TRY(dalloc(&F->d_inc_ptr, F->h_inc_ptr.size()));
+  GDP_TRY(dalloc(&F->d_inc_item, std::max<size_t>(1, F->h_inc_item.size())));
+  GDP_TRY(h2d(F->d_inc_ptr, F->h_inc_ptr.data(), F->h_inc_ptr.size()));
+  GDP_TRY(h2d(F->d_inc_item, F->h_inc_item.data(), F->h_inc_item.size()));
+  GDP_TRY(dalloc(&F->d_row_space, std::max(1, N)));
+  GDP_TRY(dalloc(&F->d_row_ent, std::max(1, N)));
+  GDP_TRY(h2d(F->d_row_space, F->h_row_space.data(), (size_t)N));
+  GDP_TRY(h2d(F->d_row_ent, F->h_row_ent.data(), (size_t)N));
+  GDP_TRY(dalloc(&F->d_ia, (size_t)N + 1));
+  GDP_TRY(dalloc(&F->d_ja, std::max<size_t>(1, (size_t)F->nnz)));
+  GDP_TRY(h2d(F->d_ia, F->h_ia.data(), (size_t)N + 1));
+  GDP_TRY(h2d(F->d_ja, F->h_ja.data(), (size_t)F->nnz));
+  DevPlan &P = F->plan;
+  P.family = F->family;
+  P.nspaces = nspaces;
+  P.nterms = nterms;
+  P.N = N;
+  P.nreg = F->nreg;
+  P.etype = F->d_etype;
+  P.region = F->d_region;
+  P.conn = F->d_conn;
+  P.edges = F->d_edges;
+  P.geokind = F->d_geokind;
+  P.geo = F->d_geo;
+  for(int s = 0; s < nspaces; s++) {
+    P.dofnum[s] = F->d_dofnum[s];
+    P.fixval[s] = F->d_fixval[s];
+  }
+  P.inc_ptr = F->d_inc_ptr;
+  P.inc_item = F->d_inc_item;
+  P.row_space = F->d_row_space;
+  P.row_ent = F->d_row_ent;
+  P.ia = F->d_ia;
+  P.ja = F->d_ja;
+  for(int ti = 0; ti < nterms; ti++) {
+    const gdp_term_desc &t = F->h_terms[ti];
+    GDP_TRY(dalloc(&F->d_coef[ti], F->h_coef[ti].size()));
+    GDP_TRY(h2d(F->d_coef[ti], F->h_coef[ti].data(), F->h_coef[ti].size()));
+    GDP_TRY(dalloc(&F->d_active[ti], (size_t)F->nreg));
+    GDP_TRY(h2d(F->d_active[ti], F->h_active[ti].data(), (size_t)F->nreg));
+    DevTerm &d = P.term[ti];
+    d.kind = t.kind;
+    d.equ = t.equ;
+    d.dof = t.dof;
+    d.npoints = t.npoints;
+    d.ncoef = t.ncoef;
+    d.coef = F->d_coef[ti];
+    d.active = F->d_active[ti];
+  }
+  for(auto &e : F->ev) GDP_HIP(hipEventCreate(&e));
+  GDP_TRY(sync());
+  F->uploaded = true;
+  return GDP_OK;
+}
+
+extern "C" int gdp_fe_destroy(gdp_fe *fe)
+{
+  if(!fe) return GDP_OK;
+  (void)sync();
+  delete fe;
+  return GDP_OK;
+}
+
+extern "C" int gdp_fe_num_dofs(const gdp_fe *fe, int *N, long *nnz)
+{
+  if(!fe) return set_error(GDP_E_ARG, "null fe");
+  if(N) *N = fe->N;
+  if(nnz) *nnz = fe->nnz;
+  return GDP_OK;
+}
+
+extern "C" int gdp_fe_dof_table(const gdp_fe *fe, int space, int *dofnum, int nmax)
+{
+  if(!fe || space < 0 || space >= fe->nspaces || !dofnum || nmax != fe->nmax)
+    return set_error(GDP_E_ARG, "gdp_fe_dof_table: bad arguments (nmax %d != %d)", nmax,
+                     fe ? fe->nmax : -1);
+  std::copy(fe->h_dofnum[space].begin(), fe->h_dofnum[space].end(), dofnum);
+  return GDP_OK;
+}
+
+extern "C" int gdp_fe_generate(gdp_fe *fe, gdp_matrix *A, gdp_vector *b)
+{
+  if(!fe || !A || !b) return set_error(GDP_E_ARG, "gdp_fe_generate: null argument");
+  if(A->n != fe->N || b->n != fe->N)
+    return set_error(GDP_E_ARG, "gdp_fe_generate: system size %d, matrix %d, vector %d", fe->N,
+                     A->n, b->n);
+  GDP_TRY(fe_upload(fe));
+  hipStream_t s = ctx().stream;
+  // adopt the formulation's pattern (LinAlg_CreateMatrix + first Generate)
+  if(A->csr.nnz != fe->nnz || A->h_ia != fe->h_ia || A->h_ja != fe->h_ja) {
+    A->csr.release();
+    A->csr.n = fe->N;
+    A->csr.nnz = fe->nnz;
+    GDP_TRY(dalloc(&A->csr.ia, (size_t)fe->N + 1));
+    GDP_TRY(dalloc(&A->csr.ja, std::max<size_t>(1, (size_t)fe->nnz)));
+    GDP_TRY(dalloc(&A->csr.a, std::max<size_t>(1, (size_t)fe->nnz)));
+    GDP_HIP(hipMemcpyAsync(A->csr.ia, fe->d_ia, sizeof(int) * (fe->N + 1), hipMemcpyDeviceToDevice, s));
+    GDP_HIP(hipMemcpyAsync(A->csr.ja, fe->d_ja, sizeof(int) * fe->nnz, hipMemcpyDeviceToDevice, s));
+    A->h_ia = fe->h_ia;
+    A->h_ja = fe->h_ja;
+    static uint64_t pv = 1ull << 40;
+    A->pattern_version = ++pv;
+  }
+  A->staged.clear();
+  b->staged.clear();
+  GDP_HIP(hipEventRecord(fe->ev[0], s));
+  if(fe->nnz) GDP_HIP(hipMemsetAsync(A->csr.a, 0, sizeof(double) * fe->nnz, s));
+  if(fe->nelem)
+    hipLaunchKernelGGL(k_geometry, dim3((fe->nelem + 255) / 256), dim3(256), 0, s, fe->nelem,
+                       fe->d_geokind, fe->d_conn, fe->d_xyz, fe->d_geo);
+  GDP_HIP(hipGetLastError());
+  GDP_HIP(hipEventRecord(fe->ev[1], s));
+  if(fe->N) {
+    dim3 grid((fe->N + 127) / 128);
+    if(fe->family == FAM_NODAL)
+      hipLaunchKernelGGL(k_gather_nodal, grid, dim3(128), 0, s, fe->plan, A->csr.a, b->d);
+    else
+      hipLaunchKernelGGL(k_gather_edge, grid, dim3(128), 0, s, fe->plan, A->csr.a, b->d);
+    GDP_HIP(hipGetLastError());
+  }
+  GDP_HIP(hipEventRecord(fe->ev[2], s));
+  GDP_TRY(sync());
+  float t01 = 0, t12 = 0;
+  GDP_HIP(hipEventElapsedTime(&t01, fe->ev[0], fe->ev[1]));
+  GDP_HIP(hipEventElapsedTime(&t12, fe->ev[1], fe->ev[2]));
+  fe->ms_geo = t01;
+  fe->ms_gather = t12;
+  fe->ms_total = t01 + t12;
+  static uint64_t vv = 1ull << 50;
+  A->value_version = ++vv;
+  return GDP_OK;
+}
+
+extern "C" int gdp_fe_last_timing(const gdp_fe *fe, double *ms_total, double *ms_geometry,
+                                  double *ms_gather)
+{
+  if(!fe) return set_error(GDP_E_ARG, "null fe");
+  if(ms_total) *ms_total = fe->ms_total;
+  if(ms_geometry) *ms_geometry = fe->ms_geo;
+  if(ms_gather) *ms_gather = fe->ms_gather;
+  return GDP_OK;
+}
+
+extern "C" int gdp_fe_pattern(const gdp_fe *fe, int *ia, int *ja)
+{
+  if(!fe) return set_error(GDP_E_ARG, "null fe");
+  if(ia) std::copy(fe->h_ia.begin(), fe->h_ia.end(), ia);
+  if(ja) std::copy(fe->h_ja.begin(), fe->h_ja.end(), ja);
+  return GDP_OK;
+}

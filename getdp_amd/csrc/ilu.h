@@ -1,0 +1,38 @@
+// ILU(0) factor object shared by the solve driver and the Krylov methods.
+#pragma once
+
+#include "gdp_device.h"
+
+namespace gdp {
+
+struct IluFactor {
+  int n = 0;
+  long nnz = 0;
+  const int *ia = nullptr, *ja = nullptr;  // pattern (not owned)
+  int *diag = nullptr;                      // diagonal position per row
+  float *lu32 = nullptr;
+  double *lu64 = nullptr;
+  bool is_f64 = false;
+  int nlevels = 0, upper_nlevels = 0;
+  std::vector<int> level_ptr;               // host: rows of level l
+  int *level_rows = nullptr;
+  double *tmp64 = nullptr;                  // forward-sweep output (sentinel-filled)
+  float *tmp32 = nullptr;
+  int *err = nullptr;                       // [0] zero pivot, [1] timeout, [2] missing diag
+
+  ~IluFactor() { release(); }
+  void release();
+  // Dependency levels and diagonal positions for pattern B (kept by pointer).
+  int analyse(const DevCSR &B, const std::vector<int> &hia, const std::vector<int> &hja);
+  // Factor values a (B's CSR layout); FP32 or FP64 storage.
+  int factor(const double *a, bool f64, int *zero_pivot_row);
+  // x = (LU)^{-1} (scale * r); may be enqueued repeatedly without host sync.
+  // gate/gmode: skip when a Krylov control block says the work has ended.
+  template <typename TV>
+  int apply(const TV *r, TV scale, TV *x, const int *gate = nullptr, int gmode = 0);
+  int reset_sweep_state();
+  int check_timeout();
+  unsigned long long wait_limit() const;
+};
+
+}  // namespace gdp

@@ -1,0 +1,380 @@
+// ILU(0) on the device: factorisation and the two triangular sweeps.
+//
+// Storage: the factors live in the CSR pattern of the (renumbered) matrix:
+// strictly-lower entries hold L (unit diagonal implied), the diagonal entry
+// holds the inverted pivot, strictly-upper entries hold U -- the same
+// information as the reference's MSR arrays alu/jlu/ju (ilut.F:1818-1941),
+// in the matrix's own layout so that no second index structure is needed.
+// Factor values are stored in T = float (the reference's default
+// HAVE_ILU_FLOAT, CMakeLists.txt:552) or double.
+//
+// Factorisation: rows are grouped into dependency levels (host analysis);
+// one launch per level processes its rows in parallel, each row performing
+// exactly the reference's sequence of operations (compiled without
+// contraction), so the factors are bit-identical to the reference's.
+//
+// Sweeps (lusol, ilut.F:2297-2347): one thread per row, rows taken in
+// dependency order; a row reads the solution entries it depends on as soon as
+// they are published.  Outputs are pre-filled with a signalling-NaN sentinel
+// that arithmetic never produces; a published entry is any non-sentinel
+// value, read and written with agent-scope atomics (one 64-bit access carries
+// both the value and its readiness).  Every wait is bounded by a wall-clock
+// limit: on expiry the kernel records an error flag and exits.
+#include "ilu.h"
+
+namespace gdp {
+
+namespace {
+
+// ---- factorisation -----------------------------------------------------------
+
+template <typename T> __device__ inline double prod_ll(T a, T b);
+template <> __device__ inline double prod_ll<float>(float a, float b)
+{
+  float p = a * b;  // real*4 * real*4 is formed in single precision
+  return (double)p;
+}
+template <> __device__ inline double prod_ll<double>(double a, double b) { return a * b; }
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ilu_copy(long nnz, const double *__restrict__ a,
+                                                  T *__restrict__ lu)
+{
+  long k = (long)blockIdx.x * 256 + threadIdx.x;
+  if(k < nnz) lu[k] = (T)a[k];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(128) k_ilu_level(int nrows, const int *__restrict__ rows,
+                                                   const int *__restrict__ ia,
+                                                   const int *__restrict__ ja,
+                                                   const int *__restrict__ diag,
+                                                   T *__restrict__ lu, int *__restrict__ pivot_err)
+{
+  int t = blockIdx.x * 128 + threadIdx.x;
+  if(t >= nrows) return;
+  const int i = rows[t];
+  const int rs = ia[i], re = ia[i + 1], di = diag[i];
+  for(int k = rs; k < di; k++) {
+    const int jrow = ja[k];
+    const double tl = prod_ll<T>(lu[k], lu[diag[jrow]]);
+    lu[k] = (T)tl;
+    // subtract tl * U(jrow, :) from the matching entries of row i
+    int q = k + 1;
+    const int je = ia[jrow + 1];
+    for(int jj = diag[jrow] + 1; jj < je; jj++) {
+      const int c = ja[jj];
+      while(q < re && ja[q] < c) q++;
+      if(q >= re) break;
+      if(ja[q] == c) lu[q] = (T)((double)lu[q] - tl * (double)lu[jj]);
+    }
+  }
+  const T piv = lu[di];
+  if(piv == (T)0) {
+    atomicMin(pivot_err, i + 1);
+    return;
+  }
+  lu[di] = (T)(1.0 / (double)piv);
+}
+
+// ---- sweeps -----------------------------------------------------------------------
+template <typename T> struct Sentinel;
+template <> struct Sentinel<double> {
+  using U = unsigned long long;
+  static constexpr U bits = 0x7FF0DEADBEEF0001ULL;  // signalling NaN
+};
+template <> struct Sentinel<float> {
+  using U = unsigned int;
+  static constexpr U bits = 0x7F80BEEFu;  // signalling NaN
+};
+
+template <typename T> __device__ inline typename Sentinel<T>::U ld_bits(const T *p)
+{
+  using U = typename Sentinel<T>::U;
+  return __hip_atomic_load((const U *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ inline void st_val(T *p, T v)
+{
+  using U = typename Sentinel<T>::U;
+  U b;
+  __builtin_memcpy(&b, &v, sizeof(T));
+  __hip_atomic_store((U *)p, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ inline T as_val(typename Sentinel<T>::U b)
+{
+  T v;
+  __builtin_memcpy(&v, &b, sizeof(T));
+  return v;
+}
+
+// Kernels enqueued inside a Krylov cycle are skipped once the solver's
+// control block says the solve (gmode 1) or the cycle (gmode 2) has ended;
+// gate[0] is the done flag, gate[3] the end-of-cycle flag.
+__device__ inline bool is_gated(const int *gate, int gmode)
+{
+  if(!gmode) return false;
+  if(gmode == 1) return gate[0] != 0;
+  return (gate[0] | gate[3]) != 0;
+}
+
+// Bounded wait: a thread may spend at most tlimit wall-clock ticks waiting
+// in one sweep; once any thread has given up, every other waiter gives up
+// too, so a sweep always ends.
+__device__ inline bool wait_expired(unsigned long long &t0, unsigned long long tlimit, int *err)
+{
+  unsigned long long now = wall_clock64();
+  if(!t0) {
+    t0 = now;
+    return false;
+  }
+  if(now - t0 > tlimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    atomicOr(err, 1);
+    return true;
+  }
+  return false;
+}
+
+// Forward sweep: y = L^{-1} r (unit lower), rows ascending.  Also fills
+// `next` (the backward sweep's output) with the sentinel.
+template <typename TF, typename TV>
+__global__ void __launch_bounds__(256) k_fwd(int n, const int *__restrict__ ia,
+                                             const int *__restrict__ ja,
+                                             const int *__restrict__ diag,
+                                             const TF *__restrict__ lu, const TV *__restrict__ r,
+                                             TV scale, TV *y, TV *next, int *err,
+                                             unsigned long long tlimit, const int *gate,
+                                             int gmode)
+{
+  if(is_gated(gate, gmode)) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if(i >= n) return;
+  TV acc = r[i] * scale;
+  int k = ia[i];
+  const int kend = diag[i];
+  st_val(next + i, as_val<TV>(Sentinel<TV>::bits));
+  bool done = false;
+  unsigned long long t0 = 0;
+  while(!done) {
+    if(k < kend) {
+      auto b = ld_bits(y + ja[k]);
+      if(b != Sentinel<TV>::bits) {
+        acc = acc - (TV)lu[k] * as_val<TV>(b);
+        k++;
+      }
+      else {
+        if(wait_expired(t0, tlimit, err)) k = kend;
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if(k >= kend) {
+      st_val(y + i, acc);
+      done = true;
+    }
+  }
+}
+
+// Backward sweep: x = U^{-1} y with inverted stored pivots, rows descending.
+// Resets y[i] to the sentinel once consumed (ready for the next forward sweep).
+template <typename TF, typename TV>
+__global__ void __launch_bounds__(256) k_bwd(int n, const int *__restrict__ ia,
+                                             const int *__restrict__ ja,
+                                             const int *__restrict__ diag,
+                                             const TF *__restrict__ lu, TV *y, TV *x, int *err,
+                                             unsigned long long tlimit, const int *gate,
+                                             int gmode)
+{
+  if(is_gated(gate, gmode)) return;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if(t >= n) return;
+  const int i = n - 1 - t;
+  TV acc = y[i];
+  st_val(y + i, as_val<TV>(Sentinel<TV>::bits));
+  const int di = diag[i];
+  int k = di + 1;
+  const int kend = ia[i + 1];
+  bool done = false;
+  unsigned long long t0 = 0;
+  while(!done) {
+    if(k < kend) {
+      auto b = ld_bits(x + ja[k]);
+      if(b != Sentinel<TV>::bits) {
+        acc = acc - (TV)lu[k] * as_val<TV>(b);
+        k++;
+      }
+      else {
+        if(wait_expired(t0, tlimit, err)) k = kend;
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if(k >= kend) {
+      st_val(x + i, (TV)lu[di] * acc);
+      done = true;
+    }
+  }
+}
+
+template <typename T> __global__ void k_fill_bits(long n, T *p)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if(i < n) p[i] = as_val<T>(Sentinel<T>::bits);
+}
+
+__global__ void k_diag_pos(int n, const int *ia, const int *ja, int *diag, int *missing)
+{
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if(i >= n) return;
+  int lo = ia[i], hi = ia[i + 1];
+  while(lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if(ja[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  if(lo < ia[i + 1] && ja[lo] == i) diag[i] = lo;
+  else {
+    diag[i] = lo;
+    atomicMin(missing, i + 1);
+  }
+}
+
+}  // namespace
+
+// ---- host side ------------------------------------------------------------------------
+
+void IluFactor::release()
+{
+  dfree(lu32);
+  dfree(lu64);
+  dfree(diag);
+  dfree(level_rows);
+  dfree(tmp64);
+  dfree(tmp32);
+  dfree(err);
+  level_ptr.clear();
+  n = 0;
+}
+
+int IluFactor::analyse(const DevCSR &B, const std::vector<int> &hia, const std::vector<int> &hja)
+{
+  release();
+  n = B.n;
+  nnz = B.nnz;
+  ia = B.ia;
+  ja = B.ja;
+  std::vector<int> lev;
+  nlevels = lower_levels(n, hia.data(), hja.data(), lev);
+  std::vector<int> cnt(nlevels + 1, 0);
+  for(int i = 0; i < n; i++) cnt[lev[i] + 1]++;
+  for(int l = 0; l < nlevels; l++) cnt[l + 1] += cnt[l];
+  level_ptr = cnt;
+  std::vector<int> rows(n);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for(int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
+  std::vector<int> ulev;
+  upper_nlevels = upper_levels(n, hia.data(), hja.data(), ulev);
+  GDP_TRY(dalloc(&level_rows, (size_t)std::max(1, n)));
+  GDP_TRY(h2d(level_rows, rows.data(), (size_t)n));
+  GDP_TRY(dalloc(&diag, (size_t)std::max(1, n)));
+  GDP_TRY(dalloc(&err, 4));
+  int init[4] = {0x7fffffff, 0, 0x7fffffff, 0};
+  GDP_TRY(h2d(err, init, 4));
+  if(n)
+    hipLaunchKernelGGL(k_diag_pos, dim3((n + 255) / 256), dim3(256), 0, ctx().stream, n, ia, ja,
+                       diag, err + 2);
+  GDP_HIP(hipGetLastError());
+  int h[4];
+  GDP_TRY(d2h(h, err, 4));
+  if(h[2] != 0x7fffffff)
+    return set_error(GDP_E_ZERO_PIVOT, "Zero pivot on line %d in ILU (missing diagonal)", h[2]);
+  GDP_TRY(dalloc(&tmp64, (size_t)std::max(1, n)));
+  GDP_TRY(dalloc(&tmp32, (size_t)std::max(1, n)));
+  GDP_TRY(reset_sweep_state());
+  return GDP_OK;
+}
+
+int IluFactor::reset_sweep_state()
+{
+  hipStream_t s = ctx().stream;
+  if(n) {
+    hipLaunchKernelGGL(k_fill_bits<double>, dim3((n + 255) / 256), dim3(256), 0, s, (long)n, tmp64);
+    hipLaunchKernelGGL(k_fill_bits<float>, dim3((n + 255) / 256), dim3(256), 0, s, (long)n, tmp32);
+  }
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+int IluFactor::factor(const double *a, bool f64, int *zero_pivot_row)
+{
+  hipStream_t s = ctx().stream;
+  is_f64 = f64;
+  dfree(lu32);
+  dfree(lu64);
+  if(f64) GDP_TRY(dalloc(&lu64, (size_t)std::max<long>(1, nnz)));
+  else GDP_TRY(dalloc(&lu32, (size_t)std::max<long>(1, nnz)));
+  int init = 0x7fffffff;
+  GDP_TRY(h2d(err, &init, 1));
+  dim3 g((unsigned)((nnz + 255) / 256));
+  if(nnz) {
+    if(f64) hipLaunchKernelGGL(k_ilu_copy<double>, g, dim3(256), 0, s, nnz, a, lu64);
+    else hipLaunchKernelGGL(k_ilu_copy<float>, g, dim3(256), 0, s, nnz, a, lu32);
+  }
+  for(int l = 0; l < nlevels; l++) {
+    int cnt = level_ptr[l + 1] - level_ptr[l];
+    dim3 gl((cnt + 127) / 128);
+    if(f64)
+      hipLaunchKernelGGL(k_ilu_level<double>, gl, dim3(128), 0, s, cnt, level_rows + level_ptr[l],
+                         ia, ja, diag, lu64, err);
+    else
+      hipLaunchKernelGGL(k_ilu_level<float>, gl, dim3(128), 0, s, cnt, level_rows + level_ptr[l], ia,
+                         ja, diag, lu32, err);
+  }
+  GDP_HIP(hipGetLastError());
+  int h;
+  GDP_TRY(d2h(&h, err, 1));
+  *zero_pivot_row = (h == 0x7fffffff) ? 0 : h;
+  int zero = 0;
+  GDP_TRY(h2d(err + 1, &zero, 1));
+  return GDP_OK;
+}
+
+unsigned long long IluFactor::wait_limit() const
+{
+  return 100000000ull;  // wall_clock64 runs at 100 MHz: 1 s per dependency wait
+}
+
+template <typename TV>
+int IluFactor::apply(const TV *r, TV scale, TV *x, const int *gate, int gmode)
+{
+  hipStream_t s = ctx().stream;
+  if(!n) return GDP_OK;
+  TV *t;
+  if constexpr(sizeof(TV) == 8) t = tmp64;
+  else t = tmp32;
+  dim3 g((n + 255) / 256);
+  if(is_f64) {
+    hipLaunchKernelGGL((k_fwd<double, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu64, r, scale, t,
+                       x, err + 1, wait_limit(), gate, gmode);
+    hipLaunchKernelGGL((k_bwd<double, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu64, t, x,
+                       err + 1, wait_limit(), gate, gmode);
+  }
+  else {
+    hipLaunchKernelGGL((k_fwd<float, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu32, r, scale, t, x,
+                       err + 1, wait_limit(), gate, gmode);
+    hipLaunchKernelGGL((k_bwd<float, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu32, t, x, err + 1,
+                       wait_limit(), gate, gmode);
+  }
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+int IluFactor::check_timeout()
+{
+  int h = 0;
+  GDP_TRY(d2h(&h, err + 1, 1));
+  if(h) return set_error(GDP_E_TIMEOUT, "ILU sweep: a dependency wait exceeded its limit");
+  return GDP_OK;
+}
+
+template int IluFactor::apply<double>(const double *, double, double *, const int *, int);
+template int IluFactor::apply<float>(const float *, float, float *, const int *, int);
+
+}  // namespace gdp

@@ -1,0 +1,67 @@
+// Krylov solvers on the device (CG, GMRES, FGMRES of contrib/Sparskit/iters.f).
+#pragma once
+
+#include "gdp_device.h"
+#include "ilu.h"
+
+namespace gdp {
+
+constexpr int kMaxKrylov = 64;
+
+// Control block kept in device memory: the scalars of Sparskit's ipar/fpar
+// arrays that the iteration needs, the Givens rotations and the Hessenberg
+// matrix.  Only `done`, `status` and `its` are read back by the host.
+struct KCtl {
+  int done, status, its, cycle_end;
+  int kfinal, ierr12, ipar11, pend_flag;
+  int maxits, m, nb, pad;
+  double rtol, fpar3, fpar4, fpar5, fpar6, fpar7, alpha, beta;
+  double scal, nrm0, thr, pend;
+  unsigned ticket[8];
+  double g[kMaxKrylov + 2], c[kMaxKrylov + 1], s[kMaxKrylov + 1];
+  double h1[kMaxKrylov + 2], h2[kMaxKrylov + 2];
+  double H[(kMaxKrylov + 1) * (kMaxKrylov + 2)];  // column j: H[j*(kMaxKrylov+2) + i]
+};
+
+// Operator: y = A x with values of type TA, vectors of type TV.
+template <typename TA> struct DevOp {
+  int n = 0;
+  long nnz = 0;
+  const int *ia = nullptr, *ja = nullptr;
+  const TA *a = nullptr;
+};
+
+// Preconditioner applied as z = M^{-1} v (right preconditioning).
+enum PrecKind { PREC_NONE = 0, PREC_ILU = 1, PREC_USER = 2 };
+struct UserPrec;  // learned preconditioner (unet.hip)
+
+// z = M^{-1} v for the learned preconditioner; skipped when the control block
+// says so (gmode as for the other kernels of a cycle).
+int user_prec_apply(UserPrec *u, const double *in, double *out, KCtl *C, int gmode);
+
+template <typename TV> struct PrecOp {
+  PrecKind kind = PREC_NONE;
+  IluFactor *ilu = nullptr;
+  UserPrec *user = nullptr;
+};
+
+struct KrylovOptions {
+  int algorithm = 8;  // 1 CG, 8 GMRES, 9 FGMRES
+  int m = 30;
+  int maxits = 1000;
+  double rtol = 1e-10;
+  int orth = 0;  // 0 MGS + selective reorthogonalisation, 1 CGS2
+  bool use_graphs = true;
+};
+
+struct KrylovResult {
+  int its = 0, status = 0;
+  double res0 = 0, res_est = 0;
+};
+
+// Solve A sol = rhs from sol = 0 on the library stream.
+template <typename TA, typename TV>
+int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol,
+                 const KrylovOptions &opt, KrylovResult &res);
+
+}  // namespace gdp

@@ -1,0 +1,841 @@
+// Krylov solvers on the device, following contrib/Sparskit/iters.f:
+//   gmres  (iters.f:1912-2217)  restarted GMRES, right preconditioning
+//   fgmres (iters.f:2522-2807)  flexible GMRES (stores Z = M^{-1} V)
+//   cg     (iters.f:379-562)    Sparskit's CG on A M^{-1} (not PCG)
+// with x0 = 0 and the stopping rule of ipar(3) = 1: stop when the (estimated)
+// residual norm falls to Stopping_Test times the first residual norm.
+//
+// Every scalar decision (Givens rotations, residual estimate, restart,
+// stopping, breakdown) is taken on the device by the last block of a
+// reduction or by a one-thread kernel, so a whole restart cycle (or a batch
+// of CG iterations) is enqueued without host round trips and is captured once
+// into a hipGraph and replayed.  Kernels of a cycle that has already ended
+// return immediately.  Reductions are deterministic (fixed-order partial
+// sums); dot products are accumulated in FP64 whatever the vector type.
+#include <cmath>
+#include <cstring>
+
+#include "krylov.h"
+
+namespace gdp {
+
+namespace {
+
+constexpr int B = 256;        // threads per block for vector kernels
+constexpr int LDH = kMaxKrylov + 2;
+
+// gate modes: 0 none, 1 skip when done, 2 skip when done or cycle ended
+__device__ inline bool gated(const KCtl *C, int mode)
+{
+  if(mode == 0) return false;
+  if(mode == 1) return C->done != 0;
+  return (C->done | C->cycle_end) != 0;
+}
+
+template <int G, typename TA, typename TV>
+__global__ void __launch_bounds__(256) k_op(int n, const int *__restrict__ ia,
+                                            const int *__restrict__ ja, const TA *__restrict__ a,
+                                            const TV *__restrict__ x, TV *__restrict__ y,
+                                            const KCtl *C, int gmode)
+{
+  if(gated(C, gmode)) return;
+  long gt = (long)blockIdx.x * 256 + threadIdx.x;
+  int row = (int)(gt / G), lane = (int)(gt % G);
+  if(row >= n) return;
+  int s = ia[row], e = ia[row + 1];
+  double sum = 0;
+  for(int k = s + lane; k < e; k += G) {
+    TA v = __builtin_nontemporal_load(a + k);
+    int c = __builtin_nontemporal_load(ja + k);
+    sum += (double)v * (double)x[c];
+  }
+#pragma unroll
+  for(int o = G / 2; o > 0; o >>= 1) sum += __shfl_down(sum, o, G);
+  if(lane == 0) y[row] = (TV)sum;
+}
+
+template <typename TA, typename TV>
+int op_apply(const DevOp<TA> &A, const TV *x, TV *y, const KCtl *C, int gmode)
+{
+  int G = spmv_lanes(A.nnz, A.n);
+  long threads = (long)A.n * G;
+  dim3 grid((unsigned)((threads + 255) / 256));
+  hipStream_t s = ctx().stream;
+  switch(G) {
+  case 2: hipLaunchKernelGGL((k_op<2, TA, TV>), grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, y, C, gmode); break;
+  case 4: hipLaunchKernelGGL((k_op<4, TA, TV>), grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, y, C, gmode); break;
+  case 8: hipLaunchKernelGGL((k_op<8, TA, TV>), grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, y, C, gmode); break;
+  case 16: hipLaunchKernelGGL((k_op<16, TA, TV>), grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, y, C, gmode); break;
+  default: hipLaunchKernelGGL((k_op<32, TA, TV>), grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, y, C, gmode); break;
+  }
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+// ---- GMRES pieces -----------------------------------------------------------------
+
+// label 10/20 of gmres: w = rhs - A*sol (A*sol already in w), alpha = |w|,
+// first-iteration tolerance, convergence test before the cycle.
+template <typename TV>
+__global__ void __launch_bounds__(B) k_gm_resid(int n, const TV *__restrict__ rhs, TV *w, KCtl *C,
+                                               double *part)
+{
+  if(C->done) return;
+  double s = 0;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B) {
+    TV v = rhs[i] - w[i];
+    w[i] = v;
+    s += (double)v * (double)v;
+  }
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+  if(grid_last_block(&C->ticket[0])) {
+    double tot = sum_partials<double, B>(part, gridDim.x);
+    if(threadIdx.x == 0) {
+      double alpha = sqrt(tot);
+      C->its++;
+      if(C->its == 1) {
+        C->fpar4 = C->rtol * alpha + 0.0;
+        C->fpar3 = alpha;
+      }
+      C->fpar5 = alpha;
+      C->g[0] = alpha;
+      C->cycle_end = 0;
+      C->kfinal = 0;
+      C->ierr12 = 0;
+      C->pend_flag = 0;
+      if(alpha <= C->fpar4) {
+        C->status = 0;
+        C->fpar6 = alpha;
+        C->done = 1;
+      }
+      else
+        C->scal = 1.0 / alpha;
+    }
+  }
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(B) k_scale(int n, TV *v, const KCtl *C, int gmode)
+{
+  if(gated(C, gmode)) return;
+  double f = C->scal;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B)
+    v[i] = (TV)(f * (double)v[i]);
+}
+
+// |v|^2 of the new Krylov vector (mgsro's nrm0, iters.f:3532-3541).
+template <typename TV>
+__global__ void __launch_bounds__(B) k_nrm0(int n, const TV *v, KCtl *C, double *part)
+{
+  if(gated(C, 2)) return;
+  double s = 0;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B)
+    s += (double)v[i] * (double)v[i];
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+  if(grid_last_block(&C->ticket[1])) {
+    double tot = sum_partials<double, B>(part, gridDim.x);
+    if(threadIdx.x == 0) {
+      C->nrm0 = tot;
+      C->thr = tot * 0.98;
+      C->pend_flag = 0;
+      if(tot <= 0.0) C->ierr12 = -1;
+      else if(!(tot > 0.0 && 1.0 / tot > 0.0)) C->ierr12 = -2;
+    }
+  }
+}
+
+// MGS step i, part 1: apply a pending re-orthogonalisation against V_{i-1},
+// then h = V_k . V_i.
+template <typename TV>
+__global__ void __launch_bounds__(B) k_mgs_dot(int n, int i, TV *__restrict__ vk,
+                                              const TV *__restrict__ V, KCtl *C, double *part)
+{
+  if(gated(C, 2) || C->ierr12) return;
+  const TV *vi = V + (long)i * n;
+  const TV *vp = i > 0 ? V + (long)(i - 1) * n : nullptr;
+  const bool pend = C->pend_flag != 0;
+  const double pc = C->pend;
+  double s = 0;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) {
+    TV x = vk[j];
+    if(pend) {
+      x = (TV)((double)x - pc * (double)vp[j]);
+      vk[j] = x;
+    }
+    s += (double)x * (double)vi[j];
+  }
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+  if(grid_last_block(&C->ticket[2])) {
+    double tot = sum_partials<double, B>(part, gridDim.x);
+    if(threadIdx.x == 0) {
+      C->h1[i] = tot;
+      C->pend_flag = 0;
+    }
+  }
+}
+
+// MGS step i, part 2: V_k -= h_i V_i, then the re-orthogonalisation test of
+// mgsro (iters.f:3563-3575): if h_i^2 > thr the correction V_k . V_i is added
+// to h_i and subtracted by the next kernel.
+template <typename TV>
+__global__ void __launch_bounds__(B) k_mgs_axpy(int n, int i, TV *__restrict__ vk,
+                                               const TV *__restrict__ V, KCtl *C, double *part)
+{
+  if(gated(C, 2) || C->ierr12) return;
+  const TV *vi = V + (long)i * n;
+  const double f = C->h1[i];
+  double s = 0;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) {
+    TV x = (TV)((double)vk[j] - f * (double)vi[j]);
+    vk[j] = x;
+    s += (double)x * (double)vi[j];
+  }
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+  if(grid_last_block(&C->ticket[3])) {
+    double tot = sum_partials<double, B>(part, gridDim.x);
+    if(threadIdx.x == 0) {
+      if(f * f > C->thr) {
+        C->h1[i] = C->h1[i] + tot;
+        C->pend = tot;
+        C->pend_flag = 1;
+      }
+      else
+        C->pend_flag = 0;
+      double nrm0 = C->nrm0 - C->h1[i] * C->h1[i];
+      if(nrm0 < 0.0) nrm0 = 0.0;
+      C->nrm0 = nrm0;
+      C->thr = nrm0 * 0.98;
+    }
+  }
+}
+
+// MGS end: apply the last pending correction, norm of V_k, Hessenberg column.
+template <typename TV>
+__global__ void __launch_bounds__(B) k_mgs_norm(int n, int k, TV *__restrict__ vk,
+                                               const TV *__restrict__ V, KCtl *C, double *part)
+{
+  if(gated(C, 2) || C->ierr12) return;
+  const TV *vp = V + (long)(k - 1) * n;
+  const bool pend = C->pend_flag != 0;
+  const double pc = C->pend;
+  double s = 0;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) {
+    TV x = vk[j];
+    if(pend) {
+      x = (TV)((double)x - pc * (double)vp[j]);
+      vk[j] = x;
+    }
+    s += (double)x * (double)x;
+  }
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+  if(grid_last_block(&C->ticket[4])) {
+    double tot = sum_partials<double, B>(part, gridDim.x);
+    if(threadIdx.x == 0) {
+      double nrm1 = sqrt(tot);
+      double *hc = C->H + (long)(k - 1) * LDH;
+      for(int i = 0; i < k; i++) hc[i] = C->h1[i];
+      hc[k] = nrm1;
+      C->pend_flag = 0;
+      if(nrm1 <= 0.0) C->ierr12 = -3;
+      else C->scal = 1.0 / nrm1;
+    }
+  }
+}
+
+// CGS2: pass 1 computes h = V^T v; pass 2 subtracts V h and computes
+// h' = V^T v; pass 3 subtracts V h' and computes |v|.
+// One block owns one chunk of CGS_PER*B entries (grid = ceil(n / chunk)); the
+// chunk of v stays in registers while the basis vectors stream past once.
+constexpr int CGS_PER = 4;
+
+template <typename TV, int PASS>
+__global__ void __launch_bounds__(B) k_cgs(int n, int k, TV *__restrict__ vk,
+                                          const TV *__restrict__ V, KCtl *C, double *part)
+{
+  if(gated(C, 2) || C->ierr12) return;
+  const int nb = gridDim.x;
+  const long base = (long)blockIdx.x * B * CGS_PER;
+  double x[CGS_PER];
+  long jj[CGS_PER];
+#pragma unroll
+  for(int u = 0; u < CGS_PER; u++) {
+    jj[u] = base + u * B + threadIdx.x;
+    x[u] = jj[u] < n ? (double)vk[jj[u]] : 0.;
+  }
+  if(PASS >= 2) {
+    const double *hsub = (PASS == 2) ? C->h1 : C->h2;
+    for(int i = 0; i < k; i++) {
+      const double h = hsub[i];
+      const TV *vi = V + (long)i * n;
+#pragma unroll
+      for(int u = 0; u < CGS_PER; u++)
+        if(jj[u] < n) x[u] -= h * (double)vi[jj[u]];
+    }
+#pragma unroll
+    for(int u = 0; u < CGS_PER; u++)
+      if(jj[u] < n) {
+        vk[jj[u]] = (TV)x[u];
+        x[u] = (double)(TV)x[u];
+      }
+  }
+  if(PASS <= 2) {
+    for(int i = 0; i < k; i++) {
+      const TV *vi = V + (long)i * n;
+      double a = 0.;
+#pragma unroll
+      for(int u = 0; u < CGS_PER; u++)
+        if(jj[u] < n) a += (double)vi[jj[u]] * x[u];
+      double t = block_sum<double, B>(a);
+      if(threadIdx.x == 0) part[(long)i * nb + blockIdx.x] = t;
+    }
+  }
+  else {
+    double ss = 0.;
+#pragma unroll
+    for(int u = 0; u < CGS_PER; u++) ss += x[u] * x[u];
+    double t = block_sum<double, B>(ss);
+    if(threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+  if(grid_last_block(&C->ticket[5])) {
+    if(PASS <= 2) {
+      for(int i = 0; i < k; i++) {
+        double t = sum_partials<double, B>(part + (long)i * nb, nb);
+        if(threadIdx.x == 0) {
+          if(PASS == 1) C->h1[i] = t;
+          else C->h2[i] = t;
+        }
+      }
+    }
+    else {
+      double t = sum_partials<double, B>(part, nb);
+      if(threadIdx.x == 0) {
+        double nrm1 = sqrt(t);
+        double *hc = C->H + (long)(k - 1) * LDH;
+        for(int i = 0; i < k; i++) hc[i] = C->h1[i] + C->h2[i];
+        hc[k] = nrm1;
+        if(nrm1 <= 0.0) C->ierr12 = -3;
+        else C->scal = 1.0 / nrm1;
+      }
+    }
+  }
+}
+
+// givens (iters.f:3235-3272)
+__device__ inline void givens(double &x, double &y, double &c, double &s)
+{
+  if(x == 0.0 && y == 0.0) {
+    c = 1.0;
+    s = 0.0;
+  }
+  else if(fabs(y) > fabs(x)) {
+    double t = x / y;
+    x = sqrt(1.0 + t * t);
+    s = copysign(1.0 / x, y);
+    c = t * s;
+  }
+  else if(fabs(y) <= fabs(x)) {
+    double t = y / x;
+    y = sqrt(1.0 + t * t);
+    c = copysign(1.0 / y, x);
+    s = t * c;
+  }
+  else {
+    x = 0.0;
+    y = 0.0;
+    c = 1.0;
+    s = 0.0;
+  }
+  x = fabs(x * y);
+}
+
+// End of Arnoldi step k (gmres label 50, iters.f:2076-2109).
+__global__ void k_gm_givens(int k, KCtl *C)
+{
+  if(threadIdx.x || blockIdx.x || gated(C, 2)) return;
+  C->its++;
+  if(C->ierr12 < 0) {
+    C->cycle_end = 1;
+    C->kfinal = k;
+    return;
+  }
+  double *hc = C->H + (long)(k - 1) * LDH;
+  for(int i = 0; i < k - 1; i++) {
+    double a0 = hc[i];
+    hc[i] = C->c[i] * a0 + C->s[i] * hc[i + 1];
+    hc[i + 1] = C->c[i] * hc[i + 1] - C->s[i] * a0;
+  }
+  double c, s;
+  givens(hc[k - 1], hc[k], c, s);
+  C->c[k - 1] = c;
+  C->s[k - 1] = s;
+  double alpha = -s * C->g[k - 1];
+  C->g[k - 1] = c * C->g[k - 1];
+  C->g[k] = alpha;
+  alpha = fabs(alpha);
+  C->fpar5 = alpha;
+  if(!(k < C->m && !(alpha <= C->fpar4) && (C->maxits <= 0 || C->its < C->maxits))) {
+    C->cycle_end = 1;
+    C->kfinal = k;
+  }
+}
+
+// label 200: back substitution of the rotated Hessenberg system.
+__global__ void k_gm_backsub(KCtl *C)
+{
+  if(threadIdx.x || blockIdx.x || C->done) return;
+  int k = C->kfinal;
+  while(k > 0 && C->H[(long)(k - 1) * LDH + (k - 1)] == 0.0) k--;
+  if(k <= 0) {
+    C->status = -3;
+    C->ierr12 = -4;
+    C->done = 1;
+    C->kfinal = 0;
+    return;
+  }
+  C->g[k - 1] = C->g[k - 1] / C->H[(long)(k - 1) * LDH + (k - 1)];
+  for(int i = k - 1; i >= 1; i--) {
+    const double *hcol = C->H + (long)i * LDH;
+    for(int ii = 0; ii < i; ii++) C->g[ii] = C->g[ii] - C->g[i] * hcol[ii];
+    C->g[i - 1] = C->g[i - 1] / C->H[(long)(i - 1) * LDH + (i - 1)];
+  }
+  C->kfinal = k;
+}
+
+// GMRES update vector u = sum_i y_i V_i, accumulated into V_0 as the reference
+// does (w(ii) = w(ii)*y_1, then w += y_i v_i).
+template <typename TV>
+__global__ void __launch_bounds__(B) k_gm_combine(int n, TV *V, const KCtl *C)
+{
+  if(C->done) return;
+  const int k = C->kfinal;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) {
+    double u = (double)V[j] * C->g[0];
+    for(int i = 1; i < k; i++) u = u + C->g[i] * (double)V[(long)i * n + j];
+    V[j] = (TV)u;
+  }
+}
+
+// FGMRES update: sol += y_i Z_i, basis by basis.
+template <typename TV>
+__global__ void __launch_bounds__(B) k_fgm_update(int n, TV *sol, const TV *Z, const KCtl *C)
+{
+  if(C->done) return;
+  const int k = C->kfinal;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) {
+    double x = (double)sol[j];
+    for(int i = 0; i < k; i++) x = x + C->g[i] * (double)Z[(long)i * n + j];
+    sol[j] = (TV)x;
+  }
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(B) k_axpy1(int n, TV *sol, const TV *d, const KCtl *C)
+{
+  if(C->done) return;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B)
+    sol[j] = sol[j] + d[j];
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(B) k_copy_gated(int n, const TV *src, TV *dst, const KCtl *C,
+                                                 int gmode)
+{
+  if(gated(C, gmode)) return;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) dst[j] = src[j];
+}
+
+// label 60/70: restart or terminate.
+__global__ void k_gm_cycle_end(KCtl *C)
+{
+  if(threadIdx.x || blockIdx.x || C->done) return;
+  C->fpar6 = C->fpar5;
+  if(C->ierr12 != 0) {
+    C->status = -3;
+    C->done = 1;
+    return;
+  }
+  if((C->its < C->maxits || C->maxits <= 0) && C->fpar6 > C->fpar4) return;  // restart
+  if(C->fpar6 <= C->fpar4) C->status = 0;
+  else if(C->its >= C->maxits && C->maxits > 0) C->status = -1;
+  else C->status = -10;
+  C->done = 1;
+}
+
+// ---- CG pieces ----------------------------------------------------------------------
+template <typename TV>
+__global__ void __launch_bounds__(B) k_cg_init(int n, const TV *rhs, const TV *ap, TV *r, TV *p,
+                                              KCtl *C, double *part)
+{
+  double s = 0;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B) {
+    TV v = rhs[i] - ap[i];
+    p[i] = v;
+    r[i] = v;
+    s += (double)v * (double)v;
+  }
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+  if(grid_last_block(&C->ticket[0])) {
+    double tot = sum_partials<double, B>(part, gridDim.x);
+    if(threadIdx.x == 0) {
+      C->its = 1;
+      C->fpar7 = tot;
+      C->fpar3 = sqrt(tot);
+      C->fpar5 = C->fpar3;
+      C->fpar4 = C->rtol * C->fpar3 + 0.0;
+      C->fpar6 = 0.0;
+      C->ipar11 = 0;
+      C->status = 1;
+    }
+  }
+}
+
+// iters.f:478-486 (label 60): alpha = <p, Ap>, breakdown test, alpha = rr/alpha
+template <typename TV>
+__global__ void __launch_bounds__(B) k_cg_pap(int n, const TV *p, const TV *ap, KCtl *C,
+                                             double *part)
+{
+  if(C->done) return;
+  double s = 0;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B)
+    s += (double)p[i] * (double)ap[i];
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+  if(grid_last_block(&C->ticket[1])) {
+    double alpha = sum_partials<double, B>(part, gridDim.x);
+    if(threadIdx.x == 0) {
+      C->its++;
+      int bk = 0;  // brkdn, iters.f:3368-3401
+      if(alpha > 0.0) bk = (1.0 / alpha > 0.0) ? 0 : -9;
+      else if(alpha < 0.0) bk = (1.0 / alpha < 0.0) ? 0 : -9;
+      else if(alpha == 0.0) bk = -3;
+      else bk = -9;
+      if(bk) {
+        C->status = bk;
+        C->done = 1;
+      }
+      else
+        C->alpha = C->fpar7 / alpha;
+    }
+  }
+}
+
+// dx += alpha p, r -= alpha Ap, then stopbis (iters.f:3275-3330).
+template <typename TV>
+__global__ void __launch_bounds__(B) k_cg_update(int n, const TV *p, const TV *ap, TV *r, TV *dx,
+                                                KCtl *C, double *part)
+{
+  if(C->done) return;
+  const double alpha = C->alpha;
+  double s = 0;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B) {
+    dx[i] = (TV)((double)dx[i] + alpha * (double)p[i]);
+    TV v = (TV)((double)r[i] - alpha * (double)ap[i]);
+    r[i] = v;
+    s += (double)v * (double)v;
+  }
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+  if(grid_last_block(&C->ticket[2])) {
+    double tot = sum_partials<double, B>(part, gridDim.x);
+    if(threadIdx.x == 0) {
+      if(C->maxits > 0 && C->its >= C->maxits) {
+        C->status = -1;
+        C->done = 1;
+        return;
+      }
+      C->fpar5 = sqrt(tot);
+      C->fpar6 = C->fpar5;
+      if(C->fpar6 > C->fpar4) {
+        C->ipar11 = 0;
+        C->beta = C->fpar5 * C->fpar5 / C->fpar7;
+        C->fpar7 = C->fpar5 * C->fpar5;
+      }
+      else {
+        C->ipar11 = 1;
+        C->done = 1;
+      }
+    }
+  }
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(B) k_cg_p(int n, const TV *r, TV *p, const KCtl *C)
+{
+  if(C->done) return;
+  const double beta = C->beta;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B)
+    p[i] = (TV)((double)r[i] + beta * (double)p[i]);
+}
+
+int reduce_blocks(int n)
+{
+  long nb = ((long)n + B * 4 - 1) / (B * 4);
+  return (int)std::max(1L, std::min(nb, 1024L));
+}
+
+struct Workspace {
+  KCtl *ctl = nullptr;
+  double *part = nullptr;
+  void *vec = nullptr;
+  size_t vec_bytes = 0, part_size = 0;
+  ~Workspace()
+  {
+    dfree(ctl);
+    dfree(part);
+    if(vec) (void)hipFree(vec);
+  }
+};
+
+Workspace &ws()
+{
+  static Workspace w;
+  return w;
+}
+
+int ensure_ws(size_t vec_bytes, int n)
+{
+  Workspace &W = ws();
+  size_t nblk = (size_t)std::max(1024L, ((long)n + B * CGS_PER - 1) / (B * CGS_PER));
+  size_t need = nblk * (kMaxKrylov + 2);
+  if(!W.ctl) GDP_TRY(dalloc(&W.ctl, 1));
+  if(W.part_size < need) {
+    dfree(W.part);
+    GDP_TRY(dalloc(&W.part, need));
+    W.part_size = need;
+  }
+  if(W.vec_bytes < vec_bytes) {
+    if(W.vec) (void)hipFree(W.vec);
+    W.vec = nullptr;
+    W.vec_bytes = 0;
+    GDP_HIP(hipMalloc(&W.vec, vec_bytes));
+    W.vec_bytes = vec_bytes;
+  }
+  return GDP_OK;
+}
+
+template <typename TV> int prec_apply(const PrecOp<TV> &M, const TV *in, TV *out, KCtl *C, int gmode, int n)
+{
+  if(M.kind == PREC_ILU) return M.ilu->template apply<TV>(in, (TV)1, out, (const int *)C, gmode);
+  if(M.kind == PREC_USER) {
+    if constexpr(sizeof(TV) == 8) return user_prec_apply(M.user, in, out, C, gmode);
+    else return set_error(GDP_E_UNSUPPORTED, "learned preconditioner needs FP64 vectors");
+  }
+  dim3 g(reduce_blocks(n));
+  hipLaunchKernelGGL(k_copy_gated<TV>, g, dim3(B), 0, ctx().stream, n, in, out, C, gmode);
+  return GDP_OK;
+}
+
+// Record or replay: enqueue fn() into a graph once, then replay it.
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  ~GraphCache()
+  {
+    if(exec) (void)hipGraphExecDestroy(exec);
+    if(graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+template <typename Fn> int run_maybe_graph(bool use, GraphCache &gc, Fn fn)
+{
+  hipStream_t s = ctx().stream;
+  if(!use) return fn();
+  if(!gc.exec) {
+    GDP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int r = fn();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if(r) {
+      if(g) (void)hipGraphDestroy(g);
+      return r;
+    }
+    if(e != hipSuccess) return set_error(GDP_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    gc.graph = g;
+    GDP_HIP(hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0));
+  }
+  GDP_HIP(hipGraphLaunch(gc.exec, s));
+  return GDP_OK;
+}
+
+int read_ctl(KCtl *d, int *done, int *status, int *its)
+{
+  int h[3];
+  GDP_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx().stream));
+  GDP_HIP(hipStreamSynchronize(ctx().stream));
+  *done = h[0];
+  *status = h[1];
+  *its = h[2];
+  return GDP_OK;
+}
+
+}  // namespace
+
+template <typename TA, typename TV>
+int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol,
+                 const KrylovOptions &opt, KrylovResult &res)
+{
+  const int n = A.n;
+  hipStream_t s = ctx().stream;
+  const int m = std::max(2, std::min(opt.m <= 1 ? 15 : opt.m, kMaxKrylov));
+  const bool flexible = opt.algorithm == 9;
+  const bool cg = opt.algorithm == 1;
+  if(!cg && opt.algorithm != 8 && !flexible)
+    return set_error(GDP_E_UNSUPPORTED, "Unknown algorithm for sparse matrix solver (%d)", opt.algorithm);
+  size_t nvec = cg ? 6 : (size_t)(m + 2) + (flexible ? m : 1);
+  GDP_TRY(ensure_ws(nvec * (size_t)std::max(1, n) * sizeof(TV) + 256, n));
+  Workspace &W = ws();
+  TV *base = (TV *)W.vec;
+  KCtl hc;
+  std::memset(&hc, 0, sizeof(hc));
+  hc.rtol = opt.rtol;
+  hc.maxits = opt.maxits;
+  hc.m = m;
+  GDP_TRY(h2d(W.ctl, &hc, 1));
+  GDP_HIP(hipMemsetAsync(sol, 0, sizeof(TV) * n, s));  // x0 = 0 (Sparskit.cpp:226)
+  KCtl *C = W.ctl;
+  double *part = W.part;
+  const int nb = reduce_blocks(n);
+  dim3 g(nb), blk(B);
+  int done = 0, status = 0, its = 0;
+
+  if(cg) {
+    TV *r = base, *p = base + n, *ap = base + 2 * (size_t)n, *z = base + 3 * (size_t)n,
+       *dx = base + 4 * (size_t)n;
+    GDP_HIP(hipMemsetAsync(dx, 0, sizeof(TV) * n, s));
+    GDP_TRY(op_apply(A, sol, ap, C, 0));
+    hipLaunchKernelGGL(k_cg_init<TV>, g, blk, 0, s, n, rhs, ap, r, p, C, part);
+    const bool rp = M.kind != PREC_NONE;
+    GraphCache gc;
+    const int batch = 16;
+    auto body = [&]() -> int {
+      for(int it = 0; it < batch; it++) {
+        if(rp) {
+          GDP_TRY(prec_apply(M, p, z, C, 1, n));
+          GDP_TRY(op_apply(A, z, ap, C, 1));
+        }
+        else
+          GDP_TRY(op_apply(A, p, ap, C, 1));
+        hipLaunchKernelGGL(k_cg_pap<TV>, g, blk, 0, s, n, p, ap, C, part);
+        hipLaunchKernelGGL(k_cg_update<TV>, g, blk, 0, s, n, p, ap, r, dx, C, part);
+        hipLaunchKernelGGL(k_cg_p<TV>, g, blk, 0, s, n, r, p, C);
+      }
+      GDP_HIP(hipGetLastError());
+      return GDP_OK;
+    };
+    for(;;) {
+      GDP_TRY(run_maybe_graph(opt.use_graphs, gc, body));
+      GDP_TRY(read_ctl(C, &done, &status, &its));
+      if(done) break;
+    }
+    // tidycg (iters.f:3333-3365): sol += M^{-1} dx (right preconditioning) or dx
+    KCtl tmp;
+    GDP_TRY(d2h(&tmp, C, 1));
+    int final_status = tmp.status;
+    if(final_status > 0) {
+      if(tmp.fpar6 <= tmp.fpar4) final_status = 0;
+      else if(tmp.its >= tmp.maxits && tmp.maxits > 0) final_status = -1;
+      else final_status = -10;
+    }
+    int zero = 0;
+    GDP_TRY(h2d(&C->done, &zero, 1));
+    if(rp) {
+      GDP_TRY(prec_apply(M, dx, z, C, 0, n));
+      hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, z, C);
+    }
+    else
+      hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, dx, C);
+    GDP_HIP(hipGetLastError());
+    GDP_TRY(sync());
+    if(M.kind == PREC_ILU) GDP_TRY(M.ilu->check_timeout());
+    res.its = tmp.its;
+    res.status = final_status;
+    res.res0 = tmp.fpar3;
+    res.res_est = tmp.fpar6;
+    return GDP_OK;
+  }
+
+  // ---- (F)GMRES ----
+  TV *V = base;                              // m+1 basis vectors
+  TV *Z = base + (size_t)(m + 1) * n;        // FGMRES: m vectors; GMRES: 1 temp
+  TV *tmpz = Z;
+  const bool rp = M.kind != PREC_NONE;
+  GraphCache gc;
+  auto cycle = [&]() -> int {
+    GDP_TRY(op_apply(A, sol, V, C, 1));
+    hipLaunchKernelGGL(k_gm_resid<TV>, g, blk, 0, s, n, rhs, V, C, part);
+    hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, V, C, 1);
+    for(int k = 1; k <= m; k++) {
+      TV *vk1 = V + (size_t)(k - 1) * n, *vk = V + (size_t)k * n;
+      const TV *src = vk1;
+      if(flexible) {
+        TV *zk = Z + (size_t)(k - 1) * n;
+        GDP_TRY(prec_apply(M, vk1, zk, C, 2, n));
+        src = zk;
+      }
+      else if(rp) {
+        GDP_TRY(prec_apply(M, vk1, tmpz, C, 2, n));
+        src = tmpz;
+      }
+      GDP_TRY(op_apply(A, src, vk, C, 2));
+      hipLaunchKernelGGL(k_nrm0<TV>, g, blk, 0, s, n, vk, C, part);
+      if(opt.orth == 1) {
+        dim3 gc((unsigned)std::max(1L, ((long)n + B * CGS_PER - 1) / (B * CGS_PER)));
+        hipLaunchKernelGGL((k_cgs<TV, 1>), gc, blk, 0, s, n, k, vk, V, C, part);
+        hipLaunchKernelGGL((k_cgs<TV, 2>), gc, blk, 0, s, n, k, vk, V, C, part);
+        hipLaunchKernelGGL((k_cgs<TV, 3>), gc, blk, 0, s, n, k, vk, V, C, part);
+      }
+      else {
+        for(int i = 0; i < k; i++) {
+          hipLaunchKernelGGL(k_mgs_dot<TV>, g, blk, 0, s, n, i, vk, V, C, part);
+          hipLaunchKernelGGL(k_mgs_axpy<TV>, g, blk, 0, s, n, i, vk, V, C, part);
+        }
+        hipLaunchKernelGGL(k_mgs_norm<TV>, g, blk, 0, s, n, k, vk, V, C, part);
+      }
+      hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, vk, C, 2);
+      hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, s, k, C);
+    }
+    hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(1), 0, s, C);
+    if(flexible)
+      hipLaunchKernelGGL(k_fgm_update<TV>, g, blk, 0, s, n, sol, Z, C);
+    else {
+      hipLaunchKernelGGL(k_gm_combine<TV>, g, blk, 0, s, n, V, C);
+      if(rp) {
+        GDP_TRY(prec_apply(M, V, tmpz, C, 1, n));
+        hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, tmpz, C);
+      }
+      else
+        hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, V, C);
+    }
+    hipLaunchKernelGGL(k_gm_cycle_end, dim3(1), dim3(1), 0, s, C);
+    GDP_HIP(hipGetLastError());
+    return GDP_OK;
+  };
+  for(;;) {
+    GDP_TRY(run_maybe_graph(opt.use_graphs, gc, cycle));
+    GDP_TRY(read_ctl(C, &done, &status, &its));
+    if(done) break;
+  }
+  if(M.kind == PREC_ILU) GDP_TRY(M.ilu->check_timeout());
+  KCtl tmp;
+  GDP_TRY(d2h(&tmp, C, 1));
+  res.its = tmp.its;
+  res.status = tmp.status;
+  res.res0 = tmp.fpar3;
+  res.res_est = tmp.fpar6;
+  return GDP_OK;
+}
+
+template int krylov_solve<double, double>(const DevOp<double> &, const PrecOp<double> &,
+                                          const double *, double *, const KrylovOptions &,
+                                          KrylovResult &);
+template int krylov_solve<float, float>(const DevOp<float> &, const PrecOp<float> &,
+                                        const float *, float *, const KrylovOptions &,
+                                        KrylovResult &);
+
+}  // namespace gdp

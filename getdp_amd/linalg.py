@@ -1,0 +1,240 @@
+"""Python mirror of the reference's LinAlg interface (src/kernel/LinAlg.h:90-213)
+over the device backend's C-ABI.
+
+Names follow the reference: ``LinAlg_CreateSolver`` -> :class:`Solver`,
+``LinAlg_CreateVector`` -> :class:`Vector`, ``LinAlg_CreateMatrix`` ->
+:class:`Matrix`, ``LinAlg_Solve`` -> :func:`solve`, ``LinAlg_SolveAgain`` ->
+:func:`solve_again`, ``LinAlg_ProdMatrixVector`` -> :func:`prod_matrix_vector`;
+``Generate[A]`` of a formulation -> :meth:`FESystem.generate`.  Errors raise
+:class:`GdpError` carrying the backend's status code and message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from ._lib import (GdpError, MeshDesc, SolveInfo, SpaceDesc, TermDesc, check, dptr, iptr,
+                   lib)
+
+__all__ = ["init", "Solver", "Vector", "Matrix", "FESystem", "solve", "solve_again",
+           "prod_matrix_vector", "GdpError"]
+
+
+def init(device: int = 0):
+    check(lib().gdp_init(device), "gdp_init")
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self):
+        self.h = C.c_void_p()
+
+    def close(self):
+        if self.h and self.h.value:
+            getattr(lib(), self._destroy)(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver(_Handle):
+    """Solver_Params (+ cached renumbering/factorisation), LinAlg_CreateSolver."""
+    _destroy = "gdp_solver_destroy"
+
+    def __init__(self, solver_par: Optional[str] = None, **options):
+        super().__init__()
+        path = solver_par.encode() if solver_par else None
+        check(lib().gdp_solver_create(path, C.byref(self.h)), "gdp_solver_create")
+        for k, v in options.items():
+            self.set(k, v)
+
+    def set(self, name: str, value):
+        check(lib().gdp_solver_set(self.h, name.encode(), str(value).encode()), f"set {name}")
+
+    def get(self, name: str) -> float:
+        v = C.c_double()
+        check(lib().gdp_solver_get(self.h, name.encode(), C.byref(v)), f"get {name}")
+        return v.value
+
+    def info(self) -> dict:
+        s = SolveInfo()
+        check(lib().gdp_solver_info(self.h, C.byref(s)), "gdp_solver_info")
+        return s.as_dict()
+
+
+class Vector(_Handle):
+    _destroy = "gdp_vector_destroy"
+
+    def __init__(self, n: int, values: Optional[np.ndarray] = None):
+        super().__init__()
+        self.n = int(n)
+        check(lib().gdp_vector_create(self.n, C.byref(self.h)), "gdp_vector_create")
+        if values is not None:
+            self.set(values)
+
+    def set(self, values):
+        v = np.ascontiguousarray(values, np.float64)
+        check(lib().gdp_vector_set(self.h, dptr(v), v.shape[0]), "gdp_vector_set")
+
+    def get(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        check(lib().gdp_vector_get(self.h, dptr(out), self.n), "gdp_vector_get")
+        return out
+
+    def zero(self):
+        check(lib().gdp_vector_zero(self.h), "gdp_vector_zero")
+
+    def add_double(self, i: int, d: float):
+        check(lib().gdp_vector_add_double(self.h, int(i), float(d)), "gdp_vector_add_double")
+
+    def assemble(self):
+        check(lib().gdp_vector_assemble(self.h), "gdp_vector_assemble")
+
+    def norm2(self) -> float:
+        v = C.c_double()
+        check(lib().gdp_vector_norm2(self.h, C.byref(v)), "gdp_vector_norm2")
+        return v.value
+
+
+class Matrix(_Handle):
+    _destroy = "gdp_matrix_destroy"
+
+    def __init__(self, n: int):
+        super().__init__()
+        self.n = int(n)
+        check(lib().gdp_matrix_create(self.n, self.n, C.byref(self.h)), "gdp_matrix_create")
+
+    def zero(self):
+        check(lib().gdp_matrix_zero(self.h), "gdp_matrix_zero")
+
+    def add_double(self, i: int, j: int, d: float):
+        check(lib().gdp_matrix_add_double(self.h, int(i), int(j), float(d)), "add_double")
+
+    def add_coo(self, i, j, d):
+        i = np.ascontiguousarray(i, np.int32)
+        j = np.ascontiguousarray(j, np.int32)
+        d = np.ascontiguousarray(d, np.float64)
+        check(lib().gdp_matrix_add_coo(self.h, i.shape[0], iptr(i), iptr(j), dptr(d)), "add_coo")
+
+    def assemble(self):
+        check(lib().gdp_matrix_assemble(self.h), "gdp_matrix_assemble")
+
+    def set_csr(self, ia, ja, a):
+        ia = np.ascontiguousarray(ia, np.int32)
+        ja = np.ascontiguousarray(ja, np.int32)
+        a = np.ascontiguousarray(a, np.float64)
+        check(lib().gdp_matrix_set_csr(self.h, self.n, iptr(ia), iptr(ja), dptr(a)), "set_csr")
+
+    @property
+    def nnz(self) -> int:
+        v = C.c_long()
+        check(lib().gdp_matrix_nnz(self.h, C.byref(v)), "gdp_matrix_nnz")
+        return v.value
+
+    def get_csr(self):
+        nnz = self.nnz
+        ia = np.zeros(self.n + 1, np.int32)
+        ja = np.zeros(max(nnz, 1), np.int32)
+        a = np.zeros(max(nnz, 1))
+        check(lib().gdp_matrix_get_csr(self.h, iptr(ia), iptr(ja), dptr(a)), "get_csr")
+        return ia, ja[:nnz], a[:nnz]
+
+
+def prod_matrix_vector(A: Matrix, x: Vector, y: Vector):
+    check(lib().gdp_prod_matrix_vector(A.h, x.h, y.h), "gdp_prod_matrix_vector")
+
+
+def solve(A: Matrix, b: Vector, s: Solver, x: Vector) -> dict:
+    check(lib().gdp_solve(A.h, b.h, s.h, x.h), "gdp_solve")
+    return s.info()
+
+
+def solve_again(A: Matrix, b: Vector, s: Solver, x: Vector) -> dict:
+    check(lib().gdp_solve_again(A.h, b.h, s.h, x.h), "gdp_solve_again")
+    return s.info()
+
+
+class FESystem(_Handle):
+    """One formulation's device assembly (PRE pass at construction,
+    ``generate`` = Generate[A])."""
+    _destroy = "gdp_fe_destroy"
+
+    def __init__(self, problem):
+        super().__init__()
+        p = problem
+        self._keep = []
+
+        def keep(a):
+            self._keep.append(a)
+            return a
+
+        xyz = keep(np.ascontiguousarray(p.xyz, np.float64))
+        etype = keep(np.ascontiguousarray(p.etype, np.int32))
+        region = keep(np.ascontiguousarray(p.region, np.int32))
+        conn = keep(np.ascontiguousarray(p.conn, np.int32))
+        md = MeshDesc(p.nnodes, dptr(xyz), p.nelem, iptr(etype), iptr(region), iptr(conn), None, 0)
+        if p.edges is not None:
+            edges = keep(np.ascontiguousarray(p.edges, np.int32))
+            md.edges = iptr(edges)
+            md.nedges = int(p.edge_nodes.shape[0])
+        sds = (SpaceDesc * len(p.spaces))()
+        for k, s in enumerate(p.spaces):
+            sup = keep(np.ascontiguousarray(s.support, np.int32))
+            fe = keep(np.ascontiguousarray(s.fixed_entity, np.int32))
+            fv = keep(np.ascontiguousarray(s.fixed_value, np.float64))
+            sds[k] = SpaceDesc(s.bf, s.numtype, sup.shape[0], iptr(sup), fe.shape[0], iptr(fe),
+                               dptr(fv))
+        nreg = int(p.region.max()) + 1
+        tds = (TermDesc * len(p.terms))()
+        for k, t in enumerate(p.terms):
+            regs = keep(np.ascontiguousarray(t.regions, np.int32))
+            coef = np.zeros((max(nreg, t.coef.shape[0]), t.ncoef))
+            coef[: t.coef.shape[0]] = t.coef
+            coef = keep(np.ascontiguousarray(coef))
+            tds[k] = TermDesc(t.kind, t.equ, t.dof, regs.shape[0], iptr(regs), t.npoints,
+                              t.jac_sur, t.ncoef, coef.shape[0], dptr(coef))
+        self._mesh = md
+        check(lib().gdp_fe_create(C.byref(md), len(p.spaces), sds, len(p.terms), tds,
+                                  C.byref(self.h)), "gdp_fe_create")
+        self._keep = []
+        self.nmax = max(p.nnodes, 0 if p.edge_nodes is None else int(p.edge_nodes.shape[0]))
+        n = C.c_int()
+        nnz = C.c_long()
+        check(lib().gdp_fe_num_dofs(self.h, C.byref(n), C.byref(nnz)), "gdp_fe_num_dofs")
+        self.N, self.nnz = n.value, nnz.value
+
+    def dof_table(self, space: int = 0) -> np.ndarray:
+        out = np.zeros(self.nmax + 1, np.int32)
+        check(lib().gdp_fe_dof_table(self.h, space, iptr(out), self.nmax), "gdp_fe_dof_table")
+        return out
+
+    def pattern(self):
+        ia = np.zeros(self.N + 1, np.int32)
+        ja = np.zeros(max(self.nnz, 1), np.int32)
+        check(lib().gdp_fe_pattern(self.h, iptr(ia), iptr(ja)), "gdp_fe_pattern")
+        return ia, ja[: self.nnz]
+
+    def generate(self, A: Matrix, b: Vector):
+        check(lib().gdp_fe_generate(self.h, A.h, b.h), "gdp_fe_generate")
+
+    def timing(self) -> dict:
+        t = [C.c_double() for _ in range(3)]
+        check(lib().gdp_fe_last_timing(self.h, *[C.byref(x) for x in t]), "timing")
+        return dict(ms_total=t[0].value, ms_geometry=t[1].value, ms_gather=t[2].value)
+
+
+def assemble(problem, fe: Optional[FESystem] = None):
+    """Convenience: build the FE system, generate A and b on the device and
+    return (fe, A, b)."""
+    fe = fe or FESystem(problem)
+    A = Matrix(fe.N)
+    b = Vector(fe.N)
+    fe.generate(A, b)
+    return fe, A, b

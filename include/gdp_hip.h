@@ -1,0 +1,227 @@
+/*
+ * gdp_hip.h -- C-ABI of the MI355X (gfx950) finite-element backend.
+ *
+ * This library is a LinAlg backend in the sense of GetDP's src/kernel/LinAlg.h
+ * (the typedef switch at LinAlg.h:20-88 selects LinAlg_SPARSKIT.cpp or
+ * LinAlg_PETSC.cpp; this is a third branch), plus a batched device path for
+ * the per-element Galerkin assembly of src/kernel/Cal_GalerkinTermOfFemEquation.cpp.
+ *
+ * All entry points are `extern "C"`, take plain pointers and sizes, and return
+ * an int status: GDP_OK (0) or a negative GDP_E_* code; gdp_last_error()
+ * returns the message (the reference reports the same conditions through
+ * Message::Error / Message::Warning, e.g. contrib/Sparskit/Sparskit.cpp:429-453,
+ * 556-567).  Indices at this interface are 0-based, as at the LinAlg interface
+ * (DofData.cpp:1477 passes NumDof-1).
+ *
+ * Every device entry point runs on the library's single HIP stream and
+ * returns after the work it enqueued has completed, unless documented
+ * otherwise.  The library is not re-entrant (the reference is single-threaded,
+ * LinAlg_SPARSKIT.cpp:27 keeps static solver options).
+ */
+#ifndef GDP_HIP_H
+#define GDP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GDP_OK 0
+#define GDP_E_ARG -1        /* invalid argument                               */
+#define GDP_E_HIP -2        /* HIP runtime error                              */
+#define GDP_E_STATE -3      /* call out of sequence                           */
+#define GDP_E_UNSUPPORTED -4/* term/element not supported by the device path  */
+#define GDP_E_ZERO_PIVOT -5 /* ILU zero pivot (row in gdp_solve_info.ilu_row)  */
+#define GDP_E_ZERO_ROW -6   /* "Zero row in matrix" (Sparskit.cpp:220-223)    */
+#define GDP_E_NOMEM -7      /* device or host allocation failure              */
+#define GDP_E_TIMEOUT -8    /* a device wait exceeded its time bound          */
+#define GDP_E_NOT_CONVERGED -9 /* Krylov stopped without reaching the tolerance */
+
+/* ---- library / device ------------------------------------------------------ */
+/* Select the device and create the stream.  Replaces LinAlg_InitializeSolver
+ * (LinAlg_SPARSKIT.cpp:29-67) as the first call. */
+int gdp_init(int device);
+int gdp_finalize(void);
+const char *gdp_last_error(void);
+int gdp_device_synchronize(void);
+
+/* ---- solver parameters (Solver_Params, contrib/Sparskit/Sparskit.h:83-103) */
+typedef struct gdp_solver gdp_solver;
+/* LinAlg_CreateSolver (LinAlg_SPARSKIT.cpp:74-110): reads a solver.par file
+ * (NULL: defaults of Sparskit.cpp:1847-1868).  Unknown file -> defaults. */
+int gdp_solver_create(const char *solver_par_file, gdp_solver **s);
+/* init_solver_option (Sparskit.cpp:2000-2042): set one parameter by name,
+ * e.g. ("Preconditioner", "6").  Extra keys of this backend:
+ *   Precision           0 FP64 (default), 1 FP32 inner solve + FP64
+ *                        iterative refinement
+ *   ILU_Ordering        0 as the matrix is ordered after Renumbering_Technique
+ *                        (default: the reference's ordering), 1 multicolour
+ *   Orthogonalization   0 modified Gram-Schmidt with selective
+ *                        re-orthogonalisation (iters.f:3475, default),
+ *                        1 classical Gram-Schmidt applied twice
+ *   Refinement_Steps    max outer steps for Precision 1 (default 20)
+ *   Inner_Tolerance     inner relative tolerance for Precision 1 (1e-6)   */
+int gdp_solver_set(gdp_solver *s, const char *name, const char *value);
+int gdp_solver_get(const gdp_solver *s, const char *name, double *value);
+int gdp_solver_destroy(gdp_solver *s); /* LinAlg_DestroySolver */
+
+typedef struct {
+  int its;            /* matrix-vector products, ipar(7) (iters.f)         */
+  int status;         /* ipar(1) at exit: 0 converged, -1 max its, -3 ...  */
+  int ilu_row;        /* zero-pivot row (1-based) or 0                     */
+  int ilu_levels;     /* dependency depth of the L factor                  */
+  int outer_its;      /* iterative-refinement steps (Precision 1)          */
+  double res0;        /* initial residual norm                             */
+  double res_est;     /* estimated residual norm at exit (fpar(6))         */
+  double true_res;    /* |A x - b|_2 (solved system) recomputed in FP64     */
+  double rel_res;     /* true_res / |b|_2                                  */
+  double t_setup_ms;  /* renumbering + factorisation                       */
+  double t_solve_ms;  /* Krylov iterations                                 */
+} gdp_solve_info;
+int gdp_solver_info(const gdp_solver *s, gdp_solve_info *info);
+
+/* ---- vectors (gVector) --------------------------------------------------- */
+typedef struct gdp_vector gdp_vector;
+int gdp_vector_create(int n, gdp_vector **v);                 /* LinAlg_CreateVector  */
+int gdp_vector_destroy(gdp_vector *v);                        /* LinAlg_DestroyVector */
+int gdp_vector_zero(gdp_vector *v);                           /* LinAlg_ZeroVector    */
+int gdp_vector_size(const gdp_vector *v, int *n);             /* LinAlg_GetVectorSize */
+int gdp_vector_set(gdp_vector *v, const double *host, int n); /* LinAlg_SetDoubleInVector, bulk */
+int gdp_vector_get(const gdp_vector *v, double *host, int n); /* LinAlg_GetDoubleInVector, bulk */
+/* LinAlg_AddDoubleInVector (LinAlg_SPARSKIT.cpp:356-364): staged on the host
+ * and applied in call order by gdp_vector_assemble. */
+int gdp_vector_add_double(gdp_vector *v, int i, double d);
+int gdp_vector_assemble(gdp_vector *v);                       /* LinAlg_AssembleVector */
+int gdp_vector_norm2(const gdp_vector *v, double *norm);      /* LinAlg_VectorNorm2   */
+int gdp_vector_copy(const gdp_vector *src, gdp_vector *dst);  /* LinAlg_CopyVector    */
+/* Device address of the data (FP64, n entries) for zero-copy interop. */
+int gdp_vector_device_ptr(gdp_vector *v, double **dptr);
+
+/* ---- matrices (gMatrix) -------------------------------------------------- */
+typedef struct gdp_matrix gdp_matrix;
+int gdp_matrix_create(int n, int m, gdp_matrix **A);          /* LinAlg_CreateMatrix  */
+int gdp_matrix_destroy(gdp_matrix *A);                        /* LinAlg_DestroyMatrix */
+/* LinAlg_ZeroMatrix: keeps a device pattern (values zeroed) when one exists;
+ * staged host entries are discarded. */
+int gdp_matrix_zero(gdp_matrix *A);
+/* LinAlg_AddDoubleInMatrix (LinAlg_SPARSKIT.cpp:392-400): row i, column j.
+ * Entries are staged on the host; duplicates are summed in call order by
+ * gdp_matrix_assemble (add_matrix_double, Sparskit.cpp:809-862). */
+int gdp_matrix_add_double(gdp_matrix *A, int i, int j, double d);
+int gdp_matrix_add_coo(gdp_matrix *A, long n, const int *i, const int *j,
+                       const double *d);
+/* LinAlg_AssembleMatrix: builds the sorted CSR (csr_format, Sparskit.cpp:2186)
+ * from the staged entries and uploads it. */
+int gdp_matrix_assemble(gdp_matrix *A);
+int gdp_matrix_set_csr(gdp_matrix *A, int n, const int *ia, const int *ja,
+                       const double *a);
+int gdp_matrix_nnz(const gdp_matrix *A, long *nnz);
+int gdp_matrix_get_csr(const gdp_matrix *A, int *ia, int *ja, double *a);
+/* LinAlg_ProdMatrixVector: y = A x on the device (CSR SpMV, FP64). */
+int gdp_prod_matrix_vector(gdp_matrix *A, const gdp_vector *x, gdp_vector *y);
+
+/* ---- solve ----------------------------------------------------------------- */
+/* LinAlg_Solve (LinAlg_SPARSKIT.cpp:572 -> solve_matrix, Sparskit.cpp:108):
+ * scaling, RCMK renumbering, ILU(0) and CG / GMRES / FGMRES on the device,
+ * x0 = 0, stop on the estimated residual <= Stopping_Test * |r0|. */
+int gdp_solve(gdp_matrix *A, gdp_vector *b, gdp_solver *s, gdp_vector *x);
+/* LinAlg_SolveAgain (:578): reuses permutation and factors of the last solve
+ * with the same matrix. */
+int gdp_solve_again(gdp_matrix *A, gdp_vector *b, gdp_solver *s, gdp_vector *x);
+
+/* ---- batched Galerkin assembly ---------------------------------------------- */
+/* Element types: Gmsh codes 1 line, 2 triangle, 4 tetrahedron. */
+#define GDP_LINE 1
+#define GDP_TRI 2
+#define GDP_TET 4
+/* Basis functions: BF_Node (BF_Node.cpp:28) on NodesOf, BF_Edge (BF_Edge.cpp:23)
+ * on EdgesOf. */
+#define GDP_BF_NODE 0
+#define GDP_BF_EDGE 1
+/* Canonical terms recognised by Cal_InitGalerkinTermOfFemEquation
+ * (Cal_GalerkinTermOfFemEquation.cpp:39-316) and supported on the device: */
+#define GDP_T_GRAD_GRAD 1  /* [ c * Dof{d u}, {d v} ]: c scalar or 3x3 tensor  */
+#define GDP_T_CURL_CURL 2  /* [ c * Dof{d a}, {d a} ], Whitney edges (tets)    */
+#define GDP_T_EDGE_MASS 3  /* [ c * Dof{a}, {a} ], Whitney edges (tets)        */
+#define GDP_T_NODE_MASS 4  /* [ c * Dof{u}, {u} ]                              */
+#define GDP_T_SRC_NODE 10  /* [ f, {u} ]: no Dof{}, f scalar                    */
+#define GDP_T_SRC_EDGE 11  /* [ f, {a} ]: no Dof{}, f vector                    */
+
+typedef struct {
+  int nnodes;
+  const double *xyz;  /* 3 per node                                          */
+  int nelem;          /* elements in ascending element number               */
+  const int *etype;   /* nelem                                               */
+  const int *region;  /* nelem, physical region ids (>= 0)                   */
+  const int *conn;    /* 4 per element, 1-based node numbers                 */
+  const int *edges;   /* 6 per element, signed edge numbers (NULL: no edges) */
+  int nedges;         /* number of edges (0 if edges == NULL)                */
+} gdp_mesh_desc;
+
+typedef struct {
+  int bf;                    /* GDP_BF_NODE / GDP_BF_EDGE                     */
+  int numtype;               /* BasisFunction.Num (DOF key, DofData.cpp:36)   */
+  int nsupport;
+  const int *support;        /* support regions                              */
+  int nfixed;                /* Dirichlet-type constraint                     */
+  const int *fixed_entity;   /* node numbers (1-based) or edge numbers        */
+  const double *fixed_value;
+} gdp_space_desc;
+
+typedef struct {
+  int kind;                  /* GDP_T_*                                       */
+  int equ, dof;              /* space indices; dof = -1 for source terms      */
+  int nregions;
+  const int *regions;        /* In ...                                        */
+  int npoints;               /* Gauss points (tri 1/3/4, tet 1/4)             */
+  int jac_sur;               /* 1: Jacobian Sur (triangles in 3D)             */
+  int ncoef;                 /* 1 scalar, 3 vector, 9 tensor                  */
+  int coef_nreg;             /* table rows = region ids 0..coef_nreg-1        */
+  const double *coef;        /* coef[r*ncoef + c], sign included              */
+} gdp_term_desc;
+
+typedef struct gdp_fe gdp_fe;
+/* PRE pass for one formulation (SolvingAnalyse.cpp:445 -> Treatment_Formulation
+ * in STATUS_PRE): DOF numbering (DofData.cpp:1279-1323), CSR pattern
+ * (every DOF pair sharing an element, stored zeros included), gather plan and
+ * device upload of mesh and coefficient tables.  Fails with
+ * GDP_E_UNSUPPORTED for terms the device path does not cover. */
+int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
+                  const gdp_space_desc *spaces, int nterms,
+                  const gdp_term_desc *terms, gdp_fe **fe);
+int gdp_fe_destroy(gdp_fe *fe);
+int gdp_fe_num_dofs(const gdp_fe *fe, int *N, long *nnz);
+/* dofnum[entity] for entities 0..nmax (index 0 unused): > 0 unknown number
+ * (1-based), < 0 -(index into fixed_entity + 1), 0 not a DOF. */
+int gdp_fe_dof_table(const gdp_fe *fe, int space, int *dofnum, int nmax);
+/* The formulation's CSR pattern (rows = equation DOFs, sorted columns):
+ * ia (N+1) and ja (nnz); host data, available without a device. */
+int gdp_fe_pattern(const gdp_fe *fe, int *ia, int *ja);
+/* Generate[A] (SolvingOperations.cpp:276-500 -> per element
+ * Cal_GalerkinTermOfFemEquation + Dof_AssembleInMat): zeroes A and b, then
+ * assembles every term of every element on the device, summing each entry in
+ * ascending element order as the reference does.  A takes the formulation's
+ * pattern on first use. */
+int gdp_fe_generate(gdp_fe *fe, gdp_matrix *A, gdp_vector *b);
+/* Device time of the last gdp_fe_generate (ms) and of its kernels. */
+int gdp_fe_last_timing(const gdp_fe *fe, double *ms_total, double *ms_geometry,
+                       double *ms_gather);
+
+/* ---- host-side pre-processing (no device work) ----------------------------- */
+/* Edge creation (GeoEntity.cpp:437-490): visiting elements in `visit` order,
+ * each new edge (sorted node pair) gets the next number; signed by local
+ * orientation (+ when it runs from the lower to the higher node number).
+ * edges_out: 6 per element, edge_nodes: 2 per edge.  Returns #edges or < 0. */
+int gdp_number_edges(int nelem, const int *etype, const int *conn, int nvisit,
+                     const int *visit, int *edges_out, int *edge_nodes,
+                     int max_edges);
+/* RCMK renumbering (cmkreord.f:3-62 with perphn/BFS of reordering.f):
+ * perm[old] = new, rperm[new] = old (0-based).  Returns #levels or < 0. */
+int gdp_rcmk(int n, const int *ia, const int *ja, int *perm, int *rperm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GDP_HIP_H */
