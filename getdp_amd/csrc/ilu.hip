@@ -154,22 +154,31 @@ __global__ void __launch_bounds__(256) k_fwd(int n, const int *__restrict__ ia,
   st_val(next + i, as_val<TV>(Sentinel<TV>::bits));
   bool done = false;
   unsigned long long t0 = 0;
-  while(!done) {
-    if(k < kend) {
-      auto b = ld_bits(y + ja[k]);
-      if(b != Sentinel<TV>::bits) {
-        acc = acc - (TV)lu[k] * as_val<TV>(b);
-        k++;
+  // The loop runs until every lane of the wave is done (uniform exit), and a
+  // lane publishes its entry inside the iteration in which it finishes: a
+  // lane waiting for a row of the same wave then sees it on a later
+  // iteration.  The compiler barrier keeps the store from being moved to the
+  // loop exit, where it would wait for the whole wave.
+  for(;;) {
+    if(!done) {
+      if(k < kend) {
+        auto b = ld_bits(y + ja[k]);
+        if(b != Sentinel<TV>::bits) {
+          acc = acc - (TV)lu[k] * as_val<TV>(b);
+          k++;
+        }
+        else {
+          if(wait_expired(t0, tlimit, err)) k = kend;
+          __builtin_amdgcn_s_sleep(1);
+        }
       }
-      else {
-        if(wait_expired(t0, tlimit, err)) k = kend;
-        __builtin_amdgcn_s_sleep(1);
+      if(k >= kend) {
+        st_val(y + i, acc);
+        done = true;
       }
     }
-    if(k >= kend) {
-      st_val(y + i, acc);
-      done = true;
-    }
+    __asm__ volatile("" ::: "memory");
+    if(__all(done)) break;
   }
 }
 
@@ -194,22 +203,26 @@ __global__ void __launch_bounds__(256) k_bwd(int n, const int *__restrict__ ia,
   const int kend = ia[i + 1];
   bool done = false;
   unsigned long long t0 = 0;
-  while(!done) {
-    if(k < kend) {
-      auto b = ld_bits(x + ja[k]);
-      if(b != Sentinel<TV>::bits) {
-        acc = acc - (TV)lu[k] * as_val<TV>(b);
-        k++;
+  for(;;) {  // same structure as the forward sweep
+    if(!done) {
+      if(k < kend) {
+        auto b = ld_bits(x + ja[k]);
+        if(b != Sentinel<TV>::bits) {
+          acc = acc - (TV)lu[k] * as_val<TV>(b);
+          k++;
+        }
+        else {
+          if(wait_expired(t0, tlimit, err)) k = kend;
+          __builtin_amdgcn_s_sleep(1);
+        }
       }
-      else {
-        if(wait_expired(t0, tlimit, err)) k = kend;
-        __builtin_amdgcn_s_sleep(1);
+      if(k >= kend) {
+        st_val(x + i, (TV)lu[di] * acc);
+        done = true;
       }
     }
-    if(k >= kend) {
-      st_val(x + i, (TV)lu[di] * acc);
-      done = true;
-    }
+    __asm__ volatile("" ::: "memory");
+    if(__all(done)) break;
   }
 }
 

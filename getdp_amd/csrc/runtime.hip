@@ -475,3 +475,24 @@ extern "C" int gdp_prod_matrix_vector(gdp_matrix *A, const gdp_vector *x, gdp_ve
   GDP_TRY(spmv(A->csr, x->d, y->d));
   return sync();
 }
+
+extern "C" int gdp_bench_spmv(gdp_matrix *A, const gdp_vector *x, gdp_vector *y, int reps,
+                              double *avg_ms)
+{
+  if(!A || !x || !y || reps < 1 || !avg_ms || x->n != A->m || y->n != A->n)
+    return set_error(GDP_E_ARG, "gdp_bench_spmv: bad arguments");
+  if(!A->csr.ia) return set_error(GDP_E_STATE, "matrix has no assembled pattern");
+  hipEvent_t e0, e1;
+  GDP_HIP(hipEventCreate(&e0));
+  GDP_HIP(hipEventCreate(&e1));
+  GDP_HIP(hipEventRecord(e0, ctx().stream));
+  for(int r = 0; r < reps; r++) GDP_TRY(spmv(A->csr, x->d, y->d));
+  GDP_HIP(hipEventRecord(e1, ctx().stream));
+  GDP_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  GDP_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = ms / reps;
+  return GDP_OK;
+}

@@ -151,6 +151,13 @@ def prod_matrix_vector(A: Matrix, x: Vector, y: Vector):
     check(lib().gdp_prod_matrix_vector(A.h, x.h, y.h), "gdp_prod_matrix_vector")
 
 
+def bench_spmv(A: Matrix, x: Vector, y: Vector, reps: int = 20) -> float:
+    """Average device time (ms) of one CSR SpMV launch over `reps` launches."""
+    v = C.c_double()
+    check(lib().gdp_bench_spmv(A.h, x.h, y.h, int(reps), C.byref(v)), "gdp_bench_spmv")
+    return v.value
+
+
 def solve(A: Matrix, b: Vector, s: Solver, x: Vector) -> dict:
     check(lib().gdp_solve(A.h, b.h, s.h, x.h), "gdp_solve")
     return s.info()

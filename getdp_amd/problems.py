@@ -261,7 +261,11 @@ def elasticity3d(n: int = 64, seed: int = 0, sigma: float = 1.0, poisson: float 
     Ct = lame_tensors(Evals, poisson)            # (nbins, 9, 9)
     dom = list(range(nbins))
     clamped = (np.flatnonzero(xyz[:, 0] == 0.0) + 1).astype(np.int32)
-    spaces = [Space(BF_NODE, s + 1, dom, clamped, np.zeros(clamped.shape[0])) for s in range(3)]
+    # the displacement spaces' support covers the loaded boundary as well
+    # (Dom_Mec = Vol_Mec + Sur_Neu_Mec in the elasticity template)
+    support = dom + [RTRAC]
+    spaces = [Space(BF_NODE, s + 1, support, clamped, np.zeros(clamped.shape[0]))
+              for s in range(3)]
     terms = []
     for t9 in range(9):
         coef = np.zeros((nreg, 9))

@@ -122,6 +122,12 @@ int gdp_matrix_get_csr(const gdp_matrix *A, int *ia, int *ja, double *a);
 /* LinAlg_ProdMatrixVector: y = A x on the device (CSR SpMV, FP64). */
 int gdp_prod_matrix_vector(gdp_matrix *A, const gdp_vector *x, gdp_vector *y);
 
+/* Measurement helper, no counterpart in the reference (for benchmarks and
+ * profiling): `reps` back-to-back launches of y = A x on the library stream,
+ * bracketed by HIP events; *avg_ms = average kernel time per launch. */
+int gdp_bench_spmv(gdp_matrix *A, const gdp_vector *x, gdp_vector *y, int reps,
+                   double *avg_ms);
+
 /* ---- solve ----------------------------------------------------------------- */
 /* LinAlg_Solve (LinAlg_SPARSKIT.cpp:572 -> solve_matrix, Sparskit.cpp:108):
  * scaling, RCMK renumbering, ILU(0) and CG / GMRES / FGMRES on the device,

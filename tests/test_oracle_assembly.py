@@ -1,0 +1,181 @@
+"""The oracle's assembly restatement pinned by closed forms and invariants.
+
+No golden element matrices exist in the reference (its only test is a smoke
+run, CMakeLists.txt:1142-1151), so each term is checked against mathematics
+that a plausible mistake (dropped quadrature factor, wrong sign, transposed
+Jacobian, wrong edge orientation) would violate.
+"""
+import numpy as np
+import pytest
+
+from getdp_amd import problems as P
+
+W3 = 3 * 0.16666666666666      # reference's truncated 3-point triangle weights
+W4 = 4 * 0.0416666666667       # reference's truncated 4-point tetrahedron weights
+
+
+def one_element(xyz, conn, etype, kind, ncoef=1, coef=None, npoints=None, bf=P.BF_NODE,
+                edges=None, jac_sur=0, dof=0):
+    xyz = np.asarray(xyz, float)
+    if xyz.shape[1] == 2:
+        xyz = np.c_[xyz, np.zeros(len(xyz))]
+    c = np.zeros((1, 4), np.int32)
+    c[0, :len(conn)] = conn
+    cf = np.asarray(coef if coef is not None else [1.0] * ncoef, float).reshape(1, -1)
+    if npoints is None:
+        npoints = 3 if etype == P.TRI else 4
+    sp = P.Space(bf, 1, [0])
+    t = P.Term(kind, 0, dof, [0], npoints, cf, jac_sur=jac_sur)
+    return P.Problem("one", xyz, np.array([etype], np.int32), np.array([0], np.int32), c, [sp], [t],
+                     edges=edges, edge_nodes=None if edges is None else np.zeros((6, 2), np.int32))
+
+
+def test_p1_triangle_stiffness_closed_form(oracle):
+    pr = one_element([[0, 0], [1, 0], [0, 1]], [1, 2, 3], P.TRI, P.T_GRAD_GRAD)
+    K = oracle.element_matrix(pr, 0, 0)
+    expect = W3 * np.array([[2, -1, -1], [-1, 1, 0], [-1, 0, 1]]) * 1.0
+    assert np.allclose(K, expect, rtol=1e-15, atol=1e-16)
+    assert abs(W3 - 0.5) < 1e-13
+
+
+def test_p1_triangle_stiffness_general_triangle(oracle):
+    """K = area * coef * G G^T with G the physical gradients (cotangent formula)."""
+    X = np.array([[0.1, 0.2], [1.3, 0.4], [0.5, 1.7]])
+    pr = one_element(X, [1, 2, 3], P.TRI, P.T_GRAD_GRAD, coef=[2.5])
+    K = oracle.element_matrix(pr, 0, 0)
+    T = np.c_[np.ones(3), X]
+    G = np.linalg.inv(T)[1:].T          # rows: gradient of barycentric i
+    area = 0.5 * abs(np.linalg.det(T))
+    assert np.allclose(K, (W3 / 0.5) * area * 2.5 * G @ G.T, rtol=1e-13, atol=1e-15)
+
+
+def test_p1_tet_stiffness_and_tensor_form(oracle):
+    X = np.array([[0.0, 0.1, 0.0], [1.1, 0.0, 0.2], [0.2, 0.9, 0.1], [0.1, 0.3, 1.2]])
+    T = np.c_[np.ones(4), X]
+    G = np.linalg.inv(T)[1:].T
+    vol = abs(np.linalg.det(T)) / 6
+    pr = one_element(X, [1, 2, 3, 4], P.TET, P.T_GRAD_GRAD, coef=[1.7])
+    K = oracle.element_matrix(pr, 0, 0)
+    assert np.allclose(K, (W4 / (1 / 6)) * vol * 1.7 * G @ G.T, rtol=1e-12)
+    # tensor coefficient (asymmetric on purpose): K_ij = g_i . (C g_j)
+    C = np.array([[2.0, 0.3, 0.0], [0.1, 1.0, 0.5], [0.0, 0.2, 3.0]])
+    pr = one_element(X, [1, 2, 3, 4], P.TET, P.T_GRAD_GRAD, ncoef=9, coef=C.ravel())
+    pr.terms[0].dof = 0
+    K = oracle.element_matrix(pr, 0, 0)
+    # symmetric term (same quantity): the reference evaluates j <= i and
+    # mirrors, i.e. K_ij = g_max(i,j) . C g_min(i,j)
+    ref = np.zeros((4, 4))
+    for i in range(4):
+        for j in range(4):
+            a, b = max(i, j), min(i, j)
+            ref[i, j] = G[a] @ (C @ G[b])
+    assert np.allclose(K, (W4 / (1 / 6)) * vol * ref, rtol=1e-12)
+
+
+def test_elasticity_rigid_modes_in_kernel(oracle):
+    """Without constraints the elasticity operator annihilates the six rigid
+    body motions (3 translations, 3 rotations)."""
+    pr = P.elasticity3d(2, nbins=4)
+    for s in pr.spaces:
+        s.fixed_entity = np.zeros(0, np.int32)
+        s.fixed_value = np.zeros(0)
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    N = ia.shape[0] - 1
+    xyz = pr.xyz
+    modes = []
+    for d in range(3):
+        u = np.zeros((pr.nnodes, 3)); u[:, d] = 1.0; modes.append(u)
+    for (p, q) in ((0, 1), (1, 2), (2, 0)):
+        u = np.zeros((pr.nnodes, 3)); u[:, p] = -xyz[:, q]; u[:, q] = xyz[:, p]; modes.append(u)
+    for u in modes:
+        v = np.zeros(N)
+        for s in range(3):
+            d = tab[s, 1:pr.nnodes + 1]
+            v[d[d > 0] - 1] = u[d > 0, s]
+        r = oracle.amux(ia, ja, a, v)
+        assert np.abs(r).max() < 1e-12 * np.abs(a).max()
+    # block DOF numbering: all ux, then all uy, then all uz (F7 contract)
+    assert tab[0, 1:].max() < tab[1, 1:][tab[1, 1:] > 0].min()
+    assert tab[1, 1:].max() < tab[2, 1:][tab[2, 1:] > 0].min()
+
+
+def test_poisson_neumann_row_sums_and_source(oracle):
+    pr = P.poisson2d(8)
+    pr.spaces[0].fixed_entity = np.zeros(0, np.int32)
+    pr.spaces[0].fixed_value = np.zeros(0)
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    N = ia.shape[0] - 1
+    assert N == pr.nnodes
+    rows = oracle.amux(ia, ja, a, np.ones(N))
+    assert np.abs(rows).max() < 1e-14
+    # symmetric matrix with the full element-adjacency pattern, zeros stored
+    A = np.zeros((N, N))
+    for i in range(N):
+        A[i, ja[ia[i]:ia[i + 1]]] = a[ia[i]:ia[i + 1]]
+    assert np.allclose(A, A.T, rtol=0, atol=0)
+    assert (a == 0).sum() > 0  # hypotenuse couplings of the right triangles are exact zeros
+    # sum of b = js * area of the source region (partition of unity)
+    ntri = pr.meta["ntri"]
+    src = pr.region[:ntri] > pr.meta["ncls"]
+    tri = pr.conn[:ntri, :3] - 1
+    X = pr.xyz
+    ar = 0.5 * np.abs(np.cross(X[tri[:, 1]] - X[tri[:, 0]], X[tri[:, 2]] - X[tri[:, 0]])[:, 2])
+    assert abs(b.sum() - W3 / 0.5 * ar[src].sum()) < 1e-12
+
+
+def test_curl_curl_kills_gradients_and_edge_mass(oracle):
+    """curl grad = 0: the curl-curl matrix maps the edge DOFs of any discrete
+    gradient (phi(hi) - phi(lo) on each edge lo->hi) to zero; the edge mass
+    reproduces |t|^2 |Omega| for a constant field t."""
+    pr = P.curlcurl3d(2)
+    pr.spaces[0].fixed_entity = np.zeros(0, np.int32)
+    pr.spaces[0].fixed_value = np.zeros(0)
+    cc, ms, src = pr.terms
+    rng = np.random.default_rng(0)
+    phi = rng.standard_normal(pr.nnodes)
+    en = pr.edge_nodes
+    pr.terms = [cc]
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    N = ia.shape[0] - 1
+    d = tab[0, 1:en.shape[0] + 1]
+    g = np.zeros(N)
+    g[d - 1] = phi[en[:, 1] - 1] - phi[en[:, 0] - 1]
+    assert np.abs(oracle.amux(ia, ja, a, g)).max() < 1e-11 * np.abs(a).max()
+    # mass term alone, unit coefficient
+    ms.coef = np.where(ms.coef != 0, 1.0, 0.0)
+    pr.terms = [ms]
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    t = np.array([0.3, -1.2, 0.7])
+    X = pr.xyz
+    dofs = np.zeros(N)
+    dofs[d - 1] = (X[en[:, 1] - 1] - X[en[:, 0] - 1]) @ t
+    energy = dofs @ oracle.amux(ia, ja, a, dofs)
+    assert abs(energy - (t @ t) * 1.0 * (W4 / (1 / 6))) < 1e-10
+
+
+def test_edge_source_integrates_field(oracle):
+    """[ f, {a} ] with constant f: b . dofs(t) = -(f . t) |Omega|."""
+    pr = P.curlcurl3d(2)
+    pr.spaces[0].fixed_entity = np.zeros(0, np.int32)
+    pr.spaces[0].fixed_value = np.zeros(0)
+    f = np.array([0.2, -0.5, 1.5])
+    src = pr.terms[2]
+    src.coef = np.tile(f, (src.coef.shape[0], 1))
+    src.regions = [1, 2, 3, 4]
+    pr.terms = [pr.terms[1], src]
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    en = pr.edge_nodes
+    d = tab[0, 1:en.shape[0] + 1]
+    t = np.array([1.0, 2.0, -0.5])
+    dofs = np.zeros(ia.shape[0] - 1)
+    dofs[d - 1] = (pr.xyz[en[:, 1] - 1] - pr.xyz[en[:, 0] - 1]) @ t
+    assert abs(b @ dofs + (f @ t) * (W4 / (1 / 6))) < 1e-12
+
+
+def test_traction_surface_term(oracle):
+    """Unit traction on x=1: the ux equations receive -sum = -area(face)."""
+    pr = P.elasticity3d(3, nbins=2)
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    assert abs(b.sum() + W3 / 0.5 * 1.0) < 1e-12
+    nx = (tab[0] > 0).sum()
+    assert np.all(b[nx:] == 0.0)
