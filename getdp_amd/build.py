@@ -36,6 +36,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
 # (source, extra flags)
 SOURCES = [
     ("runtime.hip", []),
+    ("sell.hip", []),
     ("fe.hip", ["-ffp-contract=off"]),
     ("ilu.hip", ["-ffp-contract=off"]),
     ("krylov.hip", []),

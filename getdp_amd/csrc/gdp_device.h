@@ -139,6 +139,28 @@ struct DevCSR {
   }
 };
 
+// Sliced-ELLPACK copy of a CSR matrix (layout described in sell.h).
+struct DevSell {
+  int n = 0, nslices = 0;
+  long nnz = 0, nnz_pad = 0;
+  long *sp = nullptr;     // slice start offsets (nslices + 1)
+  int *w = nullptr;       // slice widths
+  int *col = nullptr;     // nnz_pad
+  double *val = nullptr;  // nnz_pad
+  float *val32 = nullptr;
+  void release()
+  {
+    dfree(sp);
+    dfree(w);
+    dfree(col);
+    dfree(val);
+    dfree(val32);
+    n = nslices = 0;
+    nnz = nnz_pad = 0;
+  }
+  ~DevSell() { release(); }
+};
+
 // y = A x (FP64 CSR), launched on the library stream.
 int spmv(const DevCSR &A, const double *x, double *y);
 int spmv_f32(int n, const int *ia, const int *ja, const float *a, const float *x, float *y,
@@ -157,6 +179,8 @@ struct gdp_vector {
 struct gdp_matrix {
   int n = 0, m = 0;
   gdp::DevCSR csr;
+  gdp::DevSell sell;                    // SpMV layout, rebuilt lazily
+  uint64_t sell_pattern = 0, sell_values = 0;
   std::vector<int> h_ia, h_ja;          // host copy of the pattern
   uint64_t pattern_version = 0;         // bumps when the pattern changes
   uint64_t value_version = 0;           // bumps when values change

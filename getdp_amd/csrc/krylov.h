@@ -23,12 +23,12 @@ struct KCtl {
   double H[(kMaxKrylov + 1) * (kMaxKrylov + 2)];  // column j: H[j*(kMaxKrylov+2) + i]
 };
 
-// Operator: y = A x with values of type TA, vectors of type TV.
+// Operator: y = A x with values of type TA (SELL-64 layout, sell.h).
 template <typename TA> struct DevOp {
   int n = 0;
-  long nnz = 0;
-  const int *ia = nullptr, *ja = nullptr;
-  const TA *a = nullptr;
+  const long *sp = nullptr;
+  const int *w = nullptr, *col = nullptr;
+  const TA *val = nullptr;
 };
 
 // Preconditioner applied as z = M^{-1} v (right preconditioning).

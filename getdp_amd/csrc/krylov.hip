@@ -6,23 +6,31 @@
 // residual norm falls to Stopping_Test times the first residual norm.
 //
 // Every scalar decision (Givens rotations, residual estimate, restart,
-// stopping, breakdown) is taken on the device by the last block of a
-// reduction or by a one-thread kernel, so a whole restart cycle (or a batch
-// of CG iterations) is enqueued without host round trips and is captured once
-// into a hipGraph and replayed.  Kernels of a cycle that has already ended
-// return immediately.  Reductions are deterministic (fixed-order partial
-// sums); dot products are accumulated in FP64 whatever the vector type.
+// stopping, breakdown) is taken on the device, so a whole restart cycle (or a
+// batch of CG iterations) is enqueued without host round trips; it is
+// captured once into a hipGraph and replayed.  Kernels of a cycle that has
+// already ended return immediately.
+//
+// Reductions: a streaming kernel writes one partial sum per block; a
+// one-block finalize kernel sums the partials in a fixed order and applies
+// the scalar logic.  The kernel boundary makes the partials visible (no
+// device-scope fences, which cost an L2 write-back on a multi-die GPU), and
+// the result does not depend on scheduling.  Dot products accumulate in FP64
+// whatever the vector type.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "krylov.h"
+#include "sell.h"
 
 namespace gdp {
 
 namespace {
 
-constexpr int B = 256;        // threads per block for vector kernels
+constexpr int B = 256;  // threads per block for vector kernels
 constexpr int LDH = kMaxKrylov + 2;
+constexpr int CGS_PER = 4;  // entries per thread in the CGS kernels
 
 // gate modes: 0 none, 1 skip when done, 2 skip when done or cycle ended
 __device__ inline bool gated(const KCtl *C, int mode)
@@ -32,53 +40,61 @@ __device__ inline bool gated(const KCtl *C, int mode)
   return (C->done | C->cycle_end) != 0;
 }
 
-template <int G, typename TA, typename TV>
-__global__ void __launch_bounds__(256) k_op(int n, const int *__restrict__ ia,
-                                            const int *__restrict__ ja, const TA *__restrict__ a,
-                                            const TV *__restrict__ x, TV *__restrict__ y,
-                                            const KCtl *C, int gmode)
+// ---- operator -------------------------------------------------------------------
+template <typename TA, typename TV>
+__global__ void __launch_bounds__(256) k_op(int n, const long *__restrict__ sp,
+                                            const int *__restrict__ w, const int *__restrict__ col,
+                                            const TA *__restrict__ val, const TV *__restrict__ x,
+                                            TV *__restrict__ y, const KCtl *C, int gmode)
 {
   if(gated(C, gmode)) return;
-  long gt = (long)blockIdx.x * 256 + threadIdx.x;
-  int row = (int)(gt / G), lane = (int)(gt % G);
+  const int row = blockIdx.x * 256 + threadIdx.x;
   if(row >= n) return;
-  int s = ia[row], e = ia[row + 1];
-  double sum = 0;
-  for(int k = s + lane; k < e; k += G) {
-    TA v = __builtin_nontemporal_load(a + k);
-    int c = __builtin_nontemporal_load(ja + k);
-    sum += (double)v * (double)x[c];
+  double sum;
+  sell_row<TA, TV>(row, sp, w, col, val, x, sum);
+  y[row] = (TV)sum;
+}
+
+// y = A x fused with the partial sums of |y|^2 (nrm0 of mgsro) or of <d, y>.
+template <typename TA, typename TV, bool WITH_D>
+__global__ void __launch_bounds__(256) k_op_dot(int n, const long *__restrict__ sp,
+                                                const int *__restrict__ w,
+                                                const int *__restrict__ col,
+                                                const TA *__restrict__ val,
+                                                const TV *__restrict__ x, const TV *__restrict__ d,
+                                                TV *__restrict__ y, const KCtl *C, int gmode,
+                                                double *part)
+{
+  if(gated(C, gmode)) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  double t = 0.;
+  if(row < n) {
+    double sum;
+    sell_row<TA, TV>(row, sp, w, col, val, x, sum);
+    TV yv = (TV)sum;
+    y[row] = yv;
+    t = WITH_D ? (double)d[row] * (double)yv : (double)yv * (double)yv;
   }
-#pragma unroll
-  for(int o = G / 2; o > 0; o >>= 1) sum += __shfl_down(sum, o, G);
-  if(lane == 0) y[row] = (TV)sum;
+  t = block_sum<double, 256>(t);
+  if(threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 template <typename TA, typename TV>
 int op_apply(const DevOp<TA> &A, const TV *x, TV *y, const KCtl *C, int gmode)
 {
-  int G = spmv_lanes(A.nnz, A.n);
-  long threads = (long)A.n * G;
-  dim3 grid((unsigned)((threads + 255) / 256));
-  hipStream_t s = ctx().stream;
-  switch(G) {
-  case 2: hipLaunchKernelGGL((k_op<2, TA, TV>), grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, y, C, gmode); break;
-  case 4: hipLaunchKernelGGL((k_op<4, TA, TV>), grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, y, C, gmode); break;
-  case 8: hipLaunchKernelGGL((k_op<8, TA, TV>), grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, y, C, gmode); break;
-  case 16: hipLaunchKernelGGL((k_op<16, TA, TV>), grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, y, C, gmode); break;
-  default: hipLaunchKernelGGL((k_op<32, TA, TV>), grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, y, C, gmode); break;
-  }
+  if(!A.n) return GDP_OK;
+  hipLaunchKernelGGL((k_op<TA, TV>), dim3((A.n + 255) / 256), dim3(256), 0, ctx().stream, A.n, A.sp,
+                     A.w, A.col, A.val, x, y, C, gmode);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
 
-// ---- GMRES pieces -----------------------------------------------------------------
+// ---- streaming kernels with per-block partial sums ------------------------------------
 
-// label 10/20 of gmres: w = rhs - A*sol (A*sol already in w), alpha = |w|,
-// first-iteration tolerance, convergence test before the cycle.
+// w = rhs - w (A*sol already in w); partial |w|^2
 template <typename TV>
-__global__ void __launch_bounds__(B) k_gm_resid(int n, const TV *__restrict__ rhs, TV *w, KCtl *C,
-                                               double *part)
+__global__ void __launch_bounds__(B) k_gm_resid(int n, const TV *__restrict__ rhs, TV *w,
+                                               const KCtl *C, double *part)
 {
   if(C->done) return;
   double s = 0;
@@ -89,30 +105,6 @@ __global__ void __launch_bounds__(B) k_gm_resid(int n, const TV *__restrict__ rh
   }
   s = block_sum<double, B>(s);
   if(threadIdx.x == 0) part[blockIdx.x] = s;
-  if(grid_last_block(&C->ticket[0])) {
-    double tot = sum_partials<double, B>(part, gridDim.x);
-    if(threadIdx.x == 0) {
-      double alpha = sqrt(tot);
-      C->its++;
-      if(C->its == 1) {
-        C->fpar4 = C->rtol * alpha + 0.0;
-        C->fpar3 = alpha;
-      }
-      C->fpar5 = alpha;
-      C->g[0] = alpha;
-      C->cycle_end = 0;
-      C->kfinal = 0;
-      C->ierr12 = 0;
-      C->pend_flag = 0;
-      if(alpha <= C->fpar4) {
-        C->status = 0;
-        C->fpar6 = alpha;
-        C->done = 1;
-      }
-      else
-        C->scal = 1.0 / alpha;
-    }
-  }
 }
 
 template <typename TV>
@@ -124,33 +116,11 @@ __global__ void __launch_bounds__(B) k_scale(int n, TV *v, const KCtl *C, int gm
     v[i] = (TV)(f * (double)v[i]);
 }
 
-// |v|^2 of the new Krylov vector (mgsro's nrm0, iters.f:3532-3541).
-template <typename TV>
-__global__ void __launch_bounds__(B) k_nrm0(int n, const TV *v, KCtl *C, double *part)
-{
-  if(gated(C, 2)) return;
-  double s = 0;
-  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B)
-    s += (double)v[i] * (double)v[i];
-  s = block_sum<double, B>(s);
-  if(threadIdx.x == 0) part[blockIdx.x] = s;
-  if(grid_last_block(&C->ticket[1])) {
-    double tot = sum_partials<double, B>(part, gridDim.x);
-    if(threadIdx.x == 0) {
-      C->nrm0 = tot;
-      C->thr = tot * 0.98;
-      C->pend_flag = 0;
-      if(tot <= 0.0) C->ierr12 = -1;
-      else if(!(tot > 0.0 && 1.0 / tot > 0.0)) C->ierr12 = -2;
-    }
-  }
-}
-
 // MGS step i, part 1: apply a pending re-orthogonalisation against V_{i-1},
-// then h = V_k . V_i.
+// then partial sums of V_k . V_i.
 template <typename TV>
 __global__ void __launch_bounds__(B) k_mgs_dot(int n, int i, TV *__restrict__ vk,
-                                              const TV *__restrict__ V, KCtl *C, double *part)
+                                              const TV *__restrict__ V, const KCtl *C, double *part)
 {
   if(gated(C, 2) || C->ierr12) return;
   const TV *vi = V + (long)i * n;
@@ -168,21 +138,13 @@ __global__ void __launch_bounds__(B) k_mgs_dot(int n, int i, TV *__restrict__ vk
   }
   s = block_sum<double, B>(s);
   if(threadIdx.x == 0) part[blockIdx.x] = s;
-  if(grid_last_block(&C->ticket[2])) {
-    double tot = sum_partials<double, B>(part, gridDim.x);
-    if(threadIdx.x == 0) {
-      C->h1[i] = tot;
-      C->pend_flag = 0;
-    }
-  }
 }
 
-// MGS step i, part 2: V_k -= h_i V_i, then the re-orthogonalisation test of
-// mgsro (iters.f:3563-3575): if h_i^2 > thr the correction V_k . V_i is added
-// to h_i and subtracted by the next kernel.
+// MGS step i, part 2: V_k -= h_i V_i, then partial sums of V_k . V_i for the
+// re-orthogonalisation test of mgsro (iters.f:3563-3575).
 template <typename TV>
 __global__ void __launch_bounds__(B) k_mgs_axpy(int n, int i, TV *__restrict__ vk,
-                                               const TV *__restrict__ V, KCtl *C, double *part)
+                                               const TV *__restrict__ V, const KCtl *C, double *part)
 {
   if(gated(C, 2) || C->ierr12) return;
   const TV *vi = V + (long)i * n;
@@ -195,28 +157,12 @@ __global__ void __launch_bounds__(B) k_mgs_axpy(int n, int i, TV *__restrict__ v
   }
   s = block_sum<double, B>(s);
   if(threadIdx.x == 0) part[blockIdx.x] = s;
-  if(grid_last_block(&C->ticket[3])) {
-    double tot = sum_partials<double, B>(part, gridDim.x);
-    if(threadIdx.x == 0) {
-      if(f * f > C->thr) {
-        C->h1[i] = C->h1[i] + tot;
-        C->pend = tot;
-        C->pend_flag = 1;
-      }
-      else
-        C->pend_flag = 0;
-      double nrm0 = C->nrm0 - C->h1[i] * C->h1[i];
-      if(nrm0 < 0.0) nrm0 = 0.0;
-      C->nrm0 = nrm0;
-      C->thr = nrm0 * 0.98;
-    }
-  }
 }
 
-// MGS end: apply the last pending correction, norm of V_k, Hessenberg column.
+// MGS end: apply the last pending correction; partial |V_k|^2.
 template <typename TV>
 __global__ void __launch_bounds__(B) k_mgs_norm(int n, int k, TV *__restrict__ vk,
-                                               const TV *__restrict__ V, KCtl *C, double *part)
+                                               const TV *__restrict__ V, const KCtl *C, double *part)
 {
   if(gated(C, 2) || C->ierr12) return;
   const TV *vp = V + (long)(k - 1) * n;
@@ -233,29 +179,13 @@ __global__ void __launch_bounds__(B) k_mgs_norm(int n, int k, TV *__restrict__ v
   }
   s = block_sum<double, B>(s);
   if(threadIdx.x == 0) part[blockIdx.x] = s;
-  if(grid_last_block(&C->ticket[4])) {
-    double tot = sum_partials<double, B>(part, gridDim.x);
-    if(threadIdx.x == 0) {
-      double nrm1 = sqrt(tot);
-      double *hc = C->H + (long)(k - 1) * LDH;
-      for(int i = 0; i < k; i++) hc[i] = C->h1[i];
-      hc[k] = nrm1;
-      C->pend_flag = 0;
-      if(nrm1 <= 0.0) C->ierr12 = -3;
-      else C->scal = 1.0 / nrm1;
-    }
-  }
 }
 
-// CGS2: pass 1 computes h = V^T v; pass 2 subtracts V h and computes
-// h' = V^T v; pass 3 subtracts V h' and computes |v|.
-// One block owns one chunk of CGS_PER*B entries (grid = ceil(n / chunk)); the
-// chunk of v stays in registers while the basis vectors stream past once.
-constexpr int CGS_PER = 4;
-
+// CGS2: pass 1 h = V^T v; pass 2 v -= V h, h' = V^T v; pass 3 v -= V h', |v|^2.
+// One block owns one chunk of CGS_PER*B entries (grid = ceil(n / chunk)).
 template <typename TV, int PASS>
 __global__ void __launch_bounds__(B) k_cgs(int n, int k, TV *__restrict__ vk,
-                                          const TV *__restrict__ V, KCtl *C, double *part)
+                                          const TV *__restrict__ V, const KCtl *C, double *part)
 {
   if(gated(C, 2) || C->ierr12) return;
   const int nb = gridDim.x;
@@ -301,26 +231,189 @@ __global__ void __launch_bounds__(B) k_cgs(int n, int k, TV *__restrict__ vk,
     double t = block_sum<double, B>(ss);
     if(threadIdx.x == 0) part[blockIdx.x] = t;
   }
-  if(grid_last_block(&C->ticket[5])) {
-    if(PASS <= 2) {
-      for(int i = 0; i < k; i++) {
-        double t = sum_partials<double, B>(part + (long)i * nb, nb);
-        if(threadIdx.x == 0) {
-          if(PASS == 1) C->h1[i] = t;
-          else C->h2[i] = t;
-        }
+}
+
+// CG: p = r = rhs - Ap (A*sol in ap); partial |r|^2
+template <typename TV>
+__global__ void __launch_bounds__(B) k_cg_init(int n, const TV *rhs, const TV *ap, TV *r, TV *p,
+                                              double *part)
+{
+  double s = 0;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B) {
+    TV v = rhs[i] - ap[i];
+    p[i] = v;
+    r[i] = v;
+    s += (double)v * (double)v;
+  }
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// CG: dx += alpha p, r -= alpha Ap; partial |r|^2
+template <typename TV>
+__global__ void __launch_bounds__(B) k_cg_update(int n, const TV *p, const TV *ap, TV *r, TV *dx,
+                                                const KCtl *C, double *part)
+{
+  if(C->done) return;
+  const double alpha = C->alpha;
+  double s = 0;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B) {
+    dx[i] = (TV)((double)dx[i] + alpha * (double)p[i]);
+    TV v = (TV)((double)r[i] - alpha * (double)ap[i]);
+    r[i] = v;
+    s += (double)v * (double)v;
+  }
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(B) k_cg_p(int n, const TV *r, TV *p, const KCtl *C)
+{
+  if(C->done) return;
+  const double beta = C->beta;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B)
+    p[i] = (TV)((double)r[i] + beta * (double)p[i]);
+}
+
+// ---- finalize kernels (one block) -------------------------------------------------------
+
+enum FinOp {
+  FIN_GM_RESID,   // gmres label 10/20: alpha = |w|, first tolerance, early exit
+  FIN_NRM0,       // mgsro: nrm0 = |v|^2, abnormal-number checks
+  FIN_MGS_DOT,    // h_i = V_k . V_i
+  FIN_MGS_AXPY,   // re-orthogonalisation test, running norm update
+  FIN_MGS_NORM,   // nrm1 = |V_k|, Hessenberg column
+  FIN_CGS1,
+  FIN_CGS2,
+  FIN_CGS3,
+  FIN_CG_INIT,
+  FIN_CG_PAP,     // brkdn + alpha = <r,r>/<p,Ap>
+  FIN_CG_UPDATE   // stopbis (iters.f:3275-3330)
+};
+
+template <int OP>
+__global__ void __launch_bounds__(B) k_fin(KCtl *C, const double *part, int nparts, int idx)
+{
+  // same gate as the producing kernel
+  if(OP == FIN_GM_RESID || OP == FIN_CG_PAP || OP == FIN_CG_UPDATE) {
+    if(C->done) return;
+  }
+  else if(OP == FIN_NRM0) {
+    if(gated(C, 2)) return;
+  }
+  else if(OP != FIN_CG_INIT) {
+    if(gated(C, 2) || C->ierr12) return;
+  }
+  if(OP == FIN_CGS1 || OP == FIN_CGS2) {
+    for(int i = 0; i < idx; i++) {
+      double t = sum_partials<double, B>(part + (long)i * nparts, nparts);
+      if(threadIdx.x == 0) {
+        if(OP == FIN_CGS1) C->h1[i] = t;
+        else C->h2[i] = t;
       }
     }
+    return;
+  }
+  const double tot = sum_partials<double, B>(part, nparts);
+  if(threadIdx.x) return;
+  if(OP == FIN_GM_RESID) {
+    double alpha = sqrt(tot);
+    C->its++;
+    if(C->its == 1) {
+      C->fpar4 = C->rtol * alpha + 0.0;
+      C->fpar3 = alpha;
+    }
+    C->fpar5 = alpha;
+    C->g[0] = alpha;
+    C->cycle_end = 0;
+    C->kfinal = 0;
+    C->ierr12 = 0;
+    C->pend_flag = 0;
+    if(alpha <= C->fpar4) {
+      C->status = 0;
+      C->fpar6 = alpha;
+      C->done = 1;
+    }
+    else
+      C->scal = 1.0 / alpha;
+  }
+  else if(OP == FIN_NRM0) {
+    C->nrm0 = tot;
+    C->thr = tot * 0.98;
+    C->pend_flag = 0;
+    if(tot <= 0.0) C->ierr12 = -1;
+    else if(!(tot > 0.0 && 1.0 / tot > 0.0)) C->ierr12 = -2;
+  }
+  else if(OP == FIN_MGS_DOT) {
+    C->h1[idx] = tot;
+    C->pend_flag = 0;
+  }
+  else if(OP == FIN_MGS_AXPY) {
+    const double f = C->h1[idx];
+    if(f * f > C->thr) {
+      C->h1[idx] = C->h1[idx] + tot;
+      C->pend = tot;
+      C->pend_flag = 1;
+    }
+    else
+      C->pend_flag = 0;
+    double nrm0 = C->nrm0 - C->h1[idx] * C->h1[idx];
+    if(nrm0 < 0.0) nrm0 = 0.0;
+    C->nrm0 = nrm0;
+    C->thr = nrm0 * 0.98;
+  }
+  else if(OP == FIN_MGS_NORM || OP == FIN_CGS3) {
+    const int k = idx;
+    double nrm1 = sqrt(tot);
+    double *hc = C->H + (long)(k - 1) * LDH;
+    for(int i = 0; i < k; i++) hc[i] = (OP == FIN_CGS3) ? C->h1[i] + C->h2[i] : C->h1[i];
+    hc[k] = nrm1;
+    C->pend_flag = 0;
+    if(nrm1 <= 0.0) C->ierr12 = -3;
+    else C->scal = 1.0 / nrm1;
+  }
+  else if(OP == FIN_CG_INIT) {
+    C->its = 1;
+    C->fpar7 = tot;
+    C->fpar3 = sqrt(tot);
+    C->fpar5 = C->fpar3;
+    C->fpar4 = C->rtol * C->fpar3 + 0.0;
+    C->fpar6 = 0.0;
+    C->ipar11 = 0;
+    C->status = 1;
+  }
+  else if(OP == FIN_CG_PAP) {
+    const double alpha = tot;
+    C->its++;
+    int bk = 0;  // brkdn, iters.f:3368-3401
+    if(alpha > 0.0) bk = (1.0 / alpha > 0.0) ? 0 : -9;
+    else if(alpha < 0.0) bk = (1.0 / alpha < 0.0) ? 0 : -9;
+    else if(alpha == 0.0) bk = -3;
+    else bk = -9;
+    if(bk) {
+      C->status = bk;
+      C->done = 1;
+    }
+    else
+      C->alpha = C->fpar7 / alpha;
+  }
+  else if(OP == FIN_CG_UPDATE) {
+    if(C->maxits > 0 && C->its >= C->maxits) {
+      C->status = -1;
+      C->done = 1;
+      return;
+    }
+    C->fpar5 = sqrt(tot);
+    C->fpar6 = C->fpar5;
+    if(C->fpar6 > C->fpar4) {
+      C->ipar11 = 0;
+      C->beta = C->fpar5 * C->fpar5 / C->fpar7;
+      C->fpar7 = C->fpar5 * C->fpar5;
+    }
     else {
-      double t = sum_partials<double, B>(part, nb);
-      if(threadIdx.x == 0) {
-        double nrm1 = sqrt(t);
-        double *hc = C->H + (long)(k - 1) * LDH;
-        for(int i = 0; i < k; i++) hc[i] = C->h1[i] + C->h2[i];
-        hc[k] = nrm1;
-        if(nrm1 <= 0.0) C->ierr12 = -3;
-        else C->scal = 1.0 / nrm1;
-      }
+      C->ipar11 = 1;
+      C->done = 1;
     }
   }
 }
@@ -466,113 +559,6 @@ __global__ void k_gm_cycle_end(KCtl *C)
   C->done = 1;
 }
 
-// ---- CG pieces ----------------------------------------------------------------------
-template <typename TV>
-__global__ void __launch_bounds__(B) k_cg_init(int n, const TV *rhs, const TV *ap, TV *r, TV *p,
-                                              KCtl *C, double *part)
-{
-  double s = 0;
-  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B) {
-    TV v = rhs[i] - ap[i];
-    p[i] = v;
-    r[i] = v;
-    s += (double)v * (double)v;
-  }
-  s = block_sum<double, B>(s);
-  if(threadIdx.x == 0) part[blockIdx.x] = s;
-  if(grid_last_block(&C->ticket[0])) {
-    double tot = sum_partials<double, B>(part, gridDim.x);
-    if(threadIdx.x == 0) {
-      C->its = 1;
-      C->fpar7 = tot;
-      C->fpar3 = sqrt(tot);
-      C->fpar5 = C->fpar3;
-      C->fpar4 = C->rtol * C->fpar3 + 0.0;
-      C->fpar6 = 0.0;
-      C->ipar11 = 0;
-      C->status = 1;
-    }
-  }
-}
-
-// iters.f:478-486 (label 60): alpha = <p, Ap>, breakdown test, alpha = rr/alpha
-template <typename TV>
-__global__ void __launch_bounds__(B) k_cg_pap(int n, const TV *p, const TV *ap, KCtl *C,
-                                             double *part)
-{
-  if(C->done) return;
-  double s = 0;
-  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B)
-    s += (double)p[i] * (double)ap[i];
-  s = block_sum<double, B>(s);
-  if(threadIdx.x == 0) part[blockIdx.x] = s;
-  if(grid_last_block(&C->ticket[1])) {
-    double alpha = sum_partials<double, B>(part, gridDim.x);
-    if(threadIdx.x == 0) {
-      C->its++;
-      int bk = 0;  // brkdn, iters.f:3368-3401
-      if(alpha > 0.0) bk = (1.0 / alpha > 0.0) ? 0 : -9;
-      else if(alpha < 0.0) bk = (1.0 / alpha < 0.0) ? 0 : -9;
-      else if(alpha == 0.0) bk = -3;
-      else bk = -9;
-      if(bk) {
-        C->status = bk;
-        C->done = 1;
-      }
-      else
-        C->alpha = C->fpar7 / alpha;
-    }
-  }
-}
-
-// dx += alpha p, r -= alpha Ap, then stopbis (iters.f:3275-3330).
-template <typename TV>
-__global__ void __launch_bounds__(B) k_cg_update(int n, const TV *p, const TV *ap, TV *r, TV *dx,
-                                                KCtl *C, double *part)
-{
-  if(C->done) return;
-  const double alpha = C->alpha;
-  double s = 0;
-  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B) {
-    dx[i] = (TV)((double)dx[i] + alpha * (double)p[i]);
-    TV v = (TV)((double)r[i] - alpha * (double)ap[i]);
-    r[i] = v;
-    s += (double)v * (double)v;
-  }
-  s = block_sum<double, B>(s);
-  if(threadIdx.x == 0) part[blockIdx.x] = s;
-  if(grid_last_block(&C->ticket[2])) {
-    double tot = sum_partials<double, B>(part, gridDim.x);
-    if(threadIdx.x == 0) {
-      if(C->maxits > 0 && C->its >= C->maxits) {
-        C->status = -1;
-        C->done = 1;
-        return;
-      }
-      C->fpar5 = sqrt(tot);
-      C->fpar6 = C->fpar5;
-      if(C->fpar6 > C->fpar4) {
-        C->ipar11 = 0;
-        C->beta = C->fpar5 * C->fpar5 / C->fpar7;
-        C->fpar7 = C->fpar5 * C->fpar5;
-      }
-      else {
-        C->ipar11 = 1;
-        C->done = 1;
-      }
-    }
-  }
-}
-
-template <typename TV>
-__global__ void __launch_bounds__(B) k_cg_p(int n, const TV *r, TV *p, const KCtl *C)
-{
-  if(C->done) return;
-  const double beta = C->beta;
-  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B)
-    p[i] = (TV)((double)r[i] + beta * (double)p[i]);
-}
-
 int reduce_blocks(int n)
 {
   long nb = ((long)n + B * 4 - 1) / (B * 4);
@@ -601,8 +587,8 @@ Workspace &ws()
 int ensure_ws(size_t vec_bytes, int n)
 {
   Workspace &W = ws();
-  size_t nblk = (size_t)std::max(1024L, ((long)n + B * CGS_PER - 1) / (B * CGS_PER));
-  size_t need = nblk * (kMaxKrylov + 2);
+  size_t rows = (size_t)std::max(1024L, ((long)n + 255) / 256);
+  size_t need = rows * (kMaxKrylov + 2);
   if(!W.ctl) GDP_TRY(dalloc(&W.ctl, 1));
   if(W.part_size < need) {
     dfree(W.part);
@@ -619,7 +605,8 @@ int ensure_ws(size_t vec_bytes, int n)
   return GDP_OK;
 }
 
-template <typename TV> int prec_apply(const PrecOp<TV> &M, const TV *in, TV *out, KCtl *C, int gmode, int n)
+template <typename TV>
+int prec_apply(const PrecOp<TV> &M, const TV *in, TV *out, KCtl *C, int gmode, int n)
 {
   if(M.kind == PREC_ILU) return M.ilu->template apply<TV>(in, (TV)1, out, (const int *)C, gmode);
   if(M.kind == PREC_USER) {
@@ -642,10 +629,22 @@ struct GraphCache {
   }
 };
 
+// GDP_NO_GRAPHS=1 in the environment launches the cycle's kernels directly
+// (for tools that trace individual dispatches).
+bool graphs_disabled()
+{
+  static int v = -1;
+  if(v < 0) {
+    const char *e = std::getenv("GDP_NO_GRAPHS");
+    v = (e && e[0] && e[0] != '0') ? 1 : 0;
+  }
+  return v == 1;
+}
+
 template <typename Fn> int run_maybe_graph(bool use, GraphCache &gc, Fn fn)
 {
   hipStream_t s = ctx().stream;
-  if(!use) return fn();
+  if(!use || graphs_disabled()) return fn();
   if(!gc.exec) {
     GDP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int r = fn();
@@ -672,6 +671,11 @@ int read_ctl(KCtl *d, int *done, int *status, int *its)
   *status = h[1];
   *its = h[2];
   return GDP_OK;
+}
+
+template <int OP> void fin(KCtl *C, const double *part, int nparts, int idx)
+{
+  hipLaunchKernelGGL(k_fin<OP>, dim3(1), dim3(B), 0, ctx().stream, C, part, nparts, idx);
 }
 
 }  // namespace
@@ -701,7 +705,9 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
   KCtl *C = W.ctl;
   double *part = W.part;
   const int nb = reduce_blocks(n);
-  dim3 g(nb), blk(B);
+  const int nrow = std::max(1, (n + 255) / 256);
+  const int ncgs = (int)std::max(1L, ((long)n + B * CGS_PER - 1) / (B * CGS_PER));
+  dim3 g(nb), blk(B), grow(nrow);
   int done = 0, status = 0, its = 0;
 
   if(cg) {
@@ -709,20 +715,23 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
        *dx = base + 4 * (size_t)n;
     GDP_HIP(hipMemsetAsync(dx, 0, sizeof(TV) * n, s));
     GDP_TRY(op_apply(A, sol, ap, C, 0));
-    hipLaunchKernelGGL(k_cg_init<TV>, g, blk, 0, s, n, rhs, ap, r, p, C, part);
+    hipLaunchKernelGGL(k_cg_init<TV>, g, blk, 0, s, n, rhs, ap, r, p, part);
+    fin<FIN_CG_INIT>(C, part, nb, 0);
     const bool rp = M.kind != PREC_NONE;
     GraphCache gc;
     const int batch = 16;
     auto body = [&]() -> int {
       for(int it = 0; it < batch; it++) {
+        const TV *src = p;
         if(rp) {
           GDP_TRY(prec_apply(M, p, z, C, 1, n));
-          GDP_TRY(op_apply(A, z, ap, C, 1));
+          src = z;
         }
-        else
-          GDP_TRY(op_apply(A, p, ap, C, 1));
-        hipLaunchKernelGGL(k_cg_pap<TV>, g, blk, 0, s, n, p, ap, C, part);
+        hipLaunchKernelGGL((k_op_dot<TA, TV, true>), grow, dim3(256), 0, s, n, A.sp, A.w, A.col,
+                           A.val, src, p, ap, C, 1, part);
+        fin<FIN_CG_PAP>(C, part, nrow, 0);
         hipLaunchKernelGGL(k_cg_update<TV>, g, blk, 0, s, n, p, ap, r, dx, C, part);
+        fin<FIN_CG_UPDATE>(C, part, nb, 0);
         hipLaunchKernelGGL(k_cg_p<TV>, g, blk, 0, s, n, r, p, C);
       }
       GDP_HIP(hipGetLastError());
@@ -761,14 +770,15 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
   }
 
   // ---- (F)GMRES ----
-  TV *V = base;                              // m+1 basis vectors
-  TV *Z = base + (size_t)(m + 1) * n;        // FGMRES: m vectors; GMRES: 1 temp
+  TV *V = base;                        // m+1 basis vectors
+  TV *Z = base + (size_t)(m + 1) * n;  // FGMRES: m vectors; GMRES: 1 temp
   TV *tmpz = Z;
   const bool rp = M.kind != PREC_NONE;
   GraphCache gc;
   auto cycle = [&]() -> int {
     GDP_TRY(op_apply(A, sol, V, C, 1));
     hipLaunchKernelGGL(k_gm_resid<TV>, g, blk, 0, s, n, rhs, V, C, part);
+    fin<FIN_GM_RESID>(C, part, nb, 0);
     hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, V, C, 1);
     for(int k = 1; k <= m; k++) {
       TV *vk1 = V + (size_t)(k - 1) * n, *vk = V + (size_t)k * n;
@@ -782,20 +792,27 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
         GDP_TRY(prec_apply(M, vk1, tmpz, C, 2, n));
         src = tmpz;
       }
-      GDP_TRY(op_apply(A, src, vk, C, 2));
-      hipLaunchKernelGGL(k_nrm0<TV>, g, blk, 0, s, n, vk, C, part);
+      hipLaunchKernelGGL((k_op_dot<TA, TV, false>), grow, dim3(256), 0, s, n, A.sp, A.w, A.col,
+                         A.val, src, (const TV *)nullptr, vk, C, 2, part);
+      fin<FIN_NRM0>(C, part, nrow, 0);
       if(opt.orth == 1) {
-        dim3 gc((unsigned)std::max(1L, ((long)n + B * CGS_PER - 1) / (B * CGS_PER)));
-        hipLaunchKernelGGL((k_cgs<TV, 1>), gc, blk, 0, s, n, k, vk, V, C, part);
-        hipLaunchKernelGGL((k_cgs<TV, 2>), gc, blk, 0, s, n, k, vk, V, C, part);
-        hipLaunchKernelGGL((k_cgs<TV, 3>), gc, blk, 0, s, n, k, vk, V, C, part);
+        dim3 gc2(ncgs);
+        hipLaunchKernelGGL((k_cgs<TV, 1>), gc2, blk, 0, s, n, k, vk, V, C, part);
+        fin<FIN_CGS1>(C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 2>), gc2, blk, 0, s, n, k, vk, V, C, part);
+        fin<FIN_CGS2>(C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 3>), gc2, blk, 0, s, n, k, vk, V, C, part);
+        fin<FIN_CGS3>(C, part, ncgs, k);
       }
       else {
         for(int i = 0; i < k; i++) {
           hipLaunchKernelGGL(k_mgs_dot<TV>, g, blk, 0, s, n, i, vk, V, C, part);
+          fin<FIN_MGS_DOT>(C, part, nb, i);
           hipLaunchKernelGGL(k_mgs_axpy<TV>, g, blk, 0, s, n, i, vk, V, C, part);
+          fin<FIN_MGS_AXPY>(C, part, nb, i);
         }
         hipLaunchKernelGGL(k_mgs_norm<TV>, g, blk, 0, s, n, k, vk, V, C, part);
+        fin<FIN_MGS_NORM>(C, part, nb, k);
       }
       hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, vk, C, 2);
       hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, s, k, C);

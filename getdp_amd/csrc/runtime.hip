@@ -15,6 +15,7 @@
 #include <numeric>
 
 #include "gdp_device.h"
+#include "sell.h"
 
 namespace gdp {
 
@@ -467,12 +468,28 @@ extern "C" int gdp_matrix_get_csr(const gdp_matrix *A, int *ia, int *ja, double 
   return GDP_OK;
 }
 
+// (Re)build the matrix's SELL copy when its pattern or values changed.
+static int matrix_sell(gdp_matrix *A)
+{
+  if(A->sell_pattern != A->pattern_version || !A->sell.n) {
+    GDP_TRY(sell_build(A->h_ia, A->csr, A->sell));
+    A->sell_pattern = A->pattern_version;
+    A->sell_values = A->value_version;
+  }
+  else if(A->sell_values != A->value_version) {
+    GDP_TRY(sell_copy_values(A->csr, A->sell, false));
+    A->sell_values = A->value_version;
+  }
+  return GDP_OK;
+}
+
 extern "C" int gdp_prod_matrix_vector(gdp_matrix *A, const gdp_vector *x, gdp_vector *y)
 {
   if(!A || !x || !y || x->n != A->m || y->n != A->n)
     return set_error(GDP_E_ARG, "gdp_prod_matrix_vector: size mismatch");
   if(!A->csr.ia) return set_error(GDP_E_STATE, "matrix has no assembled pattern");
-  GDP_TRY(spmv(A->csr, x->d, y->d));
+  GDP_TRY(matrix_sell(A));
+  GDP_TRY(sell_spmv(A->sell, x->d, y->d));
   return sync();
 }
 
@@ -485,8 +502,10 @@ extern "C" int gdp_bench_spmv(gdp_matrix *A, const gdp_vector *x, gdp_vector *y,
   hipEvent_t e0, e1;
   GDP_HIP(hipEventCreate(&e0));
   GDP_HIP(hipEventCreate(&e1));
+  GDP_TRY(matrix_sell(A));
+  GDP_TRY(sync());
   GDP_HIP(hipEventRecord(e0, ctx().stream));
-  for(int r = 0; r < reps; r++) GDP_TRY(spmv(A->csr, x->d, y->d));
+  for(int r = 0; r < reps; r++) GDP_TRY(sell_spmv(A->sell, x->d, y->d));
   GDP_HIP(hipEventRecord(e1, ctx().stream));
   GDP_HIP(hipEventSynchronize(e1));
   float ms = 0;

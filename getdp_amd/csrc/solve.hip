@@ -15,6 +15,7 @@
 #include <memory>
 
 #include "krylov.h"
+#include "sell.h"
 
 namespace gdp {
 
@@ -79,23 +80,20 @@ __global__ void __launch_bounds__(BT) k_to_f32(long n, const double *a, float *b
   if(i < n) b[i] = (float)a[i];
 }
 
-// r = rhs - A x (FP64), partial |r|^2 and |rhs|^2 per block
-template <int G>
-__global__ void __launch_bounds__(256) k_residual(int n, const int *__restrict__ ia,
-                                                  const int *__restrict__ ja,
-                                                  const double *__restrict__ a,
+// r = rhs - A x (FP64, SELL layout)
+__global__ void __launch_bounds__(256) k_residual(int n, const long *__restrict__ sp,
+                                                  const int *__restrict__ w,
+                                                  const int *__restrict__ col,
+                                                  const double *__restrict__ val,
                                                   const double *__restrict__ x,
                                                   const double *__restrict__ rhs,
                                                   double *__restrict__ r)
 {
-  long gt = (long)blockIdx.x * 256 + threadIdx.x;
-  int row = (int)(gt / G), lane = (int)(gt % G);
+  const int row = blockIdx.x * 256 + threadIdx.x;
   if(row >= n) return;
-  double sum = 0;
-  for(int k = ia[row] + lane; k < ia[row + 1]; k += G) sum += a[k] * x[ja[k]];
-#pragma unroll
-  for(int o = G / 2; o > 0; o >>= 1) sum += __shfl_down(sum, o, G);
-  if(lane == 0) r[row] = rhs[row] - sum;
+  double sum;
+  sell_row<double, double>(row, sp, w, col, val, x, sum);
+  r[row] = rhs[row] - sum;
 }
 
 template <int BLOCK>
@@ -137,19 +135,11 @@ int norm2_dev(const double *x, int n, double *out)
   return GDP_OK;
 }
 
-int residual_dev(const DevCSR &A, const double *x, const double *rhs, double *r)
+int residual_dev(const DevSell &A, const double *x, const double *rhs, double *r)
 {
-  int G = spmv_lanes(A.nnz, A.n);
-  long threads = (long)A.n * G;
-  dim3 grid((unsigned)((threads + 255) / 256));
-  hipStream_t s = ctx().stream;
-  switch(G) {
-  case 2: hipLaunchKernelGGL(k_residual<2>, grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, rhs, r); break;
-  case 4: hipLaunchKernelGGL(k_residual<4>, grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, rhs, r); break;
-  case 8: hipLaunchKernelGGL(k_residual<8>, grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, rhs, r); break;
-  case 16: hipLaunchKernelGGL(k_residual<16>, grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, rhs, r); break;
-  default: hipLaunchKernelGGL(k_residual<32>, grid, dim3(256), 0, s, A.n, A.ia, A.ja, A.a, x, rhs, r); break;
-  }
+  if(!A.n) return GDP_OK;
+  hipLaunchKernelGGL(k_residual, dim3((A.n + 255) / 256), dim3(256), 0, ctx().stream, A.n, A.sp, A.w,
+                     A.col, A.val, x, rhs, r);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
@@ -165,6 +155,7 @@ struct SolveCache {
   std::vector<int> perm, rperm, ib, jb, map;
   int *d_perm = nullptr, *d_rperm = nullptr, *d_map = nullptr, *d_row_of = nullptr;
   DevCSR Bm;              // renumbered (scaled) matrix
+  DevSell sell;           // its SpMV layout (FP64 values, FP32 copy on demand)
   float *b32 = nullptr;   // FP32 copy of Bm's values
   double *rs = nullptr;   // scaling factors (null when not scaling)
   double *rhs = nullptr, *sol = nullptr, *work = nullptr;
@@ -180,6 +171,7 @@ struct SolveCache {
     dfree(d_map);
     dfree(d_row_of);
     Bm.release();
+    sell.release();
     dfree(b32);
     dfree(rs);
     dfree(rhs);
@@ -324,15 +316,14 @@ int prepare(gdp_solver *S, const gdp_matrix *A, bool reuse_factor)
       hipLaunchKernelGGL(k_gather_scaled, dim3((unsigned)((nnz + BT - 1) / BT)), dim3(BT), 0, s, nnz,
                          c.d_map, c.d_row_of, c.Bm.ja, c.d_rperm, A->csr.a, c.rs, c.Bm.a);
     GDP_HIP(hipGetLastError());
-    dfree(c.b32);
+    if(!c.sell.n) GDP_TRY(sell_build(c.ib, c.Bm, c.sell));
+    else GDP_TRY(sell_copy_values(c.Bm, c.sell, false));
+    dfree(c.sell.val32);
     c.value_version = A->value_version;
     c.scaling = p.Scaling;
     c.have_factor = false;
   }
-  if(p.Precision == 1 && !c.b32) {
-    GDP_TRY(dalloc(&c.b32, (size_t)std::max(1L, nnz)));
-    if(nnz) hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((nnz + BT - 1) / BT)), dim3(BT), 0, s, nnz, c.Bm.a, c.b32);
-  }
+  if(p.Precision == 1 && !c.sell.val32) GDP_TRY(sell_copy_values(c.Bm, c.sell, true));
   // factorisation
   c.precond = prec;
   if(prec == 6) {
@@ -392,10 +383,10 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
   if(p.Precision == 0) {
     DevOp<double> op;
     op.n = n;
-    op.nnz = c.nnz;
-    op.ia = c.Bm.ia;
-    op.ja = c.Bm.ja;
-    op.a = c.Bm.a;
+    op.sp = c.sell.sp;
+    op.w = c.sell.w;
+    op.col = c.sell.col;
+    op.val = c.sell.val;
     PrecOp<double> M;
     if(up && p.Algorithm == 9) {
       M.kind = PREC_USER;
@@ -419,10 +410,10 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
     }
     DevOp<float> op;
     op.n = n;
-    op.nnz = c.nnz;
-    op.ia = c.Bm.ia;
-    op.ja = c.Bm.ja;
-    op.a = c.b32;
+    op.sp = c.sell.sp;
+    op.w = c.sell.w;
+    op.col = c.sell.col;
+    op.val = c.sell.val32;
     PrecOp<float> M;
     if(p.Preconditioner == 6) {
       M.kind = PREC_ILU;
@@ -436,7 +427,7 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
     double rn = bnorm;
     info.res0 = bnorm;
     for(outer = 0; outer <= p.Refinement_Steps; outer++) {
-      GDP_TRY(residual_dev(c.Bm, c.sol, c.rhs, c.work));
+      GDP_TRY(residual_dev(c.sell, c.sol, c.rhs, c.work));
       GDP_TRY(norm2_dev(c.work, n, &rn));
       if(rn <= target || outer == p.Refinement_Steps) break;
       hipLaunchKernelGGL(k_scale_to_f32, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, c.work, 1.0 / rn, c.rhs32);
@@ -447,7 +438,7 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
       hipLaunchKernelGGL(k_add_scaled_f32, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, rn, c.sol32, c.sol);
       GDP_HIP(hipGetLastError());
       if(p.Nb_Iter_Max > 0 && total >= p.Nb_Iter_Max) {
-        GDP_TRY(residual_dev(c.Bm, c.sol, c.rhs, c.work));
+        GDP_TRY(residual_dev(c.sell, c.sol, c.rhs, c.work));
         GDP_TRY(norm2_dev(c.work, n, &rn));
         break;
       }
@@ -461,7 +452,7 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
   double t2 = now_ms();
   info.t_solve_ms = t2 - t1;
   // true residual of the solved (renumbered, scaled) system
-  GDP_TRY(residual_dev(c.Bm, c.sol, c.rhs, c.work));
+  GDP_TRY(residual_dev(c.sell, c.sol, c.rhs, c.work));
   GDP_TRY(norm2_dev(c.work, n, &info.true_res));
   info.rel_res = bnorm > 0 ? info.true_res / bnorm : info.true_res;
   hipLaunchKernelGGL(k_unpermute_sol, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, c.sol, c.rs, c.d_perm, x->d);
