@@ -128,7 +128,7 @@ def main():
     achieved = sb / (spmv_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_spmv (CSR, FP64, 16 lanes/row)", "algorithmic_bytes": sb,
+                "kernel": "k_sell_spmv (SELL-64, FP64, one row per lane)", "algorithmic_bytes": sb,
                 "avg_ms": round(spmv_ms, 5)}
 
     out = {

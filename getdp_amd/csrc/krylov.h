@@ -17,6 +17,7 @@ struct KCtl {
   int maxits, m, nb, pad;
   double rtol, fpar3, fpar4, fpar5, fpar6, fpar7, alpha, beta;
   double scal, nrm0, thr, pend;
+  double rr[2];  // CG: <r,r> of consecutive iterations (parity slots)
   unsigned ticket[8];
   double g[kMaxKrylov + 2], c[kMaxKrylov + 1], s[kMaxKrylov + 1];
   double h1[kMaxKrylov + 2], h2[kMaxKrylov + 2];

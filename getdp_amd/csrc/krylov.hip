@@ -276,6 +276,109 @@ __global__ void __launch_bounds__(B) k_cg_p(int n, const TV *r, TV *p, const KCt
     p[i] = (TV)((double)r[i] + beta * (double)p[i]);
 }
 
+// ---- CG without separate finalize kernels -------------------------------------------------
+// Each block of the next streaming kernel sums the previous kernel's partials
+// itself (same fixed order in every block, hence the same value), takes the
+// scalar decision, and block 0 alone records it.  <r,r> of the previous
+// iteration is read while the new one is written, so it lives in two slots
+// selected by the iteration's parity (rr[2]).
+
+__device__ inline double block_sum_partials(const double *part, int nparts)
+{
+  return sum_partials<double, B>(part, nparts);  // valid in thread 0
+}
+
+// dx += alpha p, r -= alpha Ap with alpha = rr_old / <p, Ap> (iters.f:478-486);
+// partial |r|^2.
+template <typename TV>
+__global__ void __launch_bounds__(B) k_cg_update2(int n, const TV *p, const TV *ap, TV *r, TV *dx,
+                                                 KCtl *C, const double *part_pap, int npap,
+                                                 double *part, int par)
+{
+  __shared__ double s_alpha;
+  __shared__ int s_skip;
+  if(C->done) return;
+  double pap = block_sum_partials(part_pap, npap);
+  if(threadIdx.x == 0) {
+    int bk = 0;  // brkdn, iters.f:3368-3401
+    if(pap > 0.0) bk = (1.0 / pap > 0.0) ? 0 : -9;
+    else if(pap < 0.0) bk = (1.0 / pap < 0.0) ? 0 : -9;
+    else if(pap == 0.0) bk = -3;
+    else bk = -9;
+    s_skip = bk;
+    s_alpha = bk ? 0. : C->rr[par] / pap;
+    if(blockIdx.x == 0) {
+      C->its++;
+      if(bk) {
+        C->status = bk;
+        C->done = 1;
+      }
+      else
+        C->alpha = s_alpha;
+    }
+  }
+  __syncthreads();
+  if(s_skip) return;
+  const double alpha = s_alpha;
+  double s = 0;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B) {
+    dx[i] = (TV)((double)dx[i] + alpha * (double)p[i]);
+    TV v = (TV)((double)r[i] - alpha * (double)ap[i]);
+    r[i] = v;
+    s += (double)v * (double)v;
+  }
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// stopbis on |r| (iters.f:3275-3330), then p = r + beta p.
+template <typename TV>
+__global__ void __launch_bounds__(B) k_cg_p2(int n, const TV *r, TV *p, KCtl *C,
+                                            const double *part_rr, int nrr, int par)
+{
+  __shared__ double s_beta;
+  __shared__ int s_stop;
+  if(C->done) return;
+  double rr = block_sum_partials(part_rr, nrr);
+  if(threadIdx.x == 0) {
+    int stop = 0, status = 0;
+    double fpar5 = 0., beta = 0.;
+    if(C->maxits > 0 && C->its >= C->maxits) {
+      stop = 1;
+      status = -1;
+    }
+    else {
+      fpar5 = sqrt(rr);
+      if(fpar5 > C->fpar4) beta = fpar5 * fpar5 / C->rr[par];
+      else stop = 2;
+    }
+    s_stop = stop;
+    s_beta = beta;
+    if(blockIdx.x == 0) {
+      if(stop == 1) {
+        C->status = status;
+        C->done = 1;
+      }
+      else {
+        C->fpar5 = fpar5;
+        C->fpar6 = fpar5;
+        C->rr[1 - par] = fpar5 * fpar5;
+        C->fpar7 = fpar5 * fpar5;
+        C->beta = beta;
+        if(stop == 2) {
+          C->ipar11 = 1;
+          C->done = 1;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if(s_stop) return;
+  const double beta = s_beta;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B)
+    p[i] = (TV)((double)r[i] + beta * (double)p[i]);
+}
+
 // ---- finalize kernels (one block) -------------------------------------------------------
 
 enum FinOp {
@@ -376,6 +479,7 @@ __global__ void __launch_bounds__(B) k_fin(KCtl *C, const double *part, int npar
   else if(OP == FIN_CG_INIT) {
     C->its = 1;
     C->fpar7 = tot;
+    C->rr[0] = tot;
     C->fpar3 = sqrt(tot);
     C->fpar5 = C->fpar3;
     C->fpar4 = C->rtol * C->fpar3 + 0.0;
@@ -720,6 +824,10 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
     const bool rp = M.kind != PREC_NONE;
     GraphCache gc;
     const int batch = 16;
+    // three kernels per iteration: SpMV + <p,Ap> partials, update + |r|^2
+    // partials (each block derives alpha), p update (each block derives beta)
+    const int ncg = std::min(nb, 256);
+    double *part2 = part + nrow;
     auto body = [&]() -> int {
       for(int it = 0; it < batch; it++) {
         const TV *src = p;
@@ -729,10 +837,9 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
         }
         hipLaunchKernelGGL((k_op_dot<TA, TV, true>), grow, dim3(256), 0, s, n, A.sp, A.w, A.col,
                            A.val, src, p, ap, C, 1, part);
-        fin<FIN_CG_PAP>(C, part, nrow, 0);
-        hipLaunchKernelGGL(k_cg_update<TV>, g, blk, 0, s, n, p, ap, r, dx, C, part);
-        fin<FIN_CG_UPDATE>(C, part, nb, 0);
-        hipLaunchKernelGGL(k_cg_p<TV>, g, blk, 0, s, n, r, p, C);
+        hipLaunchKernelGGL(k_cg_update2<TV>, dim3(ncg), blk, 0, s, n, p, ap, r, dx, C, part, nrow,
+                           part2, it & 1);
+        hipLaunchKernelGGL(k_cg_p2<TV>, dim3(ncg), blk, 0, s, n, r, p, C, part2, ncg, it & 1);
       }
       GDP_HIP(hipGetLastError());
       return GDP_OK;

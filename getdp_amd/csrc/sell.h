@@ -21,8 +21,8 @@ int sell_build(const std::vector<int> &h_ia, const DevCSR &A, DevSell &S);
 int sell_copy_values(const DevCSR &A, DevSell &S, bool also_f32);
 int sell_spmv(const DevSell &S, const double *x, double *y);
 
-// y = A x, one row per lane; UNROLL independent entries in flight per lane.
-template <typename TA, typename TV>
+// y = A x, one row per lane; U independent entries in flight per lane.
+template <typename TA, typename TV, int U = 4>
 __device__ inline void sell_row(int row, const long *__restrict__ sp, const int *__restrict__ w,
                                 const int *__restrict__ col, const TA *__restrict__ val,
                                 const TV *__restrict__ x, double &sum)
@@ -33,20 +33,19 @@ __device__ inline void sell_row(int row, const long *__restrict__ sp, const int 
   const int width = w[s];
   sum = 0.;
   int k = 0;
-  for(; k + 4 <= width; k += 4) {
-    TA v0 = __builtin_nontemporal_load(val + base + (long)(k + 0) * kSlice);
-    TA v1 = __builtin_nontemporal_load(val + base + (long)(k + 1) * kSlice);
-    TA v2 = __builtin_nontemporal_load(val + base + (long)(k + 2) * kSlice);
-    TA v3 = __builtin_nontemporal_load(val + base + (long)(k + 3) * kSlice);
-    int c0 = __builtin_nontemporal_load(col + base + (long)(k + 0) * kSlice);
-    int c1 = __builtin_nontemporal_load(col + base + (long)(k + 1) * kSlice);
-    int c2 = __builtin_nontemporal_load(col + base + (long)(k + 2) * kSlice);
-    int c3 = __builtin_nontemporal_load(col + base + (long)(k + 3) * kSlice);
-    double x0 = (double)x[c0], x1 = (double)x[c1], x2 = (double)x[c2], x3 = (double)x[c3];
-    sum = sum + (double)v0 * x0;
-    sum = sum + (double)v1 * x1;
-    sum = sum + (double)v2 * x2;
-    sum = sum + (double)v3 * x3;
+  for(; k + U <= width; k += U) {
+    TA v[U];
+    int c[U];
+    double xv[U];
+#pragma unroll
+    for(int u = 0; u < U; u++) {
+      v[u] = __builtin_nontemporal_load(val + base + (long)(k + u) * kSlice);
+      c[u] = __builtin_nontemporal_load(col + base + (long)(k + u) * kSlice);
+    }
+#pragma unroll
+    for(int u = 0; u < U; u++) xv[u] = (double)x[c[u]];
+#pragma unroll
+    for(int u = 0; u < U; u++) sum = sum + (double)v[u] * xv[u];
   }
   for(; k < width; k++) {
     TA v = __builtin_nontemporal_load(val + base + (long)k * kSlice);

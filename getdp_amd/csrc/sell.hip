@@ -1,4 +1,6 @@
 // SELL-64 construction and the plain FP64 SpMV entry (see sell.h).
+#include <cstdlib>
+
 #include "sell.h"
 
 namespace gdp {
@@ -36,6 +38,7 @@ __global__ void __launch_bounds__(256) k_sell_fill(int n, const int *__restrict_
   }
 }
 
+template <int U>
 __global__ void __launch_bounds__(256) k_sell_spmv(int n, const long *__restrict__ sp,
                                                    const int *__restrict__ w,
                                                    const int *__restrict__ col,
@@ -46,7 +49,7 @@ __global__ void __launch_bounds__(256) k_sell_spmv(int n, const long *__restrict
   int row = blockIdx.x * 256 + threadIdx.x;
   if(row >= n) return;
   double sum;
-  sell_row<double, double>(row, sp, w, col, val, x, sum);
+  sell_row<double, double, U>(row, sp, w, col, val, x, sum);
   y[row] = sum;
 }
 
@@ -101,8 +104,18 @@ int sell_copy_values(const DevCSR &A, DevSell &S, bool also_f32)
 int sell_spmv(const DevSell &S, const double *x, double *y)
 {
   if(!S.n) return GDP_OK;
-  hipLaunchKernelGGL(k_sell_spmv, dim3((S.n + 255) / 256), dim3(256), 0, ctx().stream, S.n, S.sp,
-                     S.w, S.col, S.val, x, y);
+  // GDP_SELL_UNROLL=8 selects the 8-deep variant (launch-shape experiments)
+  static int unroll = -1;
+  if(unroll < 0) {
+    const char *e = std::getenv("GDP_SELL_UNROLL");
+    unroll = (e && std::atoi(e) == 8) ? 8 : 4;
+  }
+  if(unroll == 8)
+    hipLaunchKernelGGL(k_sell_spmv<8>, dim3((S.n + 255) / 256), dim3(256), 0, ctx().stream, S.n,
+                       S.sp, S.w, S.col, S.val, x, y);
+  else
+    hipLaunchKernelGGL(k_sell_spmv<4>, dim3((S.n + 255) / 256), dim3(256), 0, ctx().stream, S.n,
+                       S.sp, S.w, S.col, S.val, x, y);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
