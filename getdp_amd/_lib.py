@@ -18,6 +18,7 @@ _pp = C.POINTER(C.c_void_p)
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
 _lp = C.POINTER(C.c_long)
+_fp = C.POINTER(C.c_float)
 
 
 class MeshDesc(C.Structure):
@@ -91,6 +92,18 @@ SIGNATURES = [
     ("gdp_fe_last_timing", C.c_int, [_vp, _dp, _dp, _dp]),
     ("gdp_number_edges", C.c_int, [C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip, C.c_int]),
     ("gdp_rcmk", C.c_int, [C.c_int, _ip, _ip, _ip, _ip]),
+    ("gdp_unet_create", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint, _pp]),
+    ("gdp_unet_destroy", C.c_int, [_vp]),
+    ("gdp_unet_num_params", C.c_int, [_vp, _lp]),
+    ("gdp_unet_get_params", C.c_int, [_vp, _fp]),
+    ("gdp_unet_set_params", C.c_int, [_vp, _fp]),
+    ("gdp_unet_get_grads", C.c_int, [_vp, _fp]),
+    ("gdp_unet_forward", C.c_int, [_vp, C.c_int, _fp, _fp, _fp]),
+    ("gdp_unet_train_step", C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _fp, C.c_float, _fp]),
+    ("gdp_unet_get_activation", C.c_int, [_vp, C.c_int, C.c_int, _fp, _lp]),
+    ("gdp_unet_attach", C.c_int, [_vp, _vp, C.c_int, _ip, _fp, _dp]),
+    ("gdp_unet_detach", C.c_int, [_vp]),
+    ("gdp_unet_set_mode", C.c_int, [_vp, C.c_int]),
 ]
 
 _LIB = None
@@ -138,3 +151,8 @@ def iptr(a: np.ndarray):
 def dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous, (a.dtype, a.flags)
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_fp)

@@ -39,6 +39,10 @@ struct UserPrec;  // learned preconditioner (unet.hip)
 // z = M^{-1} v for the learned preconditioner; skipped when the control block
 // says so (gmode as for the other kernels of a cycle).
 int user_prec_apply(UserPrec *u, const double *in, double *out, KCtl *C, int gmode);
+int user_prec_apply(UserPrec *u, const float *in, float *out, KCtl *C, int gmode);
+// the solver's renumbering (original -> internal, internal -> original) and
+// diagonal scaling (by original index, may be null) for the next applications
+void user_prec_bind(UserPrec *u, const int *perm, const int *rperm, const double *rs);
 
 template <typename TV> struct PrecOp {
   PrecKind kind = PREC_NONE;

@@ -646,6 +646,22 @@ __global__ void __launch_bounds__(B) k_copy_gated(int n, const TV *src, TV *dst,
   for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) dst[j] = src[j];
 }
 
+// t = v - t (gated): residual of a first preconditioning stage
+template <typename TV>
+__global__ void __launch_bounds__(B) k_rsub_gated(int n, const TV *v, TV *t, const KCtl *C, int gmode)
+{
+  if(gated(C, gmode)) return;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) t[j] = v[j] - t[j];
+}
+
+// z += u (gated)
+template <typename TV>
+__global__ void __launch_bounds__(B) k_add_gated(int n, TV *z, const TV *u, const KCtl *C, int gmode)
+{
+  if(gated(C, gmode)) return;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) z[j] = z[j] + u[j];
+}
+
 // label 60/70: restart or terminate.
 __global__ void k_gm_cycle_end(KCtl *C)
 {
@@ -713,12 +729,25 @@ template <typename TV>
 int prec_apply(const PrecOp<TV> &M, const TV *in, TV *out, KCtl *C, int gmode, int n)
 {
   if(M.kind == PREC_ILU) return M.ilu->template apply<TV>(in, (TV)1, out, (const int *)C, gmode);
-  if(M.kind == PREC_USER) {
-    if constexpr(sizeof(TV) == 8) return user_prec_apply(M.user, in, out, C, gmode);
-    else return set_error(GDP_E_UNSUPPORTED, "learned preconditioner needs FP64 vectors");
-  }
+  if(M.kind == PREC_USER) return user_prec_apply(M.user, in, out, C, gmode);
   dim3 g(reduce_blocks(n));
   hipLaunchKernelGGL(k_copy_gated<TV>, g, dim3(B), 0, ctx().stream, n, in, out, C, gmode);
+  return GDP_OK;
+}
+
+// Two-stage flexible preconditioner: the learned map, then an ILU(0)
+// correction of the residual it leaves, z = N(v) + (LU)^{-1} (v - A N(v)).
+template <typename TA, typename TV>
+int prec_apply_two_stage(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *in, TV *out, TV *t,
+                         TV *u, KCtl *C, int gmode, int n)
+{
+  GDP_TRY(user_prec_apply(M.user, in, out, C, gmode));
+  GDP_TRY(op_apply(A, out, t, C, gmode));
+  dim3 g(reduce_blocks(n));
+  hipLaunchKernelGGL(k_rsub_gated<TV>, g, dim3(B), 0, ctx().stream, n, in, t, C, gmode);
+  GDP_TRY(M.ilu->template apply<TV>(t, (TV)1, u, (const int *)C, gmode));
+  hipLaunchKernelGGL(k_add_gated<TV>, g, dim3(B), 0, ctx().stream, n, out, u, C, gmode);
+  GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
 
@@ -795,7 +824,8 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
   const bool cg = opt.algorithm == 1;
   if(!cg && opt.algorithm != 8 && !flexible)
     return set_error(GDP_E_UNSUPPORTED, "Unknown algorithm for sparse matrix solver (%d)", opt.algorithm);
-  size_t nvec = cg ? 6 : (size_t)(m + 2) + (flexible ? m : 1);
+  const bool two_stage = flexible && M.kind == PREC_USER && M.ilu != nullptr;
+  size_t nvec = cg ? 6 : (size_t)(m + 2) + (flexible ? m : 1) + (two_stage ? 1 : 0);
   GDP_TRY(ensure_ws(nvec * (size_t)std::max(1, n) * sizeof(TV) + 256, n));
   Workspace &W = ws();
   TV *base = (TV *)W.vec;
@@ -890,7 +920,13 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
     for(int k = 1; k <= m; k++) {
       TV *vk1 = V + (size_t)(k - 1) * n, *vk = V + (size_t)k * n;
       const TV *src = vk1;
-      if(flexible) {
+      if(two_stage) {
+        TV *zk = Z + (size_t)(k - 1) * n;
+        TV *t1 = Z + (size_t)m * n, *t2 = t1 + n;
+        GDP_TRY((prec_apply_two_stage<TA, TV>(A, M, vk1, zk, t1, t2, C, 2, n)));
+        src = zk;
+      }
+      else if(flexible) {
         TV *zk = Z + (size_t)(k - 1) * n;
         GDP_TRY(prec_apply(M, vk1, zk, C, 2, n));
         src = zk;

@@ -380,6 +380,7 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
     return set_error(GDP_E_UNSUPPORTED, "only right preconditioning (Preconditioner_Position 2)");
   KrylovResult kr;
   UserPrec *up = S->user;
+  if(up) user_prec_bind(up, c.d_perm, c.d_rperm, c.rs);
   if(p.Precision == 0) {
     DevOp<double> op;
     op.n = n;
@@ -391,6 +392,7 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
     if(up && p.Algorithm == 9) {
       M.kind = PREC_USER;
       M.user = up;
+      if(p.Preconditioner == 6) M.ilu = &c.ilu;  // network, then ILU(0) correction
     }
     else if(p.Preconditioner == 6) {
       M.kind = PREC_ILU;
@@ -415,7 +417,12 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
     op.col = c.sell.col;
     op.val = c.sell.val32;
     PrecOp<float> M;
-    if(p.Preconditioner == 6) {
+    if(up && p.Algorithm == 9) {
+      M.kind = PREC_USER;
+      M.user = up;
+      if(p.Preconditioner == 6) M.ilu = &c.ilu;
+    }
+    else if(p.Preconditioner == 6) {
       M.kind = PREC_ILU;
       M.ilu = &c.ilu;
     }

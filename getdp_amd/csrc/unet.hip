@@ -1,20 +1,1287 @@
-// Learned preconditioner (placeholder until the U-Net kernels land).
+// Learned preconditioner: a 4-level convolutional U-Net on the nodal grid,
+// run and trained with BF16 MFMA kernels (FP32 accumulation) on gfx950.
+//
+// This component has no counterpart in the reference; it is used as the
+// (nonlinear) right preconditioner of flexible GMRES (iters.f:2522-2807
+// restated in krylov.hip), with ILU(0)-GMRES as the fallback.
+//
+// Network (channels 16-32-64-128, 3x3 convolutions, ReLU):
+//   E0: conv(in 2 -> 16), conv(16 -> 16)                          -> s0  (H)
+//   E1: conv(pool s0 -> 32), conv(32 -> 32)                        -> s1  (H/2)
+//   E2: conv(pool s1 -> 64), conv(64 -> 64)                        -> s2  (H/4)
+//   E3: conv(pool s2 -> 128), conv(128 -> 128)                     -> s3  (H/8)
+//   D2: conv(up s3 ++ s2 -> 64), conv(64 -> 64)                    -> d2
+//   D1: conv(up d2 ++ s1 -> 32), conv(32 -> 32)                    -> d1
+//   D0: conv(up d1 ++ s0 -> 16), conv(16 -> 16)                    -> d0
+//   out: 1x1 conv(16 -> 1), no activation                          -> z
+// pool = 2x2 average, up = nearest neighbour, ++ = channel concatenation;
+// both are fused into the operand loaders of the next convolution.
+//
+// Layout: activations NHWC in BF16 with channels padded to a multiple of 8,
+// so that one lane loads 8 consecutive channels (16 bytes) of one pixel.
+// A 3x3 convolution is an implicit GEMM: rows = pixels, columns = output
+// channels, reduction index k = tap*Cin_pad + ci (padded to a multiple of
+// 32).  MFMA 16x16x32 BF16 tiles: operand lane l holds row (l % 16) and
+// reduction entries 8*(l/16)..+7; the accumulator lane l holds column l % 16,
+// rows 4*(l/16)..+3.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+
 #include "krylov.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
 
 namespace gdp {
 
-struct UserPrec {
-  int n = 0;
+namespace {
+
+enum LoadMode { M_PLAIN = 0, M_POOL = 1, M_UPCAT = 2 };
+
+__device__ inline float bf2f(u16 b)
+{
+  unsigned int u = ((unsigned int)b) << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+__device__ __host__ inline u16 f2bf(float f)
+{
+  unsigned int u;
+  __builtin_memcpy(&u, &f, 4);
+  if((u & 0x7f800000u) == 0x7f800000u) return (u16)(u >> 16);  // inf/nan
+  u += 0x7fffu + ((u >> 16) & 1u);                            // round to nearest even
+  return (u16)(u >> 16);
+}
+
+struct ConvArgs {
+  int B, H, W;        // output (= logical input) spatial size
+  int cin_pad, cout;  // logical input channels (padded), output channels
+  int K_pad;          // reduction length, multiple of 32
+  int c1;             // UPCAT: channels of the coarse tensor
+  const u16 *in, *in2;
+  const u16 *wt;      // [cout][K_pad] (forward) or [cin][K2_pad] (adjoint)
+  const float *bias;  // may be null
+  u16 *out_bf;        // BF16 output with bias + ReLU, or
+  float *out_f;       // FP32 output (plain)
+  int relu;
+  int flip;           // adjoint: taps mirrored (tap -> 8 - tap)
+  const int *gate;
+  int gmode;
+  int kchunk;         // split-K: reduction entries per blockIdx.z (multiple of 32)
+  float *split;       // split-K partials [gridDim.z][P][cout] (null: fused epilogue)
 };
+
+__device__ inline bool unet_gated(const int *gate, int gmode)
+{
+  if(!gmode || !gate) return false;
+  if(gmode == 1) return gate[0] != 0;
+  return (gate[0] | gate[3]) != 0;
+}
+
+// 8 consecutive logical input channels ci0.. at pixel (b, y, x) of the
+// logical input of a convolution (zeros outside the image).
+template <int MODE>
+__device__ inline uint4 load8(const ConvArgs &a, int b, int y, int x, int ci0)
+{
+  uint4 z = {0u, 0u, 0u, 0u};
+  if(y < 0 || x < 0 || y >= a.H || x >= a.W) return z;
+  if(MODE == M_PLAIN) {
+    const u16 *p = a.in + (((long)b * a.H + y) * a.W + x) * a.cin_pad + ci0;
+    return *(const uint4 *)p;
+  }
+  else if(MODE == M_POOL) {
+    const int W2 = 2 * a.W, H2 = 2 * a.H;
+    const u16 *p = a.in + (((long)b * H2 + 2 * y) * W2 + 2 * x) * a.cin_pad + ci0;
+    uint4 q[4];
+    q[0] = *(const uint4 *)p;
+    q[1] = *(const uint4 *)(p + a.cin_pad);
+    q[2] = *(const uint4 *)(p + (long)W2 * a.cin_pad);
+    q[3] = *(const uint4 *)(p + (long)W2 * a.cin_pad + a.cin_pad);
+    const u16 *h[4] = {(const u16 *)&q[0], (const u16 *)&q[1], (const u16 *)&q[2], (const u16 *)&q[3]};
+    uint4 r;
+    u16 *o = (u16 *)&r;
+#pragma unroll
+    for(int c = 0; c < 8; c++) o[c] = f2bf(0.25f * (bf2f(h[0][c]) + bf2f(h[1][c]) + bf2f(h[2][c]) + bf2f(h[3][c])));
+    return r;
+  }
+  else {
+    if(ci0 < a.c1) {
+      const int Wc = a.W >> 1, Hc = a.H >> 1;
+      const u16 *p = a.in + (((long)b * Hc + (y >> 1)) * Wc + (x >> 1)) * a.c1 + ci0;
+      return *(const uint4 *)p;
+    }
+    const int c2 = a.cin_pad - a.c1;
+    const u16 *p = a.in2 + (((long)b * a.H + y) * a.W + x) * c2 + (ci0 - a.c1);
+    return *(const uint4 *)p;
+  }
+}
+
+// one logical input element (scalar form of load8)
+template <int MODE>
+__device__ inline float load1(const ConvArgs &a, int b, int y, int x, int ci)
+{
+  if(y < 0 || x < 0 || y >= a.H || x >= a.W) return 0.f;
+  if(MODE == M_PLAIN) return bf2f(a.in[(((long)b * a.H + y) * a.W + x) * a.cin_pad + ci]);
+  if(MODE == M_POOL) {
+    const int W2 = 2 * a.W, H2 = 2 * a.H;
+    const u16 *p = a.in + (((long)b * H2 + 2 * y) * W2 + 2 * x) * a.cin_pad + ci;
+    return bf2f(f2bf(0.25f * (bf2f(p[0]) + bf2f(p[a.cin_pad]) + bf2f(p[(long)W2 * a.cin_pad]) +
+                              bf2f(p[(long)W2 * a.cin_pad + a.cin_pad]))));
+  }
+  if(ci < a.c1) {
+    const int Wc = a.W >> 1, Hc = a.H >> 1;
+    return bf2f(a.in[(((long)b * Hc + (y >> 1)) * Wc + (x >> 1)) * a.c1 + ci]);
+  }
+  const int c2 = a.cin_pad - a.c1;
+  return bf2f(a.in2[(((long)b * a.H + y) * a.W + x) * c2 + (ci - a.c1)]);
+}
+
+__device__ inline bf16x8 as_bf8(uint4 v)
+{
+  bf16x8 r;
+  __builtin_memcpy(&r, &v, 16);
+  return r;
+}
+
+// Implicit-GEMM convolution: a wave computes 16*TM pixels x 16*TN channels
+// over the reduction range of its blockIdx.z; a block of 4 waves takes 4
+// consecutive pixel tiles.  Without split-K the epilogue adds the bias,
+// applies ReLU and stores; with split-K it stores FP32 partials.
+template <int MODE, int TM, int TN>
+__global__ void __launch_bounds__(256) k_conv(ConvArgs a)
+{
+  if(unet_gated(a.gate, a.gmode)) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long P = (long)a.B * a.H * a.W;
+  const long pbase = ((long)blockIdx.x * 4 + wave) * (TM * 16);
+  const int cobase = blockIdx.y * 16 * TN;
+  const int r16 = lane & 15, kq = lane >> 4;
+  int pb[TM], py[TM], px[TM];
+  bool pok[TM];
+#pragma unroll
+  for(int t = 0; t < TM; t++) {
+    long p = pbase + t * 16 + r16;
+    pok[t] = p < P;
+    long q = pok[t] ? p : 0;
+    px[t] = (int)(q % a.W);
+    long q2 = q / a.W;
+    py[t] = (int)(q2 % a.H);
+    pb[t] = (int)(q2 / a.H);
+  }
+  f32x4 acc[TM][TN];
+#pragma unroll
+  for(int t = 0; t < TM; t++)
+#pragma unroll
+    for(int n = 0; n < TN; n++) acc[t][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const uint4 zero4 = {0u, 0u, 0u, 0u};
+  const int kbeg = blockIdx.z * a.kchunk;
+  const int kend = min(a.K_pad, kbeg + a.kchunk);
+  for(int kk = kbeg; kk < kend; kk += 32) {
+    const int k0 = kk + 8 * kq;
+    int tap = k0 / a.cin_pad;
+    const int ci0 = k0 - tap * a.cin_pad;
+    const bool kvalid = tap < 9;
+    if(a.flip) tap = 8 - tap;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    bf16x8 av[TM], bv[TN];
+#pragma unroll
+    for(int t = 0; t < TM; t++)
+      av[t] = as_bf8((kvalid && pok[t]) ? load8<MODE>(a, pb[t], py[t] + dy, px[t] + dx, ci0) : zero4);
+#pragma unroll
+    for(int n = 0; n < TN; n++) {
+      const int co = cobase + n * 16 + r16;
+      bv[n] = as_bf8(co < a.cout ? *(const uint4 *)(a.wt + (long)co * a.K_pad + k0) : zero4);
+    }
+#pragma unroll
+    for(int t = 0; t < TM; t++)
+#pragma unroll
+      for(int n = 0; n < TN; n++)
+        acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv[n], acc[t][n], 0, 0, 0);
+  }
+  // epilogue: lane holds column (l % 16), rows 4*(l/16)..+3
+#pragma unroll
+  for(int t = 0; t < TM; t++)
+#pragma unroll
+    for(int n = 0; n < TN; n++) {
+      const int co = cobase + n * 16 + r16;
+      if(co >= a.cout) continue;
+      if(a.split) {
+        float *dst = a.split + (long)blockIdx.z * P * a.cout;
+#pragma unroll
+        for(int i = 0; i < 4; i++) {
+          const long p = pbase + t * 16 + 4 * kq + i;
+          if(p < P) dst[p * a.cout + co] = acc[t][n][i];
+        }
+        continue;
+      }
+      const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+      for(int i = 0; i < 4; i++) {
+        const long p = pbase + t * 16 + 4 * kq + i;
+        if(p >= P) continue;
+        float v = acc[t][n][i] + bias;
+        if(a.relu) v = v > 0.f ? v : 0.f;
+        if(a.out_bf) a.out_bf[p * a.cout + co] = f2bf(v);
+        else a.out_f[p * a.cout + co] = v;
+      }
+    }
+}
+
+// split-K epilogue: sum the partials in a fixed order, bias, ReLU, store
+__global__ void __launch_bounds__(256) k_conv_split_sum(ConvArgs a, int nsplit)
+{
+  if(unet_gated(a.gate, a.gmode)) return;
+  const long P = (long)a.B * a.H * a.W;
+  const long n = P * a.cout;
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if(i >= n) return;
+  float v = 0.f;
+  for(int z = 0; z < nsplit; z++) v += a.split[(long)z * n + i];
+  if(a.bias) v += a.bias[i % a.cout];
+  if(a.relu) v = v > 0.f ? v : 0.f;
+  if(a.out_bf) a.out_bf[i] = f2bf(v);
+  else a.out_f[i] = v;
+}
+
+// Weight gradient: dW[co][k] = sum_p G[p][co] * Xcol[p][k] (k in the forward
+// weight layout), split over pixel chunks; rows = co, columns = k, the MFMA
+// reduction runs over pixels.  Partial results per chunk.
+struct WgradArgs {
+  ConvArgs x;          // logical input (MODE loader) and sizes
+  const u16 *g;        // [P][cout] BF16 output gradient (ReLU-masked)
+  float *part;         // [nchunks][cout][K_pad]
+  int chunk;           // pixels per chunk (multiple of 32)
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_wgrad(WgradArgs w)
+{
+  const ConvArgs &a = w.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, kq = lane >> 4;
+  const long P = (long)a.B * a.H * a.W;
+  const long p_begin = (long)blockIdx.x * w.chunk;
+  const long p_end = p_begin + w.chunk < P ? p_begin + w.chunk : P;
+  const int co = blockIdx.y * 16 + r16;     // A row
+  const int kcol0 = (blockIdx.z * 4 + wave) * 16;
+  const int kcol = kcol0 + r16;              // B column
+  // decode the column's tap / channel once
+  int tap = kcol / a.cin_pad;
+  const int ci = kcol - tap * a.cin_pad;
+  const bool kvalid = tap < 9 && kcol < a.K_pad;
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for(long p0 = p_begin; p0 < p_end; p0 += 32) {
+    bf16x8 av, bv;
+    u16 *ah = (u16 *)&av, *bh = (u16 *)&bv;
+#pragma unroll
+    for(int j = 0; j < 8; j++) {
+      const long p = p0 + 8 * kq + j;
+      const bool ok = p < p_end;
+      ah[j] = (ok && co < a.cout) ? w.g[p * a.cout + co] : (u16)0;
+      float xv = 0.f;
+      if(ok && kvalid) {
+        const int x = (int)(p % a.W);
+        const long q = p / a.W;
+        const int y = (int)(q % a.H), b = (int)(q / a.H);
+        xv = load1<MODE>(a, b, y + dy, x + dx, ci);
+      }
+      bh[j] = f2bf(xv);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+  }
+  // lane holds column kcol0 + r16, rows (co) blockIdx.y*16 + 4*kq + i
+#pragma unroll
+  for(int i = 0; i < 4; i++) {
+    const int orow = blockIdx.y * 16 + 4 * kq + i;
+    const int ocol = kcol0 + r16;
+    if(orow < a.cout && ocol < a.K_pad)
+      w.part[((long)blockIdx.x * a.cout + orow) * a.K_pad + ocol] = acc[i];
+  }
+}
+
+__global__ void k_sum_chunks(int nchunks, long n, const float *part, float *out)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if(i >= n) return;
+  float s = 0.f;
+  for(int c = 0; c < nchunks; c++) s += part[(long)c * n + i];
+  out[i] = s;
+}
+
+// G = BF16(dY * (Y > 0)), with Y the layer's (ReLU) output
+__global__ void k_relu_grad(long n, const float *dy, const u16 *y, u16 *g)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if(i < n) g[i] = f2bf(bf2f(y[i]) > 0.f ? dy[i] : 0.f);
+}
+
+// bias gradient: db[c] = sum_p G[p][c]  (one block per channel)
+__global__ void k_bias_grad(long P, int C, const u16 *g, float *db)
+{
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for(long p = threadIdx.x; p < P; p += 256) s += bf2f(g[p * C + c]);
+  s = block_sum<float, 256>(s);
+  if(threadIdx.x == 0) db[c] = s;
+}
+
+// adjoints of the fused loaders, accumulating into FP32 gradient tensors
+__global__ void k_add(long n, const float *src, float *dst)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if(i < n) dst[i] += src[i];
+}
+
+// pool^T: dFine[2y+a][2x+b][c] += dX[y][x][c] / 4
+__global__ void k_pool_adj(int B, int H, int W, int C, const float *dx, float *dfine)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  long n = (long)B * 2 * H * 2 * W * C;
+  if(i >= n) return;
+  int c = (int)(i % C);
+  long q = i / C;
+  int xf = (int)(q % (2 * W));
+  q /= 2 * W;
+  int yf = (int)(q % (2 * H));
+  int b = (int)(q / (2 * H));
+  dfine[i] += 0.25f * dx[(((long)b * H + (yf >> 1)) * W + (xf >> 1)) * C + c];
+}
+
+// up^T (++ split): dCoarse[yc][xc][c] += sum of the 4 fine dX[.][.][c];
+// dSkip[y][x][c] += dX[y][x][c1 + c]
+__global__ void k_upcat_adj(int B, int H, int W, int c1, int c2, const float *dx, float *dcoarse,
+                            float *dskip)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  const int C = c1 + c2;
+  long ncoarse = (long)B * (H / 2) * (W / 2) * c1;
+  long nskip = (long)B * H * W * c2;
+  if(i < ncoarse) {
+    int c = (int)(i % c1);
+    long q = i / c1;
+    int xc = (int)(q % (W / 2));
+    q /= W / 2;
+    int yc = (int)(q % (H / 2));
+    int b = (int)(q / (H / 2));
+    float s = 0.f;
+    for(int a2 = 0; a2 < 2; a2++)
+      for(int b2 = 0; b2 < 2; b2++)
+        s += dx[(((long)b * H + 2 * yc + a2) * W + 2 * xc + b2) * C + c];
+    dcoarse[i] += s;
+  }
+  else if(i < ncoarse + nskip) {
+    long j = i - ncoarse;
+    int c = (int)(j % c2);
+    long pix = j / c2;
+    dskip[j] += dx[pix * C + c1 + c];
+  }
+}
+
+// 1x1 output convolution (16 -> 1), FP32
+__global__ void k_out_fwd(long P, int C, const u16 *x, const float *w, const float *b, float *z,
+                          const int *gate, int gmode)
+{
+  if(unet_gated(gate, gmode)) return;
+  long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if(p >= P) return;
+  float s = b[0];
+  for(int c = 0; c < C; c++) s += bf2f(x[p * C + c]) * w[c];
+  z[p] = s;
+}
+
+__global__ void k_out_bwd_x(long P, int C, const float *dz, const float *w, float *dx)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if(i >= P * C) return;
+  dx[i] += dz[i / C] * w[i % C];
+}
+
+// gw[c] = sum_p dz[p] x[p][c] (c < C), gb = sum_p dz[p] (c == C); one block each
+__global__ void k_out_bwd_w(long P, int C, const float *dz, const u16 *x, float *gw, float *gb)
+{
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for(long p = threadIdx.x; p < P; p += 256) s += dz[p] * (c < C ? bf2f(x[p * C + c]) : 1.f);
+  s = block_sum<float, 256>(s);
+  if(threadIdx.x == 0) {
+    if(c < C) gw[c] = s;
+    else gb[0] = s;
+  }
+}
+
+// scale-invariant loss per sample: L = | z/|z| - t |^2 over the mask, t unit
+// norm; dz = -(2/|z|) (t - (zhat.t) zhat).  One block per sample.
+__global__ void k_loss(int HW, const float *z, const float *t, const float *mask, float *dz,
+                       float *loss)
+{
+  const int s = blockIdx.x;
+  const float *zs = z + (long)s * HW, *ts = t + (long)s * HW, *ms = mask;
+  float zz = 0.f, zt = 0.f;
+  for(int i = threadIdx.x; i < HW; i += 256) {
+    float m = ms[i], zv = zs[i] * m;
+    zz += zv * zv;
+    zt += zv * ts[i];
+  }
+  __shared__ float sh[2];
+  float a = block_sum<float, 256>(zz);
+  if(threadIdx.x == 0) sh[0] = a;
+  __syncthreads();
+  float b = block_sum<float, 256>(zt);
+  if(threadIdx.x == 0) sh[1] = b;
+  __syncthreads();
+  const float nz = sqrtf(fmaxf(sh[0], 1e-30f));
+  const float c = sh[1] / nz;  // zhat . t
+  if(threadIdx.x == 0) loss[s] = 2.f - 2.f * c;
+  for(int i = threadIdx.x; i < HW; i += 256) {
+    float m = ms[i];
+    float zh = zs[i] * m / nz;
+    dz[(long)s * HW + i] = -(2.f / nz) * (ts[i] - c * zh) * m;
+  }
+}
+
+// Adam (FP32 master weights)
+__global__ void k_adam(long n, float *w, const float *g, float *m, float *v, float lr, float b1,
+                       float b2, float eps, float bc1, float bc2, float wd)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if(i >= n) return;
+  float gi = g[i] + wd * w[i];
+  float mi = b1 * m[i] + (1.f - b1) * gi;
+  float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// BF16 forward weights and the tap-mirrored, transposed adjoint weights
+__global__ void k_prep_weights(int cout, int cin_pad, int K_pad, int K2_pad, const float *w32,
+                               u16 *wt, u16 *wtT)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  long n = (long)cout * K_pad;
+  if(i < n) wt[i] = f2bf(w32[i]);
+  long n2 = (long)cin_pad * K2_pad;
+  if(i < n2) {
+    int ci = (int)(i / K2_pad), k2 = (int)(i % K2_pad);
+    int tap2 = k2 / cout, co = k2 - tap2 * cout;
+    float v = 0.f;
+    if(tap2 < 9) v = w32[(long)co * K_pad + (8 - tap2) * cin_pad + ci];
+    wtT[i] = f2bf(v);
+  }
+}
+
+// network input: ch0 = residual (normalised), ch1 = coefficient feature, rest 0
+__global__ void k_make_input(long P, const float *r, const float *feat, u16 *x)
+{
+  long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if(p >= P) return;
+  uint4 v = {0u, 0u, 0u, 0u};
+  u16 *h = (u16 *)&v;
+  h[0] = f2bf(r[p]);
+  h[1] = f2bf(feat[p]);
+  *(uint4 *)(x + p * 8) = v;
+}
+
+// ---- preconditioner glue (solver vectors <-> grid) ----
+// The solver works on the renumbered, diagonally scaled system
+//   Ah = D P A P^T D,  rh = D P r,  x = P^T D xh      (D = rs, P = perm)
+// so Ah^{-1} rh = D^{-1} P A^{-1} r: unscale/unpermute the input, run the
+// network on the original grid, then permute/unscale the output.
+// perm: original -> internal index, rperm: internal -> original, rs indexed
+// by original DOF (null: no scaling).
+template <typename TV>
+__global__ void __launch_bounds__(256) k_pc_unscale(int npix, const int *pix2dof, const int *perm,
+                                                    const double *rs, const double *dscale,
+                                                    const TV *in, float *r32, double *part,
+                                                    const int *gate, int gmode)
+{
+  if(unet_gated(gate, gmode)) return;
+  double s = 0.;
+  for(long p = (long)blockIdx.x * 256 + threadIdx.x; p < npix; p += (long)gridDim.x * 256) {
+    const int d = pix2dof[p];
+    double v = 0.;
+    if(d >= 0) {
+      v = (double)in[perm ? perm[d] : d];
+      if(rs) v /= rs[d];
+      if(dscale) v *= dscale[d];  // network space: D^{-1/2} r
+    }
+    r32[p] = (float)v;
+    s += v * v;
+  }
+  s = block_sum<double, 256>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// network input: ch0 = r * sqrt(ndof)/|r| (unit RMS over the DOFs), ch1 = feature
+__global__ void __launch_bounds__(256) k_pc_gather(int npix, int ndof, const float *r32,
+                                                   const double *part, int nparts, const u16 *feat,
+                                                   u16 *x, int odd, const int *gate, int gmode)
+{
+  if(unet_gated(gate, gmode)) return;
+  __shared__ double s_inv;
+  double t = sum_partials<double, 256>(part, nparts);
+  if(threadIdx.x == 0) s_inv = t > 0. ? sqrt((double)ndof / t) : 0.;
+  __syncthreads();
+  const float inv = (float)s_inv;
+  for(long p = (long)blockIdx.x * 256 + threadIdx.x; p < npix; p += (long)gridDim.x * 256) {
+    uint4 o = {0u, 0u, 0u, 0u};
+    u16 *h = (u16 *)&o;
+    h[0] = f2bf(r32[p] * inv);
+    h[1] = feat[p];
+    *(uint4 *)(x + p * 8) = o;
+    if(odd) {  // second sample: the negated residual
+      h[0] ^= 0x8000;
+      *(uint4 *)(x + (npix + p) * 8) = o;
+    }
+  }
+}
+
+template <typename TV>
+__global__ void __launch_bounds__(256) k_pc_scatter(int n, int ndof, const int *rperm,
+                                                    const int *dof2pix, const double *rs,
+                                                    const double *dscale,
+                                                    const float *z, const double *part, int nparts,
+                                                    int npix, int odd, TV *out, const int *gate,
+                                                    int gmode)
+{
+  if(unet_gated(gate, gmode)) return;
+  __shared__ double s_sc;
+  double t = sum_partials<double, 256>(part, nparts);
+  if(threadIdx.x == 0) s_sc = sqrt(t / (double)ndof);
+  __syncthreads();
+  const double sc = s_sc;
+  for(long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const int d = rperm ? rperm[i] : (int)i;
+    const int px = dof2pix[d];
+    // odd part N(r) - N(-r) removes what does not depend on the residual
+    double v = sc * (odd ? 0.5 * ((double)z[px] - (double)z[npix + px]) : (double)z[px]);
+    if(dscale) v *= dscale[d];  // back from D^{1/2} e
+    if(rs) v /= rs[d];
+    out[i] = (TV)v;
+  }
+}
+
+long rup(long a, long b) { return (a + b - 1) / b * b; }
+
+}  // namespace
+
+// =============================================================================
+// Network object
+// =============================================================================
+struct ConvLayer {
+  int mode = M_PLAIN, cin = 0, cin_pad = 0, c1 = 0, cout = 0, K_pad = 0, K2_pad = 0;
+  int in_tensor = -1, in2_tensor = -1, level = 0;  // activation indices; level: spatial /2^level
+  float *w32 = nullptr, *b32 = nullptr, *gw = nullptr, *gb = nullptr;
+  float *mw = nullptr, *vw = nullptr, *mb = nullptr, *vb = nullptr;
+  u16 *wt = nullptr, *wtT = nullptr;
+};
+
+struct UserPrec {
+  int H = 0, W = 0, Bmax = 0;  // grid and batch capacity
+  std::vector<ConvLayer> L;    // 14 convolutions
+  float *ow = nullptr, *ob = nullptr, *gow = nullptr, *gob = nullptr;  // 1x1 output (16 + 1)
+  float *mow = nullptr, *vow = nullptr, *mob = nullptr, *vob = nullptr;
+  // activations (BF16) and their gradients (FP32), one per layer output + input
+  std::vector<u16 *> act;    // [0] = input X0 (8 ch), [i+1] = output of conv i
+  std::vector<float *> gact; // gradients of act
+  std::vector<int> act_ch, act_lvl;
+  float *z = nullptr, *dz = nullptr, *target = nullptr, *mask = nullptr, *loss = nullptr;
+  float *dx = nullptr;  // scratch for the adjoint of one convolution's input
+  u16 *g = nullptr;     // ReLU-masked gradient (BF16) of one layer
+  float *part = nullptr;
+  size_t part_size = 0;
+  float *split = nullptr;  // split-K partials of the convolutions
+  size_t split_size = 0;
+  int adam_t = 0;
+  // preconditioner binding
+  int n_dof = 0;
+  int *pix2dof = nullptr, *dof2pix = nullptr;
+  u16 *feat = nullptr;
+  float *r32 = nullptr;
+  double *dscale = nullptr;  // per original DOF, null: the network works on unscaled vectors
+  double *pc_part = nullptr;
+  int pc_nparts = 0;
+  int odd = 0;  // apply the odd part of the network (needs batch >= 2)
+  // the solver's renumbering and scaling of the current solve
+  const int *perm = nullptr, *rperm = nullptr;
+  const double *rs = nullptr;
+  ~UserPrec();
+};
+
+namespace {
+
+template <typename T> void zfree(T *&p) { dfree(p); }
+
+ConvArgs conv_args(const UserPrec &U, const ConvLayer &l, int B)
+{
+  ConvArgs a{};
+  a.B = B;
+  a.H = U.H >> l.level;
+  a.W = U.W >> l.level;
+  a.cin_pad = l.cin_pad;
+  a.cout = l.cout;
+  a.K_pad = l.K_pad;
+  a.c1 = l.c1;
+  a.in = U.act[l.in_tensor];
+  a.in2 = l.in2_tensor >= 0 ? U.act[l.in2_tensor] : nullptr;
+  a.wt = l.wt;
+  a.bias = l.b32;
+  a.relu = 1;
+  a.flip = 0;
+  return a;
+}
+
+// Launch shape of one convolution: channel tile TN from cout, pixel tile TM
+// (64 or 16 pixels per wave) and split-K so that a launch has ~1024 waves
+// (4 per SIMD of the 256 CUs) even on the coarse levels of a single grid.
+struct ConvPlan {
+  int TM, TN, nsplit, kchunk;
+  long split_floats;  // partial-sum buffer needed (0 without split-K)
+};
+
+ConvPlan plan_conv(long P, int cout, int K_pad)
+{
+  ConvPlan c;
+  c.TN = cout <= 16 ? 1 : (cout <= 32 ? 2 : 4);
+  const long ntn = (cout + 16 * c.TN - 1) / (16 * c.TN);
+  const long target = 1024;
+  c.TM = 4;
+  long waves = (P + 63) / 64 * ntn;
+  if(waves < target) {
+    c.TM = 1;
+    waves = (P + 15) / 16 * ntn;
+  }
+  const int ksteps = K_pad / 32;
+  c.nsplit = 1;
+  while(waves * c.nsplit < target && c.nsplit * 4 <= ksteps) c.nsplit *= 2;
+  c.kchunk = ((ksteps + c.nsplit - 1) / c.nsplit) * 32;
+  c.nsplit = (K_pad + c.kchunk - 1) / c.kchunk;
+  c.split_floats = c.nsplit > 1 ? (long)c.nsplit * P * cout : 0;
+  return c;
+}
+
+template <int MODE, int TM> int launch_conv_tm(const ConvArgs &a, int TN, dim3 grid)
+{
+  hipStream_t s = ctx().stream;
+  if(TN == 1) hipLaunchKernelGGL((k_conv<MODE, TM, 1>), grid, dim3(256), 0, s, a);
+  else if(TN == 2) hipLaunchKernelGGL((k_conv<MODE, TM, 2>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_conv<MODE, TM, 4>), grid, dim3(256), 0, s, a);
+  return GDP_OK;
+}
+
+template <int MODE> int launch_conv_mode(ConvArgs a, const ConvPlan &c, float *split)
+{
+  const long P = (long)a.B * a.H * a.W;
+  a.kchunk = c.kchunk;
+  a.split = c.nsplit > 1 ? split : nullptr;
+  const long ptile = 4L * 16 * c.TM;
+  dim3 grid((unsigned)((P + ptile - 1) / ptile), (unsigned)((a.cout + 16 * c.TN - 1) / (16 * c.TN)),
+            (unsigned)c.nsplit);
+  if(c.TM == 4) launch_conv_tm<MODE, 4>(a, c.TN, grid);
+  else launch_conv_tm<MODE, 1>(a, c.TN, grid);
+  if(c.nsplit > 1)
+    hipLaunchKernelGGL(k_conv_split_sum, dim3((unsigned)((P * a.cout + 255) / 256)), dim3(256), 0,
+                       ctx().stream, a, c.nsplit);
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+dim3 g1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// Plan and launch one convolution.  The split-K buffer grows on demand only
+// when may_alloc (not while a caller captures a graph); the preconditioner
+// path reserves it when the network is attached.
+int launch_conv(UserPrec &U, const ConvArgs &a, int mode, bool may_alloc)
+{
+  const ConvPlan c = plan_conv((long)a.B * a.H * a.W, a.cout, a.K_pad);
+  if((size_t)c.split_floats > U.split_size) {
+    if(!may_alloc)
+      return set_error(GDP_E_STATE, "convolution scratch not reserved (%ld floats)", c.split_floats);
+    GDP_TRY(sync());
+    dfree(U.split);
+    U.split_size = 0;
+    GDP_TRY(dalloc(&U.split, (size_t)c.split_floats));
+    U.split_size = (size_t)c.split_floats;
+  }
+  if(mode == M_POOL) return launch_conv_mode<M_POOL>(a, c, U.split);
+  if(mode == M_UPCAT) return launch_conv_mode<M_UPCAT>(a, c, U.split);
+  return launch_conv_mode<M_PLAIN>(a, c, U.split);
+}
+
+// split-K scratch needed by a forward pass at batch B
+size_t forward_split_floats(const UserPrec &U, int B)
+{
+  size_t m = 0;
+  for(const auto &l : U.L) {
+    long P = (long)B * (U.H >> l.level) * (U.W >> l.level);
+    m = std::max(m, (size_t)plan_conv(P, l.cout, l.K_pad).split_floats);
+  }
+  return m;
+}
+
+int forward(UserPrec &U, int B, const int *gate, int gmode, bool may_alloc)
+{
+  for(size_t i = 0; i < U.L.size(); i++) {
+    ConvArgs a = conv_args(U, U.L[i], B);
+    a.out_bf = U.act[i + 1];
+    a.gate = gate;
+    a.gmode = gmode;
+    GDP_TRY(launch_conv(U, a, U.L[i].mode, may_alloc));
+  }
+  long P = (long)B * U.H * U.W;
+  hipLaunchKernelGGL(k_out_fwd, g1(P), dim3(256), 0, ctx().stream, P, 16, U.act.back(), U.ow, U.ob,
+                     U.z, gate, gmode);
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+int backward(UserPrec &U, int B)
+{
+  hipStream_t s = ctx().stream;
+  const long P0 = (long)B * U.H * U.W;
+  for(size_t t = 0; t < U.gact.size(); t++) {
+    long n = (long)B * (U.H >> U.act_lvl[t]) * (U.W >> U.act_lvl[t]) * U.act_ch[t];
+    GDP_HIP(hipMemsetAsync(U.gact[t], 0, sizeof(float) * n, s));
+  }
+  // output 1x1
+  hipLaunchKernelGGL(k_out_bwd_w, dim3(17), dim3(256), 0, s, P0, 16, U.dz, U.act.back(), U.gow, U.gob);
+  hipLaunchKernelGGL(k_out_bwd_x, g1(P0 * 16), dim3(256), 0, s, P0, 16, U.dz, U.ow, U.gact.back());
+  for(int i = (int)U.L.size() - 1; i >= 0; i--) {
+    const ConvLayer &l = U.L[i];
+    const int H = U.H >> l.level, W = U.W >> l.level;
+    const long P = (long)B * H * W;
+    // G = relu'(y) * dY (BF16)
+    hipLaunchKernelGGL(k_relu_grad, g1(P * l.cout), dim3(256), 0, s, P * l.cout, U.gact[i + 1],
+                       U.act[i + 1], U.g);
+    hipLaunchKernelGGL(k_bias_grad, dim3(l.cout), dim3(256), 0, s, P, l.cout, U.g, l.gb);
+    // weight gradient
+    ConvArgs x = conv_args(U, l, B);
+    WgradArgs wa{};
+    wa.x = x;
+    wa.g = U.g;
+    wa.chunk = 2048;
+    int nchunks = (int)((P + wa.chunk - 1) / wa.chunk);
+    size_t need = (size_t)nchunks * l.cout * l.K_pad;
+    if(need > U.part_size) {
+      dfree(U.part);
+      GDP_TRY(dalloc(&U.part, need));
+      U.part_size = need;
+    }
+    wa.part = U.part;
+    dim3 gw((unsigned)nchunks, (unsigned)((l.cout + 15) / 16), (unsigned)((l.K_pad + 63) / 64));
+    if(l.mode == M_POOL) hipLaunchKernelGGL(k_wgrad<M_POOL>, gw, dim3(256), 0, s, wa);
+    else if(l.mode == M_UPCAT) hipLaunchKernelGGL(k_wgrad<M_UPCAT>, gw, dim3(256), 0, s, wa);
+    else hipLaunchKernelGGL(k_wgrad<M_PLAIN>, gw, dim3(256), 0, s, wa);
+    long nw = (long)l.cout * l.K_pad;
+    hipLaunchKernelGGL(k_sum_chunks, g1(nw), dim3(256), 0, s, nchunks, nw, U.part, l.gw);
+    if(i == 0) continue;  // no gradient needed for the network input
+    // input gradient: adjoint convolution (mirrored taps) on G
+    ConvArgs d{};
+    d.B = B;
+    d.H = H;
+    d.W = W;
+    d.cin_pad = l.cout;
+    d.cout = l.cin_pad;
+    d.K_pad = l.K2_pad;
+    d.in = U.g;
+    d.wt = l.wtT;
+    d.out_f = U.dx;
+    d.relu = 0;
+    d.flip = 0;  // the taps are already mirrored in wtT
+    GDP_TRY(launch_conv(U, d, M_PLAIN, true));
+    long nx = P * l.cin_pad;
+    if(l.mode == M_PLAIN)
+      hipLaunchKernelGGL(k_add, g1(nx), dim3(256), 0, s, nx, U.dx, U.gact[l.in_tensor]);
+    else if(l.mode == M_POOL)
+      hipLaunchKernelGGL(k_pool_adj, g1(nx * 4), dim3(256), 0, s, B, H, W, l.cin_pad, U.dx,
+                         U.gact[l.in_tensor]);
+    else {
+      long nt = (long)B * (H / 2) * (W / 2) * l.c1 + P * (l.cin_pad - l.c1);
+      hipLaunchKernelGGL(k_upcat_adj, g1(nt), dim3(256), 0, s, B, H, W, l.c1, l.cin_pad - l.c1,
+                         U.dx, U.gact[l.in_tensor], U.gact[l.in2_tensor]);
+    }
+  }
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+int prep_weights(UserPrec &U)
+{
+  for(auto &l : U.L) {
+    long n = std::max((long)l.cout * l.K_pad, (long)l.cin_pad * l.K2_pad);
+    hipLaunchKernelGGL(k_prep_weights, g1(n), dim3(256), 0, ctx().stream, l.cout, l.cin_pad, l.K_pad,
+                       l.K2_pad, l.w32, l.wt, l.wtT);
+  }
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+}  // namespace
+
+UserPrec::~UserPrec()
+{
+  for(auto &l : L) {
+    zfree(l.w32);
+    zfree(l.b32);
+    zfree(l.gw);
+    zfree(l.gb);
+    zfree(l.mw);
+    zfree(l.vw);
+    zfree(l.mb);
+    zfree(l.vb);
+    zfree(l.wt);
+    zfree(l.wtT);
+  }
+  for(auto p : act) dfree(p);
+  for(auto p : gact) dfree(p);
+  zfree(ow);
+  zfree(ob);
+  zfree(gow);
+  zfree(gob);
+  zfree(mow);
+  zfree(vow);
+  zfree(mob);
+  zfree(vob);
+  zfree(z);
+  zfree(dz);
+  zfree(target);
+  zfree(mask);
+  zfree(loss);
+  zfree(dx);
+  zfree(g);
+  zfree(part);
+  zfree(split);
+  zfree(pix2dof);
+  zfree(dof2pix);
+  zfree(feat);
+  zfree(r32);
+  zfree(dscale);
+  zfree(pc_part);
+}
+
+void user_prec_bind(UserPrec *u, const int *perm, const int *rperm, const double *rs)
+{
+  u->perm = perm;
+  u->rperm = rperm;
+  u->rs = rs;
+}
+
+// z = M^{-1} v through the network: map the solver vector to the grid
+// (unpermute, unscale), normalise to unit RMS, forward pass, restore the
+// scale and map back.
+template <typename TV>
+int user_prec_apply_t(UserPrec *u, const TV *in, TV *out, KCtl *C, int gmode)
+{
+  if(!u || !u->pix2dof) return set_error(GDP_E_STATE, "learned preconditioner not bound to a system");
+  hipStream_t s = ctx().stream;
+  const int *gate = (const int *)C;
+  const int nparts = u->pc_nparts;
+  const int npix = u->H * u->W;
+  hipLaunchKernelGGL(k_pc_unscale<TV>, dim3(nparts), dim3(256), 0, s, npix, u->pix2dof, u->perm, u->rs,
+                     u->dscale, in, u->r32, u->pc_part, gate, gmode);
+  const int odd = u->odd && u->Bmax >= 2;
+  hipLaunchKernelGGL(k_pc_gather, dim3((unsigned)std::min(1024, (npix + 255) / 256)), dim3(256), 0, s,
+                     npix, u->n_dof, u->r32, u->pc_part, nparts, u->feat, u->act[0], odd, gate, gmode);
+  GDP_TRY(forward(*u, odd ? 2 : 1, gate, gmode, false));
+  hipLaunchKernelGGL(k_pc_scatter<TV>, dim3((unsigned)std::min(1024, (u->n_dof + 255) / 256)), dim3(256), 0,
+                     s, u->n_dof, u->n_dof, u->rperm, u->dof2pix, u->rs, u->dscale, u->z, u->pc_part,
+                     nparts, npix,
+                     odd, out, gate, gmode);
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
 
 int user_prec_apply(UserPrec *u, const double *in, double *out, KCtl *C, int gmode)
 {
-  (void)u;
-  (void)in;
-  (void)out;
-  (void)C;
-  (void)gmode;
-  return set_error(GDP_E_UNSUPPORTED, "no learned preconditioner attached");
+  return user_prec_apply_t<double>(u, in, out, C, gmode);
+}
+int user_prec_apply(UserPrec *u, const float *in, float *out, KCtl *C, int gmode)
+{
+  return user_prec_apply_t<float>(u, in, out, C, gmode);
 }
 
+UserPrec *solver_user_prec(gdp_solver *s);
+void solver_set_user_prec(gdp_solver *s, UserPrec *u);
+
 }  // namespace gdp
+
+using namespace gdp;
+
+// =============================================================================
+// C-ABI
+// =============================================================================
+struct gdp_unet {
+  std::unique_ptr<UserPrec> u;
+};
+
+extern "C" int gdp_unet_create(int H, int W, int batch, unsigned seed, gdp_unet **out)
+{
+  if(!out || H <= 0 || W <= 0 || H % 8 || W % 8 || batch < 1)
+    return set_error(GDP_E_ARG, "gdp_unet_create: grid must be a positive multiple of 8");
+  GDP_TRY(ensure_ready());
+  auto *N = new gdp_unet;
+  N->u.reset(new UserPrec);
+  UserPrec &U = *N->u;
+  U.H = H;
+  U.W = W;
+  U.Bmax = batch;
+  // activations: 0 = input (8 channels)
+  auto add_act = [&](int ch, int lvl) {
+    U.act_ch.push_back(ch);
+    U.act_lvl.push_back(lvl);
+    return (int)U.act_ch.size() - 1;
+  };
+  add_act(8, 0);
+  struct Spec {
+    int mode, cout, lvl, in, in2;
+  };
+  // tensor indices: conv i writes tensor i+1
+  const Spec spec[14] = {
+    {M_PLAIN, 16, 0, 0, -1},   {M_PLAIN, 16, 0, 1, -1},                    // E0 -> s0 = t2
+    {M_POOL, 32, 1, 2, -1},    {M_PLAIN, 32, 1, 3, -1},                    // E1 -> s1 = t4
+    {M_POOL, 64, 2, 4, -1},    {M_PLAIN, 64, 2, 5, -1},                    // E2 -> s2 = t6
+    {M_POOL, 128, 3, 6, -1},   {M_PLAIN, 128, 3, 7, -1},                   // E3 -> s3 = t8
+    {M_UPCAT, 64, 2, 8, 6},    {M_PLAIN, 64, 2, 9, -1},                    // D2 -> t10
+    {M_UPCAT, 32, 1, 10, 4},   {M_PLAIN, 32, 1, 11, -1},                   // D1 -> t12
+    {M_UPCAT, 16, 0, 12, 2},   {M_PLAIN, 16, 0, 13, -1},                   // D0 -> t14
+  };
+  std::mt19937 rng(seed);
+  for(int i = 0; i < 14; i++) {
+    ConvLayer l;
+    l.mode = spec[i].mode;
+    l.cout = spec[i].cout;
+    l.level = spec[i].lvl;
+    l.in_tensor = spec[i].in;
+    l.in2_tensor = spec[i].in2;
+    if(l.mode == M_UPCAT) {
+      l.c1 = U.act_ch[l.in_tensor];
+      l.cin_pad = l.c1 + U.act_ch[l.in2_tensor];
+      l.cin = l.cin_pad;
+    }
+    else {
+      l.cin_pad = U.act_ch[l.in_tensor];
+      l.cin = (i == 0) ? 2 : l.cin_pad;
+    }
+    l.K_pad = (int)rup(9L * l.cin_pad, 32);
+    l.K2_pad = (int)rup(9L * l.cout, 32);
+    add_act(l.cout, l.level);
+    long nw = (long)l.cout * l.K_pad;
+    std::vector<float> w(nw, 0.f), b(l.cout, 0.f);
+    // He initialisation over the true fan-in
+    std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (9.f * l.cin)));
+    for(int co = 0; co < l.cout; co++)
+      for(int tap = 0; tap < 9; tap++)
+        for(int ci = 0; ci < l.cin; ci++) w[(long)co * l.K_pad + tap * l.cin_pad + ci] = nd(rng);
+    GDP_TRY(dalloc(&l.w32, nw));
+    GDP_TRY(dalloc(&l.gw, nw));
+    GDP_TRY(dalloc(&l.mw, nw));
+    GDP_TRY(dalloc(&l.vw, nw));
+    GDP_TRY(dalloc(&l.b32, l.cout));
+    GDP_TRY(dalloc(&l.gb, l.cout));
+    GDP_TRY(dalloc(&l.mb, l.cout));
+    GDP_TRY(dalloc(&l.vb, l.cout));
+    GDP_TRY(dalloc(&l.wt, nw));
+    GDP_TRY(dalloc(&l.wtT, (size_t)l.cin_pad * l.K2_pad));
+    GDP_TRY(h2d(l.w32, w.data(), nw));
+    GDP_TRY(h2d(l.b32, b.data(), l.cout));
+    GDP_HIP(hipMemsetAsync(l.mw, 0, sizeof(float) * nw, ctx().stream));
+    GDP_HIP(hipMemsetAsync(l.vw, 0, sizeof(float) * nw, ctx().stream));
+    GDP_HIP(hipMemsetAsync(l.mb, 0, sizeof(float) * l.cout, ctx().stream));
+    GDP_HIP(hipMemsetAsync(l.vb, 0, sizeof(float) * l.cout, ctx().stream));
+    U.L.push_back(l);
+  }
+  {
+    std::vector<float> w(16), b(1, 0.f);
+    std::normal_distribution<float> nd(0.f, std::sqrt(1.f / 16.f));
+    for(auto &x : w) x = nd(rng);
+    for(float **p : {&U.ow, &U.gow, &U.mow, &U.vow}) GDP_TRY(dalloc(p, 16));
+    for(float **p : {&U.ob, &U.gob, &U.mob, &U.vob}) GDP_TRY(dalloc(p, 1));
+    GDP_TRY(h2d(U.ow, w.data(), 16));
+    GDP_TRY(h2d(U.ob, b.data(), 1));
+    for(float *p : {U.mow, U.vow}) GDP_HIP(hipMemsetAsync(p, 0, 16 * sizeof(float), ctx().stream));
+    for(float *p : {U.mob, U.vob}) GDP_HIP(hipMemsetAsync(p, 0, sizeof(float), ctx().stream));
+  }
+  size_t maxx = 0;
+  for(size_t t = 0; t < U.act_ch.size(); t++) {
+    size_t n = (size_t)batch * (H >> U.act_lvl[t]) * (W >> U.act_lvl[t]) * U.act_ch[t];
+    u16 *a = nullptr;
+    float *g = nullptr;
+    GDP_TRY(dalloc(&a, n));
+    GDP_TRY(dalloc(&g, n));
+    GDP_HIP(hipMemsetAsync(a, 0, n * sizeof(u16), ctx().stream));
+    U.act.push_back(a);
+    U.gact.push_back(g);
+    maxx = std::max(maxx, n);
+  }
+  size_t maxin = 0;
+  for(auto &l : U.L) maxin = std::max(maxin, (size_t)batch * (H >> l.level) * (W >> l.level) * l.cin_pad);
+  GDP_TRY(dalloc(&U.dx, std::max(maxx, maxin)));
+  GDP_TRY(dalloc(&U.g, maxx));
+  size_t P = (size_t)batch * H * W;
+  GDP_TRY(dalloc(&U.z, P));
+  GDP_TRY(dalloc(&U.dz, P));
+  GDP_TRY(dalloc(&U.target, P));
+  GDP_TRY(dalloc(&U.mask, (size_t)H * W));
+  GDP_TRY(dalloc(&U.loss, (size_t)batch));
+  GDP_TRY(prep_weights(U));
+  GDP_TRY(sync());
+  *out = N;
+  return GDP_OK;
+}
+
+extern "C" int gdp_unet_destroy(gdp_unet *n)
+{
+  if(!n) return GDP_OK;
+  if(ctx().ready) (void)sync();
+  delete n;
+  return GDP_OK;
+}
+
+extern "C" int gdp_unet_num_params(const gdp_unet *n, long *count)
+{
+  if(!n || !count) return set_error(GDP_E_ARG, "null argument");
+  long c = 17;
+  for(auto &l : n->u->L) c += (long)l.cout * 9 * l.cin + l.cout;
+  *count = c;
+  return GDP_OK;
+}
+
+// Parameters as one FP32 array in layer order: for each convolution its
+// weights [cout][9][cin] then biases [cout]; then the output weights [16] and
+// bias [1].  get/set for checkpoints and for gradient checks.
+static int params_io(gdp_unet *n, float *host, bool get, bool grads)
+{
+  UserPrec &U = *n->u;
+  long off = 0;
+  for(auto &l : U.L) {
+    long nw = (long)l.cout * l.K_pad;
+    std::vector<float> w(nw, 0.f), b(l.cout);
+    float *dw = grads ? l.gw : l.w32, *db = grads ? l.gb : l.b32;
+    GDP_TRY(d2h(w.data(), dw, nw));
+    GDP_TRY(d2h(b.data(), db, l.cout));
+    for(int co = 0; co < l.cout; co++)
+      for(int tap = 0; tap < 9; tap++)
+        for(int ci = 0; ci < l.cin; ci++) {
+          float &slot = w[(long)co * l.K_pad + tap * l.cin_pad + ci];
+          if(get) host[off] = slot;
+          else slot = host[off];
+          off++;
+        }
+    for(int co = 0; co < l.cout; co++) {
+      if(get) host[off] = b[co];
+      else b[co] = host[off];
+      off++;
+    }
+    if(!get) {
+      GDP_TRY(h2d(dw, w.data(), nw));
+      GDP_TRY(h2d(db, b.data(), l.cout));
+    }
+  }
+  float *ow = grads ? U.gow : U.ow, *ob = grads ? U.gob : U.ob;
+  if(get) {
+    GDP_TRY(d2h(host + off, ow, 16));
+    GDP_TRY(d2h(host + off + 16, ob, 1));
+  }
+  else {
+    GDP_TRY(h2d(ow, host + off, 16));
+    GDP_TRY(h2d(ob, host + off + 16, 1));
+    GDP_TRY(prep_weights(U));
+  }
+  return sync();
+}
+
+extern "C" int gdp_unet_get_params(gdp_unet *n, float *host)
+{
+  if(!n || !host) return set_error(GDP_E_ARG, "null argument");
+  return params_io(n, host, true, false);
+}
+
+extern "C" int gdp_unet_set_params(gdp_unet *n, const float *host)
+{
+  if(!n || !host) return set_error(GDP_E_ARG, "null argument");
+  return params_io(n, const_cast<float *>(host), false, false);
+}
+
+extern "C" int gdp_unet_get_grads(gdp_unet *n, float *host)
+{
+  if(!n || !host) return set_error(GDP_E_ARG, "null argument");
+  return params_io(n, host, true, true);
+}
+
+// Forward pass on a batch of grids: r, feat (per pixel) -> z (FP32).
+extern "C" int gdp_unet_forward(gdp_unet *n, int batch, const float *r, const float *feat,
+                                float *z_out)
+{
+  if(!n || !r || !feat || batch < 1 || batch > n->u->Bmax) return set_error(GDP_E_ARG, "gdp_unet_forward");
+  UserPrec &U = *n->u;
+  const long HW = (long)U.H * U.W, P = batch * HW;
+  float *dr = nullptr, *df = nullptr;
+  GDP_TRY(dalloc(&dr, P));
+  GDP_TRY(dalloc(&df, P));
+  int rc = h2d(dr, r, P);
+  if(!rc) rc = h2d(df, feat, P);
+  if(!rc) {
+    hipLaunchKernelGGL(k_make_input, g1(P), dim3(256), 0, ctx().stream, P, dr, df, U.act[0]);
+    rc = forward(U, batch, nullptr, 0, true);
+  }
+  if(!rc && z_out) rc = d2h(z_out, U.z, P);
+  dfree(dr);
+  dfree(df);
+  return rc;
+}
+
+// One training step on a batch (scale-invariant loss on the masked grid).
+// Returns the mean loss in *loss.  lr = 0 computes gradients only.
+extern "C" int gdp_unet_train_step(gdp_unet *n, int batch, const float *r, const float *feat,
+                                   const float *target, const float *mask, float lr,
+                                   float *loss)
+{
+  if(!n || !r || !feat || !target || !mask || batch < 1 || batch > n->u->Bmax)
+    return set_error(GDP_E_ARG, "gdp_unet_train_step");
+  UserPrec &U = *n->u;
+  hipStream_t s = ctx().stream;
+  const long HW = (long)U.H * U.W, P = batch * HW;
+  float *dr = nullptr, *df = nullptr;
+  GDP_TRY(dalloc(&dr, P));
+  GDP_TRY(dalloc(&df, P));
+  int rc = h2d(dr, r, P);
+  if(!rc) rc = h2d(df, feat, P);
+  if(!rc) rc = h2d(U.target, target, P);
+  if(!rc) rc = h2d(U.mask, mask, HW);
+  if(!rc) {
+    hipLaunchKernelGGL(k_make_input, g1(P), dim3(256), 0, s, P, dr, df, U.act[0]);
+    rc = forward(U, batch, nullptr, 0, true);
+  }
+  if(!rc) {
+    hipLaunchKernelGGL(k_loss, dim3(batch), dim3(256), 0, s, (int)HW, U.z, U.target, U.mask, U.dz,
+                       U.loss);
+    rc = backward(U, batch);
+  }
+  if(!rc && lr > 0.f) {
+    U.adam_t++;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float bc1 = 1.f - std::pow(b1, (float)U.adam_t), bc2 = 1.f - std::pow(b2, (float)U.adam_t);
+    for(auto &l : U.L) {
+      long nw = (long)l.cout * l.K_pad;
+      hipLaunchKernelGGL(k_adam, g1(nw), dim3(256), 0, s, nw, l.w32, l.gw, l.mw, l.vw, lr, b1, b2, eps, bc1, bc2, 0.f);
+      hipLaunchKernelGGL(k_adam, g1(l.cout), dim3(256), 0, s, (long)l.cout, l.b32, l.gb, l.mb, l.vb, lr, b1, b2, eps, bc1, bc2, 0.f);
+    }
+    hipLaunchKernelGGL(k_adam, dim3(1), dim3(256), 0, s, 16L, U.ow, U.gow, U.mow, U.vow, lr, b1, b2, eps, bc1, bc2, 0.f);
+    hipLaunchKernelGGL(k_adam, dim3(1), dim3(256), 0, s, 1L, U.ob, U.gob, U.mob, U.vob, lr, b1, b2, eps, bc1, bc2, 0.f);
+    rc = prep_weights(U);
+  }
+  if(!rc && loss) {
+    std::vector<float> h(batch);
+    rc = d2h(h.data(), U.loss, batch);
+    float m = 0.f;
+    for(float v : h) m += v;
+    *loss = m / batch;
+  }
+  dfree(dr);
+  dfree(df);
+  return rc;
+}
+
+// Bind the network as the FGMRES preconditioner of a solver: pix2dof maps grid
+// pixels to (renumbered) DOF indices (-1: not a DOF), feat holds the per-pixel
+// coefficient feature.  Used when the solver's Algorithm is FGMRES (9).
+extern "C" int gdp_unet_attach(gdp_unet *n, gdp_solver *s, int n_dof, const int *pix2dof,
+                               const float *feat, const double *dscale)
+{
+  if(!n || !s || !pix2dof || !feat || n_dof < 1) return set_error(GDP_E_ARG, "gdp_unet_attach");
+  UserPrec &U = *n->u;
+  const long npix = (long)U.H * U.W;
+  std::vector<int> d2p(n_dof, -1);
+  for(long p = 0; p < npix; p++) {
+    int d = pix2dof[p];
+    if(d >= n_dof) return set_error(GDP_E_ARG, "pix2dof[%ld] = %d out of range", p, d);
+    if(d >= 0) d2p[d] = (int)p;
+  }
+  for(int d = 0; d < n_dof; d++)
+    if(d2p[d] < 0) return set_error(GDP_E_ARG, "DOF %d has no grid pixel", d);
+  dfree(U.pix2dof);
+  dfree(U.dof2pix);
+  dfree(U.feat);
+  dfree(U.r32);
+  dfree(U.dscale);
+  dfree(U.pc_part);
+  GDP_TRY(dalloc(&U.pix2dof, npix));
+  GDP_TRY(dalloc(&U.dof2pix, n_dof));
+  GDP_TRY(dalloc(&U.feat, npix));
+  GDP_TRY(dalloc(&U.r32, npix));
+  U.pc_nparts = (int)std::min<long>(256, (npix + 255) / 256);
+  GDP_TRY(dalloc(&U.pc_part, U.pc_nparts));
+  std::vector<u16> fb(npix);
+  for(long p = 0; p < npix; p++) fb[p] = f2bf(feat[p]);
+  GDP_TRY(h2d(U.pix2dof, pix2dof, npix));
+  GDP_TRY(h2d(U.dof2pix, d2p.data(), n_dof));
+  GDP_TRY(h2d(U.feat, fb.data(), npix));
+  if(dscale) {
+    for(int d = 0; d < n_dof; d++)
+      if(!(dscale[d] > 0.)) return set_error(GDP_E_ARG, "dscale[%d] must be positive", d);
+    GDP_TRY(dalloc(&U.dscale, n_dof));
+    GDP_TRY(h2d(U.dscale, dscale, n_dof));
+  }
+  U.n_dof = n_dof;
+  // reserve the convolution scratch of the preconditioner's passes
+  const size_t need = std::max(forward_split_floats(U, 1), forward_split_floats(U, std::min(2, U.Bmax)));
+  if(need > U.split_size) {
+    GDP_TRY(sync());
+    dfree(U.split);
+    U.split_size = 0;
+    GDP_TRY(dalloc(&U.split, need));
+    U.split_size = need;
+  }
+  solver_set_user_prec(s, &U);
+  return sync();
+}
+
+extern "C" int gdp_unet_detach(gdp_solver *s)
+{
+  if(!s) return set_error(GDP_E_ARG, "null solver");
+  solver_set_user_prec(s, nullptr);
+  return GDP_OK;
+}
+
+// Copy activation tensor t (0 = network input, i + 1 = output of convolution
+// i, 15 = network output z as one FP32 channel) of the last forward pass to
+// the host as FP32, NHWC with the stored (padded) channel count.
+extern "C" int gdp_unet_get_activation(gdp_unet *n, int t, int batch, float *host, long *count)
+{
+  if(!n || batch < 1 || batch > n->u->Bmax) return set_error(GDP_E_ARG, "gdp_unet_get_activation");
+  UserPrec &U = *n->u;
+  const int nt = (int)U.act.size();
+  if(t < 0 || t > nt) return set_error(GDP_E_ARG, "activation index %d out of range", t);
+  if(t == nt) {
+    long c = (long)batch * U.H * U.W;
+    if(count) *count = c;
+    if(host) GDP_TRY(d2h(host, U.z, (size_t)c));
+    return GDP_OK;
+  }
+  long c = (long)batch * (U.H >> U.act_lvl[t]) * (U.W >> U.act_lvl[t]) * U.act_ch[t];
+  if(count) *count = c;
+  if(!host) return GDP_OK;
+  std::vector<u16> h(c);
+  GDP_TRY(d2h(h.data(), U.act[t], (size_t)c));
+  for(long i = 0; i < c; i++) {
+    unsigned int u = ((unsigned int)h[i]) << 16;
+    std::memcpy(host + i, &u, 4);
+  }
+  return GDP_OK;
+}
+
+// Preconditioner options: bit 0 = apply the odd part (N(r) - N(-r))/2 of the
+// network (one batch-2 pass; the network must have been created with
+// batch >= 2).
+extern "C" int gdp_unet_set_mode(gdp_unet *n, int flags)
+{
+  if(!n) return set_error(GDP_E_ARG, "null network");
+  if((flags & 1) && n->u->Bmax < 2) return set_error(GDP_E_ARG, "odd mode needs a network created with batch >= 2");
+  n->u->odd = flags & 1;
+  return GDP_OK;
+}

@@ -227,6 +227,47 @@ int gdp_number_edges(int nelem, const int *etype, const int *conn, int nvisit,
  * perm[old] = new, rperm[new] = old (0-based).  Returns #levels or < 0. */
 int gdp_rcmk(int n, const int *ia, const int *ja, int *perm, int *rperm);
 
+/* ---- learned preconditioner (no counterpart in the reference) --------------
+ * A 4-level convolutional U-Net (16-32-64-128 channels, 3x3 kernels, ReLU,
+ * 2x2 average pooling, nearest upsampling + skip concatenation, 1x1 output)
+ * on an H x W grid of nodal values, BF16 operands with FP32 accumulation on
+ * the MFMA units, FP32 master weights trained with Adam.  Input channels:
+ * the residual (normalised to unit RMS) and a per-pixel coefficient feature
+ * (log nu); output: the correction, in the same normalisation.  Bound to a
+ * solver with gdp_unet_attach it is the right preconditioner of flexible
+ * GMRES (Algorithm 9, iters.f:2522-2807), FP64 or FP32 (Precision 1). */
+typedef struct gdp_unet gdp_unet;
+/* H, W multiples of 8; batch = largest batch of train/forward calls. */
+int gdp_unet_create(int H, int W, int batch, unsigned seed, gdp_unet **out);
+int gdp_unet_destroy(gdp_unet *net);
+int gdp_unet_num_params(const gdp_unet *net, long *count);
+/* Parameters as one FP32 array: per convolution weights [cout][3][3][cin]
+ * then bias [cout]; then the output weights [16] and bias [1]. */
+int gdp_unet_get_params(gdp_unet *net, float *host);
+int gdp_unet_set_params(gdp_unet *net, const float *host);
+/* Gradients of the last gdp_unet_train_step, in the parameter layout. */
+int gdp_unet_get_grads(gdp_unet *net, float *host);
+/* r, feat: batch x H x W (residuals already normalised; mask: H x W). */
+int gdp_unet_forward(gdp_unet *net, int batch, const float *r, const float *feat, float *z);
+/* One Adam step (lr = 0: gradients only) on the loss
+ * |z m / |z m| - t|^2 per sample (t unit-norm target on the mask m). */
+int gdp_unet_train_step(gdp_unet *net, int batch, const float *r, const float *feat,
+                        const float *target, const float *mask, float lr, float *loss);
+/* Tensor t of the last forward pass as FP32 NHWC (t = 0 input, 1..14 the
+ * convolution outputs, 15 the network output); host = NULL returns the count. */
+int gdp_unet_get_activation(gdp_unet *net, int t, int batch, float *host, long *count);
+/* pix2dof[H*W]: the DOF (0-based, original numbering) at each pixel or -1;
+ * every DOF must appear.  feat[H*W]: coefficient feature per pixel.
+ * dscale[n_dof] (may be NULL): the network works on s*r and returns e/s per
+ * DOF, s = 1/sqrt(diag A) making the operator unit-diagonal. */
+int gdp_unet_attach(gdp_unet *net, gdp_solver *solver, int n_dof, const int *pix2dof,
+                    const float *feat, const double *dscale);
+int gdp_unet_detach(gdp_solver *solver);
+/* Preconditioner options, bit 0: apply the odd part (N(r) - N(-r))/2 of the
+ * network (batch >= 2 at creation).  With Preconditioner 6 on the solver the
+ * preconditioner is two-stage: z = N(v) + (LU)^{-1}(v - A N(v)). */
+int gdp_unet_set_mode(gdp_unet *net, int flags);
+
 #ifdef __cplusplus
 }
 #endif
