@@ -92,6 +92,10 @@ SIGNATURES = [
     ("gdp_fe_last_timing", C.c_int, [_vp, _dp, _dp, _dp]),
     ("gdp_number_edges", C.c_int, [C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip, C.c_int]),
     ("gdp_rcmk", C.c_int, [C.c_int, _ip, _ip, _ip, _ip]),
+    ("gdp_batch_create", C.c_int, [_vp, C.c_int, _ip, _ip, _pp]),
+    ("gdp_batch_destroy", C.c_int, [_vp]),
+    ("gdp_batch_run", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    ("gdp_batch_info", C.c_int, [_vp, _ip, _ip, _dp, _dp, _dp, _dp, _dp]),
     ("gdp_unet_create", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint, _pp]),
     ("gdp_unet_destroy", C.c_int, [_vp]),
     ("gdp_unet_num_params", C.c_int, [_vp, _lp]),

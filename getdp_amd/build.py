@@ -42,6 +42,7 @@ SOURCES = [
     ("krylov.hip", []),
     ("solve.hip", []),
     ("unet.hip", []),
+    ("batch.hip", []),
     ("host_prep.cpp", []),
     ("solver_params.cpp", []),
 ]

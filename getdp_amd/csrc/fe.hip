@@ -969,6 +969,43 @@ extern "C" int gdp_fe_generate(gdp_fe *fe, gdp_matrix *A, gdp_vector *b)
   return GDP_OK;
 }
 
+namespace gdp {
+namespace {
+__global__ void k_zero_f64(long n, double *p)
+{
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if(i < n) p[i] = 0.;
+}
+}  // namespace
+
+// Generate[A] enqueued only: same kernels as gdp_fe_generate, the values
+// cleared by a kernel instead of a memset, no events and no host
+// synchronisation, so that a caller may capture it in a graph.  A must
+// already carry the formulation's pattern (one gdp_fe_generate before).
+int fe_generate_enqueue(gdp_fe *fe, gdp_matrix *A, gdp_vector *b)
+{
+  if(!fe || !A || !b) return set_error(GDP_E_ARG, "fe_generate_enqueue: null argument");
+  if(!fe->uploaded || A->csr.nnz != fe->nnz || A->h_ia != fe->h_ia || b->n != fe->N)
+    return set_error(GDP_E_STATE, "fe_generate_enqueue: generate once before capturing");
+  hipStream_t s = ctx().stream;
+  if(fe->nnz)
+    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((fe->nnz + 255) / 256)), dim3(256), 0, s, fe->nnz,
+                       A->csr.a);
+  if(fe->nelem)
+    hipLaunchKernelGGL(k_geometry, dim3((fe->nelem + 255) / 256), dim3(256), 0, s, fe->nelem,
+                       fe->d_geokind, fe->d_conn, fe->d_xyz, fe->d_geo);
+  if(fe->N) {
+    dim3 grid((fe->N + 127) / 128);
+    if(fe->family == FAM_NODAL)
+      hipLaunchKernelGGL(k_gather_nodal, grid, dim3(128), 0, s, fe->plan, A->csr.a, b->d);
+    else
+      hipLaunchKernelGGL(k_gather_edge, grid, dim3(128), 0, s, fe->plan, A->csr.a, b->d);
+  }
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+}  // namespace gdp
+
 extern "C" int gdp_fe_last_timing(const gdp_fe *fe, double *ms_total, double *ms_geometry,
                                   double *ms_gather)
 {

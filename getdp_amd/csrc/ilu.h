@@ -26,6 +26,12 @@ struct IluFactor {
   int analyse(const DevCSR &B, const std::vector<int> &hia, const std::vector<int> &hja);
   // Factor values a (B's CSR layout); FP32 or FP64 storage.
   int factor(const double *a, bool f64, int *zero_pivot_row);
+  // The same factorisation, enqueued only (no allocation, copy or host
+  // synchronisation: usable under graph capture).  The storage must have
+  // been allocated by reserve(); factor_status() reads the pivot flag.
+  int reserve(bool f64);
+  int factor_enqueue(const double *a);
+  int factor_status(int *zero_pivot_row);
   // x = (LU)^{-1} (scale * r); may be enqueued repeatedly without host sync.
   // gate/gmode: skip when a Krylov control block says the work has ended.
   template <typename TV>

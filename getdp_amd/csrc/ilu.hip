@@ -349,6 +349,52 @@ int IluFactor::factor(const double *a, bool f64, int *zero_pivot_row)
   return GDP_OK;
 }
 
+namespace {
+__global__ void k_set_int(int *p, int v) { *p = v; }
+}  // namespace
+
+int IluFactor::reserve(bool f64)
+{
+  is_f64 = f64;
+  dfree(lu32);
+  dfree(lu64);
+  if(f64) GDP_TRY(dalloc(&lu64, (size_t)std::max<long>(1, nnz)));
+  else GDP_TRY(dalloc(&lu32, (size_t)std::max<long>(1, nnz)));
+  return GDP_OK;
+}
+
+int IluFactor::factor_enqueue(const double *a)
+{
+  hipStream_t s = ctx().stream;
+  if(!(is_f64 ? (void *)lu64 : (void *)lu32)) return set_error(GDP_E_STATE, "ILU storage not reserved");
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, err, 0x7fffffff);
+  dim3 g((unsigned)((nnz + 255) / 256));
+  if(nnz) {
+    if(is_f64) hipLaunchKernelGGL(k_ilu_copy<double>, g, dim3(256), 0, s, nnz, a, lu64);
+    else hipLaunchKernelGGL(k_ilu_copy<float>, g, dim3(256), 0, s, nnz, a, lu32);
+  }
+  for(int l = 0; l < nlevels; l++) {
+    int cnt = level_ptr[l + 1] - level_ptr[l];
+    dim3 gl((cnt + 127) / 128);
+    if(is_f64)
+      hipLaunchKernelGGL(k_ilu_level<double>, gl, dim3(128), 0, s, cnt, level_rows + level_ptr[l], ia, ja,
+                         diag, lu64, err);
+    else
+      hipLaunchKernelGGL(k_ilu_level<float>, gl, dim3(128), 0, s, cnt, level_rows + level_ptr[l], ia, ja,
+                         diag, lu32, err);
+  }
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+int IluFactor::factor_status(int *zero_pivot_row)
+{
+  int h;
+  GDP_TRY(d2h(&h, err, 1));
+  *zero_pivot_row = (h == 0x7fffffff) ? 0 : h;
+  return GDP_OK;
+}
+
 unsigned long long IluFactor::wait_limit() const
 {
   return 100000000ull;  // wall_clock64 runs at 100 MHz: 1 s per dependency wait

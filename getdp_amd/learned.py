@@ -363,3 +363,113 @@ def solve_learned(A, b, x, net: UNet, maps: GridMaps, rtol: float = 1e-10,
     LA.prod_matrix_vector(A, x, y)
     out["rel_res"] = float(np.linalg.norm(b.get() - y.get()) / np.linalg.norm(b.get()))
     return out
+
+
+# ---------------------------------------------------------------------------
+# evaluation and command line
+
+
+def plain_solve(A, b, N, ordering: int = 0):
+    """The plain method: ILU(0)-GMRES(30) to 1e-10 (ordering 0: the
+    reference's RCMK renumbering; 1: multicolour ILU ordering)."""
+    from . import linalg as LA
+    s = _solver(ILU_Ordering=ordering, Orthogonalization=1)
+    x = LA.Vector(N)
+    t0 = time.perf_counter()
+    info = LA.solve(A, b, s, x)
+    info["t_wall_ms"] = 1e3 * (time.perf_counter() - t0)
+    return x, info
+
+
+def evaluate(params: np.ndarray, n: int, seeds, log=None, odd: bool = False):
+    """Learned two-stage FGMRES against plain ILU(0)-GMRES on unseen fields
+    (the problem's own source term).  Times are wall-clock of the solve call
+    (set-up included), each after one untimed warm-up solve."""
+    from . import linalg as LA
+    rows = []
+    for seed in seeds:
+        pr = random_field_problem(n, seed)
+        fe, A, b = LA.assemble(pr)
+        maps = grid_maps(pr, fe.dof_table(0), A)
+        net = UNet(n, n, batch=2, seed=0)
+        net.set_params(params)
+        net.set_mode(odd=odd)
+        row = {"n": n, "seed": seed, "field": "checker" if seed % 2 == 0 else "lognormal",
+               "dofs": fe.N}
+        for ordering, key in ((0, "plain_rcmk"), (1, "plain_multicolour")):
+            plain_solve(A, b, fe.N, ordering)
+            x, info = plain_solve(A, b, fe.N, ordering)
+            row[key] = {"its": info["its"], "ms": round(info["t_wall_ms"], 2),
+                        "rel_res": info["rel_res"]}
+        for ordering, key in ((0, "learned_rcmk"), (1, "learned_multicolour")):
+            x = LA.Vector(fe.N)
+            solve_learned(A, b, x, net, maps, maxits=1000, two_stage=True, ilu_ordering=ordering)
+            x = LA.Vector(fe.N)
+            out = solve_learned(A, b, x, net, maps, maxits=1000, two_stage=True,
+                                ilu_ordering=ordering)
+            row[key] = {"its": out["learned"]["its"], "ms": round(1e3 * out["t_total_s"], 2),
+                        "fallback_its": None if out["fallback"] is None else out["fallback"]["its"],
+                        "rel_res": out["rel_res"]}
+        conv = [row[k]["ms"] for k in ("plain_rcmk", "plain_multicolour") if row[k]["rel_res"] <= 1e-10]
+        best_plain = min(conv) if conv else float("nan")
+        best_learned = min(row["learned_rcmk"]["ms"], row["learned_multicolour"]["ms"])
+        row["speedup_vs_plain_rcmk"] = round(row["plain_rcmk"]["ms"] / best_learned, 2)
+        row["speedup_vs_best_plain"] = round(best_plain / best_learned, 2)
+        rows.append(row)
+        if log:
+            log(f"eval n={n} seed={seed} ({row['field']}): " + ", ".join(
+                f"{k} {row[k]['its']} its {row[k]['ms']} ms" for k in
+                ("plain_rcmk", "plain_multicolour", "learned_rcmk", "learned_multicolour"))
+                + f"; speedup {row['speedup_vs_plain_rcmk']}x / {row['speedup_vs_best_plain']}x")
+    return rows
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import sys
+    from . import linalg as LA
+    ap = argparse.ArgumentParser(description="train / evaluate the learned preconditioner")
+    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--fields", type=int, default=500)
+    ap.add_argument("--rhs", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=8000)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--eval-n", type=int, nargs="*", default=[128, 256])
+    ap.add_argument("--eval-seeds", type=int, nargs="*", default=[90000, 90001, 90002, 90003])
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+
+    def log(*m):
+        print(*m, file=sys.stderr, flush=True)
+
+    LA.init(0)
+    if a.cmd == "train":
+        r, e, f, mask = generate_pairs(a.n, a.fields, seed0=a.seed, rhs_per_field=a.rhs, log=log,
+                                       ILU_Ordering=1)
+        log(f"{r.shape[0]} pairs")
+        net = UNet(a.n, a.n, batch=a.batch, seed=a.seed)
+        hist = train(net, r, e, f, mask, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
+                     log=log, log_every=500)
+        del r, e, f
+        out = a.weights or "unet_poisson2d.npz"
+        net.save(out)
+        log(f"saved {out}; final loss {hist[-200:].mean():.4f}")
+        params = net.get_params()
+    else:
+        with np.load(a.weights, allow_pickle=False) as z:
+            params = z["params"]
+    rows = []
+    for n in a.eval_n:
+        rows += evaluate(params, n, a.eval_seeds, log=log)
+    if a.json:
+        with open(a.json, "w") as fh:
+            json.dump(rows, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()

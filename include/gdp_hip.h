@@ -227,6 +227,30 @@ int gdp_number_edges(int nelem, const int *etype, const int *conn, int nvisit,
  * perm[old] = new, rperm[new] = old (0-based).  Returns #levels or < 0. */
 int gdp_rcmk(int n, const int *ia, const int *ja, int *perm, int *rperm);
 
+/* ---- batched solves (no single counterpart in the reference) ----------------
+ * Many independent systems solved as the reference solves one (Algorithm 8,
+ * Preconditioner 6: solve_matrix, Sparskit.cpp:108-634, per instance).  The
+ * systems are the diagonal blocks of one matrix -- instance i owns rows
+ * [row_offsets[i], row_offsets[i+1]) -- typically assembled as one
+ * formulation over concatenated meshes.  One persistent kernel per launch
+ * configuration runs whole instances per workgroup, largest first; the
+ * assembly (when fe is given), renumbering of values, ILU(0) factorisation,
+ * the solves and the unpermutation are one captured hipGraph, recorded on
+ * the first gdp_batch_run and replayed after.
+ * config (may be NULL): 3 ints per instance -- workgroup size (256, 512,
+ * 1024), orthogonalisation (0 modified, 1 classical twice), ILU ordering
+ * (0 as Renumbering_Technique, 1 multicolour on top).  Krylov_Size <= 32. */
+typedef struct gdp_batch gdp_batch;
+int gdp_batch_create(const gdp_solver *params, int ninst, const int *row_offsets,
+                     const int *config, gdp_batch **out);
+int gdp_batch_destroy(gdp_batch *batch);
+int gdp_batch_run(gdp_batch *batch, gdp_fe *fe, gdp_matrix *A, gdp_vector *b, gdp_vector *x);
+/* Per instance: iterations, status (0 converged, -1 iteration limit), |r0|,
+ * estimated final residual norm, start/end on the device clock in us from
+ * the earliest start; graph_ms: device time of the last run. */
+int gdp_batch_info(const gdp_batch *batch, int *its, int *status, double *res0, double *res,
+                   double *t_begin_us, double *t_end_us, double *graph_ms);
+
 /* ---- learned preconditioner (no counterpart in the reference) --------------
  * A 4-level convolutional U-Net (16-32-64-128 channels, 3x3 kernels, ReLU,
  * 2x2 average pooling, nearest upsampling + skip concatenation, 1x1 output)

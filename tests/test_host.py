@@ -127,3 +127,19 @@ def test_solver_parameters(tmp_path):
     s2 = Solver(str(f))
     assert s2.get("Preconditioner") == 6 and s2.get("Krylov_Size") == 30
     assert s2.get("Stopping_Test") == 1e-12
+
+
+def test_batch_concatenation_is_block_diagonal():
+    """The concatenated formulation numbers each instance's DOFs
+    consecutively and couples no two instances (host pre-pass only)."""
+    from getdp_amd import batch as BT
+    from getdp_amd import linalg as LA
+    probs = [P.poisson2d(n, seed=i, nblocks=4) for i, n in enumerate([5, 9, 7])]
+    pr = BT.concat(probs)
+    fe = LA.FESystem(pr)
+    assert fe.N == sum((n - 1) ** 2 for n in [5, 9, 7])
+    ia, ja = fe.pattern()
+    off = np.cumsum([0] + [(n - 1) ** 2 for n in [5, 9, 7]])
+    inst = np.searchsorted(off, np.arange(fe.N), side="right") - 1
+    rows = np.repeat(np.arange(fe.N), np.diff(ia))
+    assert np.array_equal(inst[rows], inst[ja])
