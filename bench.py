@@ -10,8 +10,10 @@ the diagonally scaled system ("Jacobi", Scaling 1) to 1e-10, FP64.
 
 Prints one JSON line.  `value` = seconds per step (assembly + solve to
 1e-10 relative residual); also reported: assembly elements/s, the SpMV
-roofline (algorithmic bytes / average kernel time vs 8 TB/s), and a bounded
-CPU baseline (the oracle, single-threaded).
+roofline (algorithmic bytes / average kernel time vs 8 TB/s), the learned
+preconditioner against plain ILU(0)-GMRES on an unseen 256^2 field (with the
+device training-step rate), the 512-instance batch (solves/s, latency
+percentiles) and a bounded CPU baseline (the oracle, single-threaded).
 """
 from __future__ import annotations
 
@@ -28,6 +30,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("s to 1e-10 residual (learned vs plain precond); assembly elems/s; SpMV %HBM")
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per SpMV launch on the elasticity3d_n matrix, from two rocprofv3
+# --pmc runs of `bench.py --roofline-only` (FETCH_SIZE x 2.000 + WRITE_SIZE x
+# 1.000, factors from a 512 MiB device copy): 447.9 MB read + 6.5 MB written.
+PMC_SPMV_TRAFFIC = {64: 454_400_000}
 
 
 def log(*a):
@@ -67,6 +73,88 @@ def cpu_baseline(pr, its_gpu: int, cg_sample: int):
                        f"device's {its_gpu} iterations")}
 
 
+def unet_flops_per_pixel() -> float:
+    """Algorithmic flops of one U-Net forward pass per full-resolution pixel:
+    2*9*cin*cout per pixel of each 3x3 convolution (true channel counts; a
+    level-l convolution runs on 4^-l of the pixels) + the 1x1 output layer."""
+    spec = [(2, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2),
+            (64, 128, 3), (128, 128, 3), (192, 64, 2), (64, 64, 2), (96, 32, 1), (32, 32, 1),
+            (48, 16, 0), (16, 16, 0)]
+    return sum(2 * 9 * ci * co / 4 ** lvl for ci, co, lvl in spec) + 2 * 16
+
+
+def learned_section(n: int = 256, seed: int = 90000):
+    """Learned two-stage FGMRES (shipped weights) against plain ILU(0)-GMRES on
+    an unseen n x n checkerboard field, and the device training-step rate."""
+    from getdp_amd import learned as L
+    from getdp_amd import linalg as LA
+    from pathlib import Path
+    w = Path(ROOT) / "getdp_amd" / "weights" / "unet_poisson2d.npz"
+    with np.load(w, allow_pickle=False) as z:
+        params = z["params"]
+    row = L.evaluate(params, n, [seed], log=log)[0]
+    # training-step throughput at batch 32, 128^2 (forward + backward + Adam)
+    net = L.UNet(128, 128, batch=32, seed=0)
+    rng = np.random.default_rng(0)
+    r = rng.standard_normal((32, 128, 128)).astype(np.float32)
+    f = np.zeros_like(r)
+    e = rng.standard_normal((32, 128, 128)).astype(np.float32)
+    e /= np.linalg.norm(e.reshape(32, -1), axis=1)[:, None, None]
+    mask = np.ones((128, 128), np.float32)
+    for _ in range(3):
+        net.train_step(r, f, e, mask, 1e-4)
+    LA.lib().gdp_device_synchronize()
+    t0 = time.perf_counter()
+    k = 20
+    for _ in range(k):
+        net.train_step(r, f, e, mask, 1e-4)
+    LA.lib().gdp_device_synchronize()
+    step = (time.perf_counter() - t0) / k
+    flops = 3 * unet_flops_per_pixel() * 32 * 128 * 128  # forward + input grad + weight grad
+    return {
+        "workload": f"poisson2d_{n} checkerboard seed {seed} (unseen), 1e-10",
+        "learned_ms": min(row["learned_rcmk"]["ms"], row["learned_multicolour"]["ms"]),
+        "learned_its": row["learned_multicolour"]["its"] if row["learned_multicolour"]["ms"] <= row["learned_rcmk"]["ms"] else row["learned_rcmk"]["its"],
+        "plain_rcmk_ms": row["plain_rcmk"]["ms"], "plain_rcmk_its": row["plain_rcmk"]["its"],
+        "plain_multicolour_ms": row["plain_multicolour"]["ms"],
+        "speedup_vs_plain_rcmk": row["speedup_vs_plain_rcmk"],
+        "speedup_vs_best_plain": row["speedup_vs_best_plain"],
+        "rel_res": max(row["learned_rcmk"]["rel_res"], row["learned_multicolour"]["rel_res"]),
+        "train_step_ms_b32_128": round(step * 1e3, 2),
+        "train_tflops": round(flops / step / 1e12, 2),
+        "train_mfma_frac_bf16": round(flops / step / 2.5e15, 4),
+    }
+
+
+def batch_section(count: int = 512):
+    from getdp_amd import batch as BT
+    return BT.bench(count, reps=1, log=log)
+
+
+def roofline_only(n: int, reps: int):
+    """For rocprofv3 --pmc runs: the C1 matrix's SpMV `reps` times, then a
+    plain device copy of 512 MiB (calibration of FETCH_SIZE/WRITE_SIZE)."""
+    import torch
+    from getdp_amd import linalg as LA
+    from getdp_amd import problems as P
+    pr = P.elasticity3d(n)
+    LA.init(0)
+    fe = LA.FESystem(pr)
+    A = LA.Matrix(fe.N)
+    b = LA.Vector(fe.N)
+    fe.generate(A, b)
+    xs = LA.Vector(fe.N, np.random.default_rng(0).standard_normal(fe.N))
+    ys = LA.Vector(fe.N)
+    ms = LA.bench_spmv(A, xs, ys, reps=reps)
+    src = torch.empty(64 * 2 ** 20, dtype=torch.float64, device="cuda:0").uniform_()
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"spmv_ms": ms, "spmv_algorithmic_bytes": spmv_bytes(fe.N, fe.nnz),
+                      "copy_bytes_each_way": src.numel() * 8, "N": fe.N, "nnz": fe.nnz}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,11 +163,19 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="cells per cube side")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cg-sample", type=int, default=60)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain line")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-instance batch line")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the SpMV launches and a calibration copy (for PMC runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.roofline_only:
+        roofline_only(args.n, 20)
+        return
 
     from getdp_amd import linalg as LA
     from getdp_amd import problems as P
@@ -127,7 +223,8 @@ def main():
     sb = spmv_bytes(fe.N, fe.nnz)
     achieved = sb / (spmv_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": PMC_SPMV_TRAFFIC.get(args.n),
                 "kernel": "k_sell_spmv (SELL-64, FP64, one row per lane)", "algorithmic_bytes": sb,
                 "avg_ms": round(spmv_ms, 5)}
 
@@ -162,6 +259,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    if not args.no_learned:
+        out["learned"] = learned_section()
+    if not args.no_batch:
+        out["batch"] = batch_section()
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(pr, out["cg_its"], args.cg_sample)
