@@ -126,6 +126,45 @@ def learned_section(n: int = 256, seed: int = 90000):
     }
 
 
+def c2_section(n: int = 48, its64: int = 1500):
+    """Curl-curl with Whitney edges (1e-6 mass, 1:1000 reluctivity): device
+    assembly, then ILU(0)-GMRES(30) (multicolour ordering) in FP64 for a fixed
+    iteration budget, and the nested mixed-precision solve (FP64 FGMRES around
+    20 FP32 GMRES steps) stopped at the residual the FP64 run reached; reports
+    both times and the difference of the two solutions."""
+    from getdp_amd import linalg as LA
+    from getdp_amd import problems as P
+    pr = P.curlcurl3d(n)
+    fe = LA.FESystem(pr)
+    A = LA.Matrix(fe.N)
+    b = LA.Vector(fe.N)
+    fe.generate(A, b)
+    fe.generate(A, b)
+    asm_ms = fe.timing()["ms_total"]
+    E = int((pr.etype == P.TET).sum())
+    base = dict(Algorithm=8, Preconditioner=6, Krylov_Size=30, ILU_Ordering=1, Orthogonalization=1)
+    s64 = LA.Solver(Nb_Iter_Max=its64, Stopping_Test=1e-10, **base)
+    x64 = LA.Vector(fe.N)
+    i64 = LA.solve(A, b, s64, x64)
+    target = max(1e-10, i64["rel_res"])
+    kw = dict(base)
+    kw.update(Algorithm=9, Precision=2, Inner_Krylov_Size=20)
+    s32 = LA.Solver(Nb_Iter_Max=its64, Stopping_Test=target, **kw)
+    x32 = LA.Vector(fe.N)
+    i32 = LA.solve(A, b, s32, x32)
+    a, c = x64.get(), x32.get()
+    return {
+        "workload": f"curlcurl3d_{n}: {E} tets, {fe.N} edge DOFs, nnz {fe.nnz}",
+        "assembly_ms": round(asm_ms, 3), "assembly_elems_per_s": round(E / (asm_ms * 1e-3), 1),
+        "fp64": {"its": i64["its"], "rel_res": i64["rel_res"], "ms": round(i64["t_solve_ms"] + i64["t_setup_ms"], 1)},
+        "mixed_nested": {"outer_its": i32["its"], "inner_fp32_steps": 20 * i32["its"],
+                         "rel_res": i32["rel_res"], "ms": round(i32["t_solve_ms"] + i32["t_setup_ms"], 1)},
+        "solution_rel_diff": float(np.abs(a - c).max() / max(np.abs(a).max(), 1e-300)),
+        "note": "ILU(0)-GMRES(30) stagnates near 6e-7 on this operator in FP64 as well; both runs are "
+                "compared at the residual the FP64 run reaches",
+    }
+
+
 def batch_section(count: int = 512):
     from getdp_amd import batch as BT
     return BT.bench(count, reps=1, log=log)
@@ -165,6 +204,7 @@ def main():
     ap.add_argument("--cg-sample", type=int, default=60)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain line")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-instance batch line")
+    ap.add_argument("--no-c2", action="store_true", help="skip the curl-curl mixed-precision line")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the SpMV launches and a calibration copy (for PMC runs)")
     args = ap.parse_args()
@@ -263,6 +303,8 @@ def main():
         out["learned"] = learned_section()
     if not args.no_batch:
         out["batch"] = batch_section()
+    if not args.no_c2:
+        out["c2_mixed_precision"] = c2_section()
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(pr, out["cg_its"], args.cg_sample)

@@ -55,6 +55,7 @@ struct Params {
   int Precision = 0, ILU_Ordering = 0, Orthogonalization = 0, ILU_Storage = 0;
   int Refinement_Steps = 20;
   double Inner_Tolerance = 1e-6;
+  int Inner_Krylov_Size = 20;  // Precision 2: FP32 GMRES steps per preconditioner application
 };
 int params_set(Params &p, const char *name, const char *value);
 int params_get(const Params &p, const char *name, double *v);

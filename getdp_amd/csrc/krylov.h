@@ -33,7 +33,8 @@ template <typename TA> struct DevOp {
 };
 
 // Preconditioner applied as z = M^{-1} v (right preconditioning).
-enum PrecKind { PREC_NONE = 0, PREC_ILU = 1, PREC_USER = 2 };
+enum PrecKind { PREC_NONE = 0, PREC_ILU = 1, PREC_USER = 2, PREC_INNER32 = 3 };
+struct InnerSolver;  // FP32 GMRES used as the preconditioner of an FP64 solve
 struct UserPrec;  // learned preconditioner (unet.hip)
 
 // z = M^{-1} v for the learned preconditioner; skipped when the control block
@@ -48,7 +49,25 @@ template <typename TV> struct PrecOp {
   PrecKind kind = PREC_NONE;
   IluFactor *ilu = nullptr;
   UserPrec *user = nullptr;
+  InnerSolver *inner = nullptr;
 };
+
+// Mixed precision by nested Krylov: z = M^{-1} v is k steps of GMRES in
+// FP32 (FP32 SELL values, FP32 vectors, ILU(0) with FP32 factors, CGS2) on
+// A z = v from z = 0, inside an FP64 flexible GMRES whose residuals,
+// Gram-Schmidt and updates stay in FP64.  Device-side only, so the outer
+// restart cycle can still be captured in one graph.
+struct InnerSolver {
+  DevOp<float> A;
+  PrecOp<float> M;
+  int k = 20, n = 0;
+  KCtl *C = nullptr;
+  double *part = nullptr;
+  float *vec = nullptr;  // (k + 4) vectors of n
+  ~InnerSolver();
+  int prepare();
+};
+int inner_apply(InnerSolver *s, const double *v, double *z, KCtl *outer, int gmode);
 
 struct KrylovOptions {
   int algorithm = 8;  // 1 CG, 8 GMRES, 9 FGMRES

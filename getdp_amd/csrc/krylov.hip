@@ -730,6 +730,10 @@ int prec_apply(const PrecOp<TV> &M, const TV *in, TV *out, KCtl *C, int gmode, i
 {
   if(M.kind == PREC_ILU) return M.ilu->template apply<TV>(in, (TV)1, out, (const int *)C, gmode);
   if(M.kind == PREC_USER) return user_prec_apply(M.user, in, out, C, gmode);
+  if(M.kind == PREC_INNER32) {
+    if constexpr(sizeof(TV) == 8) return inner_apply(M.inner, in, out, C, gmode);
+    else return set_error(GDP_E_UNSUPPORTED, "nested FP32 preconditioner inside an FP32 solve");
+  }
   dim3 g(reduce_blocks(n));
   hipLaunchKernelGGL(k_copy_gated<TV>, g, dim3(B), 0, ctx().stream, n, in, out, C, gmode);
   return GDP_OK;
@@ -747,6 +751,105 @@ int prec_apply_two_stage(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *in, 
   hipLaunchKernelGGL(k_rsub_gated<TV>, g, dim3(B), 0, ctx().stream, n, in, t, C, gmode);
   GDP_TRY(M.ilu->template apply<TV>(t, (TV)1, u, (const int *)C, gmode));
   hipLaunchKernelGGL(k_add_gated<TV>, g, dim3(B), 0, ctx().stream, n, out, u, C, gmode);
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+template <int OP> void fin(KCtl *C, const double *part, int nparts, int idx)
+{
+  hipLaunchKernelGGL(k_fin<OP>, dim3(1), dim3(B), 0, ctx().stream, C, part, nparts, idx);
+}
+
+// One restart cycle of Sparskit's gmres/fgmres (iters.f:1912-2217,
+// 2522-2807) with every decision taken on the device (control block C):
+// residual, m Arnoldi steps (preconditioning, SpMV, Gram-Schmidt, Givens),
+// back-substitution, update, restart/stop test.  Enqueue only.
+template <typename TA, typename TV> struct GmArgs {
+  DevOp<TA> A;
+  const PrecOp<TV> *Mp;
+  const TV *rhs;
+  TV *sol, *V, *Z;
+  KCtl *C;
+  double *part;
+  int n, m, orth;
+  bool flexible, two_stage;
+};
+
+template <typename TA, typename TV> int gm_cycle(const GmArgs<TA, TV> &ga)
+{
+  const DevOp<TA> &A = ga.A;
+  const PrecOp<TV> &M = *ga.Mp;
+  const TV *rhs = ga.rhs;
+  TV *sol = ga.sol, *V = ga.V, *Z = ga.Z, *tmpz = ga.Z;
+  KCtl *C = ga.C;
+  double *part = ga.part;
+  const int n = ga.n, m = ga.m;
+  const bool flexible = ga.flexible, two_stage = ga.two_stage, rp = M.kind != PREC_NONE;
+  hipStream_t s = ctx().stream;
+  const int nb = reduce_blocks(n);
+  const int nrow = std::max(1, (n + 255) / 256);
+  const int ncgs = (int)std::max(1L, ((long)n + B * CGS_PER - 1) / (B * CGS_PER));
+  dim3 g(nb), blk(B), grow(nrow);
+  GDP_TRY(op_apply(A, sol, V, C, 1));
+  hipLaunchKernelGGL(k_gm_resid<TV>, g, blk, 0, s, n, rhs, V, C, part);
+  fin<FIN_GM_RESID>(C, part, nb, 0);
+  hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, V, C, 1);
+  for(int k = 1; k <= m; k++) {
+    TV *vk1 = V + (size_t)(k - 1) * n, *vk = V + (size_t)k * n;
+    const TV *src = vk1;
+    if(two_stage) {
+      TV *zk = Z + (size_t)(k - 1) * n;
+      TV *t1 = Z + (size_t)m * n, *t2 = t1 + n;
+      GDP_TRY((prec_apply_two_stage<TA, TV>(A, M, vk1, zk, t1, t2, C, 2, n)));
+      src = zk;
+    }
+    else if(flexible) {
+      TV *zk = Z + (size_t)(k - 1) * n;
+      GDP_TRY(prec_apply(M, vk1, zk, C, 2, n));
+      src = zk;
+    }
+    else if(rp) {
+      GDP_TRY(prec_apply(M, vk1, tmpz, C, 2, n));
+      src = tmpz;
+    }
+    hipLaunchKernelGGL((k_op_dot<TA, TV, false>), grow, dim3(256), 0, s, n, A.sp, A.w, A.col,
+                       A.val, src, (const TV *)nullptr, vk, C, 2, part);
+    fin<FIN_NRM0>(C, part, nrow, 0);
+    if(ga.orth == 1) {
+      dim3 gc2(ncgs);
+      hipLaunchKernelGGL((k_cgs<TV, 1>), gc2, blk, 0, s, n, k, vk, V, C, part);
+      fin<FIN_CGS1>(C, part, ncgs, k);
+      hipLaunchKernelGGL((k_cgs<TV, 2>), gc2, blk, 0, s, n, k, vk, V, C, part);
+      fin<FIN_CGS2>(C, part, ncgs, k);
+      hipLaunchKernelGGL((k_cgs<TV, 3>), gc2, blk, 0, s, n, k, vk, V, C, part);
+      fin<FIN_CGS3>(C, part, ncgs, k);
+    }
+    else {
+      for(int i = 0; i < k; i++) {
+        hipLaunchKernelGGL(k_mgs_dot<TV>, g, blk, 0, s, n, i, vk, V, C, part);
+        fin<FIN_MGS_DOT>(C, part, nb, i);
+        hipLaunchKernelGGL(k_mgs_axpy<TV>, g, blk, 0, s, n, i, vk, V, C, part);
+        fin<FIN_MGS_AXPY>(C, part, nb, i);
+      }
+      hipLaunchKernelGGL(k_mgs_norm<TV>, g, blk, 0, s, n, k, vk, V, C, part);
+      fin<FIN_MGS_NORM>(C, part, nb, k);
+    }
+    hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, vk, C, 2);
+    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, s, k, C);
+  }
+  hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(1), 0, s, C);
+  if(flexible)
+    hipLaunchKernelGGL(k_fgm_update<TV>, g, blk, 0, s, n, sol, Z, C);
+  else {
+    hipLaunchKernelGGL(k_gm_combine<TV>, g, blk, 0, s, n, V, C);
+    if(rp) {
+      GDP_TRY(prec_apply(M, V, tmpz, C, 1, n));
+      hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, tmpz, C);
+    }
+    else
+      hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, V, C);
+  }
+  hipLaunchKernelGGL(k_gm_cycle_end, dim3(1), dim3(1), 0, s, C);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
@@ -806,10 +909,6 @@ int read_ctl(KCtl *d, int *done, int *status, int *its)
   return GDP_OK;
 }
 
-template <int OP> void fin(KCtl *C, const double *part, int nparts, int idx)
-{
-  hipLaunchKernelGGL(k_fin<OP>, dim3(1), dim3(B), 0, ctx().stream, C, part, nparts, idx);
-}
 
 }  // namespace
 
@@ -909,73 +1008,9 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
   // ---- (F)GMRES ----
   TV *V = base;                        // m+1 basis vectors
   TV *Z = base + (size_t)(m + 1) * n;  // FGMRES: m vectors; GMRES: 1 temp
-  TV *tmpz = Z;
-  const bool rp = M.kind != PREC_NONE;
   GraphCache gc;
-  auto cycle = [&]() -> int {
-    GDP_TRY(op_apply(A, sol, V, C, 1));
-    hipLaunchKernelGGL(k_gm_resid<TV>, g, blk, 0, s, n, rhs, V, C, part);
-    fin<FIN_GM_RESID>(C, part, nb, 0);
-    hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, V, C, 1);
-    for(int k = 1; k <= m; k++) {
-      TV *vk1 = V + (size_t)(k - 1) * n, *vk = V + (size_t)k * n;
-      const TV *src = vk1;
-      if(two_stage) {
-        TV *zk = Z + (size_t)(k - 1) * n;
-        TV *t1 = Z + (size_t)m * n, *t2 = t1 + n;
-        GDP_TRY((prec_apply_two_stage<TA, TV>(A, M, vk1, zk, t1, t2, C, 2, n)));
-        src = zk;
-      }
-      else if(flexible) {
-        TV *zk = Z + (size_t)(k - 1) * n;
-        GDP_TRY(prec_apply(M, vk1, zk, C, 2, n));
-        src = zk;
-      }
-      else if(rp) {
-        GDP_TRY(prec_apply(M, vk1, tmpz, C, 2, n));
-        src = tmpz;
-      }
-      hipLaunchKernelGGL((k_op_dot<TA, TV, false>), grow, dim3(256), 0, s, n, A.sp, A.w, A.col,
-                         A.val, src, (const TV *)nullptr, vk, C, 2, part);
-      fin<FIN_NRM0>(C, part, nrow, 0);
-      if(opt.orth == 1) {
-        dim3 gc2(ncgs);
-        hipLaunchKernelGGL((k_cgs<TV, 1>), gc2, blk, 0, s, n, k, vk, V, C, part);
-        fin<FIN_CGS1>(C, part, ncgs, k);
-        hipLaunchKernelGGL((k_cgs<TV, 2>), gc2, blk, 0, s, n, k, vk, V, C, part);
-        fin<FIN_CGS2>(C, part, ncgs, k);
-        hipLaunchKernelGGL((k_cgs<TV, 3>), gc2, blk, 0, s, n, k, vk, V, C, part);
-        fin<FIN_CGS3>(C, part, ncgs, k);
-      }
-      else {
-        for(int i = 0; i < k; i++) {
-          hipLaunchKernelGGL(k_mgs_dot<TV>, g, blk, 0, s, n, i, vk, V, C, part);
-          fin<FIN_MGS_DOT>(C, part, nb, i);
-          hipLaunchKernelGGL(k_mgs_axpy<TV>, g, blk, 0, s, n, i, vk, V, C, part);
-          fin<FIN_MGS_AXPY>(C, part, nb, i);
-        }
-        hipLaunchKernelGGL(k_mgs_norm<TV>, g, blk, 0, s, n, k, vk, V, C, part);
-        fin<FIN_MGS_NORM>(C, part, nb, k);
-      }
-      hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, vk, C, 2);
-      hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, s, k, C);
-    }
-    hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(1), 0, s, C);
-    if(flexible)
-      hipLaunchKernelGGL(k_fgm_update<TV>, g, blk, 0, s, n, sol, Z, C);
-    else {
-      hipLaunchKernelGGL(k_gm_combine<TV>, g, blk, 0, s, n, V, C);
-      if(rp) {
-        GDP_TRY(prec_apply(M, V, tmpz, C, 1, n));
-        hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, tmpz, C);
-      }
-      else
-        hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, V, C);
-    }
-    hipLaunchKernelGGL(k_gm_cycle_end, dim3(1), dim3(1), 0, s, C);
-    GDP_HIP(hipGetLastError());
-    return GDP_OK;
-  };
+  GmArgs<TA, TV> ga{A, &M, rhs, sol, V, Z, C, part, n, m, opt.orth, flexible, two_stage};
+  auto cycle = [&]() -> int { return gm_cycle<TA, TV>(ga); };
   for(;;) {
     GDP_TRY(run_maybe_graph(opt.use_graphs, gc, cycle));
     GDP_TRY(read_ctl(C, &done, &status, &its));
@@ -988,6 +1023,80 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
   res.status = tmp.status;
   res.res0 = tmp.fpar3;
   res.res_est = tmp.fpar6;
+  return GDP_OK;
+}
+
+// ---- nested FP32 GMRES as a preconditioner ------------------------------------------------
+
+namespace {
+
+// Fresh control block for one inner solve: k steps, no tolerance stop; when
+// the outer solve has already stopped (gate) the whole inner solve is skipped.
+__global__ void __launch_bounds__(256) k_inner_begin(KCtl *Ci, const KCtl *outer, int gmode, int k)
+{
+  double *d = (double *)Ci;
+  const int nd = (int)(sizeof(KCtl) / sizeof(double));
+  for(int i = threadIdx.x; i < nd; i += 256) d[i] = 0.;
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    Ci->done = gated(outer, gmode) ? 1 : 0;
+    Ci->maxits = k;
+    Ci->m = k;
+    Ci->rtol = 0.;
+  }
+}
+
+__global__ void __launch_bounds__(B) k_d2f_gated(int n, const double *v, float *r, float *x, const KCtl *outer,
+                                                 int gmode)
+{
+  if(gated(outer, gmode)) return;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) {
+    r[j] = (float)v[j];
+    x[j] = 0.f;
+  }
+}
+
+__global__ void __launch_bounds__(B) k_f2d_gated(int n, const float *x, double *z, const KCtl *outer, int gmode)
+{
+  if(gated(outer, gmode)) return;
+  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) z[j] = (double)x[j];
+}
+
+}  // namespace
+
+InnerSolver::~InnerSolver()
+{
+  dfree(C);
+  dfree(part);
+  dfree(vec);
+}
+
+int InnerSolver::prepare()
+{
+  n = A.n;
+  k = std::max(2, std::min(k, kMaxKrylov));
+  if(!C) GDP_TRY(dalloc(&C, 1));
+  if(!part) {
+    const size_t rows = (size_t)std::max(1024L, ((long)n + 255) / 256);
+    GDP_TRY(dalloc(&part, rows * (kMaxKrylov + 2)));
+  }
+  if(!vec) GDP_TRY(dalloc(&vec, (size_t)(k + 4) * std::max(1, n)));
+  return GDP_OK;
+}
+
+int inner_apply(InnerSolver *S, const double *v, double *z, KCtl *outer, int gmode)
+{
+  if(!S || !S->vec) return set_error(GDP_E_STATE, "inner FP32 solver not prepared");
+  hipStream_t s = ctx().stream;
+  const int n = S->n, k = S->k;
+  float *rhs = S->vec, *x = rhs + n, *V = x + n;  // V: k + 1 basis vectors, then one temporary
+  hipLaunchKernelGGL(k_inner_begin, dim3(1), dim3(256), 0, s, S->C, outer, gmode, k);
+  dim3 g(reduce_blocks(n));
+  hipLaunchKernelGGL(k_d2f_gated, g, dim3(B), 0, s, n, v, rhs, x, outer, gmode);
+  GmArgs<float, float> ga{S->A, &S->M, rhs, x, V, V + (size_t)(k + 1) * n, S->C, S->part, n, k, 1, false, false};
+  GDP_TRY((gm_cycle<float, float>(ga)));
+  hipLaunchKernelGGL(k_f2d_gated, g, dim3(B), 0, s, n, x, z, outer, gmode);
+  GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
 

@@ -161,6 +161,7 @@ struct SolveCache {
   double *rhs = nullptr, *sol = nullptr, *work = nullptr;
   float *rhs32 = nullptr, *sol32 = nullptr;
   IluFactor ilu;
+  std::unique_ptr<InnerSolver> inner;  // Precision 2
   int ilu_row = 0;
   bool have_factor = false;
 
@@ -323,7 +324,7 @@ int prepare(gdp_solver *S, const gdp_matrix *A, bool reuse_factor)
     c.scaling = p.Scaling;
     c.have_factor = false;
   }
-  if(p.Precision == 1 && !c.sell.val32) GDP_TRY(sell_copy_values(c.Bm, c.sell, true));
+  if(p.Precision >= 1 && !c.sell.val32) GDP_TRY(sell_copy_values(c.Bm, c.sell, true));
   // factorisation
   c.precond = prec;
   if(prec == 6) {
@@ -381,7 +382,46 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
   KrylovResult kr;
   UserPrec *up = S->user;
   if(up) user_prec_bind(up, c.d_perm, c.d_rperm, c.rs);
-  if(p.Precision == 0) {
+  if(p.Precision == 2) {
+    // FP64 flexible GMRES (residuals, Gram-Schmidt, updates) preconditioned
+    // by Inner_Krylov_Size steps of FP32 GMRES (FP32 SpMV + FP32 ILU(0))
+    DevOp<double> op;
+    op.n = n;
+    op.sp = c.sell.sp;
+    op.w = c.sell.w;
+    op.col = c.sell.col;
+    op.val = c.sell.val;
+    if(!c.inner) c.inner.reset(new InnerSolver);
+    InnerSolver &in = *c.inner;
+    in.A.n = n;
+    in.A.sp = c.sell.sp;
+    in.A.w = c.sell.w;
+    in.A.col = c.sell.col;
+    in.A.val = c.sell.val32;
+    in.M = PrecOp<float>();
+    if(p.Preconditioner == 6) {
+      in.M.kind = PREC_ILU;
+      in.M.ilu = &c.ilu;
+    }
+    if(in.k != p.Inner_Krylov_Size) {
+      dfree(in.vec);
+      in.k = p.Inner_Krylov_Size;
+    }
+    GDP_TRY(in.prepare());
+    PrecOp<double> M;
+    M.kind = PREC_INNER32;
+    M.inner = &in;
+    KrylovOptions o2 = opt;
+    o2.algorithm = 9;
+    GDP_TRY((krylov_solve<double, double>(op, M, c.rhs, c.sol, o2, kr)));
+    if(p.Preconditioner == 6) GDP_TRY(c.ilu.check_timeout());
+    info.its = kr.its;
+    info.status = kr.status;
+    info.res0 = kr.res0;
+    info.res_est = kr.res_est;
+    info.outer_its = kr.its;
+  }
+  else if(p.Precision == 0) {
     DevOp<double> op;
     op.n = n;
     op.sp = c.sell.sp;

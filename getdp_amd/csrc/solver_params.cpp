@@ -46,6 +46,7 @@ const Entry kTable[] = {
   {"ILU_Storage", INT, &Params::ILU_Storage, nullptr},
   {"Refinement_Steps", INT, &Params::Refinement_Steps, nullptr},
   {"Inner_Tolerance", REAL, nullptr, &Params::Inner_Tolerance},
+  {"Inner_Krylov_Size", INT, &Params::Inner_Krylov_Size, nullptr},
 };
 
 const Entry *find(const char *name)

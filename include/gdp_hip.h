@@ -62,7 +62,11 @@ int gdp_solver_create(const char *solver_par_file, gdp_solver **s);
  *                        re-orthogonalisation (iters.f:3475, default),
  *                        1 classical Gram-Schmidt applied twice
  *   Refinement_Steps    max outer steps for Precision 1 (default 20)
- *   Inner_Tolerance     inner relative tolerance for Precision 1 (1e-6)   */
+ *   Inner_Tolerance     inner relative tolerance for Precision 1 (1e-6)
+ *   Precision 2         FP64 flexible GMRES (residuals, Gram-Schmidt,
+ *                        updates) preconditioned by Inner_Krylov_Size (20)
+ *                        steps of FP32 GMRES with FP32 SpMV and the FP32
+ *                        ILU(0) of Preconditioner 6; its = outer steps   */
 int gdp_solver_set(gdp_solver *s, const char *name, const char *value);
 int gdp_solver_get(const gdp_solver *s, const char *name, double *value);
 int gdp_solver_destroy(gdp_solver *s); /* LinAlg_DestroySolver */

@@ -104,8 +104,11 @@ def test_trained_network_beats_untrained_and_ilu(gpu):
     res = {}
     for name, net in (("trained", trained), ("untrained", untrained)):
         x = gpu.Vector(fe.N)
-        out = L.solve_learned(A, b, x, net, maps, rtol=1e-10, maxits=1500, fallback=True)
+        out = L.solve_learned(A, b, x, net, maps, rtol=1e-10, maxits=1500, fallback=True,
+                              two_stage=True)
         assert out["rel_res"] <= 1e-10
         res[name] = out["learned"]["its"] if out["fallback"] is None else 10 ** 9
+        print(f"{name}: {res[name]} FGMRES iterations (plain ILU(0)-GMRES: {ip['its']})")
+    # the ablation: the same two-stage method with an untrained network
     assert res["trained"] < res["untrained"], res
-    assert res["trained"] < ip["its"], (res, ip["its"])
+    assert 2 * res["trained"] < ip["its"], (res, ip["its"])
