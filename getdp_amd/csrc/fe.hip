@@ -52,6 +52,11 @@ struct DevPlan {
   const int *row_space, *row_ent;
   const int *ia, *ja;
   DevTerm term[kMaxTerms];
+  int nelem;
+  // element pass output: elements in chunks of 64 (one wavefront); value i of
+  // element e sits at ek_cb[e / 64] + 64 * i + e % 64, so that a wavefront's
+  // stores of value i are contiguous
+  const long *ek_cb;
 };
 
 // ---- reference element data ---------------------------------------------------
@@ -283,68 +288,48 @@ __global__ void __launch_bounds__(256) k_geometry(int nelem, const int *__restri
   for(int k = 0; k < 9; k++) G[10 + k] = I[k];
 }
 
-__device__ inline int find_slot(const int *__restrict__ ja, int lo, int hi, int col)
-{
-  while(lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if(ja[mid] < col) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
 
 // Add one Ek entry of the row (Dof_AssembleInMat, DofData.cpp:1464-1500):
 // unknown column -> matrix slot; fixed column -> b -= value * Ek (Val != 0).
-__device__ inline void scatter(const DevPlan &P, int dsp, int dofent, double val,
-                               double *__restrict__ a, int rs, int re, double &bacc)
-{
-  int c = P.dofnum[dsp][dofent];
-  if(c > 0) {
-    int s = find_slot(P.ja, rs, re, c - 1);
-    a[s] += val;
-  }
-  else if(c < 0 && val != 0.) {
-    bacc += (P.fixval[dsp][-c - 1] * 1.) * (-val);
-  }
-}
 
-// ---- kernel 2, nodal spaces: one thread per unknown row -----------------------
-__global__ void __launch_bounds__(128) k_gather_nodal(DevPlan P, double *__restrict__ a,
-                                                      double *__restrict__ b)
+// ---- two-phase assembly ---------------------------------------------------------
+// Pass 1, one thread per element: every active term's element-matrix block
+// Ek[k][j] (k = local row, j = local column; source terms Ek[k]), values
+// numbered in term order and stored with the chunked layout of ek_cb.  Each value is formed with the
+// reference's operation order (Cal_GalerkinTermOfFemEquation.cpp:531-1014:
+// quadrature sum of (w * |detJ|) * product, the symmetric terms evaluated
+// for j <= i and mirrored, 838-841 and 921-924), compiled without FP
+// contraction, so it is bit-identical.
+// Pass 2, one thread per CSR entry / per row: the entry's contributions
+// summed in (element, term) order, the reference's order of additions
+// (Dof_AssembleInMat called element after element, term after term).
+__global__ void __launch_bounds__(128) k_elem_nodal(DevPlan P, double *__restrict__ ek)
 {
-  int r = blockIdx.x * 128 + threadIdx.x;
-  if(r >= P.N) return;
-  const int s = P.row_space[r], ent = P.row_ent[r];
-  const int rs = P.ia[r], re = P.ia[r + 1];
-  double bacc = 0.;
-  for(int p = P.inc_ptr[ent]; p < P.inc_ptr[ent + 1]; p++) {
-    const int item = P.inc_item[p];
-    const int e = item >> 3, k = item & 7;
-    const int reg = P.region[e];
-    const int gk = P.geokind[e];
-    const int tri = (gk != GEO_TET);
-    const int nn = tri ? 3 : 4;
-    const int *cn = P.conn + (long)e * 4;
-    const double *G = P.geo + (long)e * kGeoStride;
-    const double adet = fabs(G[9]);
-    // physical gradients of the element's nodal functions (FORM1 change of
-    // coordinates of BF_GradNode; constant on P1 elements)
-    double g[4][3];
-    if(gk != GEO_TRISUR)
-      for(int j = 0; j < nn; j++) {
-        double u[3];
-        bf_gradnode(tri, j, u);
-        form1(G + 10, u, g[j]);
-      }
-    for(int ti = 0; ti < P.nterms; ti++) {
-      const DevTerm &t = P.term[ti];
-      if(t.equ != s || !t.active[reg]) continue;
-      const double *cf = t.coef + (long)reg * t.ncoef;
-      if(t.kind == GDP_T_GRAD_GRAD) {
-        const int sym = (t.dof == t.equ);
+  const int e = blockIdx.x * 128 + threadIdx.x;
+  if(e >= P.nelem) return;
+  const int gk = P.geokind[e];
+  if(gk == GEO_NONE) return;
+  const int reg = P.region[e];
+  const int tri = (gk != GEO_TET);
+  const int nn = tri ? 3 : 4;
+  const double *G = P.geo + (long)e * kGeoStride;
+  const double adet = fabs(G[9]);
+  double g[4][3];
+  if(gk != GEO_TRISUR)
+    for(int j = 0; j < nn; j++) {
+      double u[3];
+      bf_gradnode(tri, j, u);
+      form1(G + 10, u, g[j]);
+    }
+  double *out = ek + P.ek_cb[e >> 6] + (e & 63);
+  for(int ti = 0; ti < P.nterms; ti++) {
+    const DevTerm &t = P.term[ti];
+    if(!t.active[reg]) continue;
+    const double *cf = t.coef + (long)reg * t.ncoef;
+    if(t.kind == GDP_T_GRAD_GRAD) {
+      const int sym = (t.dof == t.equ);
+      for(int k = 0; k < nn; k++)
         for(int j = 0; j < nn; j++) {
-          // symmetric terms compute Ek[i][j] for j <= i and mirror it
-          // (Cal_GalerkinTermOfFemEquation.cpp:838-841, 921-924)
           int ii = k, jj = j;
           if(sym && j > k) {
             ii = j;
@@ -359,10 +344,12 @@ __global__ void __launch_bounds__(128) k_gather_nodal(DevPlan P, double *__restr
             gauss_pt(tri, t.npoints, q, u, v, w, wt);
             acc += (wt * adet) * pr;
           }
-          scatter(P, t.dof, cn[j], acc, a, rs, re, bacc);
+          out[64 * (k * nn + j)] = acc;
         }
-      }
-      else if(t.kind == GDP_T_NODE_MASS) {
+      out += 64 * nn * nn;
+    }
+    else if(t.kind == GDP_T_NODE_MASS) {
+      for(int k = 0; k < nn; k++)
         for(int j = 0; j < nn; j++) {
           int ii = k, jj = j;
           if(j > k) {
@@ -376,46 +363,41 @@ __global__ void __launch_bounds__(128) k_gather_nodal(DevPlan P, double *__restr
             double pi = bf_node(tri, ii, u, v, w), pj = bf_node(tri, jj, u, v, w);
             acc += (wt * adet) * (pi * (cf[0] * pj));
           }
-          scatter(P, t.dof, cn[j], acc, a, rs, re, bacc);
+          out[64 * (k * nn + j)] = acc;
         }
-      }
-      else if(t.kind == GDP_T_SRC_NODE) {
-        // [ f, {u} ]: Ek[i][0] with the fixed pseudo-DOF of value 1
-        // (Dof_InitDofForNoDof, DofData.cpp:1970-1981)
+      out += 64 * nn * nn;
+    }
+    else if(t.kind == GDP_T_SRC_NODE) {
+      for(int k = 0; k < nn; k++) {
         double acc = 0.;
         for(int q = 0; q < t.npoints; q++) {
           double u, v, w, wt;
           gauss_pt(tri, t.npoints, q, u, v, w, wt);
           acc += (wt * adet) * (bf_node(tri, k, u, v, w) * cf[0]);
         }
-        if(acc != 0.) bacc += (1. * 1.) * (-acc);
+        out[64 * k] = acc;
       }
+      out += 64 * nn;
     }
   }
-  b[r] = bacc;
 }
 
-// ---- kernel 2, Whitney edge space on tetrahedra: one thread per row -------------
-__global__ void __launch_bounds__(128) k_gather_edge(DevPlan P, double *__restrict__ a,
-                                                     double *__restrict__ b)
+__global__ void __launch_bounds__(128) k_elem_edge(DevPlan P, double *__restrict__ ek)
 {
-  int r = blockIdx.x * 128 + threadIdx.x;
-  if(r >= P.N) return;
-  const int ent = P.row_ent[r];
-  const int rs = P.ia[r], re = P.ia[r + 1];
-  double bacc = 0.;
-  for(int p = P.inc_ptr[ent]; p < P.inc_ptr[ent + 1]; p++) {
-    const int item = P.inc_item[p];
-    const int e = item >> 3, k = item & 7;
-    const int reg = P.region[e];
-    const int *ed = P.edges + (long)e * 6;
-    const double *G = P.geo + (long)e * kGeoStride;
-    const double det = G[9], adet = fabs(det);
-    for(int ti = 0; ti < P.nterms; ti++) {
-      const DevTerm &t = P.term[ti];
-      if(!t.active[reg]) continue;
-      const double *cf = t.coef + (long)reg * t.ncoef;
-      if(t.kind == GDP_T_CURL_CURL) {
+  const int e = blockIdx.x * 128 + threadIdx.x;
+  if(e >= P.nelem) return;
+  if(P.geokind[e] == GEO_NONE) return;
+  const int reg = P.region[e];
+  const int *ed = P.edges + (long)e * 6;
+  const double *G = P.geo + (long)e * kGeoStride;
+  const double det = G[9], adet = fabs(det);
+  double *out = ek + P.ek_cb[e >> 6] + (e & 63);
+  for(int ti = 0; ti < P.nterms; ti++) {
+    const DevTerm &t = P.term[ti];
+    if(!t.active[reg]) continue;
+    const double *cf = t.coef + (long)reg * t.ncoef;
+    if(t.kind == GDP_T_CURL_CURL) {
+      for(int k = 0; k < 6; k++)
         for(int j = 0; j < 6; j++) {
           int ii = k, jj = j;
           if(j > k) {
@@ -435,10 +417,12 @@ __global__ void __launch_bounds__(128) k_gather_edge(DevPlan P, double *__restri
             gauss_pt(0, t.npoints, q, u, v, w, wt);
             acc += (wt * adet) * pr;
           }
-          scatter(P, 0, ed[j] < 0 ? -ed[j] : ed[j], acc, a, rs, re, bacc);
+          out[64 * (k * 6 + j)] = acc;
         }
-      }
-      else if(t.kind == GDP_T_EDGE_MASS) {
+      out += 64 * 36;
+    }
+    else if(t.kind == GDP_T_EDGE_MASS) {
+      for(int k = 0; k < 6; k++)
         for(int j = 0; j < 6; j++) {
           int ii = k, jj = j;
           if(j > k) {
@@ -457,10 +441,12 @@ __global__ void __launch_bounds__(128) k_gather_edge(DevPlan P, double *__restri
             coef_times(t.ncoef, cf, xj, cx);
             acc += (wt * adet) * dot3(xi, cx);
           }
-          scatter(P, 0, ed[j] < 0 ? -ed[j] : ed[j], acc, a, rs, re, bacc);
+          out[64 * (k * 6 + j)] = acc;
         }
-      }
-      else if(t.kind == GDP_T_SRC_EDGE) {
+      out += 64 * 36;
+    }
+    else if(t.kind == GDP_T_SRC_EDGE) {
+      for(int k = 0; k < 6; k++) {
         double acc = 0.;
         for(int q = 0; q < t.npoints; q++) {
           double u, v, w, wt;
@@ -470,9 +456,39 @@ __global__ void __launch_bounds__(128) k_gather_edge(DevPlan P, double *__restri
           form1(G + 10, si, xi);
           acc += (wt * adet) * dot3(xi, cf);
         }
-        if(acc != 0.) bacc += (1. * 1.) * (-acc);
+        out[64 * k] = acc;
       }
+      out += 64 * 6;
     }
+  }
+}
+
+// a[s] = sum of the entry's contributions in order (the reference adds each
+// Ek value to a zeroed matrix: 0 + v1 + v2 + ...)
+__global__ void __launch_bounds__(256) k_slot_sum(long nnz, const int *__restrict__ cptr,
+                                                  const int *__restrict__ cidx,
+                                                  const double *__restrict__ ek, double *__restrict__ a)
+{
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  if(s >= nnz) return;
+  double acc = 0.;
+  for(int k = cptr[s]; k < cptr[s + 1]; k++) acc += ek[cidx[k]];
+  a[s] = acc;
+}
+
+// b[r]: fixed-DOF columns and source terms, b -= (value * 1) * Ek
+// (Dof_AssembleInMat unknown x fixed branch, DofData.cpp:1489-1500)
+__global__ void __launch_bounds__(256) k_rhs_sum(int N, const int *__restrict__ rptr,
+                                                 const int *__restrict__ ridx,
+                                                 const double *__restrict__ rfv,
+                                                 const double *__restrict__ ek, double *__restrict__ b)
+{
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if(r >= N) return;
+  double bacc = 0.;
+  for(int k = rptr[r]; k < rptr[r + 1]; k++) {
+    const double v = ek[ridx[k]];
+    if(v != 0.) bacc += (rfv[k] * 1.) * (-v);
   }
   b[r] = bacc;
 }
@@ -514,6 +530,11 @@ struct gdp_fe {
   double *d_coef[kMaxTerms] = {};
   unsigned char *d_active[kMaxTerms] = {};
   DevPlan plan{};
+  // two-phase assembly tables (built on the first Generate)
+  long ek_total = 0;
+  long *d_ek_cb = nullptr;
+  double *d_ek = nullptr, *d_rfv = nullptr;
+  int *d_cptr = nullptr, *d_cidx = nullptr, *d_rptr = nullptr, *d_ridx = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   double ms_total = 0, ms_geo = 0, ms_gather = 0;
 
@@ -540,6 +561,13 @@ struct gdp_fe {
       dfree(d_coef[t]);
       dfree(d_active[t]);
     }
+    dfree(d_ek_cb);
+    dfree(d_ek);
+    dfree(d_rfv);
+    dfree(d_cptr);
+    dfree(d_cidx);
+    dfree(d_rptr);
+    dfree(d_ridx);
     for(auto &e : ev)
       if(e) (void)hipEventDestroy(e);
   }
@@ -810,6 +838,114 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
   return GDP_OK;
 }
 
+// Tables of the two-phase assembly.  Element values: for each element, its
+// active terms in order, nb x nb values (bilinear terms) or nb values
+// (sources), in the chunked layout (value i of element e at
+// cb[e / 64] + 64 i + e % 64).  For each CSR entry, the block values that add into it, in
+// (element, term) order; for each row, the values that add into b (fixed
+// columns with their fixed value, sources with 1), in (element, term, column)
+// order -- the order of the reference's element loop.
+static int fe_build_two_phase(gdp_fe *F)
+{
+  const int E = F->nelem, N = F->N, nterms = F->nterms;
+  const bool edge = F->family == FAM_EDGE;
+  auto nb_of = [&](int e) { return edge ? 6 : nodes_of(F->h_etype[e]); };
+  auto bilinear = [](int kind) {
+    return kind == GDP_T_GRAD_GRAD || kind == GDP_T_NODE_MASS || kind == GDP_T_CURL_CURL ||
+           kind == GDP_T_EDGE_MASS;
+  };
+  // values per element, then chunk bases: a chunk of 64 elements is as wide
+  // as its widest element
+  const int nch = (E + 63) / 64;
+  std::vector<long> cb((size_t)nch + 1, 0);
+  for(int c = 0; c < nch; c++) {
+    long w = 0;
+    for(int e = c * 64; e < std::min(E, c * 64 + 64); e++) {
+      long sz = 0;
+      if(F->h_geokind[e] != GEO_NONE) {
+        const int nb = nb_of(e), reg = F->h_region[e];
+        for(int ti = 0; ti < nterms; ti++)
+          if(F->h_active[ti][reg]) sz += bilinear(F->h_terms[ti].kind) ? nb * nb : nb;
+      }
+      w = std::max(w, sz);
+    }
+    cb[c + 1] = cb[c] + 64 * w;
+  }
+  F->ek_total = cb[nch];
+  if(cb[nch] > 2147483647L) return set_error(GDP_E_UNSUPPORTED, "element blocks exceed int32 indexing");
+  auto vidx = [&](int e, long i) { return (int)(cb[e >> 6] + 64 * i + (e & 63)); };
+  const int *ia = F->h_ia.data(), *ja = F->h_ja.data();
+  // visit(r, fn_slot, fn_rhs): the row's contributions in order
+  auto visit = [&](int r, auto &&on_slot, auto &&on_rhs) {
+    const int s = F->h_row_space[r], ent = F->h_row_ent[r];
+    const int rs = ia[r], re = ia[r + 1];
+    for(int p = F->h_inc_ptr[ent]; p < F->h_inc_ptr[ent + 1]; p++) {
+      const int item = F->h_inc_item[p];
+      const int e = item >> 3, k = item & 7;
+      const int reg = F->h_region[e], nb = nb_of(e);
+      long o = 0;
+      for(int ti = 0; ti < nterms; ti++) {
+        if(!F->h_active[ti][reg]) continue;
+        const gdp_term_desc &t = F->h_terms[ti];
+        const bool bil = bilinear(t.kind);
+        if(edge || t.equ == s) {
+          if(bil)
+            for(int j = 0; j < nb; j++) {
+              const int ej = edge ? std::abs(F->h_edges[(size_t)e * 6 + j]) : F->h_conn[(size_t)e * 4 + j];
+              const int c = F->h_dofnum[t.dof][ej];
+              if(c > 0) {
+                const int slot = (int)(std::lower_bound(ja + rs, ja + re, c - 1) - ja);
+                on_slot(slot, vidx(e, o + k * nb + j));
+              }
+              else if(c < 0)
+                on_rhs(vidx(e, o + k * nb + j), F->h_fixval[t.dof][-c - 1]);
+            }
+          else
+            on_rhs(vidx(e, o + k), 1.0);
+        }
+        o += bil ? nb * nb : nb;
+      }
+    }
+  };
+  const long nnz = F->nnz;
+  std::vector<int> cptr((size_t)nnz + 1, 0), rptr((size_t)N + 1, 0);
+  parallel_for(N, [&](int lo, int hi) {
+    for(int r = lo; r < hi; r++)
+      visit(r, [&](int slot, int) { cptr[(size_t)slot + 1]++; }, [&](int, double) { rptr[(size_t)r + 1]++; });
+  });
+  for(long i = 0; i < nnz; i++) cptr[i + 1] += cptr[i];
+  for(int i = 0; i < N; i++) rptr[i + 1] += rptr[i];
+  std::vector<int> cidx((size_t)std::max(1, cptr[nnz])), ridx((size_t)std::max(1, rptr[N]));
+  std::vector<double> rfv((size_t)std::max(1, rptr[N]));
+  parallel_for(N, [&](int lo, int hi) {
+    std::vector<int> cur;
+    for(int r = lo; r < hi; r++) {
+      cur.assign(cptr.begin() + ia[r], cptr.begin() + ia[r + 1]);
+      int rc = rptr[r];
+      visit(r, [&](int slot, int idx) { cidx[(size_t)cur[slot - ia[r]]++] = idx; },
+            [&](int idx, double fv) {
+              ridx[(size_t)rc] = idx;
+              rfv[(size_t)rc++] = fv;
+            });
+    }
+  });
+  GDP_TRY(dalloc(&F->d_ek_cb, cb.size()));
+  GDP_TRY(h2d(F->d_ek_cb, cb.data(), cb.size()));
+  GDP_TRY(dalloc(&F->d_ek, (size_t)std::max(1L, cb[nch])));
+  GDP_TRY(dalloc(&F->d_cptr, cptr.size()));
+  GDP_TRY(h2d(F->d_cptr, cptr.data(), cptr.size()));
+  GDP_TRY(dalloc(&F->d_cidx, cidx.size()));
+  GDP_TRY(h2d(F->d_cidx, cidx.data(), cidx.size()));
+  GDP_TRY(dalloc(&F->d_rptr, rptr.size()));
+  GDP_TRY(h2d(F->d_rptr, rptr.data(), rptr.size()));
+  GDP_TRY(dalloc(&F->d_ridx, ridx.size()));
+  GDP_TRY(h2d(F->d_ridx, ridx.data(), ridx.size()));
+  GDP_TRY(dalloc(&F->d_rfv, rfv.size()));
+  GDP_TRY(h2d(F->d_rfv, rfv.data(), rfv.size()));
+  GDP_TRY(sync());
+  return GDP_OK;
+}
+
 static int fe_upload(gdp_fe *F)
 {
   if(F->uploaded) return GDP_OK;
@@ -885,6 +1021,9 @@ static int fe_upload(gdp_fe *F)
     d.coef = F->d_coef[ti];
     d.active = F->d_active[ti];
   }
+  GDP_TRY(fe_build_two_phase(F));
+  P.nelem = E;
+  P.ek_cb = F->d_ek_cb;
   for(auto &e : F->ev) GDP_HIP(hipEventCreate(&e));
   GDP_TRY(sync());
   F->uploaded = true;
@@ -916,6 +1055,25 @@ extern "C" int gdp_fe_dof_table(const gdp_fe *fe, int space, int *dofnum, int nm
   return GDP_OK;
 }
 
+// element pass, then the CSR-entry and RHS sums (enqueue only)
+static int fe_two_phase(gdp_fe *fe, double *a, double *b)
+{
+  hipStream_t s = ctx().stream;
+  if(fe->nelem) {
+    dim3 ge((fe->nelem + 127) / 128);
+    if(fe->family == FAM_NODAL) hipLaunchKernelGGL(k_elem_nodal, ge, dim3(128), 0, s, fe->plan, fe->d_ek);
+    else hipLaunchKernelGGL(k_elem_edge, ge, dim3(128), 0, s, fe->plan, fe->d_ek);
+  }
+  if(fe->nnz)
+    hipLaunchKernelGGL(k_slot_sum, dim3((unsigned)((fe->nnz + 255) / 256)), dim3(256), 0, s, fe->nnz,
+                       fe->d_cptr, fe->d_cidx, fe->d_ek, a);
+  if(fe->N)
+    hipLaunchKernelGGL(k_rhs_sum, dim3((fe->N + 255) / 256), dim3(256), 0, s, fe->N, fe->d_rptr, fe->d_ridx,
+                       fe->d_rfv, fe->d_ek, b);
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
 extern "C" int gdp_fe_generate(gdp_fe *fe, gdp_matrix *A, gdp_vector *b)
 {
   if(!fe || !A || !b) return set_error(GDP_E_ARG, "gdp_fe_generate: null argument");
@@ -942,20 +1100,12 @@ extern "C" int gdp_fe_generate(gdp_fe *fe, gdp_matrix *A, gdp_vector *b)
   A->staged.clear();
   b->staged.clear();
   GDP_HIP(hipEventRecord(fe->ev[0], s));
-  if(fe->nnz) GDP_HIP(hipMemsetAsync(A->csr.a, 0, sizeof(double) * fe->nnz, s));
   if(fe->nelem)
     hipLaunchKernelGGL(k_geometry, dim3((fe->nelem + 255) / 256), dim3(256), 0, s, fe->nelem,
                        fe->d_geokind, fe->d_conn, fe->d_xyz, fe->d_geo);
   GDP_HIP(hipGetLastError());
   GDP_HIP(hipEventRecord(fe->ev[1], s));
-  if(fe->N) {
-    dim3 grid((fe->N + 127) / 128);
-    if(fe->family == FAM_NODAL)
-      hipLaunchKernelGGL(k_gather_nodal, grid, dim3(128), 0, s, fe->plan, A->csr.a, b->d);
-    else
-      hipLaunchKernelGGL(k_gather_edge, grid, dim3(128), 0, s, fe->plan, A->csr.a, b->d);
-    GDP_HIP(hipGetLastError());
-  }
+  GDP_TRY(fe_two_phase(fe, A->csr.a, b->d));
   GDP_HIP(hipEventRecord(fe->ev[2], s));
   GDP_TRY(sync());
   float t01 = 0, t12 = 0;
@@ -970,13 +1120,6 @@ extern "C" int gdp_fe_generate(gdp_fe *fe, gdp_matrix *A, gdp_vector *b)
 }
 
 namespace gdp {
-namespace {
-__global__ void k_zero_f64(long n, double *p)
-{
-  long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if(i < n) p[i] = 0.;
-}
-}  // namespace
 
 // Generate[A] enqueued only: same kernels as gdp_fe_generate, the values
 // cleared by a kernel instead of a memset, no events and no host
@@ -988,19 +1131,10 @@ int fe_generate_enqueue(gdp_fe *fe, gdp_matrix *A, gdp_vector *b)
   if(!fe->uploaded || A->csr.nnz != fe->nnz || A->h_ia != fe->h_ia || b->n != fe->N)
     return set_error(GDP_E_STATE, "fe_generate_enqueue: generate once before capturing");
   hipStream_t s = ctx().stream;
-  if(fe->nnz)
-    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((fe->nnz + 255) / 256)), dim3(256), 0, s, fe->nnz,
-                       A->csr.a);
   if(fe->nelem)
     hipLaunchKernelGGL(k_geometry, dim3((fe->nelem + 255) / 256), dim3(256), 0, s, fe->nelem,
                        fe->d_geokind, fe->d_conn, fe->d_xyz, fe->d_geo);
-  if(fe->N) {
-    dim3 grid((fe->N + 127) / 128);
-    if(fe->family == FAM_NODAL)
-      hipLaunchKernelGGL(k_gather_nodal, grid, dim3(128), 0, s, fe->plan, A->csr.a, b->d);
-    else
-      hipLaunchKernelGGL(k_gather_edge, grid, dim3(128), 0, s, fe->plan, A->csr.a, b->d);
-  }
+  GDP_TRY(fe_two_phase(fe, A->csr.a, b->d));
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
