@@ -283,22 +283,18 @@ __global__ void __launch_bounds__(B) k_cg_p(int n, const TV *r, TV *p, const KCt
 // iteration is read while the new one is written, so it lives in two slots
 // selected by the iteration's parity (rr[2]).
 
-__device__ inline double block_sum_partials(const double *part, int nparts)
-{
-  return sum_partials<double, B>(part, nparts);  // valid in thread 0
-}
 
 // dx += alpha p, r -= alpha Ap with alpha = rr_old / <p, Ap> (iters.f:478-486);
 // partial |r|^2.
-template <typename TV>
-__global__ void __launch_bounds__(B) k_cg_update2(int n, const TV *p, const TV *ap, TV *r, TV *dx,
-                                                 KCtl *C, const double *part_pap, int npap,
-                                                 double *part, int par)
+template <typename TV, int BS>
+__global__ void __launch_bounds__(BS) k_cg_update2(int n, const TV *p, const TV *ap, TV *r, TV *dx,
+                                                  KCtl *C, const double *part_pap, int npap,
+                                                  double *part, int par)
 {
   __shared__ double s_alpha;
   __shared__ int s_skip;
   if(C->done) return;
-  double pap = block_sum_partials(part_pap, npap);
+  double pap = sum_partials<double, BS>(part_pap, npap);
   if(threadIdx.x == 0) {
     int bk = 0;  // brkdn, iters.f:3368-3401
     if(pap > 0.0) bk = (1.0 / pap > 0.0) ? 0 : -9;
@@ -321,25 +317,25 @@ __global__ void __launch_bounds__(B) k_cg_update2(int n, const TV *p, const TV *
   if(s_skip) return;
   const double alpha = s_alpha;
   double s = 0;
-  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B) {
+  for(long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) {
     dx[i] = (TV)((double)dx[i] + alpha * (double)p[i]);
     TV v = (TV)((double)r[i] - alpha * (double)ap[i]);
     r[i] = v;
     s += (double)v * (double)v;
   }
-  s = block_sum<double, B>(s);
+  s = block_sum<double, BS>(s);
   if(threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // stopbis on |r| (iters.f:3275-3330), then p = r + beta p.
-template <typename TV>
-__global__ void __launch_bounds__(B) k_cg_p2(int n, const TV *r, TV *p, KCtl *C,
-                                            const double *part_rr, int nrr, int par)
+template <typename TV, int BS>
+__global__ void __launch_bounds__(BS) k_cg_p2(int n, const TV *r, TV *p, KCtl *C,
+                                             const double *part_rr, int nrr, int par)
 {
   __shared__ double s_beta;
   __shared__ int s_stop;
   if(C->done) return;
-  double rr = block_sum_partials(part_rr, nrr);
+  double rr = sum_partials<double, BS>(part_rr, nrr);
   if(threadIdx.x == 0) {
     int stop = 0, status = 0;
     double fpar5 = 0., beta = 0.;
@@ -375,7 +371,7 @@ __global__ void __launch_bounds__(B) k_cg_p2(int n, const TV *r, TV *p, KCtl *C,
   __syncthreads();
   if(s_stop) return;
   const double beta = s_beta;
-  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B)
+  for(long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS)
     p[i] = (TV)((double)r[i] + beta * (double)p[i]);
 }
 
@@ -955,7 +951,10 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
     const int batch = 16;
     // three kernels per iteration: SpMV + <p,Ap> partials, update + |r|^2
     // partials (each block derives alpha), p update (each block derives beta)
-    const int ncg = std::min(nb, 256);
+    // one workgroup of 1024 threads per CU for the two vector kernels: 16
+    // waves keep enough loads in flight to stream the vectors
+    constexpr int CGB = 1024;
+    const int ncg = std::max(1, std::min(ctx().cu_count, (n + CGB - 1) / CGB));
     double *part2 = part + nrow;
     auto body = [&]() -> int {
       for(int it = 0; it < batch; it++) {
@@ -966,9 +965,9 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
         }
         hipLaunchKernelGGL((k_op_dot<TA, TV, true>), grow, dim3(256), 0, s, n, A.sp, A.w, A.col,
                            A.val, src, p, ap, C, 1, part);
-        hipLaunchKernelGGL(k_cg_update2<TV>, dim3(ncg), blk, 0, s, n, p, ap, r, dx, C, part, nrow,
-                           part2, it & 1);
-        hipLaunchKernelGGL(k_cg_p2<TV>, dim3(ncg), blk, 0, s, n, r, p, C, part2, ncg, it & 1);
+        hipLaunchKernelGGL((k_cg_update2<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, p, ap, r, dx, C, part,
+                           nrow, part2, it & 1);
+        hipLaunchKernelGGL((k_cg_p2<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, r, p, C, part2, ncg, it & 1);
       }
       GDP_HIP(hipGetLastError());
       return GDP_OK;

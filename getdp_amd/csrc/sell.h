@@ -1,12 +1,13 @@
 // Sliced-ELLPACK storage (slices of 64 rows = one wavefront) for SpMV.
 //
-// Row r lives in slice s = r / 64 at lane l = r % 64; its k-th entry (in
-// column order) is stored at sp[s] + k*64 + l.  A slice is as wide as its
-// longest row; shorter rows are padded with zero values whose column is the
-// row itself.  One lane computes one row: every load instruction of a wave
-// reads 64 consecutive values (512 bytes) and 64 consecutive column indices,
-// and each row is summed sequentially in column order, the order of the
-// reference's amux (matvec.f:35-76).
+// Row r lives in slice s = r / 64 at lane l = r % 64.  Its entries, in column
+// order, are taken in pairs: entries 2q and 2q+1 are stored side by side at
+// sp[s] + 128 q + 2 l (+1), so that one 16-byte load per lane fetches a pair
+// and one load instruction of a wave reads 64 consecutive pairs (1 KiB of
+// FP64 values).  A slice is as wide as its longest row, rounded up to even;
+// shorter rows are padded with zero values whose column is the row itself.
+// One lane computes one row and sums it sequentially in column order, the
+// order of the reference's amux (matvec.f:35-76).
 #pragma once
 
 #include "gdp_device.h"
@@ -15,42 +16,60 @@ namespace gdp {
 
 constexpr int kSlice = 64;
 
+typedef double sell_d2 __attribute__((ext_vector_type(2)));
+typedef float sell_f2 __attribute__((ext_vector_type(2)));
+typedef int sell_i2 __attribute__((ext_vector_type(2)));
+template <typename T> struct SellPair;
+template <> struct SellPair<double> { typedef sell_d2 type; };
+template <> struct SellPair<float> { typedef sell_f2 type; };
+
 // Pattern from a host CSR; values copied from a device CSR with the same
 // pattern (also usable alone to refresh values).
 int sell_build(const std::vector<int> &h_ia, const DevCSR &A, DevSell &S);
 int sell_copy_values(const DevCSR &A, DevSell &S, bool also_f32);
 int sell_spmv(const DevSell &S, const double *x, double *y);
 
-// y = A x, one row per lane; U independent entries in flight per lane.
-template <typename TA, typename TV, int U = 4>
+// y = A x, one row per lane; U pairs of entries in flight per lane.
+template <typename TA, typename TV, int U = 2>
 __device__ inline void sell_row(int row, const long *__restrict__ sp, const int *__restrict__ w,
                                 const int *__restrict__ col, const TA *__restrict__ val,
                                 const TV *__restrict__ x, double &sum)
 {
 #pragma clang fp contract(off)
+  typedef typename SellPair<TA>::type VP;
   const int s = row >> 6, lane = row & 63;
-  const long base = sp[s] + lane;
+  const long base = sp[s] + 2 * lane;
   const int width = w[s];
+  const VP *vp = (const VP *)(val + base);
+  const sell_i2 *cp = (const sell_i2 *)(col + base);
   sum = 0.;
-  int k = 0;
-  for(; k + U <= width; k += U) {
-    TA v[U];
-    int c[U];
-    double xv[U];
+  int q = 0;
+  const int nq = width >> 1;
+  for(; q + U <= nq; q += U) {
+    VP v[U];
+    sell_i2 c[U];
+    double xa[U], xb[U];
 #pragma unroll
     for(int u = 0; u < U; u++) {
-      v[u] = __builtin_nontemporal_load(val + base + (long)(k + u) * kSlice);
-      c[u] = __builtin_nontemporal_load(col + base + (long)(k + u) * kSlice);
+      v[u] = __builtin_nontemporal_load(vp + (long)(q + u) * kSlice);
+      c[u] = __builtin_nontemporal_load(cp + (long)(q + u) * kSlice);
     }
 #pragma unroll
-    for(int u = 0; u < U; u++) xv[u] = (double)x[c[u]];
+    for(int u = 0; u < U; u++) {
+      xa[u] = (double)x[c[u].x];
+      xb[u] = (double)x[c[u].y];
+    }
 #pragma unroll
-    for(int u = 0; u < U; u++) sum = sum + (double)v[u] * xv[u];
+    for(int u = 0; u < U; u++) {
+      sum = sum + (double)v[u].x * xa[u];
+      sum = sum + (double)v[u].y * xb[u];
+    }
   }
-  for(; k < width; k++) {
-    TA v = __builtin_nontemporal_load(val + base + (long)k * kSlice);
-    int c = __builtin_nontemporal_load(col + base + (long)k * kSlice);
-    sum = sum + (double)v * (double)x[c];
+  for(; q < nq; q++) {
+    const VP v = __builtin_nontemporal_load(vp + (long)q * kSlice);
+    const sell_i2 c = __builtin_nontemporal_load(cp + (long)q * kSlice);
+    sum = sum + (double)v.x * (double)x[c.x];
+    sum = sum + (double)v.y * (double)x[c.y];
   }
 }
 

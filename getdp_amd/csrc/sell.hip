@@ -1,4 +1,4 @@
-// SELL-64 construction and the plain FP64 SpMV entry (see sell.h).
+// SELL-64 construction and the plain FP64 SpMV entry (layout in sell.h).
 #include <cstdlib>
 
 #include "sell.h"
@@ -21,7 +21,7 @@ __global__ void __launch_bounds__(256) k_sell_fill(int n, const int *__restrict_
   int nrows_pad = ((n + kSlice - 1) / kSlice) * kSlice;
   if(r >= nrows_pad) return;
   const int s = r >> 6, lane = r & 63;
-  const long base = sp[s] + lane;
+  const long base = sp[s] + 2 * lane;
   const int width = w[s];
   int len = 0, start = 0;
   if(r < n) {
@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(256) k_sell_fill(int n, const int *__restrict_
   }
   const int self = r < n ? r : 0;
   for(int k = 0; k < width; k++) {
-    long d = base + (long)k * kSlice;
+    long d = base + (long)(k >> 1) * (2 * kSlice) + (k & 1);
     double v = k < len ? a[start + k] : 0.;
     if(fill_cols) col[d] = k < len ? ja[start + k] : self;
     val[d] = v;
@@ -67,6 +67,7 @@ int sell_build(const std::vector<int> &h_ia, const DevCSR &A, DevSell &S)
   for(int s = 0; s < S.nslices; s++) {
     int m = 0;
     for(int r = s * kSlice; r < std::min(n, (s + 1) * kSlice); r++) m = std::max(m, h_ia[r + 1] - h_ia[r]);
+    m = (m + 1) & ~1;  // entries go in pairs
     w[s] = m;
     sp[s + 1] = sp[s] + (long)m * kSlice;
   }
@@ -104,17 +105,18 @@ int sell_copy_values(const DevCSR &A, DevSell &S, bool also_f32)
 int sell_spmv(const DevSell &S, const double *x, double *y)
 {
   if(!S.n) return GDP_OK;
-  // GDP_SELL_UNROLL=8 selects the 8-deep variant (launch-shape experiments)
+  // GDP_SELL_UNROLL=4 keeps 4 pairs per lane in flight instead of 2
+  // (launch-shape experiments)
   static int unroll = -1;
   if(unroll < 0) {
     const char *e = std::getenv("GDP_SELL_UNROLL");
-    unroll = (e && std::atoi(e) == 8) ? 8 : 4;
+    unroll = (e && std::atoi(e) == 4) ? 4 : 2;
   }
-  if(unroll == 8)
-    hipLaunchKernelGGL(k_sell_spmv<8>, dim3((S.n + 255) / 256), dim3(256), 0, ctx().stream, S.n,
+  if(unroll == 4)
+    hipLaunchKernelGGL(k_sell_spmv<4>, dim3((S.n + 255) / 256), dim3(256), 0, ctx().stream, S.n,
                        S.sp, S.w, S.col, S.val, x, y);
   else
-    hipLaunchKernelGGL(k_sell_spmv<4>, dim3((S.n + 255) / 256), dim3(256), 0, ctx().stream, S.n,
+    hipLaunchKernelGGL(k_sell_spmv<2>, dim3((S.n + 255) / 256), dim3(256), 0, ctx().stream, S.n,
                        S.sp, S.w, S.col, S.val, x, y);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
