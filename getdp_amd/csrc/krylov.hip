@@ -30,7 +30,10 @@ namespace {
 
 constexpr int B = 256;  // threads per block for vector kernels
 constexpr int LDH = kMaxKrylov + 2;
-constexpr int CGS_PER = 4;  // entries per thread in the CGS kernels
+// entries per thread in the CGS kernels: 4 on large vectors, 1 below
+// kCgsSmall entries so that a small system still launches >= 256 workgroups
+constexpr int kCgsSmall = 256 * 256 * 4;
+inline int cgs_per(int n) { return n >= kCgsSmall ? 4 : 1; }
 
 // gate modes: 0 none, 1 skip when done, 2 skip when done or cycle ended
 __device__ inline bool gated(const KCtl *C, int mode)
@@ -183,7 +186,7 @@ __global__ void __launch_bounds__(B) k_mgs_norm(int n, int k, TV *__restrict__ v
 
 // CGS2: pass 1 h = V^T v; pass 2 v -= V h, h' = V^T v; pass 3 v -= V h', |v|^2.
 // One block owns one chunk of CGS_PER*B entries (grid = ceil(n / chunk)).
-template <typename TV, int PASS>
+template <typename TV, int PASS, int CGS_PER>
 __global__ void __launch_bounds__(B) k_cgs(int n, int k, TV *__restrict__ vk,
                                           const TV *__restrict__ V, const KCtl *C, double *part)
 {
@@ -784,7 +787,8 @@ template <typename TA, typename TV> int gm_cycle(const GmArgs<TA, TV> &ga)
   hipStream_t s = ctx().stream;
   const int nb = reduce_blocks(n);
   const int nrow = std::max(1, (n + 255) / 256);
-  const int ncgs = (int)std::max(1L, ((long)n + B * CGS_PER - 1) / (B * CGS_PER));
+  const int cper = cgs_per(n);
+  const int ncgs = (int)std::max(1L, ((long)n + B * cper - 1) / (B * cper));
   dim3 g(nb), blk(B), grow(nrow);
   GDP_TRY(op_apply(A, sol, V, C, 1));
   hipLaunchKernelGGL(k_gm_resid<TV>, g, blk, 0, s, n, rhs, V, C, part);
@@ -813,12 +817,22 @@ template <typename TA, typename TV> int gm_cycle(const GmArgs<TA, TV> &ga)
     fin<FIN_NRM0>(C, part, nrow, 0);
     if(ga.orth == 1) {
       dim3 gc2(ncgs);
-      hipLaunchKernelGGL((k_cgs<TV, 1>), gc2, blk, 0, s, n, k, vk, V, C, part);
-      fin<FIN_CGS1>(C, part, ncgs, k);
-      hipLaunchKernelGGL((k_cgs<TV, 2>), gc2, blk, 0, s, n, k, vk, V, C, part);
-      fin<FIN_CGS2>(C, part, ncgs, k);
-      hipLaunchKernelGGL((k_cgs<TV, 3>), gc2, blk, 0, s, n, k, vk, V, C, part);
-      fin<FIN_CGS3>(C, part, ncgs, k);
+      if(cper == 4) {
+        hipLaunchKernelGGL((k_cgs<TV, 1, 4>), gc2, blk, 0, s, n, k, vk, V, C, part);
+        fin<FIN_CGS1>(C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 2, 4>), gc2, blk, 0, s, n, k, vk, V, C, part);
+        fin<FIN_CGS2>(C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 3, 4>), gc2, blk, 0, s, n, k, vk, V, C, part);
+        fin<FIN_CGS3>(C, part, ncgs, k);
+      }
+      else {
+        hipLaunchKernelGGL((k_cgs<TV, 1, 1>), gc2, blk, 0, s, n, k, vk, V, C, part);
+        fin<FIN_CGS1>(C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 2, 1>), gc2, blk, 0, s, n, k, vk, V, C, part);
+        fin<FIN_CGS2>(C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 3, 1>), gc2, blk, 0, s, n, k, vk, V, C, part);
+        fin<FIN_CGS3>(C, part, ncgs, k);
+      }
     }
     else {
       for(int i = 0; i < k; i++) {
@@ -935,7 +949,8 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
   double *part = W.part;
   const int nb = reduce_blocks(n);
   const int nrow = std::max(1, (n + 255) / 256);
-  const int ncgs = (int)std::max(1L, ((long)n + B * CGS_PER - 1) / (B * CGS_PER));
+  const int cper = cgs_per(n);
+  const int ncgs = (int)std::max(1L, ((long)n + B * cper - 1) / (B * cper));
   dim3 g(nb), blk(B), grow(nrow);
   int done = 0, status = 0, its = 0;
 

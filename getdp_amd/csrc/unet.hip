@@ -68,7 +68,6 @@ struct ConvArgs {
   u16 *out_bf;        // BF16 output with bias + ReLU, or
   float *out_f;       // FP32 output (plain)
   int relu;
-  int flip;           // adjoint: taps mirrored (tap -> 8 - tap)
   const int *gate;
   int gmode;
   int kchunk;         // split-K: reduction entries per blockIdx.z (multiple of 32)
@@ -151,6 +150,45 @@ __device__ inline bf16x8 as_bf8(uint4 v)
 // over the reduction range of its blockIdx.z; a block of 4 waves takes 4
 // consecutive pixel tiles.  Without split-K the epilogue adds the bias,
 // applies ReLU and stores; with split-K it stores FP32 partials.
+//
+// The reduction position of a lane's 8-channel group (tap, channel) and the
+// pixel and weight addresses are carried from one 32-wide k-step to the next
+// by additions only: no division or address rebuild inside the loop.
+__device__ inline int tap_dy(int tap) { return ((tap * 43) >> 7) - 1; }  // tap / 3 - 1, tap < 9
+__device__ inline int tap_dx(int tap) { return tap - 3 * ((tap * 43) >> 7) - 1; }
+
+// 8 input channels (ci0..ci0+7, logical input of the convolution) of the
+// pixel at offset (dy, dx) from the lane's output pixel; zeros outside.
+// pix: PLAIN/UPCAT pixel index b*H*W + y*W + x, POOL fine pixel of (2y, 2x);
+// cbh: b * H/2 (UPCAT coarse rows).
+template <int MODE>
+__device__ inline uint4 conv_load_a(const ConvArgs &a, long pix, int py, int px, int cbh, int dy, int dx,
+                                    int ci0, bool ok)
+{
+  uint4 v = {0u, 0u, 0u, 0u};
+  const int H = a.H, W = a.W, cin = a.cin_pad;
+  const int y = py + dy, x = px + dx;
+  if(!(ok && (unsigned)y < (unsigned)H && (unsigned)x < (unsigned)W)) return v;
+  if(MODE == M_PLAIN) return *(const uint4 *)(a.in + (pix + (long)dy * W + dx) * cin + ci0);
+  if(MODE == M_POOL) {
+    const long W2 = 2 * W;
+    const u16 *p0 = a.in + (pix + 2L * dy * W2 + 2 * dx) * cin + ci0;
+    uint4 q[4];
+    q[0] = *(const uint4 *)p0;
+    q[1] = *(const uint4 *)(p0 + cin);
+    q[2] = *(const uint4 *)(p0 + W2 * cin);
+    q[3] = *(const uint4 *)(p0 + W2 * cin + cin);
+    const u16 *h0 = (const u16 *)&q[0], *h1 = (const u16 *)&q[1], *h2 = (const u16 *)&q[2],
+              *h3 = (const u16 *)&q[3];
+    u16 *o = (u16 *)&v;
+#pragma unroll
+    for(int c = 0; c < 8; c++) o[c] = f2bf(0.25f * (bf2f(h0[c]) + bf2f(h1[c]) + bf2f(h2[c]) + bf2f(h3[c])));
+    return v;
+  }
+  if(ci0 < a.c1) return *(const uint4 *)(a.in + ((long)(cbh + (y >> 1)) * (W >> 1) + (x >> 1)) * a.c1 + ci0);
+  return *(const uint4 *)(a.in2 + (pix + (long)dy * W + dx) * (cin - a.c1) + (ci0 - a.c1));
+}
+
 template <int MODE, int TM, int TN>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs a)
 {
@@ -160,17 +198,23 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a)
   const long pbase = ((long)blockIdx.x * 4 + wave) * (TM * 16);
   const int cobase = blockIdx.y * 16 * TN;
   const int r16 = lane & 15, kq = lane >> 4;
-  int pb[TM], py[TM], px[TM];
+  const int H = a.H, W = a.W, cin = a.cin_pad;
+  int py[TM], px[TM];
+  long pix[TM];   // PLAIN/UPCAT: pixel index b*H*W + y*W + x; POOL: fine pixel of (2y, 2x)
+  int cbh[TM];    // UPCAT: b * (H/2) (coarse row base)
   bool pok[TM];
 #pragma unroll
   for(int t = 0; t < TM; t++) {
     long p = pbase + t * 16 + r16;
     pok[t] = p < P;
     long q = pok[t] ? p : 0;
-    px[t] = (int)(q % a.W);
-    long q2 = q / a.W;
-    py[t] = (int)(q2 % a.H);
-    pb[t] = (int)(q2 / a.H);
+    px[t] = (int)(q % W);
+    long q2 = q / W;
+    py[t] = (int)(q2 % H);
+    const int b = (int)(q2 / H);
+    if(MODE == M_POOL) pix[t] = ((long)b * 2 * H + 2 * py[t]) * (2 * W) + 2 * px[t];
+    else pix[t] = q;
+    cbh[t] = b * (H >> 1);
   }
   f32x4 acc[TM][TN];
 #pragma unroll
@@ -180,27 +224,48 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a)
   const uint4 zero4 = {0u, 0u, 0u, 0u};
   const int kbeg = blockIdx.z * a.kchunk;
   const int kend = min(a.K_pad, kbeg + a.kchunk);
-  for(int kk = kbeg; kk < kend; kk += 32) {
-    const int k0 = kk + 8 * kq;
-    int tap = k0 / a.cin_pad;
-    const int ci0 = k0 - tap * a.cin_pad;
-    const bool kvalid = tap < 9;
-    if(a.flip) tap = 8 - tap;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    bf16x8 av[TM], bv[TN];
+  int tap = (kbeg + 8 * kq) / cin;
+  int ci0 = kbeg + 8 * kq - tap * cin;
+  const u16 *wp[TN];
+  bool wok[TN];
 #pragma unroll
-    for(int t = 0; t < TM; t++)
-      av[t] = as_bf8((kvalid && pok[t]) ? load8<MODE>(a, pb[t], py[t] + dy, px[t] + dx, ci0) : zero4);
+  for(int n = 0; n < TN; n++) {
+    const int co = cobase + n * 16 + r16;
+    wok[n] = co < a.cout;
+    wp[n] = a.wt + (long)(wok[n] ? co : 0) * a.K_pad + kbeg + 8 * kq;
+  }
+  // PF k-steps per round, all loads of a round issued before its MFMAs
+  constexpr int PF = (TM * TN <= 4) ? 4 : 2;
+  for(int kk = kbeg; kk < kend; kk += 32 * PF) {
+    uint4 av[PF][TM], bv[PF][TN];
 #pragma unroll
-    for(int n = 0; n < TN; n++) {
-      const int co = cobase + n * 16 + r16;
-      bv[n] = as_bf8(co < a.cout ? *(const uint4 *)(a.wt + (long)co * a.K_pad + k0) : zero4);
+    for(int j = 0; j < PF; j++) {
+      const bool on = kk + 32 * j < kend;
+      const bool kvalid = on && tap < 9;
+      const int dy = tap_dy(tap), dx = tap_dx(tap);
+#pragma unroll
+      for(int t = 0; t < TM; t++)
+        av[j][t] = conv_load_a<MODE>(a, pix[t], py[t], px[t], cbh[t], dy, dx, ci0, kvalid && pok[t]);
+#pragma unroll
+      for(int n = 0; n < TN; n++) {
+        bv[j][n] = (on && wok[n]) ? *(const uint4 *)wp[n] : zero4;
+        wp[n] += on ? 32 : 0;
+      }
+      if(on) {
+        ci0 += 32;
+        while(ci0 >= cin) {
+          ci0 -= cin;
+          tap++;
+        }
+      }
     }
 #pragma unroll
-    for(int t = 0; t < TM; t++)
+    for(int j = 0; j < PF; j++)
 #pragma unroll
-      for(int n = 0; n < TN; n++)
-        acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv[n], acc[t][n], 0, 0, 0);
+      for(int t = 0; t < TM; t++)
+#pragma unroll
+        for(int n = 0; n < TN; n++)
+          acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av[j][t]), as_bf8(bv[j][n]), acc[t][n], 0, 0, 0);
   }
   // epilogue: lane holds column (l % 16), rows 4*(l/16)..+3
 #pragma unroll
@@ -229,6 +294,100 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a)
         else a.out_f[p * a.cout + co] = v;
       }
     }
+}
+
+// Small-image variant: the 4 waves of a block share one tile of 16 pixels x
+// 16*TN channels and split the reduction into 4 contiguous ranges; their
+// accumulators are added in LDS in wave order (deterministic) and wave 0
+// stores.  Used when one wave per tile would leave most of the GPU idle.
+template <int MODE, int TN>
+__global__ void __launch_bounds__(256) k_conv_ks(ConvArgs a)
+{
+  if(unet_gated(a.gate, a.gmode)) return;
+  __shared__ f32x4 red[3][TN][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long P = (long)a.B * a.H * a.W;
+  const long pbase = (long)blockIdx.x * 16;
+  const int cobase = blockIdx.y * 16 * TN;
+  const int r16 = lane & 15, kq = lane >> 4;
+  const int H = a.H, W = a.W, cin = a.cin_pad;
+  const long p = pbase + r16;
+  const bool pok = p < P;
+  const long q = pok ? p : 0;
+  const int px = (int)(q % W);
+  const long q2 = q / W;
+  const int py = (int)(q2 % H);
+  const int b = (int)(q2 / H);
+  const long pix = MODE == M_POOL ? ((long)b * 2 * H + 2 * py) * (2 * W) + 2 * px : q;
+  const int cbh = b * (H >> 1);
+  const int nsteps = a.K_pad / 32;
+  const int s0 = wave * nsteps / 4, s1 = (wave + 1) * nsteps / 4;
+  f32x4 acc[TN];
+#pragma unroll
+  for(int n = 0; n < TN; n++) acc[n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const uint4 zero4 = {0u, 0u, 0u, 0u};
+  int tap = (32 * s0 + 8 * kq) / cin;
+  int ci0 = 32 * s0 + 8 * kq - tap * cin;
+  const u16 *wp[TN];
+  bool wok[TN];
+#pragma unroll
+  for(int n = 0; n < TN; n++) {
+    const int co = cobase + n * 16 + r16;
+    wok[n] = co < a.cout;
+    wp[n] = a.wt + (long)(wok[n] ? co : 0) * a.K_pad + 32 * s0 + 8 * kq;
+  }
+  // up to PF k-steps per round: every load of the round is issued before the
+  // first MFMA, so a round costs one memory latency
+  constexpr int PF = 8;
+  for(int c0 = s0; c0 < s1; c0 += PF) {
+    uint4 av[PF], bv[PF][TN];
+#pragma unroll
+    for(int j = 0; j < PF; j++) {
+      const bool on = c0 + j < s1;
+      av[j] = conv_load_a<MODE>(a, pix, py, px, cbh, tap_dy(tap), tap_dx(tap), ci0, on && tap < 9 && pok);
+#pragma unroll
+      for(int n = 0; n < TN; n++) {
+        bv[j][n] = (on && wok[n]) ? *(const uint4 *)wp[n] : zero4;
+        wp[n] += on ? 32 : 0;
+      }
+      if(on) {
+        ci0 += 32;
+        while(ci0 >= cin) {
+          ci0 -= cin;
+          tap++;
+        }
+      }
+    }
+#pragma unroll
+    for(int j = 0; j < PF; j++)
+#pragma unroll
+      for(int n = 0; n < TN; n++)
+        acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av[j]), as_bf8(bv[j][n]), acc[n], 0, 0, 0);
+  }
+  if(wave > 0)
+#pragma unroll
+    for(int n = 0; n < TN; n++) red[wave - 1][n][lane] = acc[n];
+  __syncthreads();
+  if(wave > 0) return;
+#pragma unroll
+  for(int w = 0; w < 3; w++)
+#pragma unroll
+    for(int n = 0; n < TN; n++) acc[n] += red[w][n][lane];
+#pragma unroll
+  for(int n = 0; n < TN; n++) {
+    const int co = cobase + n * 16 + r16;
+    if(co >= a.cout) continue;
+    const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+    for(int i = 0; i < 4; i++) {
+      const long pp = pbase + 4 * kq + i;
+      if(pp >= P) continue;
+      float v = acc[n][i] + bias;
+      if(a.relu) v = v > 0.f ? v : 0.f;
+      if(a.out_bf) a.out_bf[pp * a.cout + co] = f2bf(v);
+      else a.out_f[pp * a.cout + co] = v;
+    }
+  }
 }
 
 // split-K epilogue: sum the partials in a fixed order, bias, ReLU, store
@@ -632,7 +791,6 @@ ConvArgs conv_args(const UserPrec &U, const ConvLayer &l, int B)
   a.wt = l.wt;
   a.bias = l.b32;
   a.relu = 1;
-  a.flip = 0;
   return a;
 }
 
@@ -641,6 +799,7 @@ ConvArgs conv_args(const UserPrec &U, const ConvLayer &l, int B)
 // (4 per SIMD of the 256 CUs) even on the coarse levels of a single grid.
 struct ConvPlan {
   int TM, TN, nsplit, kchunk;
+  bool ks;            // 4 waves per tile, reduction split inside the block
   long split_floats;  // partial-sum buffer needed (0 without split-K)
 };
 
@@ -651,10 +810,19 @@ ConvPlan plan_conv(long P, int cout, int K_pad)
   const long ntn = (cout + 16 * c.TN - 1) / (16 * c.TN);
   const long target = 1024;
   c.TM = 4;
+  c.ks = false;
   long waves = (P + 63) / 64 * ntn;
   if(waves < target) {
     c.TM = 1;
     waves = (P + 15) / 16 * ntn;
+  }
+  if(waves < target && K_pad >= 4 * 32) {
+    // share each tile between 4 waves; no partial buffer, no extra kernel
+    c.ks = true;
+    c.nsplit = 1;
+    c.kchunk = K_pad;
+    c.split_floats = 0;
+    return c;
   }
   const int ksteps = K_pad / 32;
   c.nsplit = 1;
@@ -679,6 +847,15 @@ template <int MODE> int launch_conv_mode(ConvArgs a, const ConvPlan &c, float *s
   const long P = (long)a.B * a.H * a.W;
   a.kchunk = c.kchunk;
   a.split = c.nsplit > 1 ? split : nullptr;
+  if(c.ks) {
+    dim3 grid((unsigned)((P + 15) / 16), (unsigned)((a.cout + 16 * c.TN - 1) / (16 * c.TN)));
+    hipStream_t s = ctx().stream;
+    if(c.TN == 1) hipLaunchKernelGGL((k_conv_ks<MODE, 1>), grid, dim3(256), 0, s, a);
+    else if(c.TN == 2) hipLaunchKernelGGL((k_conv_ks<MODE, 2>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_conv_ks<MODE, 4>), grid, dim3(256), 0, s, a);
+    GDP_HIP(hipGetLastError());
+    return GDP_OK;
+  }
   const long ptile = 4L * 16 * c.TM;
   dim3 grid((unsigned)((P + ptile - 1) / ptile), (unsigned)((a.cout + 16 * c.TN - 1) / (16 * c.TN)),
             (unsigned)c.nsplit);
@@ -791,8 +968,7 @@ int backward(UserPrec &U, int B)
     d.in = U.g;
     d.wt = l.wtT;
     d.out_f = U.dx;
-    d.relu = 0;
-    d.flip = 0;  // the taps are already mirrored in wtT
+    d.relu = 0;  // the taps are already mirrored in wtT
     GDP_TRY(launch_conv(U, d, M_PLAIN, true));
     long nx = P * l.cin_pad;
     if(l.mode == M_PLAIN)
