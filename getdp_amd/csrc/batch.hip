@@ -185,7 +185,7 @@ __device__ inline void b_ilu(const BatchArgs &a, int inst, int n, const double *
   }
 }
 
-template <int T, int W>
+template <int T, int W, int ORTH>
 __global__ void __launch_bounds__(T) k_batch_gmres(BatchArgs a)
 {
   extern __shared__ double zlds[];
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(T) k_batch_gmres(BatchArgs a)
         b_spmv(a, r0, n, Z, w, T);
         its++;
         __syncthreads();
-        if(a.orth == 1) {
+        if constexpr(ORTH == 1) {
           // classical Gram-Schmidt, twice
           for(int pass = 0; pass < 2; pass++) {
             double d[kBM + 1];
@@ -485,12 +485,16 @@ int zcap_rows(int threads) { return threads == 1024 ? 18000 : (threads == 512 ? 
 
 template <int T, int W> void launch_group(const BatchArgs &a, int blocks, hipStream_t s)
 {
-  hipLaunchKernelGGL((k_batch_gmres<T, W>), dim3(blocks), dim3(T), (size_t)a.zcap * sizeof(double), s, a);
+  const size_t lds = (size_t)a.zcap * sizeof(double);
+  if(a.orth == 1) hipLaunchKernelGGL((k_batch_gmres<T, W, 1>), dim3(blocks), dim3(T), lds, s, a);
+  else hipLaunchKernelGGL((k_batch_gmres<T, W, 0>), dim3(blocks), dim3(T), lds, s, a);
 }
 
 template <int T, int W> int allow_lds()
 {
-  GDP_HIP(hipFuncSetAttribute((const void *)k_batch_gmres<T, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  GDP_HIP(hipFuncSetAttribute((const void *)k_batch_gmres<T, W, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              zcap_rows(T) * (int)sizeof(double)));
+  GDP_HIP(hipFuncSetAttribute((const void *)k_batch_gmres<T, W, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               zcap_rows(T) * (int)sizeof(double)));
   return GDP_OK;
 }
