@@ -143,3 +143,16 @@ def test_batch_concatenation_is_block_diagonal():
     inst = np.searchsorted(off, np.arange(fe.N), side="right") - 1
     rows = np.repeat(np.arange(fe.N), np.diff(ia))
     assert np.array_equal(inst[rows], inst[ja])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU; --gpus N > 1 exits with a message."""
+    import json
+    import subprocess
+    import sys
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"] and msg["n_gpus"] == 2

@@ -179,3 +179,42 @@ def test_traction_surface_term(oracle):
     assert abs(b.sum() + W3 / 0.5 * 1.0) < 1e-12
     nx = (tab[0] > 0).sum()
     assert np.all(b[nx:] == 0.0)
+
+
+def test_edge_numbering_hand_computed(oracle):
+    """First-encounter edge numbering with the reference's local edge table
+    of the tetrahedron (GeoEntity.h:214-215: (0,1) (0,2) (0,3) (1,2) (1,3)
+    (2,3)) and the sign rule (+ when the local edge runs from the lower to the
+    higher node number), worked out by hand for two tetrahedra sharing a face,
+    visited in both orders."""
+    from getdp_amd.problems import TET
+    conn = np.array([[1, 2, 3, 4], [5, 3, 2, 4]], np.int32)
+    etype = np.array([TET, TET], np.int32)
+    e, en = oracle.number_edges(etype, conn, np.array([0, 1], np.int32))
+    assert e.tolist() == [[1, 2, 3, 4, 5, 6], [-7, -8, -9, -4, 6, 5]]
+    assert [tuple(sorted(p)) for p in en.tolist()] == [(1, 2), (1, 3), (1, 4), (2, 3), (2, 4), (3, 4),
+                                                      (3, 5), (2, 5), (4, 5)]
+    e, en = oracle.number_edges(etype, conn, np.array([1, 0], np.int32))
+    assert e.tolist() == [[7, 8, 9, 4, 6, 5], [-1, -2, -3, -4, 5, 6]]
+    assert [tuple(sorted(p)) for p in en.tolist()] == [(3, 5), (2, 5), (4, 5), (2, 3), (3, 4), (2, 4),
+                                                      (1, 2), (1, 3), (1, 4)]
+
+
+def test_dof_numbering_hand_computed(oracle):
+    """DOF numbers follow the key order (NumType, Entity) of fcmp_Dof
+    (DofData.cpp:36-45), fixed DOFs get none: on one cube (8 nodes, x = 0
+    clamped), the 4 free nodes 2, 4, 6, 8 are numbered 1-4 for u_x, 5-8 for
+    u_y and 9-12 for u_z; clamped nodes carry -(index into the fixed list + 1)."""
+    import getdp_amd.problems as P
+    pr = P.elasticity3d(1, nbins=4)
+    N, tab = oracle.number_dofs(pr)
+    assert N == 12
+    free = [n for n in range(1, 9) if pr.xyz[n - 1, 0] > 0.5]
+    assert free == [2, 4, 6, 8]
+    for s in range(3):
+        for r, node in enumerate(free):
+            assert tab[s, node] == 4 * s + r + 1
+        for node in range(1, 9):
+            if node not in free:
+                f = list(pr.spaces[s].fixed_entity).index(node)
+                assert tab[s, node] == -(f + 1)
