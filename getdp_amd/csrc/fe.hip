@@ -36,7 +36,8 @@ enum GeoKind { GEO_NONE = 0, GEO_TRI2D = 1, GEO_TET = 2, GEO_TRISUR = 3 };
 enum Family { FAM_NODAL = 0, FAM_EDGE = 1 };
 
 struct DevTerm {
-  int kind, equ, dof, npoints, ncoef, pad;
+  int kind, equ, dof, npoints, ncoef;
+  int perp;                     // test and DOF spaces are BF_PerpendicularEdge
   const double *coef;           // device: coef[region * ncoef + c]
   const unsigned char *active;  // device: per region id, term applies
 };
@@ -303,6 +304,68 @@ __global__ void __launch_bounds__(256) k_geometry(int nelem, const int *__restri
 // Pass 2, one thread per CSR entry / per row: the entry's contributions
 // summed in (element, term) order, the reference's order of additions
 // (Dof_AssembleInMat called element after element, term after term).
+// BF_PerpendicularEdge terms on a triangle in the plane (2D a_z forms).
+// Curl-curl: BF_CurlPerpendicularEdge, the reference-coordinate node gradient
+// turned to (dv, -du, dw) (BF_Perpendicular.cpp:70-78), through
+// ChangeOfCoord_Form2P (Get_Geometry.cpp:1938-1947), Cal_Product123.  Mass and
+// source: the value (0, 0, BF_Node) (:33-41) through ChangeOfCoord_Form1P
+// (:1931-1936), Cal_Product3 (:1976).  Returns the next output position.
+__device__ double *elem_perp(const DevTerm &t, const double *cf, const double *G, double adet,
+                             double *out)
+{
+  if(t.kind == GDP_T_CURL_CURL) {
+    double c[3][3];
+    for(int j = 0; j < 3; j++) {
+      double u[3];
+      bf_gradnode(1, j, u);
+      const double r0 = u[1], r1 = -u[0];
+      c[j][0] = (r0 * G[0] + r1 * G[3]) / G[9];
+      c[j][1] = (r0 * G[1] + r1 * G[4]) / G[9];
+      c[j][2] = 0.;
+    }
+    for(int k = 0; k < 3; k++)
+      for(int j = 0; j < 3; j++) {
+        const int ii = j > k ? j : k, jj = j > k ? k : j;
+        double cc[3];
+        coef_times(t.ncoef, cf, c[jj], cc);
+        const double pr = dot3(c[ii], cc);
+        double acc = 0.;
+        for(int q = 0; q < t.npoints; q++) {
+          double u, v, w, wt;
+          gauss_pt(1, t.npoints, q, u, v, w, wt);
+          acc += (wt * adet) * pr;
+        }
+        out[64 * (k * 3 + j)] = acc;
+      }
+    return out + 64 * 9;
+  }
+  if(t.kind == GDP_T_EDGE_MASS) {
+    for(int k = 0; k < 3; k++)
+      for(int j = 0; j < 3; j++) {
+        const int ii = j > k ? j : k, jj = j > k ? k : j;
+        double acc = 0.;
+        for(int q = 0; q < t.npoints; q++) {
+          double u, v, w, wt;
+          gauss_pt(1, t.npoints, q, u, v, w, wt);
+          const double pi = bf_node(1, ii, u, v, w) / G[8], pj = bf_node(1, jj, u, v, w) / G[8];
+          acc += (wt * adet) * (pi * (cf[0] * pj));
+        }
+        out[64 * (k * 3 + j)] = acc;
+      }
+    return out + 64 * 9;
+  }
+  for(int k = 0; k < 3; k++) {  // GDP_T_SRC_EDGE
+    double acc = 0.;
+    for(int q = 0; q < t.npoints; q++) {
+      double u, v, w, wt;
+      gauss_pt(1, t.npoints, q, u, v, w, wt);
+      acc += (wt * adet) * ((bf_node(1, k, u, v, w) / G[8]) * cf[2]);
+    }
+    out[64 * k] = acc;
+  }
+  return out + 64 * 3;
+}
+
 __global__ void __launch_bounds__(128) k_elem_nodal(DevPlan P, double *__restrict__ ek)
 {
   const int e = blockIdx.x * 128 + threadIdx.x;
@@ -326,6 +389,10 @@ __global__ void __launch_bounds__(128) k_elem_nodal(DevPlan P, double *__restric
     const DevTerm &t = P.term[ti];
     if(!t.active[reg]) continue;
     const double *cf = t.coef + (long)reg * t.ncoef;
+    if(t.perp) {
+      out = elem_perp(t, cf, G, adet, out);
+      continue;
+    }
     if(t.kind == GDP_T_GRAD_GRAD) {
       const int sym = (t.dof == t.equ);
       for(int k = 0; k < nn; k++)
@@ -511,6 +578,7 @@ struct gdp_fe {
   long nnz = 0;
   // host copies
   std::vector<int> h_etype, h_region, h_conn, h_edges, h_geokind;
+  std::vector<int> h_perp;  // per term: BF_PerpendicularEdge spaces
   std::vector<std::vector<int>> h_dofnum;  // per space, nmax+1
   std::vector<int> h_ia, h_ja;
   std::vector<int> h_inc_ptr, h_inc_item, h_row_space, h_row_ent;
@@ -585,7 +653,7 @@ bool contains(int n, const int *l, int v)
 // Entity k of element e in a space: node number or |edge number|.
 inline int entity_of(const gdp_fe &F, int bf, int e, int k)
 {
-  if(bf == GDP_BF_NODE) return F.h_conn[(size_t)e * 4 + k];
+  if(bf != GDP_BF_EDGE) return F.h_conn[(size_t)e * 4 + k];
   int v = F.h_edges[(size_t)e * 6 + k];
   return v < 0 ? -v : v;
 }
@@ -640,7 +708,7 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
   int nedge_sp = 0;
   for(int s = 0; s < nspaces; s++) {
     if(spaces[s].bf == GDP_BF_EDGE) nedge_sp++;
-    else if(spaces[s].bf != GDP_BF_NODE)
+    else if(spaces[s].bf != GDP_BF_NODE && spaces[s].bf != GDP_BF_PERP_EDGE)
       return set_error(GDP_E_UNSUPPORTED, "space %d: unknown basis function", s);
   }
   if(nedge_sp && (nspaces != 1 || !mesh->edges))
@@ -656,9 +724,20 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
     bool edgek = (t.kind == GDP_T_CURL_CURL || t.kind == GDP_T_EDGE_MASS || t.kind == GDP_T_SRC_EDGE);
     bool nodek = (t.kind == GDP_T_GRAD_GRAD || t.kind == GDP_T_NODE_MASS || t.kind == GDP_T_SRC_NODE);
     if(!edgek && !nodek) return set_error(GDP_E_UNSUPPORTED, "term %d: kind %d", ti, t.kind);
-    if(edgek != (F->family == FAM_EDGE))
+    // BF_PerpendicularEdge lives on nodes: the edge kinds in their 2D
+    // perpendicular forms (Form1P / Form2P), on both sides of the term
+    const bool perp = spaces[t.equ].bf == GDP_BF_PERP_EDGE;
+    if(perp && (!edgek || (t.dof >= 0 && spaces[t.dof].bf != GDP_BF_PERP_EDGE)))
+      return set_error(GDP_E_UNSUPPORTED, "term %d: kind %d on a perpendicular-edge space", ti, t.kind);
+    if(!perp && t.dof >= 0 && spaces[t.dof].bf == GDP_BF_PERP_EDGE)
+      return set_error(GDP_E_UNSUPPORTED, "term %d: mixed forms", ti);
+    if(!perp && edgek != (F->family == FAM_EDGE))
       return set_error(GDP_E_UNSUPPORTED, "term %d: mixes edge and nodal spaces", ti);
-    int need_coef = (t.kind == GDP_T_GRAD_GRAD) ? -1 : (t.kind == GDP_T_SRC_EDGE ? 3 : 1);
+    if(perp && t.jac_sur)
+      return set_error(GDP_E_UNSUPPORTED, "term %d: perpendicular edges need Jacobian Vol in 2D", ti);
+    F->h_perp.push_back(perp);
+    int need_coef = (t.kind == GDP_T_GRAD_GRAD || t.kind == GDP_T_CURL_CURL) ? -1
+                    : (t.kind == GDP_T_SRC_EDGE ? 3 : 1);
     if(need_coef == -1 ? (t.ncoef != 1 && t.ncoef != 9) : t.ncoef != need_coef)
       return set_error(GDP_E_UNSUPPORTED, "term %d: %d coefficient values", ti, t.ncoef);
     if(t.coef_nreg < F->nreg) return set_error(GDP_E_ARG, "term %d: coefficient table too short", ti);
@@ -667,13 +746,14 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
       if(!contains(t.nregions, t.regions, F->h_region[e])) continue;
       int et = F->h_etype[e], gk;
       if(et == GDP_TET) {
+        if(perp) return set_error(GDP_E_UNSUPPORTED, "term %d: perpendicular edges on a tetrahedron", ti);
         if(t.jac_sur) return set_error(GDP_E_UNSUPPORTED, "term %d: Jacobian Sur on a tetrahedron", ti);
         gk = GEO_TET;
         if(t.npoints != 1 && t.npoints != 4)
           return set_error(GDP_E_UNSUPPORTED, "term %d: %d Gauss points on tetrahedra", ti, t.npoints);
       }
       else if(et == GDP_TRI) {
-        if(edgek) return set_error(GDP_E_UNSUPPORTED, "term %d: edge elements on triangles", ti);
+        if(edgek && !perp) return set_error(GDP_E_UNSUPPORTED, "term %d: edge elements on triangles", ti);
         gk = t.jac_sur ? GEO_TRISUR : GEO_TRI2D;
         if(t.jac_sur && t.kind != GDP_T_SRC_NODE)
           return set_error(GDP_E_UNSUPPORTED, "term %d: Jacobian Sur only for sources", ti);
@@ -718,7 +798,7 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
       for(int q = 0; q < 2; q++) {
         int s = sps[q];
         if(s < 0) continue;
-        int nb = spaces[s].bf == GDP_BF_NODE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+        int nb = spaces[s].bf != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
         for(int k = 0; k < nb; k++) {
           int ent = entity_of(*F, spaces[s].bf, e, k);
           if(ent < 1 || ent > nmax) return set_error(GDP_E_ARG, "entity %d out of range", ent);
@@ -758,7 +838,7 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
   };
   for(int e = 0; e < E; e++) {
     if(!elem_used(e)) continue;
-    int nb = space_bf == GDP_BF_NODE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+    int nb = space_bf != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
     for(int k = 0; k < nb; k++) inc_ptr[entity_of(*F, space_bf, e, k) + 1]++;
   }
   for(int i = 0; i <= nmax; i++) inc_ptr[i + 1] += inc_ptr[i];
@@ -767,7 +847,7 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
     std::vector<int> fill(inc_ptr.begin(), inc_ptr.end() - 1);
     for(int e = 0; e < E; e++) {
       if(!elem_used(e)) continue;
-      int nb = space_bf == GDP_BF_NODE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+      int nb = space_bf != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
       for(int k = 0; k < nb; k++) inc_item[fill[entity_of(*F, space_bf, e, k)]++] = e * 8 + k;
     }
   }
@@ -787,7 +867,7 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
         for(int ti = 0; ti < nterms; ti++) {
           const gdp_term_desc &t = terms[ti];
           if(t.dof < 0 || t.equ != s || !active[ti][reg]) continue;
-          int nb = spaces[t.dof].bf == GDP_BF_NODE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+          int nb = spaces[t.dof].bf != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
           for(int j = 0; j < nb; j++) {
             int c = F->h_dofnum[t.dof][entity_of(*F, spaces[t.dof].bf, e, j)];
             if(c > 0) cand.push_back(c - 1);
@@ -1018,6 +1098,7 @@ static int fe_upload(gdp_fe *F)
     d.dof = t.dof;
     d.npoints = t.npoints;
     d.ncoef = t.ncoef;
+    d.perp = F->h_perp[ti];
     d.coef = F->d_coef[ti];
     d.active = F->d_active[ti];
   }

@@ -27,7 +27,7 @@ import numpy as np
 # Element types (Gmsh codes)
 LINE, TRI, TET = 1, 2, 4
 # Basis function kinds
-BF_NODE, BF_EDGE = 0, 1
+BF_NODE, BF_EDGE, BF_PERP_EDGE = 0, 1, 2
 # Term kinds (shared with include/gdp_hip.h and the oracle)
 T_GRAD_GRAD, T_CURL_CURL, T_EDGE_MASS, T_NODE_MASS = 1, 2, 3, 4
 T_SRC_NODE, T_SRC_EDGE = 10, 11
@@ -35,7 +35,7 @@ T_SRC_NODE, T_SRC_EDGE = 10, 11
 
 @dataclass
 class Space:
-    """A FunctionSpace with one basis function (BF_Node or BF_Edge)."""
+    """A FunctionSpace with one basis function (BF_Node, BF_Edge or BF_PerpendicularEdge)."""
     bf: int
     numtype: int                       # BasisFunction.Num: the DOF key's first field
     support: List[int]
@@ -120,12 +120,19 @@ def _square_mesh(n: int):
 
 def poisson2d(n: int = 256, seed: int = 0, nblocks: int = 16, nu_vals=(1.0, 1e-3),
               js: float = 1.0, radius: float = 0.2, field_kind: str = "checker",
-              sigma: float = 1.0, nbins: int = 64, npoints: int = 3) -> Problem:
+              sigma: float = 1.0, nbins: int = 64, npoints: int = 3,
+              basis: str = "node") -> Problem:
     """2D scalar Poisson/magnetostatics on the unit square.
 
     Regions: triangles get ``1 + cls + K*in_src`` where cls is the coefficient
     class (checkerboard: 2 classes; lognormal: ``nbins`` quantised classes),
     boundary lines region 0.
+
+    ``basis="node"``: u in BF_Node, ``[ nu * Dof{d u}, {d u} ] + [ -js, {u} ]``.
+    ``basis="perp"``: the magnetostatic a_z form of the same problem, a in
+    BF_PerpendicularEdge, ``[ nu * Dof{d a}, {d a} ] + [ -Vector[0,0,js], {a} ]``
+    (curl (phi e_z) . curl (psi e_z) = grad phi . grad psi: the same system up
+    to rounding).
     """
     xyz, tri, ci, cj, lines = _square_mesh(n)
     ntri = tri.shape[0]
@@ -161,10 +168,19 @@ def poisson2d(n: int = 256, seed: int = 0, nblocks: int = 16, nu_vals=(1.0, 1e-3
     f = np.zeros((nreg, 1))
     f[srcreg, 0] = -js
     bnodes = np.unique(lines.ravel()).astype(np.int32)
-    space = Space(BF_NODE, 1, dom, bnodes, np.zeros(bnodes.shape[0]))
-    terms = [Term(T_GRAD_GRAD, 0, 0, dom, npoints, nu, dim=2),
-             Term(T_SRC_NODE, 0, -1, srcreg, npoints, f, dim=2)]
-    return Problem(f"poisson2d_{n}", xyz, etype, region, conn, [space], terms,
+    if basis == "perp":
+        space = Space(BF_PERP_EDGE, 1, dom, bnodes, np.zeros(bnodes.shape[0]))
+        f3 = np.zeros((nreg, 3))
+        f3[:, 2] = f[:, 0]
+        terms = [Term(T_CURL_CURL, 0, 0, dom, npoints, nu, dim=2),
+                 Term(T_SRC_EDGE, 0, -1, srcreg, npoints, f3, dim=2)]
+    elif basis == "node":
+        space = Space(BF_NODE, 1, dom, bnodes, np.zeros(bnodes.shape[0]))
+        terms = [Term(T_GRAD_GRAD, 0, 0, dom, npoints, nu, dim=2),
+                 Term(T_SRC_NODE, 0, -1, srcreg, npoints, f, dim=2)]
+    else:
+        raise ValueError(f"basis {basis!r}")
+    return Problem(f"poisson2d_{n}" + ("_az" if basis == "perp" else ""), xyz, etype, region, conn, [space], terms,
                    grid=(n + 1, n + 1),
                    meta=dict(n=n, seed=seed, ncls=K, cls_val=cls_val, ntri=ntri))
 

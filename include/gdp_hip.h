@@ -147,13 +147,17 @@ int gdp_solve_again(gdp_matrix *A, gdp_vector *b, gdp_solver *s, gdp_vector *x);
 #define GDP_TRI 2
 #define GDP_TET 4
 /* Basis functions: BF_Node (BF_Node.cpp:28) on NodesOf, BF_Edge (BF_Edge.cpp:23)
- * on EdgesOf. */
+ * on EdgesOf, BF_PerpendicularEdge (BF_Perpendicular.cpp:33-41: (0, 0, BF_Node),
+ * curl BF_CurlPerpendicularEdge :70-78) on NodesOf of triangles in the plane.
+ * On a perpendicular-edge space GDP_T_CURL_CURL, GDP_T_EDGE_MASS and
+ * GDP_T_SRC_EDGE take the 2D forms Form2P / Form1P (Get_Geometry.cpp:1929-1947). */
 #define GDP_BF_NODE 0
 #define GDP_BF_EDGE 1
+#define GDP_BF_PERP_EDGE 2
 /* Canonical terms recognised by Cal_InitGalerkinTermOfFemEquation
  * (Cal_GalerkinTermOfFemEquation.cpp:39-316) and supported on the device: */
 #define GDP_T_GRAD_GRAD 1  /* [ c * Dof{d u}, {d v} ]: c scalar or 3x3 tensor  */
-#define GDP_T_CURL_CURL 2  /* [ c * Dof{d a}, {d a} ], Whitney edges (tets)    */
+#define GDP_T_CURL_CURL 2  /* [ c * Dof{d a}, {d a} ]: c scalar or 3x3 tensor  */
 #define GDP_T_EDGE_MASS 3  /* [ c * Dof{a}, {a} ], Whitney edges (tets)        */
 #define GDP_T_NODE_MASS 4  /* [ c * Dof{u}, {u} ]                              */
 #define GDP_T_SRC_NODE 10  /* [ f, {u} ]: no Dof{}, f scalar                    */
@@ -171,7 +175,7 @@ typedef struct {
 } gdp_mesh_desc;
 
 typedef struct {
-  int bf;                    /* GDP_BF_NODE / GDP_BF_EDGE                     */
+  int bf;                    /* GDP_BF_NODE / GDP_BF_EDGE / GDP_BF_PERP_EDGE  */
   int numtype;               /* BasisFunction.Num (DOF key, DofData.cpp:36)   */
   int nsupport;
   const int *support;        /* support regions                              */

@@ -863,6 +863,16 @@ static void bf_curledge_tet(int k, double s[3])
   s[2] = C[k][2];
 }
 
+/* BF_CurlPerpendicularEdge (BF_Perpendicular.cpp:70-78): the node gradient
+ * in reference coordinates turned by -90 degrees, (dv, -du, dw). */
+static void bf_curlperp(int t, int i, double s[3])
+{
+  bf_gradnode(t, i, s);
+  double ss = s[0];
+  s[0] = s[1];
+  s[1] = -ss;
+}
+
 /* JacobianVol2D / JacobianVol3D / JacobianSur3D, Get_Geometry.cpp:1099-1124,
  * 1435-1469, 1629-1680.  dndu are the P1 gradients (constant). */
 static double jacobian(int t, int sur, const double *x, const double *y,
@@ -950,6 +960,21 @@ static void form2(const m3 *J, double d, const double u[3], double x[3])
   x[2] = (u[0] * J->c13 + u[1] * J->c23 + u[2] * J->c33) / d;
 }
 
+/* ChangeOfCoord_Form1P / Form2P (Get_Geometry.cpp:1931-1947), 2D only. */
+static void form1p(const m3 *J, const double u[3], double x[3])
+{
+  x[0] = 0.;
+  x[1] = 0.;
+  x[2] = u[2] / J->c33;
+}
+
+static void form2p(const m3 *J, double d, const double u[3], double x[3])
+{
+  x[0] = (u[0] * J->c11 + u[1] * J->c21) / d;
+  x[1] = (u[0] * J->c12 + u[1] * J->c22) / d;
+  x[2] = 0.;
+}
+
 /* Cal_ProductValue (Cal_Value.cpp:626-900): coefficient times the DOF-side
  * value; scalar x scalar, scalar x vector, tensor x vector. */
 static void coef_product(int ncoef, const double *c, const double v[3],
@@ -979,15 +1004,17 @@ static int term_is_symmetric(const orc_term *t)
   return t->kind < 10 && t->equ == t->dof;
 }
 
-static int term_nbf(const orc_term *t, int etype)
+static int term_nbf(const orc_term *t, int perp, int etype)
 {
-  if(t->kind == ORC_T_CURL_CURL || t->kind == ORC_T_EDGE_MASS ||
-     t->kind == ORC_T_SRC_EDGE)
+  if(!perp && (t->kind == ORC_T_CURL_CURL || t->kind == ORC_T_EDGE_MASS ||
+               t->kind == ORC_T_SRC_EDGE))
     return nedges_of(etype);
   return nnodes_of(etype);
 }
 
-static int elem_matrix_impl(const orc_mesh *m, const orc_term *t, int e,
+/* Element matrix of one term.  perp: the test space is BF_PerpendicularEdge
+ * (triangles in the plane only). */
+static int elem_matrix_impl(const orc_mesh *m, const orc_term *t, int perp, int e,
                             double Ek[MAXK][MAXK], int *kequ, int *kdof)
 {
   int et = m->etype[e];
@@ -1003,7 +1030,8 @@ static int elem_matrix_impl(const orc_mesh *m, const orc_term *t, int e,
   if(reg < 0 || reg >= t->coef_nreg) return -1;
   const double *coef = t->coef + (size_t)reg * t->ncoef;
   int sym = term_is_symmetric(t);
-  int ke = term_nbf(t, et);
+  if(perp && (et != ORC_TRI || t->jac_sur)) return -4;
+  int ke = term_nbf(t, perp, et);
   int kd = t->kind >= 10 ? 1 : ke;
   int is_src = t->kind >= 10;
   int dim3 = (et == ORC_TET) || t->jac_sur;
@@ -1012,8 +1040,8 @@ static int elem_matrix_impl(const orc_mesh *m, const orc_term *t, int e,
   for(int i = 0; i < MAXK; i++)
     for(int j = 0; j < MAXK; j++) Ek[i][j] = 0.;
   int esign[6] = {1, 1, 1, 1, 1, 1};
-  if(t->kind == ORC_T_CURL_CURL || t->kind == ORC_T_EDGE_MASS ||
-     t->kind == ORC_T_SRC_EDGE) {
+  if(!perp && (t->kind == ORC_T_CURL_CURL || t->kind == ORC_T_EDGE_MASS ||
+                t->kind == ORC_T_SRC_EDGE)) {
     if(!m->edges) return -2;
     for(int k = 0; k < ke; k++) esign[k] = m->edges[(size_t)e * 6 + k] < 0 ? -1 : 1;
   }
@@ -1022,12 +1050,24 @@ static int elem_matrix_impl(const orc_mesh *m, const orc_term *t, int e,
     if(gauss_point(et, t->npoints, q, &u, &v, &w, &wt)) return -3;
     m3 J, I;
     double det = jacobian(et, t->jac_sur, x, y, z, &J);
-    int need_inv = (t->kind == ORC_T_GRAD_GRAD || t->kind == ORC_T_EDGE_MASS ||
-                    t->kind == ORC_T_SRC_EDGE);
+    int need_inv = !perp && (t->kind == ORC_T_GRAD_GRAD || t->kind == ORC_T_EDGE_MASS ||
+                             t->kind == ORC_T_SRC_EDGE);
     if(need_inv) inverse(dim3, det, &J, &I);
     double be[MAXK][3], bd[MAXK][3];
     for(int i = 0; i < ke; i++) {
       double s[3];
+      if(perp) {
+        if(t->kind == ORC_T_CURL_CURL) {
+          bf_curlperp(et, i, s);
+          form2p(&J, det, s, be[i]);
+        }
+        else { /* BF_PerpendicularEdge value (0, 0, BF_Node), :33-41 */
+          s[0] = s[1] = 0.;
+          s[2] = bf_node(et, i, u, v, w);
+          form1p(&J, s, be[i]);
+        }
+        continue;
+      }
       switch(t->kind) {
       case ORC_T_GRAD_GRAD:
         bf_gradnode(et, i, s);
@@ -1071,12 +1111,15 @@ static int elem_matrix_impl(const orc_mesh *m, const orc_term *t, int e,
     }
     double factor = 1. * (wt * fabs(det));
     int prod1 = (t->kind == ORC_T_NODE_MASS || t->kind == ORC_T_SRC_NODE);
+    /* Cal_Product3 for FORM1P test functions */
+    int prod3 = perp && (t->kind == ORC_T_EDGE_MASS || t->kind == ORC_T_SRC_EDGE);
     for(int i = 0; i < ke; i++) {
       int jmax = sym ? i + 1 : kd;
       for(int j = 0; j < jmax; j++) {
         double pr = prod1 ? be[i][0] * bd[j][0]
-                          : be[i][0] * bd[j][0] + be[i][1] * bd[j][1] +
-                              be[i][2] * bd[j][2];
+                    : prod3 ? be[i][2] * bd[j][2]
+                            : be[i][0] * bd[j][0] + be[i][1] * bd[j][1] +
+                                be[i][2] * bd[j][2];
         Ek[i][j] += factor * pr;
       }
     }
@@ -1087,12 +1130,12 @@ static int elem_matrix_impl(const orc_mesh *m, const orc_term *t, int e,
   return 0;
 }
 
-int orc_element_matrix(const orc_mesh *m, const orc_term *t, int e, double *Ek,
-                       int *k_out)
+int orc_element_matrix(const orc_mesh *m, const orc_space *sp, const orc_term *t, int e,
+                       double *Ek, int *k_out)
 {
   double E[MAXK][MAXK];
   int ke, kd;
-  int r = elem_matrix_impl(m, t, e, E, &ke, &kd);
+  int r = elem_matrix_impl(m, t, sp[t->equ].bf == ORC_BF_PERP_EDGE, e, E, &ke, &kd);
   if(r) return r;
   for(int i = 0; i < ke; i++)
     for(int j = 0; j < kd; j++) Ek[i * kd + j] = E[i][j];
@@ -1128,7 +1171,7 @@ static int space_entity(const orc_mesh *m, const orc_space *s, int e, int k,
                         int *sign)
 {
   *sign = 1;
-  if(s->bf == ORC_BF_NODE) return m->conn[(size_t)e * 4 + k];
+  if(s->bf != ORC_BF_EDGE) return m->conn[(size_t)e * 4 + k];
   int v = m->edges[(size_t)e * 6 + k];
   *sign = v < 0 ? -1 : 1;
   return v < 0 ? -v : v;
@@ -1148,7 +1191,7 @@ int orc_number_dofs(const orc_mesh *m, int nspaces, const orc_space *sp,
       for(int q = 0; q < 2; q++) {
         int s = sps[q];
         if(s < 0) continue;
-        int nb = sp[s].bf == ORC_BF_NODE ? nnodes_of(m->etype[e]) : nedges_of(m->etype[e]);
+        int nb = sp[s].bf != ORC_BF_EDGE ? nnodes_of(m->etype[e]) : nedges_of(m->etype[e]);
         for(int k = 0; k < nb; k++) {
           int sg;
           int ent = space_entity(m, &sp[s], e, k, &sg);
@@ -1224,7 +1267,7 @@ long orc_assemble(const orc_mesh *m, int nspaces, const orc_space *sp,
       if(!term_active(m, sp, tt, e)) continue;
       double Ek[MAXK][MAXK];
       int ke, kd;
-      if(elem_matrix_impl(m, tt, e, Ek, &ke, &kd)) {
+      if(elem_matrix_impl(m, tt, sp[tt->equ].bf == ORC_BF_PERP_EDGE, e, Ek, &ke, &kd)) {
         for(int i = 0; i < N; i++) {
           free(rows[i].col);
           free(rows[i].val);

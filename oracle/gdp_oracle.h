@@ -95,7 +95,8 @@ int orc_number_edges(int ne, const int *etype, const int *conn, int nvisit,
 /* ---- Assembly ----------------------------------------------------------- */
 enum {
   ORC_BF_NODE = 0, /* BF_Node on NodesOf */
-  ORC_BF_EDGE = 1  /* BF_Edge on EdgesOf */
+  ORC_BF_EDGE = 1, /* BF_Edge on EdgesOf */
+  ORC_BF_PERP_EDGE = 2 /* BF_PerpendicularEdge on NodesOf (2D, Form1P) */
 };
 enum {
   /* bilinear terms (Equ space, Dof space) */
@@ -107,6 +108,10 @@ enum {
   ORC_T_SRC_NODE = 10,   /* [ f, {u} ]  scalar f, FORM0 test              */
   ORC_T_SRC_EDGE = 11    /* [ f, {a} ]  vector f, FORM1 test              */
 };
+/* On an ORC_BF_PERP_EDGE space the edge kinds take the 2D perpendicular forms:
+ * CURL_CURL is FORM2P x FORM2P (ChangeOfCoord_Form2P, Cal_Product123),
+ * EDGE_MASS and SRC_EDGE are FORM1P (ChangeOfCoord_Form1P, Cal_Product3),
+ * Get_Geometry.cpp:1929-1947, 1965-1976. */
 
 typedef struct {
   int bf;           /* ORC_BF_NODE / ORC_BF_EDGE */
@@ -160,7 +165,7 @@ long orc_assemble(const orc_mesh *m, int nspaces, const orc_space *sp,
                   long max_nnz, double *b);
 
 /* Element matrix of one term on one element (for closed-form tests). */
-int orc_element_matrix(const orc_mesh *m, const orc_term *t, int e,
+int orc_element_matrix(const orc_mesh *m, const orc_space *sp, const orc_term *t, int e,
                        double *Ek /* k*k row-major */, int *k_out);
 
 #ifdef __cplusplus

@@ -307,7 +307,7 @@ def element_matrix(problem, term_index, e):
     keep, m, sps, ts = _describe(problem)
     Ek = np.zeros(144)
     k = np.zeros(2, np.int32)
-    r = lib().orc_element_matrix(C.byref(m), C.byref(ts[term_index]), int(e), _d(Ek), _i(k))
+    r = lib().orc_element_matrix(C.byref(m), sps, C.byref(ts[term_index]), int(e), _d(Ek), _i(k))
     if r:
         raise RuntimeError(f"orc_element_matrix failed {r}")
     return Ek[: k[0] * k[1]].reshape(k[0], k[1])

@@ -10,6 +10,23 @@ from getdp_amd import problems as P
 
 pytestmark = pytest.mark.gpu
 
+
+def az_general(n=12, seed=6):
+    """a_z (BF_PerpendicularEdge) on a jittered mesh: general triangles, an
+    anisotropic tensor reluctivity per region and a mass term."""
+    pr = P.poisson2d(n, seed=seed, field_kind="lognormal", sigma=1.0, nbins=8, basis="perp")
+    rng = np.random.default_rng(seed)
+    inner = np.all((pr.xyz[:, :2] > 0) & (pr.xyz[:, :2] < 1), axis=1)
+    pr.xyz[inner, :2] += rng.uniform(-0.3 / n, 0.3 / n, size=(int(inner.sum()), 2))
+    nu = pr.terms[0].coef[:, 0]
+    C = np.zeros((nu.shape[0], 9))
+    C[:, 0], C[:, 1], C[:, 3], C[:, 4], C[:, 8] = nu, 0.3 * nu, -0.2 * nu, 2.0 * nu, nu
+    pr.terms[0].coef = C
+    dom = pr.terms[0].regions
+    pr.terms.append(P.Term(P.T_EDGE_MASS, 0, 0, dom, 3, np.full((C.shape[0], 1), 1e-2), dim=2))
+    return pr
+
+
 CASES = {
     "poisson2d_32": lambda: P.poisson2d(32),
     "poisson2d_lognormal_24": lambda: P.poisson2d(24, field_kind="lognormal", sigma=1.5, nbins=16),
@@ -19,6 +36,8 @@ CASES = {
     "elasticity3d_1pt": lambda: P.elasticity3d(2, nbins=4, npoints=1),
     "curlcurl3d_4": lambda: P.curlcurl3d(4),
     "curlcurl3d_1pt": lambda: P.curlcurl3d(3, npoints=1),
+    "poisson2d_az_32": lambda: P.poisson2d(32, basis="perp"),
+    "poisson2d_az_general": az_general,
 }
 
 

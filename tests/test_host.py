@@ -67,6 +67,7 @@ PROBLEMS = {
     "poisson2d_lognormal_12": lambda: P.poisson2d(12, field_kind="lognormal", nbins=8),
     "elasticity3d_3": lambda: P.elasticity3d(3, nbins=8),
     "curlcurl3d_3": lambda: P.curlcurl3d(3),
+    "poisson2d_az_12": lambda: P.poisson2d(12, basis="perp"),
 }
 
 
@@ -96,6 +97,32 @@ def test_unsupported_term_is_rejected():
     from getdp_amd.linalg import FESystem
     pr = P.poisson2d(4)
     pr.terms[0].kind = 77
+    with pytest.raises(GdpError) as e:
+        FESystem(pr)
+    assert e.value.code == -4
+
+
+def test_perpendicular_edge_limits_are_rejected():
+    """BF_PerpendicularEdge is a 2D form: grad-grad on it, a nodal DOF side,
+    tetrahedra and Jacobian Sur are refused (the reference's CPU path keeps
+    them)."""
+    from getdp_amd.linalg import FESystem
+    pr = P.poisson2d(4, basis="perp")
+    pr.terms[0].kind = P.T_GRAD_GRAD
+    with pytest.raises(GdpError) as e:
+        FESystem(pr)
+    assert e.value.code == -4
+    pr = P.poisson2d(4, basis="perp")
+    pr.terms[0].jac_sur = 1
+    with pytest.raises(GdpError) as e:
+        FESystem(pr)
+    assert e.value.code == -4
+    pr = P.elasticity3d(1, nbins=2)
+    for s in pr.spaces:
+        s.bf = P.BF_PERP_EDGE
+    for t in pr.terms:
+        if t.kind == P.T_GRAD_GRAD:
+            t.kind = P.T_CURL_CURL
     with pytest.raises(GdpError) as e:
         FESystem(pr)
     assert e.value.code == -4

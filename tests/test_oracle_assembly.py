@@ -218,3 +218,57 @@ def test_dof_numbering_hand_computed(oracle):
             if node not in free:
                 f = list(pr.spaces[s].fixed_entity).index(node)
                 assert tab[s, node] == -(f + 1)
+
+
+def _rot(g):
+    """curl (phi e_z) in the plane: (d phi/dy, -d phi/dx, 0)."""
+    return np.array([g[1], -g[0], 0.0])
+
+
+def test_perpendicular_edge_closed_forms(oracle):
+    """BF_PerpendicularEdge (a_z): curl-curl = area * (R g_i) . C (R g_j) with
+    R the -90 degree turn of the node gradients; mass = area (1 + d_ij) / 12;
+    source = f_z area / 3 (only the z component enters, Cal_Product3).  The
+    anisotropic tensor tells a turned gradient from a swapped one."""
+    X = np.array([[0.1, 0.2], [1.3, 0.4], [0.5, 1.7]])
+    T = np.c_[np.ones(3), X]
+    G = np.linalg.inv(T)[1:].T
+    G3 = np.c_[G, np.zeros(3)]
+    area = 0.5 * abs(np.linalg.det(T))
+    s = W3 / 0.5
+    pr = one_element(X, [1, 2, 3], P.TRI, P.T_CURL_CURL, coef=[2.5], bf=P.BF_PERP_EDGE)
+    K = oracle.element_matrix(pr, 0, 0)
+    assert np.allclose(K, s * area * 2.5 * G @ G.T, rtol=1e-13, atol=1e-15)
+    C = np.array([[2.0, 0.7, 0.0], [-0.4, 1.0, 0.0], [0.0, 0.0, 3.0]])
+    pr = one_element(X, [1, 2, 3], P.TRI, P.T_CURL_CURL, ncoef=9, coef=C.ravel(), bf=P.BF_PERP_EDGE)
+    K = oracle.element_matrix(pr, 0, 0)
+    ref = np.array([[_rot(G3[max(i, j)]) @ C @ _rot(G3[min(i, j)]) for j in range(3)] for i in range(3)])
+    assert np.allclose(K, s * area * ref, rtol=1e-13, atol=1e-15)
+    swapped = np.array([[G3[max(i, j)][[1, 0, 2]] @ C @ G3[min(i, j)][[1, 0, 2]] for j in range(3)]
+                        for i in range(3)])
+    assert not np.allclose(K, s * area * swapped, rtol=1e-6)
+    pr = one_element(X, [1, 2, 3], P.TRI, P.T_EDGE_MASS, coef=[0.3], bf=P.BF_PERP_EDGE)
+    M = oracle.element_matrix(pr, 0, 0)
+    assert np.allclose(M, s * area * 0.3 * (np.ones((3, 3)) + np.eye(3)) / 12, rtol=1e-12)
+    pr = one_element(X, [1, 2, 3], P.TRI, P.T_SRC_EDGE, ncoef=3, coef=[5.0, -7.0, -1.5],
+                     bf=P.BF_PERP_EDGE, dof=-1)
+    f = oracle.element_matrix(pr, 0, 0).ravel()
+    assert np.allclose(f, s * area * -1.5 / 3, rtol=1e-12)
+
+
+def test_perpendicular_edge_is_the_nodal_problem(oracle):
+    """The a_z formulation of the 2D problem assembles the nodal Poisson
+    system: same DOF numbering, pattern and right-hand side, matrix entries
+    equal up to rounding (J/detJ against J^-1)."""
+    rng = np.random.default_rng(5)
+    prs = [P.poisson2d(10, seed=4, field_kind="lognormal", sigma=1.0, nbins=8, basis=b)
+           for b in ("node", "perp")]
+    inner = np.all((prs[0].xyz[:, :2] > 0) & (prs[0].xyz[:, :2] < 1), axis=1)
+    jit = rng.uniform(-0.03, 0.03, size=(int(inner.sum()), 2))
+    for pr in prs:
+        pr.xyz[inner, :2] += jit
+    (ia, ja, a, b, tab), (ia2, ja2, a2, b2, tab2) = (oracle.assemble(pr) for pr in prs)
+    assert np.array_equal(ia, ia2) and np.array_equal(ja, ja2) and np.array_equal(tab, tab2)
+    assert np.array_equal(b, b2)
+    assert np.abs(a - a2).max() <= 1e-13 * np.abs(a).max()
+    assert not np.array_equal(a, a2)      # the two change-of-coordinate paths round differently
