@@ -30,10 +30,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("s to 1e-10 residual (learned vs plain precond); assembly elems/s; SpMV %HBM")
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per SpMV launch on the elasticity3d_n matrix, from two rocprofv3
-# --pmc runs of `bench.py --roofline-only` (FETCH_SIZE x 2.000 + WRITE_SIZE x
-# 1.000, factors from a 512 MiB device copy): 457.7 MB read + 6.5 MB written.
-PMC_SPMV_TRAFFIC = {64: 464_200_000}
+# HBM bytes per SpMV launch on the CG operator of elasticity3d_n (RCMK order,
+# 16-bit column offsets), from two rocprofv3 --pmc runs of `bench.py
+# --roofline-only` (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
+# 512 MiB device copy): 403.5 MB read + 6.5 MB written.
+PMC_SPMV_TRAFFIC = {64: 409_978_000}
 
 
 def log(*a):
@@ -182,9 +183,12 @@ def roofline_only(n: int, reps: int):
     A = LA.Matrix(fe.N)
     b = LA.Vector(fe.N)
     fe.generate(A, b)
-    xs = LA.Vector(fe.N, np.random.default_rng(0).standard_normal(fe.N))
-    ys = LA.Vector(fe.N)
-    ms = LA.bench_spmv(A, xs, ys, reps=reps)
+    x = LA.Vector(fe.N)
+    # one CG iteration builds the solver's renumbered SELL copy (the CG operator)
+    s = LA.Solver(Algorithm=1, Preconditioner=0, Scaling=1, Nb_Iter_Max=1, Stopping_Test=1e-10,
+                  Renumbering_Technique=1)
+    LA.solve(A, b, s, x)
+    ms = LA.bench_solver_spmv(s, reps=reps)["ms"]
     src = torch.empty(64 * 2 ** 20, dtype=torch.float64, device="cuda:0").uniform_()
     dst = torch.empty_like(src)
     for _ in range(5):
@@ -256,17 +260,21 @@ def main():
     E = int((pr.etype == 4).sum())
     asm_s = float(np.mean(asm_ms)) / 1e3
 
-    # SpMV roofline on the assembled matrix (the dominant kernel of the solve)
-    xs = LA.Vector(fe.N, np.random.default_rng(0).standard_normal(fe.N))
-    ys = LA.Vector(fe.N)
-    spmv_ms = LA.bench_spmv(A, xs, ys, reps=50)
+    # SpMV roofline on the CG's operator: the solver's RCMK-renumbered, scaled
+    # SELL copy (the SpMV is the dominant kernel of the solve)
+    sp = LA.bench_solver_spmv(solver, reps=50)
+    spmv_ms = sp["ms"]
     sb = spmv_bytes(fe.N, fe.nnz)
     achieved = sb / (spmv_ms * 1e-3) / 1e9
+    cols = "16-bit column offsets" if sp["compressed"] else "32-bit columns"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": PMC_SPMV_TRAFFIC.get(args.n),
-                "kernel": "k_sell_spmv (SELL-64, FP64, one row per lane)", "algorithmic_bytes": sb,
-                "avg_ms": round(spmv_ms, 5)}
+                "kernel": f"k_sell_spmv<2> on the CG operator (RCMK order, SELL-64, FP64, {cols})",
+                "algorithmic_bytes": sb, "avg_ms": round(spmv_ms, 5)}
+    xs = LA.Vector(fe.N, np.random.default_rng(0).standard_normal(fe.N))
+    ys = LA.Vector(fe.N)
+    spmv_natural_ms = LA.bench_spmv(A, xs, ys, reps=50)
 
     out = {
         "metric": METRIC,
@@ -296,6 +304,7 @@ def main():
         "assembly_elems_per_s": round(E / asm_s, 1),
         "assembly_gbs": round(assembly_bytes(pr, fe.N, fe.nnz) / asm_s / 1e9, 1),
         "spmv_hbm_frac": roofline["frac"],
+        "spmv_natural_order_ms": round(spmv_natural_ms, 5),
         "roofline": roofline,
         "cpu_baseline": None,
     }

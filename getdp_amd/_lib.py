@@ -80,6 +80,8 @@ SIGNATURES = [
     ("gdp_matrix_get_csr", C.c_int, [_vp, _ip, _ip, _dp]),
     ("gdp_prod_matrix_vector", C.c_int, [_vp, _vp, _vp]),
     ("gdp_bench_spmv", C.c_int, [_vp, _vp, _vp, C.c_int, _dp]),
+    ("gdp_matrix_sell_info", C.c_int, [_vp, C.POINTER(C.c_long), _ip]),
+    ("gdp_solver_bench_spmv", C.c_int, [_vp, C.c_int, _dp, _ip]),
     ("gdp_solve", C.c_int, [_vp, _vp, _vp, _vp]),
     ("gdp_solve_again", C.c_int, [_vp, _vp, _vp, _vp]),
     ("gdp_fe_create", C.c_int, [C.POINTER(MeshDesc), C.c_int, C.POINTER(SpaceDesc), C.c_int,

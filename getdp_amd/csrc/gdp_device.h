@@ -145,7 +145,9 @@ struct DevSell {
   long nnz = 0, nnz_pad = 0;
   long *sp = nullptr;     // slice start offsets (nslices + 1)
   int *w = nullptr;       // slice widths
-  int *col = nullptr;     // nnz_pad
+  int *col = nullptr;     // nnz_pad (null when the columns are compressed)
+  unsigned *c16 = nullptr;  // compressed columns: nnz_pad / 2 words
+  int *cbase = nullptr;     // compressed columns: one base per pair group
   double *val = nullptr;  // nnz_pad
   float *val32 = nullptr;
   void release()
@@ -153,6 +155,8 @@ struct DevSell {
     dfree(sp);
     dfree(w);
     dfree(col);
+    dfree(c16);
+    dfree(cbase);
     dfree(val);
     dfree(val32);
     n = nslices = 0;

@@ -3,6 +3,7 @@
 
 #include "gdp_device.h"
 #include "ilu.h"
+#include "sell.h"
 
 namespace gdp {
 
@@ -25,12 +26,7 @@ struct KCtl {
 };
 
 // Operator: y = A x with values of type TA (SELL-64 layout, sell.h).
-template <typename TA> struct DevOp {
-  int n = 0;
-  const long *sp = nullptr;
-  const int *w = nullptr, *col = nullptr;
-  const TA *val = nullptr;
-};
+template <typename TA> using DevOp = SellOp<TA>;
 
 // Preconditioner applied as z = M^{-1} v (right preconditioning).
 enum PrecKind { PREC_NONE = 0, PREC_ILU = 1, PREC_USER = 2, PREC_INNER32 = 3 };

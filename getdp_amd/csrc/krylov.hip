@@ -45,35 +45,29 @@ __device__ inline bool gated(const KCtl *C, int mode)
 
 // ---- operator -------------------------------------------------------------------
 template <typename TA, typename TV>
-__global__ void __launch_bounds__(256) k_op(int n, const long *__restrict__ sp,
-                                            const int *__restrict__ w, const int *__restrict__ col,
-                                            const TA *__restrict__ val, const TV *__restrict__ x,
+__global__ void __launch_bounds__(256) k_op(SellOp<TA> A, const TV *__restrict__ x,
                                             TV *__restrict__ y, const KCtl *C, int gmode)
 {
   if(gated(C, gmode)) return;
   const int row = blockIdx.x * 256 + threadIdx.x;
-  if(row >= n) return;
+  if(row >= A.n) return;
   double sum;
-  sell_row<TA, TV>(row, sp, w, col, val, x, sum);
+  sell_row<TA, TV>(row, A, x, sum);
   y[row] = (TV)sum;
 }
 
 // y = A x fused with the partial sums of |y|^2 (nrm0 of mgsro) or of <d, y>.
 template <typename TA, typename TV, bool WITH_D>
-__global__ void __launch_bounds__(256) k_op_dot(int n, const long *__restrict__ sp,
-                                                const int *__restrict__ w,
-                                                const int *__restrict__ col,
-                                                const TA *__restrict__ val,
-                                                const TV *__restrict__ x, const TV *__restrict__ d,
-                                                TV *__restrict__ y, const KCtl *C, int gmode,
-                                                double *part)
+__global__ void __launch_bounds__(256) k_op_dot(SellOp<TA> A, const TV *__restrict__ x,
+                                                const TV *__restrict__ d, TV *__restrict__ y,
+                                                const KCtl *C, int gmode, double *part)
 {
   if(gated(C, gmode)) return;
   const int row = blockIdx.x * 256 + threadIdx.x;
   double t = 0.;
-  if(row < n) {
+  if(row < A.n) {
     double sum;
-    sell_row<TA, TV>(row, sp, w, col, val, x, sum);
+    sell_row<TA, TV>(row, A, x, sum);
     TV yv = (TV)sum;
     y[row] = yv;
     t = WITH_D ? (double)d[row] * (double)yv : (double)yv * (double)yv;
@@ -86,8 +80,8 @@ template <typename TA, typename TV>
 int op_apply(const DevOp<TA> &A, const TV *x, TV *y, const KCtl *C, int gmode)
 {
   if(!A.n) return GDP_OK;
-  hipLaunchKernelGGL((k_op<TA, TV>), dim3((A.n + 255) / 256), dim3(256), 0, ctx().stream, A.n, A.sp,
-                     A.w, A.col, A.val, x, y, C, gmode);
+  hipLaunchKernelGGL((k_op<TA, TV>), dim3((A.n + 255) / 256), dim3(256), 0, ctx().stream, A, x, y, C,
+                     gmode);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
@@ -812,8 +806,8 @@ template <typename TA, typename TV> int gm_cycle(const GmArgs<TA, TV> &ga)
       GDP_TRY(prec_apply(M, vk1, tmpz, C, 2, n));
       src = tmpz;
     }
-    hipLaunchKernelGGL((k_op_dot<TA, TV, false>), grow, dim3(256), 0, s, n, A.sp, A.w, A.col,
-                       A.val, src, (const TV *)nullptr, vk, C, 2, part);
+    hipLaunchKernelGGL((k_op_dot<TA, TV, false>), grow, dim3(256), 0, s, A, src, (const TV *)nullptr,
+                       vk, C, 2, part);
     fin<FIN_NRM0>(C, part, nrow, 0);
     if(ga.orth == 1) {
       dim3 gc2(ncgs);
@@ -978,8 +972,7 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
           GDP_TRY(prec_apply(M, p, z, C, 1, n));
           src = z;
         }
-        hipLaunchKernelGGL((k_op_dot<TA, TV, true>), grow, dim3(256), 0, s, n, A.sp, A.w, A.col,
-                           A.val, src, p, ap, C, 1, part);
+        hipLaunchKernelGGL((k_op_dot<TA, TV, true>), grow, dim3(256), 0, s, A, src, p, ap, C, 1, part);
         hipLaunchKernelGGL((k_cg_update2<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, p, ap, r, dx, C, part,
                            nrow, part2, it & 1);
         hipLaunchKernelGGL((k_cg_p2<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, r, p, C, part2, ncg, it & 1);

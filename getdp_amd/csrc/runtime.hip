@@ -483,6 +483,16 @@ static int matrix_sell(gdp_matrix *A)
   return GDP_OK;
 }
 
+extern "C" int gdp_matrix_sell_info(gdp_matrix *A, long *nnz_pad, int *compressed)
+{
+  if(!A || !nnz_pad || !compressed) return set_error(GDP_E_ARG, "gdp_matrix_sell_info: null argument");
+  if(!A->csr.ia) return set_error(GDP_E_STATE, "matrix has no assembled pattern");
+  GDP_TRY(matrix_sell(A));
+  *nnz_pad = A->sell.nnz_pad;
+  *compressed = A->sell.c16 != nullptr;
+  return sync();
+}
+
 extern "C" int gdp_prod_matrix_vector(gdp_matrix *A, const gdp_vector *x, gdp_vector *y)
 {
   if(!A || !x || !y || x->n != A->m || y->n != A->n)

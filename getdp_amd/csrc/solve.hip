@@ -81,18 +81,14 @@ __global__ void __launch_bounds__(BT) k_to_f32(long n, const double *a, float *b
 }
 
 // r = rhs - A x (FP64, SELL layout)
-__global__ void __launch_bounds__(256) k_residual(int n, const long *__restrict__ sp,
-                                                  const int *__restrict__ w,
-                                                  const int *__restrict__ col,
-                                                  const double *__restrict__ val,
-                                                  const double *__restrict__ x,
+__global__ void __launch_bounds__(256) k_residual(SellOp<double> A, const double *__restrict__ x,
                                                   const double *__restrict__ rhs,
                                                   double *__restrict__ r)
 {
   const int row = blockIdx.x * 256 + threadIdx.x;
-  if(row >= n) return;
+  if(row >= A.n) return;
   double sum;
-  sell_row<double, double>(row, sp, w, col, val, x, sum);
+  sell_row<double, double>(row, A, x, sum);
   r[row] = rhs[row] - sum;
 }
 
@@ -138,8 +134,8 @@ int norm2_dev(const double *x, int n, double *out)
 int residual_dev(const DevSell &A, const double *x, const double *rhs, double *r)
 {
   if(!A.n) return GDP_OK;
-  hipLaunchKernelGGL(k_residual, dim3((A.n + 255) / 256), dim3(256), 0, ctx().stream, A.n, A.sp, A.w,
-                     A.col, A.val, x, rhs, r);
+  hipLaunchKernelGGL(k_residual, dim3((A.n + 255) / 256), dim3(256), 0, ctx().stream, sell_op<double>(A), x,
+                     rhs, r);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
@@ -385,19 +381,10 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
   if(p.Precision == 2) {
     // FP64 flexible GMRES (residuals, Gram-Schmidt, updates) preconditioned
     // by Inner_Krylov_Size steps of FP32 GMRES (FP32 SpMV + FP32 ILU(0))
-    DevOp<double> op;
-    op.n = n;
-    op.sp = c.sell.sp;
-    op.w = c.sell.w;
-    op.col = c.sell.col;
-    op.val = c.sell.val;
+    DevOp<double> op = sell_op<double>(c.sell);
     if(!c.inner) c.inner.reset(new InnerSolver);
     InnerSolver &in = *c.inner;
-    in.A.n = n;
-    in.A.sp = c.sell.sp;
-    in.A.w = c.sell.w;
-    in.A.col = c.sell.col;
-    in.A.val = c.sell.val32;
+    in.A = sell_op<float>(c.sell);
     in.M = PrecOp<float>();
     if(p.Preconditioner == 6) {
       in.M.kind = PREC_ILU;
@@ -422,12 +409,7 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
     info.outer_its = kr.its;
   }
   else if(p.Precision == 0) {
-    DevOp<double> op;
-    op.n = n;
-    op.sp = c.sell.sp;
-    op.w = c.sell.w;
-    op.col = c.sell.col;
-    op.val = c.sell.val;
+    DevOp<double> op = sell_op<double>(c.sell);
     PrecOp<double> M;
     if(up && p.Algorithm == 9) {
       M.kind = PREC_USER;
@@ -450,12 +432,7 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
       GDP_TRY(dalloc(&c.rhs32, (size_t)n));
       GDP_TRY(dalloc(&c.sol32, (size_t)n));
     }
-    DevOp<float> op;
-    op.n = n;
-    op.sp = c.sell.sp;
-    op.w = c.sell.w;
-    op.col = c.sell.col;
-    op.val = c.sell.val32;
+    DevOp<float> op = sell_op<float>(c.sell);
     PrecOp<float> M;
     if(up && p.Algorithm == 9) {
       M.kind = PREC_USER;
@@ -564,4 +541,34 @@ extern "C" int gdp_solve(gdp_matrix *A, gdp_vector *b, gdp_solver *s, gdp_vector
 extern "C" int gdp_solve_again(gdp_matrix *A, gdp_vector *b, gdp_solver *s, gdp_vector *x)
 {
   return do_solve(s, A, b, x, true);
+}
+
+extern "C" int gdp_solver_bench_spmv(gdp_solver *s, int reps, double *avg_ms, int *compressed)
+{
+  if(!s || reps < 1 || !avg_ms || !compressed) return set_error(GDP_E_ARG, "gdp_solver_bench_spmv: bad arguments");
+  const DevSell &S = s->cache.sell;
+  if(!S.n) return set_error(GDP_E_STATE, "the solver has not solved a system yet");
+  double *x = nullptr, *y = nullptr;
+  GDP_TRY(dalloc(&x, (size_t)S.n));
+  int rc = dalloc(&y, (size_t)S.n);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0.f;
+  if(!rc) rc = hipMemsetAsync(x, 0, sizeof(double) * S.n, ctx().stream) == hipSuccess ? GDP_OK
+                                                                                     : set_error(GDP_E_HIP, "hipMemsetAsync");
+  if(!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = set_error(GDP_E_HIP, "hipEventCreate");
+  if(!rc) rc = sync();
+  if(!rc && hipEventRecord(e0, ctx().stream) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventRecord");
+  for(int r = 0; !rc && r < reps; r++) rc = sell_spmv(S, x, y);
+  if(!rc && hipEventRecord(e1, ctx().stream) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventRecord");
+  if(!rc && hipEventSynchronize(e1) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventSynchronize");
+  if(!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventElapsedTime");
+  if(e0) (void)hipEventDestroy(e0);
+  if(e1) (void)hipEventDestroy(e1);
+  dfree(x);
+  dfree(y);
+  GDP_TRY(rc);
+  *avg_ms = ms / reps;
+  *compressed = S.c16 != nullptr;
+  return GDP_OK;
 }

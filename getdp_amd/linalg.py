@@ -151,6 +151,24 @@ def prod_matrix_vector(A: Matrix, x: Vector, y: Vector):
     check(lib().gdp_prod_matrix_vector(A.h, x.h, y.h), "gdp_prod_matrix_vector")
 
 
+def sell_info(A: Matrix) -> dict:
+    """The matrix's SpMV layout: padded entries and whether the columns are
+    stored as 16-bit offsets."""
+    npad = C.c_long()
+    comp = C.c_int()
+    check(lib().gdp_matrix_sell_info(A.h, C.byref(npad), C.byref(comp)), "gdp_matrix_sell_info")
+    return {"nnz_pad": npad.value, "compressed": bool(comp.value)}
+
+
+def bench_solver_spmv(s: Solver, reps: int = 20) -> dict:
+    """Average device time (ms) of the SpMV on the solver's renumbered copy of
+    the last matrix it solved (the operator of its Krylov iteration)."""
+    ms = C.c_double()
+    comp = C.c_int()
+    check(lib().gdp_solver_bench_spmv(s.h, int(reps), C.byref(ms), C.byref(comp)), "gdp_solver_bench_spmv")
+    return {"ms": ms.value, "compressed": bool(comp.value)}
+
+
 def bench_spmv(A: Matrix, x: Vector, y: Vector, reps: int = 20) -> float:
     """Average device time (ms) of one CSR SpMV launch over `reps` launches."""
     v = C.c_double()

@@ -131,6 +131,14 @@ int gdp_prod_matrix_vector(gdp_matrix *A, const gdp_vector *x, gdp_vector *y);
  * bracketed by HIP events; *avg_ms = average kernel time per launch. */
 int gdp_bench_spmv(gdp_matrix *A, const gdp_vector *x, gdp_vector *y, int reps,
                    double *avg_ms);
+/* Measurement helpers, no counterpart in the reference.  The matrix's SpMV
+ * layout: padded entry count and whether its column indices are stored as
+ * 16-bit offsets (1) or 32-bit (0). */
+int gdp_matrix_sell_info(gdp_matrix *A, long *nnz_pad, int *compressed);
+/* `reps` launches of the SpMV on the solver's renumbered (and scaled) copy of
+ * the last matrix it solved -- the operator of its Krylov iteration --
+ * bracketed by HIP events; *avg_ms per launch, *compressed as above. */
+int gdp_solver_bench_spmv(gdp_solver *s, int reps, double *avg_ms, int *compressed);
 
 /* ---- solve ----------------------------------------------------------------- */
 /* LinAlg_Solve (LinAlg_SPARSKIT.cpp:572 -> solve_matrix, Sparskit.cpp:108):

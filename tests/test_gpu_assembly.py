@@ -112,3 +112,27 @@ def test_linalg_add_double_path(gpu, oracle):
     v.add_double(3, 2.0)
     v.assemble()
     assert v.get()[3] == 3.5
+
+
+def test_spmv_compressed_columns(gpu, oracle, monkeypatch):
+    """16-bit column offsets per pair group give the same SpMV bits as the
+    32-bit columns; an ordering whose groups span 65536 columns or more (the
+    component-blocked elasticity numbering) keeps the 32-bit columns."""
+    pr = P.poisson2d(48)
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    x = np.random.default_rng(3).standard_normal(ia.shape[0] - 1)
+    ys = []
+    for col32 in ("0", "1"):
+        monkeypatch.setenv("GDP_SELL_COL32", col32)
+        fe, A, bv = gpu.assemble(pr)
+        assert gpu.sell_info(A)["compressed"] == (col32 == "0")
+        y = gpu.Vector(fe.N)
+        gpu.prod_matrix_vector(A, gpu.Vector(fe.N, x), y)
+        ys.append(y.get())
+    assert np.array_equal(ys[0], ys[1])
+    y0 = oracle.amux(ia, ja, a, x)
+    assert np.abs(ys[0] - y0).max() <= 1e-14 * np.abs(a).max() * np.abs(x).max() * 64
+    monkeypatch.setenv("GDP_SELL_COL32", "0")
+    big = P.elasticity3d(32, nbins=4)          # ux block, then uy, uz: ~36k apart per row
+    fe, A, bv = gpu.assemble(big)
+    assert not gpu.sell_info(A)["compressed"]
