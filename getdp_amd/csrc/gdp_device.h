@@ -95,26 +95,6 @@ template <typename T, int BLOCK> __device__ inline T block_sum(T v)
   return r;
 }
 
-// "Last block finishes" grid reduction: every block writes its partial(s);
-// the block that arrives last (atomic ticket) sums all partials in a fixed
-// order, so the result does not depend on scheduling.
-__device__ inline bool grid_last_block(unsigned *ticket)
-{
-  __shared__ bool last;
-  __threadfence();
-  __syncthreads();
-  if(threadIdx.x == 0) {
-    unsigned t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if(last) {
-    __threadfence();
-    if(threadIdx.x == 0) *ticket = 0;
-  }
-  return last;
-}
-
 // Sum nparts partials (any block size), deterministic, result in thread 0.
 template <typename T, int BLOCK> __device__ inline T sum_partials(const T *p, int nparts)
 {

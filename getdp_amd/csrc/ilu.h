@@ -34,8 +34,10 @@ struct IluFactor {
   int factor_status(int *zero_pivot_row);
   // x = (LU)^{-1} (scale * r); may be enqueued repeatedly without host sync.
   // gate/gmode: skip when a Krylov control block says the work has ended.
+  // addto (optional, distinct from x): also addto += x, in the same pass.
   template <typename TV>
-  int apply(const TV *r, TV scale, TV *x, const int *gate = nullptr, int gmode = 0);
+  int apply(const TV *r, TV scale, TV *x, const int *gate = nullptr, int gmode = 0,
+            TV *addto = nullptr);
   int reset_sweep_state();
   int check_timeout();
   unsigned long long wait_limit() const;

@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(256) k_bwd(int n, const int *__restrict__ ia,
                                              const int *__restrict__ diag,
                                              const TF *__restrict__ lu, TV *y, TV *x, int *err,
                                              unsigned long long tlimit, const int *gate,
-                                             int gmode)
+                                             int gmode, TV *addto)
 {
   if(is_gated(gate, gmode)) return;
   const int t = blockIdx.x * 256 + threadIdx.x;
@@ -217,7 +217,9 @@ __global__ void __launch_bounds__(256) k_bwd(int n, const int *__restrict__ ia,
         }
       }
       if(k >= kend) {
-        st_val(x + i, (TV)lu[di] * acc);
+        const TV xi = (TV)lu[di] * acc;
+        st_val(x + i, xi);
+        if(addto) addto[i] = addto[i] + xi;  // fused z += x of the caller
         done = true;
       }
     }
@@ -401,7 +403,7 @@ unsigned long long IluFactor::wait_limit() const
 }
 
 template <typename TV>
-int IluFactor::apply(const TV *r, TV scale, TV *x, const int *gate, int gmode)
+int IluFactor::apply(const TV *r, TV scale, TV *x, const int *gate, int gmode, TV *addto)
 {
   hipStream_t s = ctx().stream;
   if(!n) return GDP_OK;
@@ -413,13 +415,13 @@ int IluFactor::apply(const TV *r, TV scale, TV *x, const int *gate, int gmode)
     hipLaunchKernelGGL((k_fwd<double, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu64, r, scale, t,
                        x, err + 1, wait_limit(), gate, gmode);
     hipLaunchKernelGGL((k_bwd<double, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu64, t, x,
-                       err + 1, wait_limit(), gate, gmode);
+                       err + 1, wait_limit(), gate, gmode, addto);
   }
   else {
     hipLaunchKernelGGL((k_fwd<float, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu32, r, scale, t, x,
                        err + 1, wait_limit(), gate, gmode);
     hipLaunchKernelGGL((k_bwd<float, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu32, t, x, err + 1,
-                       wait_limit(), gate, gmode);
+                       wait_limit(), gate, gmode, addto);
   }
   GDP_HIP(hipGetLastError());
   return GDP_OK;
@@ -433,7 +435,7 @@ int IluFactor::check_timeout()
   return GDP_OK;
 }
 
-template int IluFactor::apply<double>(const double *, double, double *, const int *, int);
-template int IluFactor::apply<float>(const float *, float, float *, const int *, int);
+template int IluFactor::apply<double>(const double *, double, double *, const int *, int, double *);
+template int IluFactor::apply<float>(const float *, float, float *, const int *, int, float *);
 
 }  // namespace gdp

@@ -19,7 +19,6 @@ struct KCtl {
   double rtol, fpar3, fpar4, fpar5, fpar6, fpar7, alpha, beta;
   double scal, nrm0, thr, pend;
   double rr[2];  // CG: <r,r> of consecutive iterations (parity slots)
-  unsigned ticket[8];
   double g[kMaxKrylov + 2], c[kMaxKrylov + 1], s[kMaxKrylov + 1];
   double h1[kMaxKrylov + 2], h2[kMaxKrylov + 2];
   double H[(kMaxKrylov + 1) * (kMaxKrylov + 2)];  // column j: H[j*(kMaxKrylov+2) + i]

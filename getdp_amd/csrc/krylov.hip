@@ -43,6 +43,102 @@ __device__ inline bool gated(const KCtl *C, int mode)
   return (C->done | C->cycle_end) != 0;
 }
 
+// givens (iters.f:3235-3272)
+__device__ inline void givens(double &x, double &y, double &c, double &s)
+{
+  if(x == 0.0 && y == 0.0) {
+    c = 1.0;
+    s = 0.0;
+  }
+  else if(fabs(y) > fabs(x)) {
+    double t = x / y;
+    x = sqrt(1.0 + t * t);
+    s = copysign(1.0 / x, y);
+    c = t * s;
+  }
+  else if(fabs(y) <= fabs(x)) {
+    double t = y / x;
+    y = sqrt(1.0 + t * t);
+    c = copysign(1.0 / y, x);
+    s = t * c;
+  }
+  else {
+    x = 0.0;
+    y = 0.0;
+    c = 1.0;
+    s = 0.0;
+  }
+  x = fabs(x * y);
+}
+
+// End of Arnoldi step k (gmres label 50, iters.f:2076-2109), one thread.
+__device__ inline void gm_givens_step(int k, KCtl *C)
+{
+  C->its++;
+  if(C->ierr12 < 0) {
+    C->cycle_end = 1;
+    C->kfinal = k;
+    return;
+  }
+  double *hc = C->H + (long)(k - 1) * LDH;
+  for(int i = 0; i < k - 1; i++) {
+    double a0 = hc[i];
+    hc[i] = C->c[i] * a0 + C->s[i] * hc[i + 1];
+    hc[i + 1] = C->c[i] * hc[i + 1] - C->s[i] * a0;
+  }
+  double c, s;
+  givens(hc[k - 1], hc[k], c, s);
+  C->c[k - 1] = c;
+  C->s[k - 1] = s;
+  double alpha = -s * C->g[k - 1];
+  C->g[k - 1] = c * C->g[k - 1];
+  C->g[k] = alpha;
+  alpha = fabs(alpha);
+  C->fpar5 = alpha;
+  if(!(k < C->m && !(alpha <= C->fpar4) && (C->maxits <= 0 || C->its < C->maxits))) {
+    C->cycle_end = 1;
+    C->kfinal = k;
+  }
+}
+
+
+// Finalisations shared by the one-block kernels and by the last block of a
+// fused producer: mgsro's nrm0 = |v|^2 with the abnormal-number checks, and
+// the Hessenberg column of step k with nrm1 = |V_k| (MGS: h1, CGS2: h1 + h2).
+__device__ inline void fin_nrm0(KCtl *C, double tot)
+{
+  C->nrm0 = tot;
+  C->thr = tot * 0.98;
+  C->pend_flag = 0;
+  if(tot <= 0.0) C->ierr12 = -1;
+  else if(!(tot > 0.0 && 1.0 / tot > 0.0)) C->ierr12 = -2;
+}
+
+__device__ inline void fin_hcol(KCtl *C, int k, double tot, bool cgs)
+{
+  double nrm1 = sqrt(tot);
+  double *hc = C->H + (long)(k - 1) * LDH;
+  for(int i = 0; i < k; i++) hc[i] = cgs ? C->h1[i] + C->h2[i] : C->h1[i];
+  hc[k] = nrm1;
+  C->pend_flag = 0;
+  if(nrm1 <= 0.0) C->ierr12 = -3;
+  else C->scal = 1.0 / nrm1;
+}
+
+// out[i] = sum of part[i * np + j] over j, i < k: one wave per i, each lane
+// adds a strided subset in order, then the wave adds its lanes (a fixed order).
+template <int BLOCK> __device__ inline void wave_sums(const double *part, int np, int k, double *out)
+{
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  for(int i = wv; i < k; i += BLOCK / kWave) {
+    const double *p = part + (long)i * np;
+    double v = 0.;
+    for(int j = lane; j < np; j += kWave) v += p[j];
+    v = wave_sum(v);
+    if(lane == 0) out[i] = v;
+  }
+}
+
 // ---- operator -------------------------------------------------------------------
 template <typename TA, typename TV>
 __global__ void __launch_bounds__(256) k_op(SellOp<TA> A, const TV *__restrict__ x,
@@ -74,6 +170,20 @@ __global__ void __launch_bounds__(256) k_op_dot(SellOp<TA> A, const TV *__restri
   }
   t = block_sum<double, 256>(t);
   if(threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// t = v - A x (gated): the residual of the learned stage, one pass.
+template <typename TA, typename TV>
+__global__ void __launch_bounds__(256) k_op_resid(SellOp<TA> A, const TV *__restrict__ x,
+                                                  const TV *__restrict__ v, TV *__restrict__ t,
+                                                  const KCtl *C, int gmode)
+{
+  if(gated(C, gmode)) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if(row >= A.n) return;
+  double sum;
+  sell_row<TA, TV>(row, A, x, sum);
+  t[row] = v[row] - (TV)sum;
 }
 
 template <typename TA, typename TV>
@@ -180,11 +290,25 @@ __global__ void __launch_bounds__(B) k_mgs_norm(int n, int k, TV *__restrict__ v
 
 // CGS2: pass 1 h = V^T v; pass 2 v -= V h, h' = V^T v; pass 3 v -= V h', |v|^2.
 // One block owns one chunk of CGS_PER*B entries (grid = ceil(n / chunk)).
+// Pass 1 starts with mgsro's nrm0 test on the partials of |w|^2 left by the
+// SpMV (pnrm, nrow of them): every block sums them in the same order and
+// takes the same decision; block 0 records it in the control block.
 template <typename TV, int PASS, int CGS_PER>
 __global__ void __launch_bounds__(B) k_cgs(int n, int k, TV *__restrict__ vk,
-                                          const TV *__restrict__ V, const KCtl *C, double *part)
+                                          const TV *__restrict__ V, KCtl *C, double *part,
+                                          const double *pnrm, int nrow)
 {
   if(gated(C, 2) || C->ierr12) return;
+  if(PASS == 1) {
+    __shared__ int s_bad;
+    const double tot = sum_partials<double, B>(pnrm, nrow);
+    if(threadIdx.x == 0) {
+      s_bad = !(tot > 0.0 && 1.0 / tot > 0.0);
+      if(blockIdx.x == 0) fin_nrm0(C, tot);
+    }
+    __syncthreads();
+    if(s_bad) return;
+  }
   const int nb = gridDim.x;
   const long base = (long)blockIdx.x * B * CGS_PER;
   double x[CGS_PER];
@@ -227,6 +351,28 @@ __global__ void __launch_bounds__(B) k_cgs(int n, int k, TV *__restrict__ vk,
     for(int u = 0; u < CGS_PER; u++) ss += x[u] * x[u];
     double t = block_sum<double, B>(ss);
     if(threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+
+// CGS2 finalisation, one block of 1024 threads.  Passes 1, 2: h (h') into the
+// control block, one wave per coefficient.  Pass 3: the Hessenberg column and
+// nrm1, then the end of the Arnoldi step (Givens rotation), which also runs
+// when an abnormal |w| stopped the passes.
+constexpr int FB = 1024;
+template <int PASS>
+__global__ void __launch_bounds__(FB) k_fin_cgs(KCtl *C, const double *part, int nparts, int k)
+{
+  if(gated(C, 2)) return;
+  const bool stopped = C->ierr12 != 0;
+  if(PASS <= 2) {
+    if(!stopped) wave_sums<FB>(part, nparts, k, PASS == 1 ? C->h1 : C->h2);
+    return;
+  }
+  double tot = 0.;
+  if(!stopped) tot = sum_partials<double, FB>(part, nparts);
+  if(threadIdx.x == 0) {
+    if(!stopped) fin_hcol(C, k, tot, true);
+    gm_givens_step(k, C);
   }
 }
 
@@ -380,9 +526,6 @@ enum FinOp {
   FIN_MGS_DOT,    // h_i = V_k . V_i
   FIN_MGS_AXPY,   // re-orthogonalisation test, running norm update
   FIN_MGS_NORM,   // nrm1 = |V_k|, Hessenberg column
-  FIN_CGS1,
-  FIN_CGS2,
-  FIN_CGS3,
   FIN_CG_INIT,
   FIN_CG_PAP,     // brkdn + alpha = <r,r>/<p,Ap>
   FIN_CG_UPDATE   // stopbis (iters.f:3275-3330)
@@ -400,16 +543,6 @@ __global__ void __launch_bounds__(B) k_fin(KCtl *C, const double *part, int npar
   }
   else if(OP != FIN_CG_INIT) {
     if(gated(C, 2) || C->ierr12) return;
-  }
-  if(OP == FIN_CGS1 || OP == FIN_CGS2) {
-    for(int i = 0; i < idx; i++) {
-      double t = sum_partials<double, B>(part + (long)i * nparts, nparts);
-      if(threadIdx.x == 0) {
-        if(OP == FIN_CGS1) C->h1[i] = t;
-        else C->h2[i] = t;
-      }
-    }
-    return;
   }
   const double tot = sum_partials<double, B>(part, nparts);
   if(threadIdx.x) return;
@@ -435,11 +568,7 @@ __global__ void __launch_bounds__(B) k_fin(KCtl *C, const double *part, int npar
       C->scal = 1.0 / alpha;
   }
   else if(OP == FIN_NRM0) {
-    C->nrm0 = tot;
-    C->thr = tot * 0.98;
-    C->pend_flag = 0;
-    if(tot <= 0.0) C->ierr12 = -1;
-    else if(!(tot > 0.0 && 1.0 / tot > 0.0)) C->ierr12 = -2;
+    fin_nrm0(C, tot);
   }
   else if(OP == FIN_MGS_DOT) {
     C->h1[idx] = tot;
@@ -459,15 +588,8 @@ __global__ void __launch_bounds__(B) k_fin(KCtl *C, const double *part, int npar
     C->nrm0 = nrm0;
     C->thr = nrm0 * 0.98;
   }
-  else if(OP == FIN_MGS_NORM || OP == FIN_CGS3) {
-    const int k = idx;
-    double nrm1 = sqrt(tot);
-    double *hc = C->H + (long)(k - 1) * LDH;
-    for(int i = 0; i < k; i++) hc[i] = (OP == FIN_CGS3) ? C->h1[i] + C->h2[i] : C->h1[i];
-    hc[k] = nrm1;
-    C->pend_flag = 0;
-    if(nrm1 <= 0.0) C->ierr12 = -3;
-    else C->scal = 1.0 / nrm1;
+  else if(OP == FIN_MGS_NORM) {
+    fin_hcol(C, idx, tot, false);
   }
   else if(OP == FIN_CG_INIT) {
     C->its = 1;
@@ -515,63 +637,10 @@ __global__ void __launch_bounds__(B) k_fin(KCtl *C, const double *part, int npar
   }
 }
 
-// givens (iters.f:3235-3272)
-__device__ inline void givens(double &x, double &y, double &c, double &s)
-{
-  if(x == 0.0 && y == 0.0) {
-    c = 1.0;
-    s = 0.0;
-  }
-  else if(fabs(y) > fabs(x)) {
-    double t = x / y;
-    x = sqrt(1.0 + t * t);
-    s = copysign(1.0 / x, y);
-    c = t * s;
-  }
-  else if(fabs(y) <= fabs(x)) {
-    double t = y / x;
-    y = sqrt(1.0 + t * t);
-    c = copysign(1.0 / y, x);
-    s = t * c;
-  }
-  else {
-    x = 0.0;
-    y = 0.0;
-    c = 1.0;
-    s = 0.0;
-  }
-  x = fabs(x * y);
-}
-
-// End of Arnoldi step k (gmres label 50, iters.f:2076-2109).
 __global__ void k_gm_givens(int k, KCtl *C)
 {
   if(threadIdx.x || blockIdx.x || gated(C, 2)) return;
-  C->its++;
-  if(C->ierr12 < 0) {
-    C->cycle_end = 1;
-    C->kfinal = k;
-    return;
-  }
-  double *hc = C->H + (long)(k - 1) * LDH;
-  for(int i = 0; i < k - 1; i++) {
-    double a0 = hc[i];
-    hc[i] = C->c[i] * a0 + C->s[i] * hc[i + 1];
-    hc[i + 1] = C->c[i] * hc[i + 1] - C->s[i] * a0;
-  }
-  double c, s;
-  givens(hc[k - 1], hc[k], c, s);
-  C->c[k - 1] = c;
-  C->s[k - 1] = s;
-  double alpha = -s * C->g[k - 1];
-  C->g[k - 1] = c * C->g[k - 1];
-  C->g[k] = alpha;
-  alpha = fabs(alpha);
-  C->fpar5 = alpha;
-  if(!(k < C->m && !(alpha <= C->fpar4) && (C->maxits <= 0 || C->its < C->maxits))) {
-    C->cycle_end = 1;
-    C->kfinal = k;
-  }
+  gm_givens_step(k, C);
 }
 
 // label 200: back substitution of the rotated Hessenberg system.
@@ -639,22 +708,6 @@ __global__ void __launch_bounds__(B) k_copy_gated(int n, const TV *src, TV *dst,
   for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) dst[j] = src[j];
 }
 
-// t = v - t (gated): residual of a first preconditioning stage
-template <typename TV>
-__global__ void __launch_bounds__(B) k_rsub_gated(int n, const TV *v, TV *t, const KCtl *C, int gmode)
-{
-  if(gated(C, gmode)) return;
-  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) t[j] = v[j] - t[j];
-}
-
-// z += u (gated)
-template <typename TV>
-__global__ void __launch_bounds__(B) k_add_gated(int n, TV *z, const TV *u, const KCtl *C, int gmode)
-{
-  if(gated(C, gmode)) return;
-  for(long j = (long)blockIdx.x * B + threadIdx.x; j < n; j += (long)gridDim.x * B) z[j] = z[j] + u[j];
-}
-
 // label 60/70: restart or terminate.
 __global__ void k_gm_cycle_end(KCtl *C)
 {
@@ -697,10 +750,13 @@ Workspace &ws()
   return w;
 }
 
+// rows of one partial-sum slot of the workspace
+size_t part_rows(int n) { return (size_t)std::max(1024L, ((long)n + 255) / 256); }
+
 int ensure_ws(size_t vec_bytes, int n)
 {
   Workspace &W = ws();
-  size_t rows = (size_t)std::max(1024L, ((long)n + 255) / 256);
+  size_t rows = part_rows(n);
   size_t need = rows * (kMaxKrylov + 2);
   if(!W.ctl) GDP_TRY(dalloc(&W.ctl, 1));
   if(W.part_size < need) {
@@ -739,11 +795,11 @@ int prec_apply_two_stage(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *in, 
                          TV *u, KCtl *C, int gmode, int n)
 {
   GDP_TRY(user_prec_apply(M.user, in, out, C, gmode));
-  GDP_TRY(op_apply(A, out, t, C, gmode));
-  dim3 g(reduce_blocks(n));
-  hipLaunchKernelGGL(k_rsub_gated<TV>, g, dim3(B), 0, ctx().stream, n, in, t, C, gmode);
-  GDP_TRY(M.ilu->template apply<TV>(t, (TV)1, u, (const int *)C, gmode));
-  hipLaunchKernelGGL(k_add_gated<TV>, g, dim3(B), 0, ctx().stream, n, out, u, C, gmode);
+  if(n)
+    hipLaunchKernelGGL((k_op_resid<TA, TV>), dim3((n + 255) / 256), dim3(256), 0, ctx().stream, A, out,
+                       in, t, C, gmode);
+  // u = (LU)^{-1} t, and out += u in the backward sweep
+  GDP_TRY(M.ilu->template apply<TV>(t, (TV)1, u, (const int *)C, gmode, out));
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
@@ -783,6 +839,10 @@ template <typename TA, typename TV> int gm_cycle(const GmArgs<TA, TV> &ga)
   const int nrow = std::max(1, (n + 255) / 256);
   const int cper = cgs_per(n);
   const int ncgs = (int)std::max(1L, ((long)n + B * cper - 1) / (B * cper));
+  // partial sums: CGS coefficient i at [i * ncgs, (i + 1) * ncgs), inside the
+  // first kMaxKrylov slots of part_rows(n); the SpMV's |w|^2 in the slot after
+  // them (the workspace holds kMaxKrylov + 2 slots, ensure_ws)
+  double *pnrm = part + (size_t)kMaxKrylov * part_rows(n);
   dim3 g(nb), blk(B), grow(nrow);
   GDP_TRY(op_apply(A, sol, V, C, 1));
   hipLaunchKernelGGL(k_gm_resid<TV>, g, blk, 0, s, n, rhs, V, C, part);
@@ -806,29 +866,34 @@ template <typename TA, typename TV> int gm_cycle(const GmArgs<TA, TV> &ga)
       GDP_TRY(prec_apply(M, vk1, tmpz, C, 2, n));
       src = tmpz;
     }
+    // w = A z with |w|^2 partials (into pnrm, apart from the CGS partials)
     hipLaunchKernelGGL((k_op_dot<TA, TV, false>), grow, dim3(256), 0, s, A, src, (const TV *)nullptr,
-                       vk, C, 2, part);
-    fin<FIN_NRM0>(C, part, nrow, 0);
+                       vk, C, 2, pnrm);
     if(ga.orth == 1) {
-      dim3 gc2(ncgs);
+      // CGS2: three passes with one-block finalisations; the third ends the
+      // step (Hessenberg column, Givens rotation)
+      dim3 gc2(ncgs), gf(1), bf(FB);
       if(cper == 4) {
-        hipLaunchKernelGGL((k_cgs<TV, 1, 4>), gc2, blk, 0, s, n, k, vk, V, C, part);
-        fin<FIN_CGS1>(C, part, ncgs, k);
-        hipLaunchKernelGGL((k_cgs<TV, 2, 4>), gc2, blk, 0, s, n, k, vk, V, C, part);
-        fin<FIN_CGS2>(C, part, ncgs, k);
-        hipLaunchKernelGGL((k_cgs<TV, 3, 4>), gc2, blk, 0, s, n, k, vk, V, C, part);
-        fin<FIN_CGS3>(C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 1, 4>), gc2, blk, 0, s, n, k, vk, V, C, part, pnrm, nrow);
+        hipLaunchKernelGGL(k_fin_cgs<1>, gf, bf, 0, s, C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 2, 4>), gc2, blk, 0, s, n, k, vk, V, C, part, pnrm, nrow);
+        hipLaunchKernelGGL(k_fin_cgs<2>, gf, bf, 0, s, C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 3, 4>), gc2, blk, 0, s, n, k, vk, V, C, part, pnrm, nrow);
       }
       else {
-        hipLaunchKernelGGL((k_cgs<TV, 1, 1>), gc2, blk, 0, s, n, k, vk, V, C, part);
-        fin<FIN_CGS1>(C, part, ncgs, k);
-        hipLaunchKernelGGL((k_cgs<TV, 2, 1>), gc2, blk, 0, s, n, k, vk, V, C, part);
-        fin<FIN_CGS2>(C, part, ncgs, k);
-        hipLaunchKernelGGL((k_cgs<TV, 3, 1>), gc2, blk, 0, s, n, k, vk, V, C, part);
-        fin<FIN_CGS3>(C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 1, 1>), gc2, blk, 0, s, n, k, vk, V, C, part, pnrm, nrow);
+        hipLaunchKernelGGL(k_fin_cgs<1>, gf, bf, 0, s, C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 2, 1>), gc2, blk, 0, s, n, k, vk, V, C, part, pnrm, nrow);
+        hipLaunchKernelGGL(k_fin_cgs<2>, gf, bf, 0, s, C, part, ncgs, k);
+        hipLaunchKernelGGL((k_cgs<TV, 3, 1>), gc2, blk, 0, s, n, k, vk, V, C, part, pnrm, nrow);
       }
+      hipLaunchKernelGGL(k_fin_cgs<3>, gf, bf, 0, s, C, part, ncgs, k);
+      // V_k = w / nrm1; skipped when the step ended the cycle (V_k is then
+      // not used: the update combines V_0..V_{k-1} or Z_0..Z_{k-1})
+      hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, vk, C, 2);
     }
     else {
+      fin<FIN_NRM0>(C, pnrm, nrow, 0);
       for(int i = 0; i < k; i++) {
         hipLaunchKernelGGL(k_mgs_dot<TV>, g, blk, 0, s, n, i, vk, V, C, part);
         fin<FIN_MGS_DOT>(C, part, nb, i);
@@ -837,9 +902,9 @@ template <typename TA, typename TV> int gm_cycle(const GmArgs<TA, TV> &ga)
       }
       hipLaunchKernelGGL(k_mgs_norm<TV>, g, blk, 0, s, n, k, vk, V, C, part);
       fin<FIN_MGS_NORM>(C, part, nb, k);
+      hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, vk, C, 2);
+      hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, s, k, C);
     }
-    hipLaunchKernelGGL(k_scale<TV>, g, blk, 0, s, n, vk, C, 2);
-    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, s, k, C);
   }
   hipLaunchKernelGGL(k_gm_backsub, dim3(1), dim3(1), 0, s, C);
   if(flexible)

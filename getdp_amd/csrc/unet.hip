@@ -541,16 +541,21 @@ __global__ void k_upcat_adj(int B, int H, int W, int c1, int c2, const float *dx
   }
 }
 
-// 1x1 output convolution (16 -> 1), FP32
+// 1x1 output convolution (16 -> 1), FP32, at pixel p
+__device__ inline float out_pixel(const u16 *x, long p, int C, const float *w, const float *b)
+{
+  float s = b[0];
+  for(int c = 0; c < C; c++) s += bf2f(x[p * C + c]) * w[c];
+  return s;
+}
+
 __global__ void k_out_fwd(long P, int C, const u16 *x, const float *w, const float *b, float *z,
                           const int *gate, int gmode)
 {
   if(unet_gated(gate, gmode)) return;
   long p = (long)blockIdx.x * 256 + threadIdx.x;
   if(p >= P) return;
-  float s = b[0];
-  for(int c = 0; c < C; c++) s += bf2f(x[p * C + c]) * w[c];
-  z[p] = s;
+  z[p] = out_pixel(x, p, C, w, b);
 }
 
 __global__ void k_out_bwd_x(long P, int C, const float *dz, const float *w, float *dx)
@@ -700,11 +705,14 @@ __global__ void __launch_bounds__(256) k_pc_gather(int npix, int ndof, const flo
   }
 }
 
+// The network's output layer (the 1x1 convolution of the last activation x,
+// evaluated here at the DOF's pixel) mapped back to the solver's vector.
 template <typename TV>
 __global__ void __launch_bounds__(256) k_pc_scatter(int n, int ndof, const int *rperm,
                                                     const int *dof2pix, const double *rs,
-                                                    const double *dscale,
-                                                    const float *z, const double *part, int nparts,
+                                                    const double *dscale, const u16 *x,
+                                                    const float *ow, const float *ob,
+                                                    const double *part, int nparts,
                                                     int npix, int odd, TV *out, const int *gate,
                                                     int gmode)
 {
@@ -718,7 +726,9 @@ __global__ void __launch_bounds__(256) k_pc_scatter(int n, int ndof, const int *
     const int d = rperm ? rperm[i] : (int)i;
     const int px = dof2pix[d];
     // odd part N(r) - N(-r) removes what does not depend on the residual
-    double v = sc * (odd ? 0.5 * ((double)z[px] - (double)z[npix + px]) : (double)z[px]);
+    const float z0 = out_pixel(x, px, 16, ow, ob);
+    double v = sc * (odd ? 0.5 * ((double)z0 - (double)out_pixel(x, (long)npix + px, 16, ow, ob))
+                         : (double)z0);
     if(dscale) v *= dscale[d];  // back from D^{1/2} e
     if(rs) v /= rs[d];
     out[i] = (TV)v;
@@ -901,7 +911,8 @@ size_t forward_split_floats(const UserPrec &U, int B)
   return m;
 }
 
-int forward(UserPrec &U, int B, const int *gate, int gmode, bool may_alloc)
+// The convolutions, then (out_layer) the 1x1 output layer into U.z.
+int forward(UserPrec &U, int B, const int *gate, int gmode, bool may_alloc, bool out_layer = true)
 {
   for(size_t i = 0; i < U.L.size(); i++) {
     ConvArgs a = conv_args(U, U.L[i], B);
@@ -911,8 +922,9 @@ int forward(UserPrec &U, int B, const int *gate, int gmode, bool may_alloc)
     GDP_TRY(launch_conv(U, a, U.L[i].mode, may_alloc));
   }
   long P = (long)B * U.H * U.W;
-  hipLaunchKernelGGL(k_out_fwd, g1(P), dim3(256), 0, ctx().stream, P, 16, U.act.back(), U.ow, U.ob,
-                     U.z, gate, gmode);
+  if(out_layer)
+    hipLaunchKernelGGL(k_out_fwd, g1(P), dim3(256), 0, ctx().stream, P, 16, U.act.back(), U.ow, U.ob,
+                       U.z, gate, gmode);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
@@ -1063,11 +1075,11 @@ int user_prec_apply_t(UserPrec *u, const TV *in, TV *out, KCtl *C, int gmode)
   const int odd = u->odd && u->Bmax >= 2;
   hipLaunchKernelGGL(k_pc_gather, dim3((unsigned)std::min(1024, (npix + 255) / 256)), dim3(256), 0, s,
                      npix, u->n_dof, u->r32, u->pc_part, nparts, u->feat, u->act[0], odd, gate, gmode);
-  GDP_TRY(forward(*u, odd ? 2 : 1, gate, gmode, false));
+  // the output layer is evaluated in the scatter, at the DOF pixels only
+  GDP_TRY(forward(*u, odd ? 2 : 1, gate, gmode, false, false));
   hipLaunchKernelGGL(k_pc_scatter<TV>, dim3((unsigned)std::min(1024, (u->n_dof + 255) / 256)), dim3(256), 0,
-                     s, u->n_dof, u->n_dof, u->rperm, u->dof2pix, u->rs, u->dscale, u->z, u->pc_part,
-                     nparts, npix,
-                     odd, out, gate, gmode);
+                     s, u->n_dof, u->n_dof, u->rperm, u->dof2pix, u->rs, u->dscale, u->act.back(), u->ow,
+                     u->ob, u->pc_part, nparts, npix, odd, out, gate, gmode);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
