@@ -320,7 +320,7 @@ def train(net: UNet, r, e, feat, mask, steps: int, batch: int, lr: float = 1e-3,
 
 
 def solve_learned(A, b, x, net: UNet, maps: GridMaps, rtol: float = 1e-10,
-                  maxits: int = 1000, krylov: int = 30, precision: int = 0,
+                  maxits: int = 1000, krylov: int = 64, precision: int = 0,
                   fallback: bool = True, two_stage: bool = False, ilu_ordering: int = 0):
     """FGMRES(krylov) right-preconditioned by the network (two_stage: the
     network, then an ILU(0) correction of the residual it leaves); if it stops
