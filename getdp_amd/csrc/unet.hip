@@ -25,6 +25,7 @@
 // reduction entries 8*(l/16)..+7; the accumulator lane l holds column l % 16,
 // rows 4*(l/16)..+3.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -185,8 +186,11 @@ __device__ inline uint4 conv_load_a(const ConvArgs &a, long pix, int py, int px,
     for(int c = 0; c < 8; c++) o[c] = f2bf(0.25f * (bf2f(h0[c]) + bf2f(h1[c]) + bf2f(h2[c]) + bf2f(h3[c])));
     return v;
   }
-  if(ci0 < a.c1) return *(const uint4 *)(a.in + ((long)(cbh + (y >> 1)) * (W >> 1) + (x >> 1)) * a.c1 + ci0);
-  return *(const uint4 *)(a.in2 + (pix + (long)dy * W + dx) * (cin - a.c1) + (ci0 - a.c1));
+  // upsampled coarse channels or skip channels: one load from a selected address
+  const bool coarse = ci0 < a.c1;
+  const u16 *pc = a.in + ((long)(cbh + (y >> 1)) * (W >> 1) + (x >> 1)) * a.c1 + ci0;
+  const u16 *pf = a.in2 + (pix + (long)dy * W + dx) * (cin - a.c1) + (ci0 - a.c1);
+  return *(const uint4 *)(coarse ? pc : pf);
 }
 
 template <int MODE, int TM, int TN>
@@ -813,22 +817,35 @@ struct ConvPlan {
   long split_floats;  // partial-sum buffer needed (0 without split-K)
 };
 
+// Launch shape of one convolution: about one wave per SIMD (1024 on 256 CUs;
+// measured at batch 1, 256^2: 4 or 16 waves per SIMD are no faster, the
+// per-layer time is the same for every shape tried): 64 pixels per wave when
+// that fills the GPU, else 16; for smaller images, a 16-pixel tile shared by
+// the 4 waves of a workgroup (reduction split 4 ways), with narrower column
+// tiles until the grid has the target number of waves; for short reductions,
+// split-K over the grid.
 ConvPlan plan_conv(long P, int cout, int K_pad)
 {
   ConvPlan c;
+  // GDP_CONV_WAVES overrides the wave target (launch-shape sweeps)
+  static long target = -1;
+  if(target < 0) {
+    const char *e = std::getenv("GDP_CONV_WAVES");
+    target = (e && std::atol(e) > 0) ? std::atol(e) : 1024;
+  }
+  auto ntn = [&](int tn) { return (long)(cout + 16 * tn - 1) / (16 * tn); };
   c.TN = cout <= 16 ? 1 : (cout <= 32 ? 2 : 4);
-  const long ntn = (cout + 16 * c.TN - 1) / (16 * c.TN);
-  const long target = 1024;
   c.TM = 4;
   c.ks = false;
-  long waves = (P + 63) / 64 * ntn;
+  long waves = (P + 63) / 64 * ntn(c.TN);
   if(waves < target) {
     c.TM = 1;
-    waves = (P + 15) / 16 * ntn;
+    waves = (P + 15) / 16 * ntn(c.TN);
   }
   if(waves < target && K_pad >= 4 * 32) {
     // share each tile between 4 waves; no partial buffer, no extra kernel
     c.ks = true;
+    while(c.TN > 1 && 4 * ((P + 15) / 16) * ntn(c.TN) < target) c.TN >>= 1;
     c.nsplit = 1;
     c.kchunk = K_pad;
     c.split_floats = 0;
