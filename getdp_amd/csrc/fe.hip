@@ -53,7 +53,8 @@ struct DevPlan {
   const int *row_space, *row_ent;
   const int *ia, *ja;
   DevTerm term[kMaxTerms];
-  int nelem;
+  int nelem, nmax;
+  int bf[kMaxSpaces];  // basis function of each space (GDP_BF_*)
   // element pass output: elements in chunks of 64 (one wavefront); value i of
   // element e sits at ek_cb[e / 64] + 64 * i + e % 64, so that a wavefront's
   // stores of value i are contiguous
@@ -560,6 +561,303 @@ __global__ void __launch_bounds__(256) k_rhs_sum(int N, const int *__restrict__ 
   b[r] = bacc;
 }
 
+// ---- PRE pass on the device ---------------------------------------------------
+// The same tables as the host pass (DOF numbering, incidence, CSR pattern,
+// two-phase contribution lists), built in parallel: marks, exclusive scans,
+// per-entity and per-row threads.  Every list comes out in the host pass's
+// order, so the results are identical.
+
+__device__ inline int nodes_of_d(int et) { return et == GDP_TET ? 4 : et == GDP_TRI ? 3 : et == GDP_LINE ? 2 : 0; }
+__device__ inline int nb_of_d(const DevPlan &P, int bf, int e)
+{
+  const int et = P.etype[e];
+  if(bf != GDP_BF_EDGE) return nodes_of_d(et);
+  return et == GDP_TET ? 6 : et == GDP_TRI ? 3 : et == GDP_LINE ? 1 : 0;
+}
+__device__ inline int ent_of_d(const DevPlan &P, int bf, int e, int k)
+{
+  if(bf != GDP_BF_EDGE) return P.conn[(long)e * 4 + k];
+  const int v = P.edges[(long)e * 6 + k];
+  return v < 0 ? -v : v;
+}
+
+struct PrepOut {
+  int *dofnum[kMaxSpaces];
+};
+
+// (space, entity) pairs touched by active terms: 1 (Pre_TermOfFemEquation.cpp:125-160)
+__global__ void __launch_bounds__(256) k_prep_mark(DevPlan P, PrepOut O, int *err)
+{
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if(e >= P.nelem) return;
+  const int reg = P.region[e];
+  for(int ti = 0; ti < P.nterms; ti++) {
+    const DevTerm &t = P.term[ti];
+    if(!t.active[reg]) continue;
+    for(int q = 0; q < 2; q++) {
+      const int sp = q ? t.dof : t.equ;
+      if(sp < 0) continue;
+      const int bf = P.bf[sp], nb = nb_of_d(P, bf, e);
+      for(int k = 0; k < nb; k++) {
+        const int ent = ent_of_d(P, bf, e, k);
+        if(ent < 1 || ent > P.nmax) {
+          atomicOr(err, 1);
+          continue;
+        }
+        O.dofnum[sp][ent] = 1;
+      }
+    }
+  }
+}
+
+// fixed entities of one space: the last occurrence in the list wins, as in
+// the serial loop (marks become f + 2)
+__global__ void __launch_bounds__(256) k_prep_fix(int nfix, const int *fixent, int nmax, int *dofnum)
+{
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  if(f >= nfix) return;
+  const int ent = fixent[f];
+  if(ent >= 1 && ent <= nmax && dofnum[ent] >= 1) atomicMax(dofnum + ent, f + 2);
+}
+
+// unknown flags (1 for a mark that stayed 1) for the numbering scan; fixed
+// marks f + 2 become -(f + 1)
+__global__ void __launch_bounds__(256) k_prep_flags(int nmax, int *dofnum, int *flag)
+{
+  const int ent = blockIdx.x * 256 + threadIdx.x;
+  if(ent > nmax + 1) return;
+  if(ent > nmax) {
+    flag[ent] = 0;
+    return;
+  }
+  const int v = dofnum[ent];
+  flag[ent] = v == 1;
+  if(v >= 2) dofnum[ent] = -(v - 1);
+}
+
+// numbers in entity order after an exclusive scan of the flags; base = the
+// unknowns of the spaces numbered before (NumType order, DofData.cpp:36-45)
+__global__ void __launch_bounds__(256) k_prep_number(int nmax, int sp, int base, const int *scan, int *dofnum,
+                                                     int *row_space, int *row_ent)
+{
+  const int ent = blockIdx.x * 256 + threadIdx.x;
+  if(ent > nmax || dofnum[ent] != 1) return;
+  const int r = base + scan[ent];
+  dofnum[ent] = r + 1;
+  row_space[r] = sp;
+  row_ent[r] = ent;
+}
+
+__device__ inline bool elem_used(const DevPlan &P, int reg)
+{
+  for(int ti = 0; ti < P.nterms; ti++)
+    if(P.term[ti].active[reg]) return true;
+  return false;
+}
+
+// incidence entity -> (element, local index): counts, then fill, then each
+// entity's list sorted (elements ascending, as the host's element loop)
+__global__ void __launch_bounds__(256) k_prep_inc(DevPlan P, int *cnt, int *cur, int *item, bool fill)
+{
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if(e >= P.nelem || !elem_used(P, P.region[e])) return;
+  const int bf = P.bf[0], nb = nb_of_d(P, bf, e);
+  for(int k = 0; k < nb; k++) {
+    const int ent = ent_of_d(P, bf, e, k);
+    if(fill) item[atomicAdd(cur + ent, 1)] = e * 8 + k;
+    else atomicAdd(cnt + ent, 1);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_prep_inc_sort(int nmax, const int *ptr, int *item)
+{
+  const int ent = blockIdx.x * 256 + threadIdx.x;
+  if(ent > nmax) return;
+  const int b = ptr[ent], n = ptr[ent + 1] - b;
+  int *a = item + b;
+  for(int i = 1; i < n; i++) {
+    const int v = a[i];
+    int j = i - 1;
+    while(j >= 0 && a[j] > v) {
+      a[j + 1] = a[j];
+      j--;
+    }
+    a[j + 1] = v;
+  }
+}
+
+// CSR pattern: one thread per row collects the unknown DOFs of the elements
+// around its entity (bilinear terms whose test space is the row's) into a
+// sorted, duplicate-free list in LDS; pass 1 counts, pass 2 writes ja.
+constexpr int kPrepRow = 128;  // longest row the device pass handles
+template <bool FILL>
+__global__ void __launch_bounds__(64) k_prep_pattern(DevPlan P, int *rowlen, int *ja, int *err)
+{
+  __shared__ int buf[kPrepRow * 64];
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if(r >= P.N) return;
+  int *b = buf + threadIdx.x;
+  const int sp = P.row_space[r], er = P.row_ent[r];
+  int len = 0;
+  for(int p = P.inc_ptr[er]; p < P.inc_ptr[er + 1]; p++) {
+    const int e = P.inc_item[p] >> 3, reg = P.region[e];
+    for(int ti = 0; ti < P.nterms; ti++) {
+      const DevTerm &t = P.term[ti];
+      if(t.dof < 0 || t.equ != sp || !t.active[reg]) continue;
+      const int bf = P.bf[t.dof], nb = nb_of_d(P, bf, e);
+      for(int j = 0; j < nb; j++) {
+        const int c = P.dofnum[t.dof][ent_of_d(P, bf, e, j)] - 1;
+        if(c < 0) continue;
+        int lo = 0, hi = len;  // first position with b[pos] >= c
+        while(lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if(b[mid * 64] < c) lo = mid + 1;
+          else hi = mid;
+        }
+        if(lo < len && b[lo * 64] == c) continue;
+        if(len == kPrepRow) {
+          atomicOr(err, 2);
+          continue;
+        }
+        for(int q = len; q > lo; q--) b[q * 64] = b[(q - 1) * 64];
+        b[lo * 64] = c;
+        len++;
+      }
+    }
+  }
+  if(!FILL) {
+    rowlen[r] = len;
+    if(len == 0) atomicMin(err + 1, r);
+    return;
+  }
+  const int o = P.ia[r];
+  for(int i = 0; i < len; i++) ja[o + i] = b[i * 64];
+}
+
+// values per element in the two-phase layout; chunk widths of 64 elements
+__device__ inline bool bilinear_d(int kind)
+{
+  return kind == GDP_T_GRAD_GRAD || kind == GDP_T_NODE_MASS || kind == GDP_T_CURL_CURL ||
+         kind == GDP_T_EDGE_MASS;
+}
+__global__ void __launch_bounds__(64) k_prep_chunks(DevPlan P, int *cw)
+{
+  const int e = blockIdx.x * 64 + threadIdx.x;
+  int sz = 0;
+  if(e < P.nelem && P.geokind[e] != GEO_NONE) {
+    const int nb = P.family == FAM_EDGE ? 6 : nodes_of_d(P.etype[e]), reg = P.region[e];
+    for(int ti = 0; ti < P.nterms; ti++)
+      if(P.term[ti].active[reg]) sz += bilinear_d(P.term[ti].kind) ? nb * nb : nb;
+  }
+#pragma unroll
+  for(int o = 32; o > 0; o >>= 1) sz = max(sz, __shfl_xor(sz, o, 64));
+  if(threadIdx.x == 0) cw[blockIdx.x] = 64 * sz;
+}
+
+__global__ void __launch_bounds__(256) k_prep_widen(int n, const int *a, long *b)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if(i <= n) b[i] = a[i];
+}
+
+// One row's contributions in (element, term, column) order: matrix entries
+// (count per CSR entry, then the value indices) and right-hand side (fixed
+// columns with their value, sources with 1).
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_prep_visit(DevPlan P, const int *cb, int *ccnt, int *cur, int *cidx,
+                                                    int *rcnt, const int *rptr, int *ridx, double *rfv)
+{
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if(r >= P.N) return;
+  const bool edge = P.family == FAM_EDGE;
+  const int sp = P.row_space[r], er = P.row_ent[r];
+  const int rs = P.ia[r], re = P.ia[r + 1];
+  int rc = FILL ? rptr[r] : 0;
+  for(int p = P.inc_ptr[er]; p < P.inc_ptr[er + 1]; p++) {
+    const int item = P.inc_item[p];
+    const int e = item >> 3, k = item & 7;
+    const int reg = P.region[e], nb = edge ? 6 : nodes_of_d(P.etype[e]);
+    const int base = cb[e >> 6] + (e & 63);
+    int o = 0;
+    for(int ti = 0; ti < P.nterms; ti++) {
+      const DevTerm &t = P.term[ti];
+      if(!t.active[reg]) continue;
+      const bool bil = bilinear_d(t.kind);
+      if(edge || t.equ == sp) {
+        if(bil)
+          for(int j = 0; j < nb; j++) {
+            const int ej = edge ? abs(P.edges[(long)e * 6 + j]) : P.conn[(long)e * 4 + j];
+            const int c = P.dofnum[t.dof][ej];
+            const int vi = base + 64 * (o + k * nb + j);
+            if(c > 0) {
+              int lo = rs, hi = re;
+              while(lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if(P.ja[mid] < c - 1) lo = mid + 1;
+                else hi = mid;
+              }
+              if(FILL) cidx[cur[lo]++] = vi;
+              else ccnt[lo]++;
+            }
+            else if(c < 0) {
+              if(FILL) {
+                ridx[rc] = vi;
+                rfv[rc] = P.fixval[t.dof][-c - 1];
+              }
+              rc++;
+            }
+          }
+        else {
+          if(FILL) {
+            ridx[rc] = base + 64 * (o + k);
+            rfv[rc] = 1.0;
+          }
+          rc++;
+        }
+      }
+      o += bil ? nb * nb : nb;
+    }
+  }
+  if(!FILL) rcnt[r] = rc;
+}
+
+// ---- exclusive scan of int arrays (in place, deterministic) ----
+__global__ void __launch_bounds__(256) k_scan_local(int *d, long n, int *bsum)
+{
+  __shared__ int sh[256];
+  const long base = (long)blockIdx.x * 2048 + threadIdx.x * 8;
+  int v[8], t = 0;
+#pragma unroll
+  for(int i = 0; i < 8; i++) {
+    v[i] = base + i < n ? d[base + i] : 0;
+    t += v[i];
+  }
+  sh[threadIdx.x] = t;
+  __syncthreads();
+  for(int o = 1; o < 256; o <<= 1) {
+    const int x = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += x;
+    __syncthreads();
+  }
+  int run = threadIdx.x ? sh[threadIdx.x - 1] : 0;
+#pragma unroll
+  for(int i = 0; i < 8; i++) {
+    if(base + i < n) d[base + i] = run;
+    run += v[i];
+  }
+  if(threadIdx.x == 255) bsum[blockIdx.x] = sh[255];
+}
+
+__global__ void __launch_bounds__(256) k_scan_add(int *d, long n, const int *boff)
+{
+  const long base = (long)blockIdx.x * 2048 + threadIdx.x * 8;
+  const int add = boff[blockIdx.x];
+#pragma unroll
+  for(int i = 0; i < 8; i++)
+    if(base + i < n) d[base + i] += add;
+}
+
 int nodes_of(int et) { return et == GDP_TET ? 4 : et == GDP_TRI ? 3 : et == GDP_LINE ? 2 : 0; }
 int edges_of(int et) { return et == GDP_TET ? 6 : et == GDP_TRI ? 3 : et == GDP_LINE ? 1 : 0; }
 
@@ -579,6 +877,8 @@ struct gdp_fe {
   // host copies
   std::vector<int> h_etype, h_region, h_conn, h_edges, h_geokind;
   std::vector<int> h_perp;  // per term: BF_PerpendicularEdge spaces
+  int h_bf[kMaxSpaces] = {0, 0, 0}, h_numtype[kMaxSpaces] = {0, 0, 0};
+  std::vector<std::vector<int>> h_fixent;  // fixed entities per space
   std::vector<std::vector<int>> h_dofnum;  // per space, nmax+1
   std::vector<int> h_ia, h_ja;
   std::vector<int> h_inc_ptr, h_inc_item, h_row_space, h_row_ent;
@@ -587,6 +887,7 @@ struct gdp_fe {
   std::vector<std::vector<unsigned char>> h_active;
   std::vector<gdp_term_desc> h_terms;
   bool uploaded = false;
+  bool prepped_on_device = false;
   // device
   int *d_etype = nullptr, *d_region = nullptr, *d_conn = nullptr, *d_edges = nullptr,
       *d_geokind = nullptr;
@@ -606,7 +907,8 @@ struct gdp_fe {
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   double ms_total = 0, ms_geo = 0, ms_gather = 0;
 
-  ~gdp_fe()
+  ~gdp_fe() { release_device(); }
+  void release_device()
   {
     dfree(d_etype);
     dfree(d_region);
@@ -636,10 +938,19 @@ struct gdp_fe {
     dfree(d_cidx);
     dfree(d_rptr);
     dfree(d_ridx);
-    for(auto &e : ev)
+    for(auto &e : ev) {
       if(e) (void)hipEventDestroy(e);
+      e = nullptr;
+    }
+    plan = DevPlan{};
+    uploaded = false;
   }
 };
+
+static bool fe_prep_on_device();
+static int fe_prep_device(gdp_fe *F);
+static void fe_free_device(gdp_fe *F) { F->release_device(); }
+constexpr int kPrepFallback = 1;  // the device PRE pass hands over to the host pass
 
 namespace {
 
@@ -676,6 +987,126 @@ template <typename Fn> void parallel_for(int n, Fn fn)
 }
 
 }  // namespace
+
+// PRE pass on the host: DOF numbering (DofData.cpp:36-45, 1279-1323),
+// entity -> element incidence, CSR pattern.
+static int fe_prep_host(gdp_fe *F)
+{
+  const int E = F->nelem, nspaces = F->nspaces, nterms = F->nterms;
+  const int nmax = F->nmax;
+  F->h_dofnum.assign(nspaces, std::vector<int>((size_t)nmax + 1, 0));
+  for(int e = 0; e < E; e++) {
+    int reg = F->h_region[e];
+    for(int ti = 0; ti < nterms; ti++) {
+      if(!F->h_active[ti][reg]) continue;
+      int sps[2] = {F->h_terms[ti].equ, F->h_terms[ti].dof};
+      for(int q = 0; q < 2; q++) {
+        int s = sps[q];
+        if(s < 0) continue;
+        int nb = F->h_bf[s] != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+        for(int k = 0; k < nb; k++) {
+          int ent = entity_of(*F, F->h_bf[s], e, k);
+          if(ent < 1 || ent > nmax) return set_error(GDP_E_ARG, "entity %d out of range", ent);
+          F->h_dofnum[s][ent] = 1;
+        }
+      }
+    }
+  }
+  for(int s = 0; s < nspaces; s++)
+    for(size_t f = 0; f < F->h_fixent[s].size(); f++) {
+      int ent = F->h_fixent[s][f];
+      if(ent >= 1 && ent <= nmax && F->h_dofnum[s][ent]) F->h_dofnum[s][ent] = -(f + 1);
+    }
+  std::vector<int> order(nspaces);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return F->h_numtype[x] < F->h_numtype[y]; });
+  std::vector<int> row_space, row_ent;
+  int N = 0;
+  for(int s : order)
+    for(int ent = 1; ent <= nmax; ent++)
+      if(F->h_dofnum[s][ent] > 0) {
+        F->h_dofnum[s][ent] = ++N;
+        row_space.push_back(s);
+        row_ent.push_back(ent);
+      }
+  F->N = N;
+
+  // ---- incidence: entity -> (element, local index), elements ascending ----
+  int space_bf = F->h_bf[0];
+  std::vector<int> inc_ptr((size_t)nmax + 2, 0), inc_item;
+  auto elem_used = [&](int e) {
+    int reg = F->h_region[e];
+    for(int ti = 0; ti < nterms; ti++)
+      if(F->h_active[ti][reg]) return true;
+    return false;
+  };
+  for(int e = 0; e < E; e++) {
+    if(!elem_used(e)) continue;
+    int nb = space_bf != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+    for(int k = 0; k < nb; k++) inc_ptr[entity_of(*F, space_bf, e, k) + 1]++;
+  }
+  for(int i = 0; i <= nmax; i++) inc_ptr[i + 1] += inc_ptr[i];
+  inc_item.resize(inc_ptr[nmax + 1]);
+  {
+    std::vector<int> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+    for(int e = 0; e < E; e++) {
+      if(!elem_used(e)) continue;
+      int nb = space_bf != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+      for(int k = 0; k < nb; k++) inc_item[fill[entity_of(*F, space_bf, e, k)]++] = e * 8 + k;
+    }
+  }
+
+  // ---- CSR pattern: all unknown DOF pairs sharing an element in a bilinear
+  // term, stored zeros included (F6; csr_format sorts columns) ----
+  std::vector<int> rowlen(N, 0);
+  std::vector<std::vector<int>> rowcols(N);
+  parallel_for(N, [&](int lo, int hi) {
+    std::vector<int> cand;
+    for(int r = lo; r < hi; r++) {
+      int s = row_space[r], ent = row_ent[r];
+      cand.clear();
+      for(int p = inc_ptr[ent]; p < inc_ptr[ent + 1]; p++) {
+        int e = inc_item[p] >> 3;
+        int reg = F->h_region[e];
+        for(int ti = 0; ti < nterms; ti++) {
+          const gdp_term_desc &t = F->h_terms[ti];
+          if(t.dof < 0 || t.equ != s || !F->h_active[ti][reg]) continue;
+          int nb = F->h_bf[t.dof] != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
+          for(int j = 0; j < nb; j++) {
+            int c = F->h_dofnum[t.dof][entity_of(*F, F->h_bf[t.dof], e, j)];
+            if(c > 0) cand.push_back(c - 1);
+          }
+        }
+      }
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      rowcols[r] = cand;
+      rowlen[r] = (int)cand.size();
+    }
+  });
+  F->h_ia.assign((size_t)N + 1, 0);
+  for(int r = 0; r < N; r++) {
+    if(!rowlen[r]) return set_error(GDP_E_ZERO_ROW, "Zero row in matrix (DOF %d)", r + 1);
+    F->h_ia[r + 1] = F->h_ia[r] + rowlen[r];
+  }
+  F->nnz = F->h_ia[N];
+  if(F->nnz > 2147483647L) return set_error(GDP_E_UNSUPPORTED, "nnz exceeds int32");
+  F->h_ja.resize((size_t)F->nnz);
+  parallel_for(N, [&](int lo, int hi) {
+    for(int r = lo; r < hi; r++)
+      std::copy(rowcols[r].begin(), rowcols[r].end(), F->h_ja.begin() + F->h_ia[r]);
+  });
+  rowcols.clear();
+  rowcols.shrink_to_fit();
+
+  // ---- keep what the device upload needs (done on first Generate) ----
+  F->h_inc_ptr.swap(inc_ptr);
+  F->h_inc_item.swap(inc_item);
+  F->h_row_space.swap(row_space);
+  F->h_row_ent.swap(row_ent);
+  return GDP_OK;
+}
 
 extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
                              const gdp_space_desc *spaces, int nterms,
@@ -742,8 +1173,11 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
       return set_error(GDP_E_UNSUPPORTED, "term %d: %d coefficient values", ti, t.ncoef);
     if(t.coef_nreg < F->nreg) return set_error(GDP_E_ARG, "term %d: coefficient table too short", ti);
     if(!t.coef || !t.regions) return set_error(GDP_E_ARG, "term %d: null table", ti);
+    std::vector<unsigned char> inreg(F->nreg, 0);  // region r in the term's regions
+    for(int q = 0; q < t.nregions; q++)
+      if(t.regions[q] >= 0 && t.regions[q] < F->nreg) inreg[t.regions[q]] = 1;
     for(int e = 0; e < E; e++) {
-      if(!contains(t.nregions, t.regions, F->h_region[e])) continue;
+      if(!inreg[F->h_region[e]]) continue;
       int et = F->h_etype[e], gk;
       if(et == GDP_TET) {
         if(perp) return set_error(GDP_E_UNSUPPORTED, "term %d: perpendicular edges on a tetrahedron", ti);
@@ -787,124 +1221,16 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
     }
   }
 
-  // ---- DOF numbering (DofData.cpp:36-45, 1279-1323) ----
-  const int nmax = F->nmax;
-  F->h_dofnum.assign(nspaces, std::vector<int>((size_t)nmax + 1, 0));
-  for(int e = 0; e < E; e++) {
-    int reg = F->h_region[e];
-    for(int ti = 0; ti < nterms; ti++) {
-      if(!active[ti][reg]) continue;
-      int sps[2] = {terms[ti].equ, terms[ti].dof};
-      for(int q = 0; q < 2; q++) {
-        int s = sps[q];
-        if(s < 0) continue;
-        int nb = spaces[s].bf != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
-        for(int k = 0; k < nb; k++) {
-          int ent = entity_of(*F, spaces[s].bf, e, k);
-          if(ent < 1 || ent > nmax) return set_error(GDP_E_ARG, "entity %d out of range", ent);
-          F->h_dofnum[s][ent] = 1;
-        }
-      }
-    }
-  }
-  for(int s = 0; s < nspaces; s++)
-    for(int f = 0; f < spaces[s].nfixed; f++) {
-      int ent = spaces[s].fixed_entity[f];
-      if(ent >= 1 && ent <= nmax && F->h_dofnum[s][ent]) F->h_dofnum[s][ent] = -(f + 1);
-    }
-  std::vector<int> order(nspaces);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int x, int y) { return spaces[x].numtype < spaces[y].numtype; });
-  std::vector<int> row_space, row_ent;
-  int N = 0;
-  for(int s : order)
-    for(int ent = 1; ent <= nmax; ent++)
-      if(F->h_dofnum[s][ent] > 0) {
-        F->h_dofnum[s][ent] = ++N;
-        row_space.push_back(s);
-        row_ent.push_back(ent);
-      }
-  F->N = N;
-
-  // ---- incidence: entity -> (element, local index), elements ascending ----
-  int space_bf = spaces[0].bf;
-  std::vector<int> inc_ptr((size_t)nmax + 2, 0), inc_item;
-  auto elem_used = [&](int e) {
-    int reg = F->h_region[e];
-    for(int ti = 0; ti < nterms; ti++)
-      if(active[ti][reg]) return true;
-    return false;
-  };
-  for(int e = 0; e < E; e++) {
-    if(!elem_used(e)) continue;
-    int nb = space_bf != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
-    for(int k = 0; k < nb; k++) inc_ptr[entity_of(*F, space_bf, e, k) + 1]++;
-  }
-  for(int i = 0; i <= nmax; i++) inc_ptr[i + 1] += inc_ptr[i];
-  inc_item.resize(inc_ptr[nmax + 1]);
-  {
-    std::vector<int> fill(inc_ptr.begin(), inc_ptr.end() - 1);
-    for(int e = 0; e < E; e++) {
-      if(!elem_used(e)) continue;
-      int nb = space_bf != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
-      for(int k = 0; k < nb; k++) inc_item[fill[entity_of(*F, space_bf, e, k)]++] = e * 8 + k;
-    }
-  }
-
-  // ---- CSR pattern: all unknown DOF pairs sharing an element in a bilinear
-  // term, stored zeros included (F6; csr_format sorts columns) ----
-  std::vector<int> rowlen(N, 0);
-  std::vector<std::vector<int>> rowcols(N);
-  parallel_for(N, [&](int lo, int hi) {
-    std::vector<int> cand;
-    for(int r = lo; r < hi; r++) {
-      int s = row_space[r], ent = row_ent[r];
-      cand.clear();
-      for(int p = inc_ptr[ent]; p < inc_ptr[ent + 1]; p++) {
-        int e = inc_item[p] >> 3;
-        int reg = F->h_region[e];
-        for(int ti = 0; ti < nterms; ti++) {
-          const gdp_term_desc &t = terms[ti];
-          if(t.dof < 0 || t.equ != s || !active[ti][reg]) continue;
-          int nb = spaces[t.dof].bf != GDP_BF_EDGE ? nodes_of(F->h_etype[e]) : edges_of(F->h_etype[e]);
-          for(int j = 0; j < nb; j++) {
-            int c = F->h_dofnum[t.dof][entity_of(*F, spaces[t.dof].bf, e, j)];
-            if(c > 0) cand.push_back(c - 1);
-          }
-        }
-      }
-      std::sort(cand.begin(), cand.end());
-      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-      rowcols[r] = cand;
-      rowlen[r] = (int)cand.size();
-    }
-  });
-  F->h_ia.assign((size_t)N + 1, 0);
-  for(int r = 0; r < N; r++) {
-    if(!rowlen[r]) return set_error(GDP_E_ZERO_ROW, "Zero row in matrix (DOF %d)", r + 1);
-    F->h_ia[r + 1] = F->h_ia[r] + rowlen[r];
-  }
-  F->nnz = F->h_ia[N];
-  if(F->nnz > 2147483647L) return set_error(GDP_E_UNSUPPORTED, "nnz exceeds int32");
-  F->h_ja.resize((size_t)F->nnz);
-  parallel_for(N, [&](int lo, int hi) {
-    for(int r = lo; r < hi; r++)
-      std::copy(rowcols[r].begin(), rowcols[r].end(), F->h_ja.begin() + F->h_ia[r]);
-  });
-  rowcols.clear();
-  rowcols.shrink_to_fit();
-
-  // ---- keep what the device upload needs (done on first Generate) ----
-  F->h_inc_ptr.swap(inc_ptr);
-  F->h_inc_item.swap(inc_item);
-  F->h_row_space.swap(row_space);
-  F->h_row_ent.swap(row_ent);
+  // ---- the formulation's tables (both pre-processing paths use them) ----
   F->h_xyz.assign(mesh->xyz, mesh->xyz + (size_t)F->nnodes * 3);
   F->h_fixval.resize(nspaces);
+  F->h_fixent.resize(nspaces);
   for(int s = 0; s < nspaces; s++) {
     F->h_fixval[s].assign(std::max(1, spaces[s].nfixed), 0.);
+    F->h_fixent[s].assign(spaces[s].fixed_entity, spaces[s].fixed_entity + spaces[s].nfixed);
     for(int f = 0; f < spaces[s].nfixed; f++) F->h_fixval[s][f] = spaces[s].fixed_value[f];
+    F->h_bf[s] = spaces[s].bf;
+    F->h_numtype[s] = spaces[s].numtype;
   }
   F->h_active.swap(active);
   F->h_coef.resize(nterms);
@@ -914,6 +1240,17 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
     F->h_terms[ti].coef = nullptr;
     F->h_terms[ti].regions = nullptr;
   }
+  // ---- PRE pass: on the device when the library is initialised ----
+  if(fe_prep_on_device()) {
+    const int rc = fe_prep_device(F.get());
+    if(rc == GDP_OK) {
+      *out = F.release();
+      return GDP_OK;
+    }
+    if(rc != kPrepFallback) return rc;
+    fe_free_device(F.get());
+  }
+  GDP_TRY(fe_prep_host(F.get()));
   *out = F.release();
   return GDP_OK;
 }
@@ -1026,11 +1363,11 @@ static int fe_build_two_phase(gdp_fe *F)
   return GDP_OK;
 }
 
-static int fe_upload(gdp_fe *F)
+// Mesh, geometry buffer, fixed values, coefficient and activity tables, and
+// the plan's fields that do not depend on the DOF numbering.
+static int fe_upload_mesh(gdp_fe *F)
 {
-  if(F->uploaded) return GDP_OK;
-  GDP_TRY(ensure_ready());
-  const int E = F->nelem, N = F->N, nmax = F->nmax, nspaces = F->nspaces, nterms = F->nterms;
+  const int E = F->nelem, nspaces = F->nspaces, nterms = F->nterms;
   GDP_TRY(dalloc(&F->d_etype, E));
   GDP_TRY(dalloc(&F->d_region, E));
   GDP_TRY(dalloc(&F->d_conn, (size_t)E * 4));
@@ -1047,29 +1384,15 @@ static int fe_upload(gdp_fe *F)
     GDP_TRY(h2d(F->d_edges, F->h_edges.data(), (size_t)E * 6));
   }
   for(int s = 0; s < nspaces; s++) {
-    GDP_TRY(dalloc(&F->d_dofnum[s], (size_t)nmax + 1));
-    GDP_TRY(h2d(F->d_dofnum[s], F->h_dofnum[s].data(), (size_t)nmax + 1));
     GDP_TRY(dalloc(&F->d_fixval[s], F->h_fixval[s].size()));
     GDP_TRY(h2d(F->d_fixval[s], F->h_fixval[s].data(), F->h_fixval[s].size()));
   }
-  GDP_TRY(dalloc(&F->d_inc_ptr, F->h_inc_ptr.size()));
-  GDP_TRY(dalloc(&F->d_inc_item, std::max<size_t>(1, F->h_inc_item.size())));
-  GDP_TRY(h2d(F->d_inc_ptr, F->h_inc_ptr.data(), F->h_inc_ptr.size()));
-  GDP_TRY(h2d(F->d_inc_item, F->h_inc_item.data(), F->h_inc_item.size()));
-  GDP_TRY(dalloc(&F->d_row_space, std::max(1, N)));
-  GDP_TRY(dalloc(&F->d_row_ent, std::max(1, N)));
-  GDP_TRY(h2d(F->d_row_space, F->h_row_space.data(), (size_t)N));
-  GDP_TRY(h2d(F->d_row_ent, F->h_row_ent.data(), (size_t)N));
-  GDP_TRY(dalloc(&F->d_ia, (size_t)N + 1));
-  GDP_TRY(dalloc(&F->d_ja, std::max<size_t>(1, (size_t)F->nnz)));
-  GDP_TRY(h2d(F->d_ia, F->h_ia.data(), (size_t)N + 1));
-  GDP_TRY(h2d(F->d_ja, F->h_ja.data(), (size_t)F->nnz));
   DevPlan &P = F->plan;
   P.family = F->family;
   P.nspaces = nspaces;
   P.nterms = nterms;
-  P.N = N;
   P.nreg = F->nreg;
+  P.nmax = F->nmax;
   P.etype = F->d_etype;
   P.region = F->d_region;
   P.conn = F->d_conn;
@@ -1077,15 +1400,9 @@ static int fe_upload(gdp_fe *F)
   P.geokind = F->d_geokind;
   P.geo = F->d_geo;
   for(int s = 0; s < nspaces; s++) {
-    P.dofnum[s] = F->d_dofnum[s];
     P.fixval[s] = F->d_fixval[s];
+    P.bf[s] = F->h_bf[s];
   }
-  P.inc_ptr = F->d_inc_ptr;
-  P.inc_item = F->d_inc_item;
-  P.row_space = F->d_row_space;
-  P.row_ent = F->d_row_ent;
-  P.ia = F->d_ia;
-  P.ja = F->d_ja;
   for(int ti = 0; ti < nterms; ti++) {
     const gdp_term_desc &t = F->h_terms[ti];
     GDP_TRY(dalloc(&F->d_coef[ti], F->h_coef[ti].size()));
@@ -1102,13 +1419,251 @@ static int fe_upload(gdp_fe *F)
     d.coef = F->d_coef[ti];
     d.active = F->d_active[ti];
   }
-  GDP_TRY(fe_build_two_phase(F));
   P.nelem = E;
+  GDP_TRY(sync());
+  return GDP_OK;
+}
+
+// The plan's pattern fields (numbering, incidence, CSR, two-phase tables).
+static void fe_plan_pattern(gdp_fe *F)
+{
+  DevPlan &P = F->plan;
+  P.N = F->N;
+  for(int s = 0; s < F->nspaces; s++) P.dofnum[s] = F->d_dofnum[s];
+  P.inc_ptr = F->d_inc_ptr;
+  P.inc_item = F->d_inc_item;
+  P.row_space = F->d_row_space;
+  P.row_ent = F->d_row_ent;
+  P.ia = F->d_ia;
+  P.ja = F->d_ja;
   P.ek_cb = F->d_ek_cb;
+}
+
+// Device copies on the first Generate after a host PRE pass.
+static int fe_upload(gdp_fe *F)
+{
+  if(F->uploaded) return GDP_OK;
+  GDP_TRY(ensure_ready());
+  const int N = F->N, nmax = F->nmax, nspaces = F->nspaces;
+  GDP_TRY(fe_upload_mesh(F));
+  for(int s = 0; s < nspaces; s++) {
+    GDP_TRY(dalloc(&F->d_dofnum[s], (size_t)nmax + 1));
+    GDP_TRY(h2d(F->d_dofnum[s], F->h_dofnum[s].data(), (size_t)nmax + 1));
+  }
+  GDP_TRY(dalloc(&F->d_inc_ptr, F->h_inc_ptr.size()));
+  GDP_TRY(dalloc(&F->d_inc_item, std::max<size_t>(1, F->h_inc_item.size())));
+  GDP_TRY(h2d(F->d_inc_ptr, F->h_inc_ptr.data(), F->h_inc_ptr.size()));
+  GDP_TRY(h2d(F->d_inc_item, F->h_inc_item.data(), F->h_inc_item.size()));
+  GDP_TRY(dalloc(&F->d_row_space, std::max(1, N)));
+  GDP_TRY(dalloc(&F->d_row_ent, std::max(1, N)));
+  GDP_TRY(h2d(F->d_row_space, F->h_row_space.data(), (size_t)N));
+  GDP_TRY(h2d(F->d_row_ent, F->h_row_ent.data(), (size_t)N));
+  GDP_TRY(dalloc(&F->d_ia, (size_t)N + 1));
+  GDP_TRY(dalloc(&F->d_ja, std::max<size_t>(1, (size_t)F->nnz)));
+  GDP_TRY(h2d(F->d_ia, F->h_ia.data(), (size_t)N + 1));
+  GDP_TRY(h2d(F->d_ja, F->h_ja.data(), (size_t)F->nnz));
+  GDP_TRY(fe_build_two_phase(F));
+  fe_plan_pattern(F);
   for(auto &e : F->ev) GDP_HIP(hipEventCreate(&e));
   GDP_TRY(sync());
   F->uploaded = true;
   return GDP_OK;
+}
+
+// In-place exclusive scan of n ints on the library stream.
+static int scan_excl(int *d, long n)
+{
+  if(n <= 0) return GDP_OK;
+  const long nb = (n + 2047) / 2048;
+  int *bs = nullptr;
+  GDP_TRY(dalloc(&bs, (size_t)nb));
+  hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(256), 0, ctx().stream, d, n, bs);
+  int rc = hipGetLastError() == hipSuccess ? GDP_OK : set_error(GDP_E_HIP, "k_scan_local launch");
+  if(!rc && nb > 1) {
+    rc = scan_excl(bs, nb);
+    if(!rc) hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(256), 0, ctx().stream, d, n, bs);
+  }
+  if(!rc) rc = sync();
+  dfree(bs);
+  return rc;
+}
+
+static int read_int(const int *d, int *h) { return d2h(h, d, 1); }
+
+static bool fe_prep_on_device()
+{
+  // GDP_HOST_PREP=1 keeps the PRE pass on the host
+  const char *e = std::getenv("GDP_HOST_PREP");
+  return ctx().ready && !(e && e[0] && e[0] != '0');
+}
+
+// PRE pass on the device: the host pass's tables, built with marks, scans and
+// per-entity / per-row threads; the mesh is uploaded at the same time.  The
+// host keeps copies of the numbering and of the pattern (API, renumbering).
+static int fe_prep_device(gdp_fe *F)
+{
+  hipStream_t st = ctx().stream;
+  const int E = F->nelem, nmax = F->nmax, nsp = F->nspaces;
+  GDP_TRY(fe_upload_mesh(F));
+  DevPlan &P = F->plan;
+  std::vector<int *> tmp;
+  auto talloc = [&](int **p, size_t n) {
+    int rc = dalloc(p, std::max<size_t>(1, n));
+    if(!rc) tmp.push_back(*p);
+    return rc;
+  };
+  auto done = [&](int rc) {
+    (void)sync();
+    for(int *p : tmp) dfree(p);
+    return rc;
+  };
+  auto zero = [&](void *p, size_t bytes) {
+    return hipMemsetAsync(p, 0, bytes, st) == hipSuccess ? GDP_OK : set_error(GDP_E_HIP, "hipMemsetAsync");
+  };
+  int *err = nullptr;
+  GDP_TRY(talloc(&err, 2));
+  const int einit[2] = {0, 0x7fffffff};
+  int rc = h2d(err, einit, 2);
+  // marks, fixed entities
+  PrepOut O{};
+  for(int s = 0; s < nsp && !rc; s++) {
+    rc = dalloc(&F->d_dofnum[s], (size_t)nmax + 1);
+    if(!rc) rc = zero(F->d_dofnum[s], sizeof(int) * ((size_t)nmax + 1));
+    O.dofnum[s] = F->d_dofnum[s];
+    P.dofnum[s] = F->d_dofnum[s];
+  }
+  if(rc) return done(rc);
+  if(E) hipLaunchKernelGGL(k_prep_mark, dim3((E + 255) / 256), dim3(256), 0, st, P, O, err);
+  for(int s = 0; s < nsp && !rc; s++) {
+    const int nf = (int)F->h_fixent[s].size();
+    if(!nf) continue;
+    int *fe_ = nullptr;
+    rc = talloc(&fe_, nf);
+    if(!rc) rc = h2d(fe_, F->h_fixent[s].data(), (size_t)nf);
+    if(!rc) hipLaunchKernelGGL(k_prep_fix, dim3((nf + 255) / 256), dim3(256), 0, st, nf, fe_, nmax, F->d_dofnum[s]);
+  }
+  int h_err[2] = {0, 0};
+  if(!rc) rc = d2h(h_err, err, 2);
+  if(rc) return done(rc);
+  if(h_err[0] & 1) return done(set_error(GDP_E_ARG, "an element entity is out of range"));
+  // numbering in NumType order
+  std::vector<int> order(nsp);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return F->h_numtype[x] < F->h_numtype[y]; });
+  int *flag = nullptr;
+  rc = talloc(&flag, (size_t)nmax + 2);
+  if(!rc) rc = dalloc(&F->d_row_space, (size_t)nsp * ((size_t)nmax + 1));
+  if(!rc) rc = dalloc(&F->d_row_ent, (size_t)nsp * ((size_t)nmax + 1));
+  long N = 0;
+  const unsigned gent = (unsigned)((nmax + 2 + 255) / 256);
+  for(int s : order) {
+    if(rc) break;
+    hipLaunchKernelGGL(k_prep_flags, dim3(gent), dim3(256), 0, st, nmax, F->d_dofnum[s], flag);
+    rc = scan_excl(flag, (long)nmax + 2);
+    int cnt = 0;
+    if(!rc) rc = read_int(flag + nmax + 1, &cnt);
+    if(!rc)
+      hipLaunchKernelGGL(k_prep_number, dim3(gent), dim3(256), 0, st, nmax, s, (int)N, flag, F->d_dofnum[s],
+                         F->d_row_space, F->d_row_ent);
+    N += cnt;
+  }
+  if(rc) return done(rc);
+  F->N = (int)N;
+  P.N = (int)N;
+  P.row_space = F->d_row_space;
+  P.row_ent = F->d_row_ent;
+  // incidence
+  int *cur = nullptr;
+  rc = dalloc(&F->d_inc_ptr, (size_t)nmax + 2);
+  if(!rc) rc = zero(F->d_inc_ptr, sizeof(int) * ((size_t)nmax + 2));
+  if(!rc && E) hipLaunchKernelGGL(k_prep_inc, dim3((E + 255) / 256), dim3(256), 0, st, P, F->d_inc_ptr, nullptr, nullptr, false);
+  if(!rc) rc = scan_excl(F->d_inc_ptr, (long)nmax + 2);
+  int ninc = 0;
+  if(!rc) rc = read_int(F->d_inc_ptr + nmax + 1, &ninc);
+  if(!rc) rc = dalloc(&F->d_inc_item, (size_t)std::max(1, ninc));
+  if(!rc) rc = talloc(&cur, (size_t)nmax + 2);
+  if(!rc && hipMemcpyAsync(cur, F->d_inc_ptr, sizeof(int) * ((size_t)nmax + 2), hipMemcpyDeviceToDevice, st) != hipSuccess)
+    rc = set_error(GDP_E_HIP, "hipMemcpyAsync");
+  if(!rc && E) hipLaunchKernelGGL(k_prep_inc, dim3((E + 255) / 256), dim3(256), 0, st, P, nullptr, cur, F->d_inc_item, true);
+  if(!rc) hipLaunchKernelGGL(k_prep_inc_sort, dim3(gent), dim3(256), 0, st, nmax, F->d_inc_ptr, F->d_inc_item);
+  if(rc) return done(rc);
+  P.inc_ptr = F->d_inc_ptr;
+  P.inc_item = F->d_inc_item;
+  // pattern
+  if(N * (long)kPrepRow > 2147483647L) return done(kPrepFallback);
+  rc = dalloc(&F->d_ia, (size_t)N + 1);
+  if(!rc) rc = zero(F->d_ia, sizeof(int) * ((size_t)N + 1));
+  const unsigned grow = (unsigned)((N + 63) / 64);
+  if(!rc && N) hipLaunchKernelGGL(k_prep_pattern<false>, dim3(grow), dim3(64), 0, st, P, F->d_ia, nullptr, err);
+  if(!rc) rc = d2h(h_err, err, 2);
+  if(rc) return done(rc);
+  if(h_err[0] & 2) return done(kPrepFallback);  // a row longer than kPrepRow
+  if(h_err[1] != 0x7fffffff) return done(set_error(GDP_E_ZERO_ROW, "Zero row in matrix (DOF %d)", h_err[1] + 1));
+  rc = scan_excl(F->d_ia, N + 1);
+  int nnz = 0;
+  if(!rc) rc = read_int(F->d_ia + N, &nnz);
+  if(rc) return done(rc);
+  F->nnz = nnz;
+  P.ia = F->d_ia;
+  rc = dalloc(&F->d_ja, (size_t)std::max(1, nnz));
+  P.ja = F->d_ja;
+  if(!rc && N) hipLaunchKernelGGL(k_prep_pattern<true>, dim3(grow), dim3(64), 0, st, P, nullptr, F->d_ja, err);
+  if(rc) return done(rc);
+  // two-phase tables
+  const int nch = (E + 63) / 64;
+  int *cb = nullptr, *ccur = nullptr;
+  rc = talloc(&cb, (size_t)nch + 1);
+  if(!rc) rc = zero(cb, sizeof(int) * ((size_t)nch + 1));
+  if(!rc && nch) hipLaunchKernelGGL(k_prep_chunks, dim3(nch), dim3(64), 0, st, P, cb);
+  if(!rc) rc = scan_excl(cb, (long)nch + 1);
+  int ektot = 0;
+  if(!rc) rc = read_int(cb + nch, &ektot);
+  if(!rc && ektot < 0) rc = set_error(GDP_E_UNSUPPORTED, "element blocks exceed int32 indexing");
+  if(!rc) rc = dalloc(&F->d_ek_cb, (size_t)nch + 1);
+  if(!rc) hipLaunchKernelGGL(k_prep_widen, dim3((nch + 256) / 256), dim3(256), 0, st, nch, cb, F->d_ek_cb);
+  if(!rc) rc = dalloc(&F->d_ek, (size_t)std::max(1, ektot));
+  F->ek_total = ektot;
+  if(!rc) rc = dalloc(&F->d_cptr, (size_t)nnz + 1);
+  if(!rc) rc = zero(F->d_cptr, sizeof(int) * ((size_t)nnz + 1));
+  if(!rc) rc = dalloc(&F->d_rptr, (size_t)N + 1);
+  if(!rc) rc = zero(F->d_rptr, sizeof(int) * ((size_t)N + 1));
+  if(!rc && N)
+    hipLaunchKernelGGL(k_prep_visit<false>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, P, cb,
+                       F->d_cptr, nullptr, nullptr, F->d_rptr, nullptr, nullptr, nullptr);
+  if(!rc) rc = scan_excl(F->d_cptr, (long)nnz + 1);
+  if(!rc) rc = scan_excl(F->d_rptr, N + 1);
+  int ncon = 0, nrhs = 0;
+  if(!rc) rc = read_int(F->d_cptr + nnz, &ncon);
+  if(!rc) rc = read_int(F->d_rptr + N, &nrhs);
+  if(!rc && (ncon < 0 || nrhs < 0)) rc = set_error(GDP_E_UNSUPPORTED, "contribution lists exceed int32 indexing");
+  if(!rc) rc = dalloc(&F->d_cidx, (size_t)std::max(1, ncon));
+  if(!rc) rc = dalloc(&F->d_ridx, (size_t)std::max(1, nrhs));
+  if(!rc) rc = dalloc(&F->d_rfv, (size_t)std::max(1, nrhs));
+  if(!rc) rc = talloc(&ccur, (size_t)nnz + 1);
+  if(!rc && hipMemcpyAsync(ccur, F->d_cptr, sizeof(int) * ((size_t)nnz + 1), hipMemcpyDeviceToDevice, st) != hipSuccess)
+    rc = set_error(GDP_E_HIP, "hipMemcpyAsync");
+  if(!rc && N)
+    hipLaunchKernelGGL(k_prep_visit<true>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, P, cb, nullptr,
+                       ccur, F->d_cidx, nullptr, F->d_rptr, F->d_ridx, F->d_rfv);
+  if(!rc && hipGetLastError() != hipSuccess) rc = set_error(GDP_E_HIP, "PRE pass launch");
+  // host copies
+  F->h_dofnum.assign(nsp, std::vector<int>((size_t)nmax + 1, 0));
+  for(int s = 0; s < nsp && !rc; s++) rc = d2h(F->h_dofnum[s].data(), F->d_dofnum[s], (size_t)nmax + 1);
+  F->h_ia.assign((size_t)N + 1, 0);
+  F->h_ja.assign((size_t)nnz, 0);
+  F->h_row_space.assign((size_t)N, 0);
+  F->h_row_ent.assign((size_t)N, 0);
+  if(!rc) rc = d2h(F->h_ia.data(), F->d_ia, (size_t)N + 1);
+  if(!rc && nnz) rc = d2h(F->h_ja.data(), F->d_ja, (size_t)nnz);
+  if(!rc && N) rc = d2h(F->h_row_space.data(), F->d_row_space, (size_t)N);
+  if(!rc && N) rc = d2h(F->h_row_ent.data(), F->d_row_ent, (size_t)N);
+  if(rc) return done(rc);
+  fe_plan_pattern(F);
+  for(auto &e : F->ev)
+    if(hipEventCreate(&e) != hipSuccess) return done(set_error(GDP_E_HIP, "hipEventCreate"));
+  F->uploaded = true;
+  F->prepped_on_device = true;
+  return done(GDP_OK);
 }
 
 extern "C" int gdp_fe_destroy(gdp_fe *fe)

@@ -136,3 +136,23 @@ def test_spmv_compressed_columns(gpu, oracle, monkeypatch):
     big = P.elasticity3d(32, nbins=4)          # ux block, then uy, uz: ~36k apart per row
     fe, A, bv = gpu.assemble(big)
     assert not gpu.sell_info(A)["compressed"]
+
+
+@pytest.mark.parametrize("name", ["poisson2d_lognormal_24", "elasticity3d_4", "curlcurl3d_4",
+                                  "poisson2d_az_general"])
+def test_device_pre_pass_matches_host(gpu, monkeypatch, name):
+    """The PRE pass on the device (numbering, incidence, pattern, assembly
+    tables) gives the host pass's DOF tables and pattern, and the same
+    assembled system."""
+    pr = CASES[name]()
+    out = {}
+    for host in ("1", "0"):
+        monkeypatch.setenv("GDP_HOST_PREP", host)
+        fe, A, bv = gpu.assemble(pr)
+        ia, ja, a = A.get_csr()
+        out[host] = ([fe.dof_table(s) for s in range(len(pr.spaces))], ia, ja, a, bv.get())
+    h, d = out["1"], out["0"]
+    for s in range(len(pr.spaces)):
+        assert np.array_equal(h[0][s], d[0][s])
+    for k in range(1, 5):
+        assert np.array_equal(h[k], d[k])
