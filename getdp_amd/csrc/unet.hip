@@ -438,22 +438,30 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs w)
   const bool kvalid = tap < 9 && kcol < a.K_pad;
   const int dy = tap / 3 - 1, dx = tap % 3 - 1;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  const int W = a.W, H = a.H;
   for(long p0 = p_begin; p0 < p_end; p0 += 32) {
     bf16x8 av, bv;
     u16 *ah = (u16 *)&av, *bh = (u16 *)&bv;
+    // coordinates of the lane's first pixel, then advanced one pixel at a time
+    const int pb = (int)(p0 + 8 * kq);
+    int x = pb % W, b = pb / W;
+    int y = b % H;
+    b /= H;
 #pragma unroll
     for(int j = 0; j < 8; j++) {
-      const long p = p0 + 8 * kq + j;
+      const long p = (long)pb + j;
       const bool ok = p < p_end;
       ah[j] = (ok && co < a.cout) ? w.g[p * a.cout + co] : (u16)0;
       float xv = 0.f;
-      if(ok && kvalid) {
-        const int x = (int)(p % a.W);
-        const long q = p / a.W;
-        const int y = (int)(q % a.H), b = (int)(q / a.H);
-        xv = load1<MODE>(a, b, y + dy, x + dx, ci);
-      }
+      if(ok && kvalid) xv = load1<MODE>(a, b, y + dy, x + dx, ci);
       bh[j] = f2bf(xv);
+      if(++x == W) {
+        x = 0;
+        if(++y == H) {
+          y = 0;
+          b++;
+        }
+      }
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
   }
@@ -484,13 +492,44 @@ __global__ void k_relu_grad(long n, const float *dy, const u16 *y, u16 *g)
 }
 
 // bias gradient: db[c] = sum_p G[p][c]  (one block per channel)
-__global__ void k_bias_grad(long P, int C, const u16 *g, float *db)
+// Column sums of a [P][C] tensor (C divides 256), in two deterministic stages.
+// Stage 1: block b takes elements [b * 8192, (b + 1) * 8192); thread t reads
+// elements t, t + 256, ... (coalesced), all in column t % C, optionally
+// weighted by dz[pixel]; the threads of one column are then added in thread
+// order.  Stage 2 adds the blocks' partials in block order.
+constexpr int kColChunk = 8192;
+template <typename T>
+__global__ void __launch_bounds__(256) k_colsum_part(long n, int C, const T *x, const float *dz, float *part)
 {
-  const int c = blockIdx.x;
+  __shared__ float sh[256];
+  const int t = threadIdx.x;
+  const long b0 = (long)blockIdx.x * kColChunk;
+  float acc = 0.f;
+  for(int k = 0; k < kColChunk / 256; k++) {
+    const long i = b0 + t + 256L * k;
+    if(i >= n) break;
+    float v;
+    if constexpr(sizeof(T) == 2) v = bf2f(x[i]);
+    else v = x[i];
+    if(dz) v *= dz[i / C];
+    acc += v;
+  }
+  sh[t] = acc;
+  __syncthreads();
+  if(t < C) {
+    float s = 0.f;
+    for(int j = t; j < 256; j += C) s += sh[j];
+    part[(long)blockIdx.x * C + t] = s;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_colsum_fin(int nblocks, int C, const float *part, float *out)
+{
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if(c >= C) return;
   float s = 0.f;
-  for(long p = threadIdx.x; p < P; p += 256) s += bf2f(g[p * C + c]);
-  s = block_sum<float, 256>(s);
-  if(threadIdx.x == 0) db[c] = s;
+  for(int b = 0; b < nblocks; b++) s += part[(long)b * C + c];
+  out[c] = s;
 }
 
 // adjoints of the fused loaders, accumulating into FP32 gradient tensors
@@ -570,17 +609,6 @@ __global__ void k_out_bwd_x(long P, int C, const float *dz, const float *w, floa
 }
 
 // gw[c] = sum_p dz[p] x[p][c] (c < C), gb = sum_p dz[p] (c == C); one block each
-__global__ void k_out_bwd_w(long P, int C, const float *dz, const u16 *x, float *gw, float *gb)
-{
-  const int c = blockIdx.x;
-  float s = 0.f;
-  for(long p = threadIdx.x; p < P; p += 256) s += dz[p] * (c < C ? bf2f(x[p * C + c]) : 1.f);
-  s = block_sum<float, 256>(s);
-  if(threadIdx.x == 0) {
-    if(c < C) gw[c] = s;
-    else gb[0] = s;
-  }
-}
 
 // scale-invariant loss per sample: L = | z/|z| - t |^2 over the mask, t unit
 // norm; dz = -(2/|z|) (t - (zhat.t) zhat).  One block per sample.
@@ -768,6 +796,8 @@ struct UserPrec {
   u16 *g = nullptr;     // ReLU-masked gradient (BF16) of one layer
   float *part = nullptr;
   size_t part_size = 0;
+  float *cpart = nullptr;  // column-sum partials (bias and output-layer gradients)
+  size_t cpart_size = 0;
   float *split = nullptr;  // split-K partials of the convolutions
   size_t split_size = 0;
   int adam_t = 0;
@@ -946,6 +976,23 @@ int forward(UserPrec &U, int B, const int *gate, int gmode, bool may_alloc, bool
   return GDP_OK;
 }
 
+// out[c] = sum over pixels of x[p][c] (times dz[p] when given); partials in U.part
+template <typename T> int colsum(UserPrec &U, const T *x, long P, int C, const float *dz, float *out)
+{
+  const long n = P * C;
+  const int nb = (int)((n + kColChunk - 1) / kColChunk);
+  const size_t need = (size_t)nb * C;
+  if(need > U.cpart_size) {
+    dfree(U.cpart);
+    GDP_TRY(dalloc(&U.cpart, need));
+    U.cpart_size = need;
+  }
+  hipLaunchKernelGGL(k_colsum_part<T>, dim3(nb), dim3(256), 0, ctx().stream, n, C, x, dz, U.cpart);
+  hipLaunchKernelGGL(k_colsum_fin, dim3((C + 255) / 256), dim3(256), 0, ctx().stream, nb, C, U.cpart, out);
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
 int backward(UserPrec &U, int B)
 {
   hipStream_t s = ctx().stream;
@@ -954,8 +1001,9 @@ int backward(UserPrec &U, int B)
     long n = (long)B * (U.H >> U.act_lvl[t]) * (U.W >> U.act_lvl[t]) * U.act_ch[t];
     GDP_HIP(hipMemsetAsync(U.gact[t], 0, sizeof(float) * n, s));
   }
-  // output 1x1
-  hipLaunchKernelGGL(k_out_bwd_w, dim3(17), dim3(256), 0, s, P0, 16, U.dz, U.act.back(), U.gow, U.gob);
+  // output 1x1: weight gradient sum_p dz[p] x[p][c], bias gradient sum_p dz[p]
+  GDP_TRY(colsum(U, U.act.back(), P0, 16, U.dz, U.gow));
+  GDP_TRY(colsum(U, U.dz, P0, 1, (const float *)nullptr, U.gob));
   hipLaunchKernelGGL(k_out_bwd_x, g1(P0 * 16), dim3(256), 0, s, P0, 16, U.dz, U.ow, U.gact.back());
   for(int i = (int)U.L.size() - 1; i >= 0; i--) {
     const ConvLayer &l = U.L[i];
@@ -964,7 +1012,7 @@ int backward(UserPrec &U, int B)
     // G = relu'(y) * dY (BF16)
     hipLaunchKernelGGL(k_relu_grad, g1(P * l.cout), dim3(256), 0, s, P * l.cout, U.gact[i + 1],
                        U.act[i + 1], U.g);
-    hipLaunchKernelGGL(k_bias_grad, dim3(l.cout), dim3(256), 0, s, P, l.cout, U.g, l.gb);
+    GDP_TRY(colsum(U, U.g, P, l.cout, (const float *)nullptr, l.gb));
     // weight gradient
     ConvArgs x = conv_args(U, l, B);
     WgradArgs wa{};
@@ -1054,6 +1102,7 @@ UserPrec::~UserPrec()
   zfree(vob);
   zfree(z);
   zfree(dz);
+  zfree(cpart);
   zfree(target);
   zfree(mask);
   zfree(loss);
