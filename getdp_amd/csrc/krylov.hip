@@ -335,14 +335,26 @@ __global__ void __launch_bounds__(B) k_cgs(int n, int k, TV *__restrict__ vk,
       }
   }
   if(PASS <= 2) {
+    // the k block sums without a barrier per coefficient: each wave reduces
+    // its lanes (the tree of block_sum), the waves' sums meet in LDS and are
+    // added in wave order -- the same arithmetic as block_sum
+    __shared__ double red[kMaxKrylov + 2][B / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     for(int i = 0; i < k; i++) {
       const TV *vi = V + (long)i * n;
       double a = 0.;
 #pragma unroll
       for(int u = 0; u < CGS_PER; u++)
         if(jj[u] < n) a += (double)vi[jj[u]] * x[u];
-      double t = block_sum<double, B>(a);
-      if(threadIdx.x == 0) part[(long)i * nb + blockIdx.x] = t;
+      a = wave_sum(a);
+      if(lane == 0) red[i][wv] = a;
+    }
+    __syncthreads();
+    for(int i = threadIdx.x; i < k; i += B) {
+      double t = 0.;
+#pragma unroll
+      for(int w = 0; w < B / kWave; w++) t += red[i][w];
+      part[(long)i * nb + blockIdx.x] = t;
     }
   }
   else {
