@@ -450,6 +450,138 @@ int orc_cg(int n, const double *rhs, double *sol, int maxits, double rtol,
   return status;
 }
 
+/* bcgstab, iters.f:1032-1306 with stopbis (:3275-3330), tidycg (:3333-3365)
+ * and brkdn (:3368-3401), for ipar(3) = 1.  Work columns of the reference:
+ * r0 = w1, r = w2, s = w3, t = w4, v = w5, p = w6, z = w7 (M^-1 input of
+ * the next product), dx = w8.  Right preconditioning iterates on A*M^-1 and
+ * maps the accumulated step back through M^-1 at the end, as in orc_cg.
+ * `its` counts matrix-vector products (ipar(7)): 1 + 2 per iteration. */
+int orc_bcgstab(int n, const double *rhs, double *sol, int maxits, double rtol,
+                orc_op matvec, orc_op precond, void *ctx, orc_info *info)
+{
+  size_t N = (size_t)n;
+  double *r0 = (double *)calloc(N, sizeof(double));
+  double *r = (double *)calloc(N, sizeof(double));
+  double *s = (double *)calloc(N, sizeof(double));
+  double *t = (double *)calloc(N, sizeof(double));
+  double *v = (double *)calloc(N, sizeof(double));
+  double *p = (double *)calloc(N, sizeof(double));
+  double *z = (double *)calloc(N, sizeof(double));
+  double *dx = (double *)calloc(N, sizeof(double));
+  int rp = precond != NULL;
+  int its = 0, status = 1, ipar12 = 0;
+  double fpar3, fpar4, fpar5, fpar6, fpar7, alpha = 0.0, omega = 0.0;
+
+  /* labels 10/20: r0 = rhs - A sol, r = p = r0 */
+  matvec(ctx, sol, t);
+  its++;
+  for(int i = 0; i < n; i++) r0[i] = rhs[i] - t[i];
+  for(int i = 0; i < n; i++) {
+    r[i] = r0[i];
+    p[i] = r0[i];
+  }
+  fpar7 = orc_ddot(n, r0, r0);
+  fpar5 = sqrt(fpar7);
+  fpar3 = fpar5;
+  fpar4 = rtol * fpar3 + 0.0;
+  fpar6 = fpar5;
+  if(!(fpar5 <= fpar4)) {
+    for(;;) {
+      /* step (1): v = A M^-1 p */
+      if(rp) {
+        precond(ctx, p, z);
+        matvec(ctx, z, v);
+      }
+      else
+        matvec(ctx, p, v);
+      its++;
+      /* step (2) */
+      double den = orc_ddot(n, r0, v);
+      int bk = brkdn(den);
+      if(bk) {
+        status = bk;
+        break;
+      }
+      alpha = fpar7 / den;
+      /* step (3) */
+      for(int i = 0; i < n; i++) s[i] = r[i] - alpha * v[i];
+      /* step (4): t = A M^-1 s */
+      if(rp) {
+        precond(ctx, s, z);
+        matvec(ctx, z, t);
+      }
+      else
+        matvec(ctx, s, t);
+      its++;
+      /* step (5) */
+      double tt = orc_ddot(n, t, t);
+      bk = brkdn(tt);
+      if(bk) {
+        status = bk;
+        break;
+      }
+      omega = orc_ddot(n, t, s) / tt;
+      bk = brkdn(omega);
+      if(bk) {
+        status = bk;
+        break;
+      }
+      /* steps (6), (7) */
+      for(int i = 0; i < n; i++) {
+        z[i] = alpha * p[i] + omega * s[i];
+        dx[i] = dx[i] + z[i];
+        r[i] = s[i] - omega * t[i];
+      }
+      /* stopbis */
+      if(maxits > 0 && its >= maxits) {
+        status = -1;
+        break;
+      }
+      fpar5 = sqrt(orc_ddot(n, r, r));
+      fpar6 = fpar5;
+      if(!(fpar6 > fpar4)) break;
+      /* step (8) */
+      double rho = fpar7;
+      fpar7 = orc_ddot(n, r, r0);
+      double beta = fpar7 * alpha / (omega * rho);
+      for(int i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+      bk = brkdn(fpar7);
+      if(bk) {
+        status = bk;
+        break;
+      }
+    }
+  }
+  /* label 900 / 110 */
+  const double *delta = dx;
+  if(rp) {
+    if(status < 0) ipar12 = status;
+    status = 5;
+    precond(ctx, dx, z);
+    delta = z;
+  }
+  if(ipar12 != 0) status = ipar12;
+  else if(status > 0) {
+    if(fpar6 <= fpar4) status = 0;
+    else if(its >= maxits && maxits > 0) status = -1;
+    else status = -10;
+  }
+  for(int i = 0; i < n; i++) sol[i] = sol[i] + delta[i];
+  info->its = its;
+  info->status = status;
+  info->res_est = fpar6;
+  info->res0 = fpar3;
+  free(r0);
+  free(r);
+  free(s);
+  free(t);
+  free(v);
+  free(p);
+  free(z);
+  free(dx);
+  return status;
+}
+
 /* ---- RCMK (cmkreord.f:3-62, reordering.f:15-391, unary.f:1292-1508) ------ */
 
 /* BFS with a single start node and natural traversal order for further
@@ -672,6 +804,9 @@ int orc_solve(int n, const int *ia, const int *ja, double *a, double *b,
     switch(p->algorithm) {
     case 1:
       orc_cg(n, rhs, sol, p->maxits, p->rtol, op_amux, prec, &sc, info);
+      break;
+    case 5:
+      orc_bcgstab(n, rhs, sol, p->maxits, p->rtol, op_amux, prec, &sc, info);
       break;
     case 9:
       orc_fgmres(n, rhs, sol, p->krylov, p->maxits, p->rtol, op_amux, prec, &sc,

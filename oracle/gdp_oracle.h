@@ -76,6 +76,8 @@ int orc_fgmres(int n, const double *rhs, double *sol, int m, int maxits,
                orc_info *info);
 int orc_cg(int n, const double *rhs, double *sol, int maxits, double rtol,
            orc_op matvec, orc_op precond, void *ctx, orc_info *info);
+int orc_bcgstab(int n, const double *rhs, double *sol, int maxits, double rtol,
+                orc_op matvec, orc_op precond, void *ctx, orc_info *info);
 
 /* ---- Mesh entities and DOF numbering ------------------------------------ */
 /* Element types use Gmsh codes: 1 line, 2 triangle, 4 tetrahedron. */

@@ -8,7 +8,8 @@ The inputs are the small assembled systems of the three problem families
 the outputs are what the reference's kernels return for them: the RCMK
 permutation (cmkreord + sortcol), the FP32 ILU(0) factors in MSR form (ilu0),
 and the solutions / iteration counts / status of solve_matrix's sequence for
-GMRES(30), FGMRES(30), CG with diagonal scaling, and CG + ILU(0).
+GMRES(30), FGMRES(30), CG with diagonal scaling, GMRES(10) without
+renumbering, BCGSTAB + ILU(0) and BCGSTAB with diagonal scaling.
 """
 import sys
 from pathlib import Path
@@ -30,6 +31,8 @@ SOLVES = {
     "fgmres_ilu0": dict(algorithm=9, preconditioner=6, krylov=30),
     "cg_jacobi": dict(algorithm=1, preconditioner=0, scaling=1),
     "gmres_none_natural": dict(algorithm=8, preconditioner=0, renumbering=0, krylov=10),
+    "bcgstab_ilu0": dict(algorithm=5, preconditioner=6),
+    "bcgstab_jacobi": dict(algorithm=5, preconditioner=0, scaling=1),
 }
 
 

@@ -2,7 +2,7 @@
 
 * Golden fixtures (tests/golden/sparskit_ref.npz) hold outputs of the
   reference's Fortran routines (cmkreord/sortcol, ilu0, solve_matrix's
-  GMRES/FGMRES/CG sequence); the restatement must reproduce them bit for bit.
+  GMRES/FGMRES/CG/BCGSTAB sequence); the restatement must reproduce them bit for bit.
 * Where oracle/_ref is built (this container), the restatement is also run
   side by side with the reference kernels on random sparse matrices.
 """
@@ -15,6 +15,8 @@ SOLVES = {
     "fgmres_ilu0": dict(algorithm=9, preconditioner=6, krylov=30),
     "cg_jacobi": dict(algorithm=1, preconditioner=0, scaling=1),
     "gmres_none_natural": dict(algorithm=8, preconditioner=0, renumbering=0, krylov=10),
+    "bcgstab_ilu0": dict(algorithm=5, preconditioner=6),
+    "bcgstab_jacobi": dict(algorithm=5, preconditioner=0, scaling=1),
 }
 
 
@@ -98,7 +100,8 @@ def test_restatement_vs_reference_kernels_random(oracle, seed):
         assert np.array_equal(oracle.lusol(f1[0], f1[1], f1[2], y), oracle.ref_lusol(f2[0], f2[1], f2[2], y))
     b = rng.standard_normal(ia.shape[0] - 1)
     for kw in (dict(algorithm=8, preconditioner=6), dict(algorithm=9, preconditioner=6),
-               dict(algorithm=8, preconditioner=0, krylov=5), dict(algorithm=1, preconditioner=0)):
+               dict(algorithm=8, preconditioner=0, krylov=5), dict(algorithm=1, preconditioner=0),
+               dict(algorithm=5, preconditioner=6), dict(algorithm=5, preconditioner=0)):
         xo, io, _ = oracle.solve(ia, ja, a, b, maxits=3000, **kw)
         xr, ir, _ = oracle.ref_solve(ia, ja, a, b, maxits=3000, **kw)
         assert io["its"] == ir["its"] and io["status"] == ir["status"]
