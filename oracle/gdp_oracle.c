@@ -582,6 +582,152 @@ int orc_bcgstab(int n, const double *rhs, double *sol, int maxits, double rtol,
   return status;
 }
 
+/* tfqmr, iters.f:1307-1584, for ipar(3) = 1, with tidycg (:3333-3365) and
+ * brkdn (:3368-3401).  Work columns: r0 = w2, w = w3, y = w4, z = w5,
+ * ay = w6 = A M^-1 y, az = w7 = A M^-1 z (A sol at start), v = w8, d = w9,
+ * M^-1 input/output w10, dx = w11; bisinit zeroes all of them.  There is no
+ * test before the first iteration: a zero residual breaks down (rho = 0).
+ * The residual estimate is tau (fpar(5) = tao, the "approximation" branch). */
+int orc_tfqmr(int n, const double *rhs, double *sol, int maxits, double rtol,
+              orc_op matvec, orc_op precond, void *ctx, orc_info *info)
+{
+  size_t N = (size_t)n;
+  double *r0 = (double *)calloc(N, sizeof(double));
+  double *w = (double *)calloc(N, sizeof(double));
+  double *y = (double *)calloc(N, sizeof(double));
+  double *z = (double *)calloc(N, sizeof(double));
+  double *ay = (double *)calloc(N, sizeof(double));
+  double *az = (double *)calloc(N, sizeof(double));
+  double *v = (double *)calloc(N, sizeof(double));
+  double *d = (double *)calloc(N, sizeof(double));
+  double *tmp = (double *)calloc(N, sizeof(double));
+  double *dx = (double *)calloc(N, sizeof(double));
+  int rp = precond != NULL;
+  int its = 0, status = 1, ipar12 = 0;
+  double fpar3, fpar4, fpar5, fpar6 = 0.0;
+  double eta, sigma, theta, te, alpha, rho, tao;
+
+  /* labels 10/20 */
+  matvec(ctx, sol, az);
+  its++;
+  for(int i = 0; i < n; i++) {
+    r0[i] = rhs[i] - az[i];
+    d[i] = 0.0;
+  }
+  for(int i = 0; i < n; i++) w[i] = r0[i];
+  fpar5 = sqrt(orc_ddot(n, r0, r0));
+  fpar3 = fpar5;
+  tao = fpar5;
+  fpar4 = rtol * tao + 0.0;
+  te = 0.0;
+  rho = 0.0;
+  for(;;) {
+    /* label 30 */
+    sigma = rho;
+    rho = orc_ddot(n, r0, w);
+    int bk = brkdn(rho);
+    if(bk) {
+      status = bk;
+      break;
+    }
+    alpha = (its == 1) ? 0.0 : rho / sigma;
+    for(int i = 0; i < n; i++) y[i] = w[i] + alpha * z[i];
+    if(rp) {
+      precond(ctx, y, tmp);
+      matvec(ctx, tmp, ay);
+    }
+    else
+      matvec(ctx, y, ay);
+    /* label 60 */
+    its++;
+    for(int i = 0; i < n; i++) v[i] = ay[i] + alpha * (az[i] + alpha * v[i]);
+    sigma = orc_ddot(n, r0, v);
+    bk = brkdn(sigma);
+    if(bk) {
+      status = bk;
+      break;
+    }
+    alpha = rho / sigma;
+    for(int i = 0; i < n; i++) z[i] = y[i] - alpha * v[i];
+    if(rp) {
+      precond(ctx, z, tmp);
+      matvec(ctx, tmp, az);
+    }
+    else
+      matvec(ctx, z, az);
+    /* label 90 */
+    its++;
+    for(int i = 0; i < n; i++) w[i] = w[i] - alpha * ay[i];
+    /* update I */
+    theta = orc_ddot(n, w, w) / (tao * tao);
+    sigma = 1.0 / (1.0 + theta);
+    tao = tao * sqrt(sigma * theta);
+    bk = brkdn(tao);
+    if(bk) {
+      status = bk;
+      break;
+    }
+    eta = sigma * alpha;
+    sigma = te / alpha;
+    te = theta * eta;
+    for(int i = 0; i < n; i++) {
+      d[i] = y[i] + sigma * d[i];
+      dx[i] = dx[i] + eta * d[i];
+      w[i] = w[i] - alpha * az[i];
+    }
+    /* update II */
+    theta = orc_ddot(n, w, w) / (tao * tao);
+    sigma = 1.0 / (1.0 + theta);
+    tao = tao * sqrt(sigma * theta);
+    bk = brkdn(tao);
+    if(bk) {
+      status = bk;
+      break;
+    }
+    eta = sigma * alpha;
+    sigma = te / alpha;
+    te = theta * eta;
+    for(int i = 0; i < n; i++) {
+      d[i] = z[i] + sigma * d[i];
+      dx[i] = dx[i] + eta * d[i];
+    }
+    fpar5 = tao;
+    fpar6 = fpar5;
+    if(!(fpar6 > fpar4 && (its < maxits || maxits <= 0))) break;
+  }
+  /* label 900 / 110 */
+  const double *delta = dx;
+  if(rp) {
+    if(status < 0) ipar12 = status;
+    status = 5;
+    precond(ctx, dx, tmp);
+    delta = tmp;
+  }
+  if(ipar12 != 0) status = ipar12;
+  else if(status > 0) {
+    if(fpar6 <= fpar4) status = 0;
+    else if(its >= maxits && maxits > 0) status = -1;
+    else status = -10;
+  }
+  for(int i = 0; i < n; i++) sol[i] = sol[i] + delta[i];
+  info->its = its;
+  info->status = status;
+  info->res_est = fpar6;
+  info->res0 = fpar3;
+  (void)fpar5;
+  free(r0);
+  free(w);
+  free(y);
+  free(z);
+  free(ay);
+  free(az);
+  free(v);
+  free(d);
+  free(tmp);
+  free(dx);
+  return status;
+}
+
 /* ---- RCMK (cmkreord.f:3-62, reordering.f:15-391, unary.f:1292-1508) ------ */
 
 /* BFS with a single start node and natural traversal order for further
@@ -807,6 +953,9 @@ int orc_solve(int n, const int *ia, const int *ja, double *a, double *b,
       break;
     case 5:
       orc_bcgstab(n, rhs, sol, p->maxits, p->rtol, op_amux, prec, &sc, info);
+      break;
+    case 6:
+      orc_tfqmr(n, rhs, sol, p->maxits, p->rtol, op_amux, prec, &sc, info);
       break;
     case 9:
       orc_fgmres(n, rhs, sol, p->krylov, p->maxits, p->rtol, op_amux, prec, &sc,

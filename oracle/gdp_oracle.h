@@ -78,6 +78,8 @@ int orc_cg(int n, const double *rhs, double *sol, int maxits, double rtol,
            orc_op matvec, orc_op precond, void *ctx, orc_info *info);
 int orc_bcgstab(int n, const double *rhs, double *sol, int maxits, double rtol,
                 orc_op matvec, orc_op precond, void *ctx, orc_info *info);
+int orc_tfqmr(int n, const double *rhs, double *sol, int maxits, double rtol,
+              orc_op matvec, orc_op precond, void *ctx, orc_info *info);
 
 /* ---- Mesh entities and DOF numbering ------------------------------------ */
 /* Element types use Gmsh codes: 1 line, 2 triangle, 4 tetrahedron. */

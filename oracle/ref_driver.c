@@ -32,6 +32,8 @@ extern void cg_(int *n, double *rhs, double *sol, int *ipar, double *fpar,
                 double *w);
 extern void bcgstab_(int *n, double *rhs, double *sol, int *ipar, double *fpar,
                      double *w);
+extern void tfqmr_(int *n, double *rhs, double *sol, int *ipar, double *fpar,
+                   double *w);
 extern void gmres_(int *n, double *rhs, double *sol, int *ipar, double *fpar,
                    double *w);
 extern void fgmres_(int *n, double *rhs, double *sol, int *ipar, double *fpar,
@@ -52,7 +54,7 @@ extern double ddot_(int *n, double *x, int *incx, double *y, int *incy);
 int ref_sscalar_size(void) { return (int)sizeof(sscalar); }
 
 /* Sparskit.h algorithm / preconditioner codes (contrib/Sparskit/Sparskit.h). */
-enum { A_CG = 1, A_BCGSTAB = 5, A_GMRES = 8, A_FGMRES = 9 };
+enum { A_CG = 1, A_BCGSTAB = 5, A_TFQMR = 6, A_GMRES = 8, A_FGMRES = 9 };
 enum { P_NONE = 0, P_ILU0 = 6, P_DIAGONAL = 8 };
 
 typedef struct {
@@ -109,8 +111,8 @@ void ref_amux(int n, double *x, double *y, double *a, int *ja, int *ia)
 }
 
 /* solve_matrix (Sparskit.cpp:108-634) for the SPARSE format, the
- * preconditioners NONE / ILU0 / DIAGONAL, the algorithms CG / BCGSTAB / GMRES /
- * FGMRES,
+ * preconditioners NONE / ILU0 / DIAGONAL, the algorithms CG / BCGSTAB / TFQMR /
+ * GMRES / FGMRES,
  * renumbering NONE / RCMK and scaling NONE / DIAG.  a, ja, ia are modified in
  * place by scaling, exactly as the reference modifies its matrix. */
 int ref_solve(int n, double *a, int *ja, int *ia, double *b, double *x,
@@ -202,6 +204,7 @@ int ref_solve(int n, double *a, int *ja, int *ia, double *b, double *x,
     switch(p->algorithm) {
     case A_CG: ipar[4] = 5 * n; break;
     case A_BCGSTAB: ipar[4] = 8 * n; break;
+    case A_TFQMR: ipar[4] = 11 * n; break;
     case A_GMRES: ipar[4] = (n + 3) * (m + 2) + (m + 1) * m / 2; break;
     case A_FGMRES:
       ipar[4] = 2 * n * (m + 1) + (m + 1) * m / 2 + 3 * m + 2;
@@ -215,6 +218,7 @@ int ref_solve(int n, double *a, int *ja, int *ia, double *b, double *x,
       switch(p->algorithm) {
       case A_CG: cg_(&n, rhs, sol, &ipar[1], &fpar[1], w); break;
       case A_BCGSTAB: bcgstab_(&n, rhs, sol, &ipar[1], &fpar[1], w); break;
+      case A_TFQMR: tfqmr_(&n, rhs, sol, &ipar[1], &fpar[1], w); break;
       case A_GMRES: gmres_(&n, rhs, sol, &ipar[1], &fpar[1], w); break;
       case A_FGMRES: fgmres_(&n, rhs, sol, &ipar[1], &fpar[1], w); break;
       }

@@ -9,7 +9,8 @@ the outputs are what the reference's kernels return for them: the RCMK
 permutation (cmkreord + sortcol), the FP32 ILU(0) factors in MSR form (ilu0),
 and the solutions / iteration counts / status of solve_matrix's sequence for
 GMRES(30), FGMRES(30), CG with diagonal scaling, GMRES(10) without
-renumbering, BCGSTAB + ILU(0) and BCGSTAB with diagonal scaling.
+renumbering, BCGSTAB + ILU(0), BCGSTAB with diagonal scaling, TFQMR + ILU(0) and
+unpreconditioned TFQMR.
 """
 import sys
 from pathlib import Path
@@ -33,6 +34,8 @@ SOLVES = {
     "gmres_none_natural": dict(algorithm=8, preconditioner=0, renumbering=0, krylov=10),
     "bcgstab_ilu0": dict(algorithm=5, preconditioner=6),
     "bcgstab_jacobi": dict(algorithm=5, preconditioner=0, scaling=1),
+    "tfqmr_ilu0": dict(algorithm=6, preconditioner=6),
+    "tfqmr_none": dict(algorithm=6, preconditioner=0),
 }
 
 

@@ -2,7 +2,7 @@
 
 * Golden fixtures (tests/golden/sparskit_ref.npz) hold outputs of the
   reference's Fortran routines (cmkreord/sortcol, ilu0, solve_matrix's
-  GMRES/FGMRES/CG/BCGSTAB sequence); the restatement must reproduce them bit for bit.
+  GMRES/FGMRES/CG/BCGSTAB/TFQMR sequence); the restatement must reproduce them bit for bit.
 * Where oracle/_ref is built (this container), the restatement is also run
   side by side with the reference kernels on random sparse matrices.
 """
@@ -17,6 +17,8 @@ SOLVES = {
     "gmres_none_natural": dict(algorithm=8, preconditioner=0, renumbering=0, krylov=10),
     "bcgstab_ilu0": dict(algorithm=5, preconditioner=6),
     "bcgstab_jacobi": dict(algorithm=5, preconditioner=0, scaling=1),
+    "tfqmr_ilu0": dict(algorithm=6, preconditioner=6),
+    "tfqmr_none": dict(algorithm=6, preconditioner=0),
 }
 
 
@@ -101,7 +103,8 @@ def test_restatement_vs_reference_kernels_random(oracle, seed):
     b = rng.standard_normal(ia.shape[0] - 1)
     for kw in (dict(algorithm=8, preconditioner=6), dict(algorithm=9, preconditioner=6),
                dict(algorithm=8, preconditioner=0, krylov=5), dict(algorithm=1, preconditioner=0),
-               dict(algorithm=5, preconditioner=6), dict(algorithm=5, preconditioner=0)):
+               dict(algorithm=5, preconditioner=6), dict(algorithm=5, preconditioner=0),
+               dict(algorithm=6, preconditioner=6), dict(algorithm=6, preconditioner=0)):
         xo, io, _ = oracle.solve(ia, ja, a, b, maxits=3000, **kw)
         xr, ir, _ = oracle.ref_solve(ia, ja, a, b, maxits=3000, **kw)
         assert io["its"] == ir["its"] and io["status"] == ir["status"]
@@ -149,3 +152,33 @@ def test_zero_pivot_reported(oracle):
     a = np.array([0.0, 1.0, 1.0, 1.0])
     alu, jlu, ju, ierr = oracle.ilu0(ia, ja, a)
     assert ierr == 1
+
+
+@pytest.mark.parametrize("algorithm,its", [(5, 5), (6, 9)])
+def test_short_iteration_limits_match_reference(oracle, algorithm, its):
+    """BCGSTAB tests the limit after the second product of an iteration
+    (4 -> 5 products), TFQMR at the end of an iteration (9 -> 9)."""
+    if oracle.ref() is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(7)
+    ia, ja, a = _random_csr(rng, 300, 900, diag_boost=1.0)
+    b = rng.standard_normal(ia.shape[0] - 1)
+    maxits = 4 if algorithm == 5 else 9
+    xo, io, _ = oracle.solve(ia, ja, a, b, maxits=maxits, algorithm=algorithm, preconditioner=6)
+    xr, ir, _ = oracle.ref_solve(ia, ja, a, b, maxits=maxits, algorithm=algorithm, preconditioner=6)
+    assert io["status"] == ir["status"] == -1
+    assert io["its"] == ir["its"] == its
+    assert np.array_equal(xo, xr)
+
+
+@pytest.mark.parametrize("algorithm,status", [(5, 0), (6, -3)])
+def test_zero_rhs_matches_reference(oracle, golden, algorithm, status):
+    """BCGSTAB stops before its first iteration on a zero residual; TFQMR has
+    no such test and reports the breakdown rho = <r0, r> = 0 (-3)."""
+    ia, ja, a, b = _sys(golden, "poisson2d_8")
+    x, info, _ = oracle.solve(ia, ja, a, np.zeros_like(b), algorithm=algorithm, preconditioner=6)
+    assert info["status"] == status and info["its"] == 1
+    assert not np.any(x)
+    if oracle.ref() is not None:
+        xr, ir, _ = oracle.ref_solve(ia, ja, a, np.zeros_like(b), algorithm=algorithm, preconditioner=6)
+        assert ir["status"] == status and ir["its"] == 1
