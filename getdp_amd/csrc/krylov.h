@@ -1,4 +1,5 @@
-// Krylov solvers on the device (CG, GMRES, FGMRES of contrib/Sparskit/iters.f).
+// Krylov solvers on the device (CG, BCGSTAB, TFQMR, GMRES, FGMRES of
+// contrib/Sparskit/iters.f).
 #pragma once
 
 #include "gdp_device.h"
@@ -17,8 +18,9 @@ struct KCtl {
   int kfinal, ierr12, ipar11, pend_flag;
   int maxits, m, nb, pad;
   double rtol, fpar3, fpar4, fpar5, fpar6, fpar7, alpha, beta;
-  double scal, nrm0, thr, pend;
-  double rr[2];  // CG: <r,r> of consecutive iterations (parity slots)
+  double scal, nrm0, thr, pend, omega;
+  double tao, te, tao1, te1;  // TFQMR: tau and theta*eta after each half step
+  double rr[2];  // CG: <r,r>, BCGSTAB: rho = <r,r0> of consecutive iterations (parity slots)
   double g[kMaxKrylov + 2], c[kMaxKrylov + 1], s[kMaxKrylov + 1];
   double h1[kMaxKrylov + 2], h2[kMaxKrylov + 2];
   double H[(kMaxKrylov + 1) * (kMaxKrylov + 2)];  // column j: H[j*(kMaxKrylov+2) + i]
@@ -65,7 +67,7 @@ struct InnerSolver {
 int inner_apply(InnerSolver *s, const double *v, double *z, KCtl *outer, int gmode);
 
 struct KrylovOptions {
-  int algorithm = 8;  // 1 CG, 8 GMRES, 9 FGMRES
+  int algorithm = 8;  // 1 CG, 5 BCGSTAB, 6 TFQMR, 8 GMRES, 9 FGMRES
   int m = 30;
   int maxits = 1000;
   double rtol = 1e-10;

@@ -2,6 +2,8 @@
 //   gmres  (iters.f:1912-2217)  restarted GMRES, right preconditioning
 //   fgmres (iters.f:2522-2807)  flexible GMRES (stores Z = M^{-1} V)
 //   cg     (iters.f:379-562)    Sparskit's CG on A M^{-1} (not PCG)
+//   bcgstab (iters.f:1032-1306) BiCGStab on A M^{-1}, step mapped back at the end
+//   tfqmr  (iters.f:1307-1584)  transpose-free QMR on A M^{-1}, likewise
 // with x0 = 0 and the stopping rule of ipar(3) = 1: stop when the (estimated)
 // residual norm falls to Stopping_Test times the first residual norm.
 //
@@ -530,6 +532,462 @@ __global__ void __launch_bounds__(BS) k_cg_p2(int n, const TV *r, TV *p, KCtl *C
     p[i] = (TV)((double)r[i] + beta * (double)p[i]);
 }
 
+// ---- BCGSTAB (iters.f:1032-1306) ----------------------------------------------------------
+// Same scheme as the CG kernels: every block re-derives the scalar of the
+// step from the previous kernel's partials, block 0 records it.  Vectors in
+// the reference's work columns: r0 = w1, r = w2, s = w3, t = w4, v = w5,
+// p = w6, dx = w8 (w7, the preconditioner's output, is a separate z).
+
+// brkdn (iters.f:3368-3401): 0, -3 (zero) or -9 (abnormal number)
+__device__ inline int brkdn_code(double a)
+{
+  if(a > 0.0) return (1.0 / a > 0.0) ? 0 : -9;
+  if(a < 0.0) return (1.0 / a < 0.0) ? 0 : -9;
+  if(a == 0.0) return -3;
+  return -9;
+}
+
+// labels 10/20: r0 = r = p = rhs - A sol (A sol in t); partial |r0|^2
+template <typename TV>
+__global__ void __launch_bounds__(B) k_bs_init(int n, const TV *rhs, const TV *t, TV *r0, TV *r, TV *p,
+                                              double *part)
+{
+  double s = 0;
+  for(long i = (long)blockIdx.x * B + threadIdx.x; i < n; i += (long)gridDim.x * B) {
+    TV v = rhs[i] - t[i];
+    r0[i] = v;
+    r[i] = v;
+    p[i] = v;
+    s += (double)v * (double)v;
+  }
+  s = block_sum<double, B>(s);
+  if(threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// t = A x fused with the partial sums of <t,t> (part) and <t,s> (part + np).
+template <typename TA, typename TV>
+__global__ void __launch_bounds__(256) k_op_dot2(SellOp<TA> A, const TV *__restrict__ x,
+                                                 const TV *__restrict__ sv, TV *__restrict__ t,
+                                                 const KCtl *C, double *part, int np)
+{
+  if(C->done) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  double tt = 0., ts = 0.;
+  if(row < A.n) {
+    double sum;
+    sell_row<TA, TV>(row, A, x, sum);
+    TV tv = (TV)sum;
+    t[row] = tv;
+    tt = (double)tv * (double)tv;
+    ts = (double)tv * (double)sv[row];
+  }
+  tt = block_sum<double, 256>(tt);
+  ts = block_sum<double, 256>(ts);
+  if(threadIdx.x == 0) {
+    part[blockIdx.x] = tt;
+    part[np + blockIdx.x] = ts;
+  }
+}
+
+// steps (2), (3): alpha = rho / <r0, v> (label 60), s = r - alpha v
+template <typename TV, int BS>
+__global__ void __launch_bounds__(BS) k_bs_s(int n, const TV *r, const TV *v, TV *s, KCtl *C,
+                                            const double *part_rv, int np, int par)
+{
+  __shared__ double s_alpha;
+  __shared__ int s_skip;
+  if(C->done) return;
+  double den = sum_partials<double, BS>(part_rv, np);
+  if(threadIdx.x == 0) {
+    int bk = brkdn_code(den);
+    s_skip = bk;
+    s_alpha = bk ? 0. : C->rr[par] / den;
+    if(blockIdx.x == 0) {
+      C->its++;
+      if(bk) {
+        C->status = bk;
+        C->done = 1;
+      }
+      else
+        C->alpha = s_alpha;
+    }
+  }
+  __syncthreads();
+  if(s_skip) return;
+  const double alpha = s_alpha;
+  for(long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS)
+    s[i] = (TV)((double)r[i] - alpha * (double)v[i]);
+}
+
+// step (5): omega = <t,s> / <t,t> (label 90) with both breakdown tests;
+// steps (6), (7): dx += alpha p + omega s, r = s - omega t; partial |r|^2
+// (part) and <r, r0> (part + np) for the stopping test and step (8).
+template <typename TV, int BS>
+__global__ void __launch_bounds__(BS) k_bs_update(int n, const TV *p, const TV *s, const TV *t,
+                                                 const TV *r0, TV *r, TV *dx, KCtl *C,
+                                                 const double *part_t, int npt, double *part, int np)
+{
+  __shared__ double s_omega;
+  __shared__ int s_skip;
+  if(C->done) return;
+  double tt = sum_partials<double, BS>(part_t, npt);
+  double ts = sum_partials<double, BS>(part_t + npt, npt);
+  if(threadIdx.x == 0) {
+    int bk = brkdn_code(tt);
+    double omega = 0.;
+    if(!bk) {
+      omega = ts / tt;
+      bk = brkdn_code(omega);
+    }
+    s_skip = bk;
+    s_omega = omega;
+    if(blockIdx.x == 0) {
+      C->its++;
+      if(bk) {
+        C->status = bk;
+        C->done = 1;
+      }
+      else
+        C->omega = omega;
+    }
+  }
+  __syncthreads();
+  if(s_skip) return;
+  const double alpha = C->alpha, omega = s_omega;
+  double rr = 0., rr0 = 0.;
+  for(long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) {
+    const double si = (double)s[i];
+    const double w7 = alpha * (double)p[i] + omega * si;
+    dx[i] = (TV)((double)dx[i] + w7);
+    TV v = (TV)(si - omega * (double)t[i]);
+    r[i] = v;
+    rr += (double)v * (double)v;
+    rr0 += (double)v * (double)r0[i];
+  }
+  rr = block_sum<double, BS>(rr);
+  rr0 = block_sum<double, BS>(rr0);
+  if(threadIdx.x == 0) {
+    part[blockIdx.x] = rr;
+    part[np + blockIdx.x] = rr0;
+  }
+}
+
+// stopbis on |r| (iters.f:3275-3330), then step (8): rho = <r, r0>,
+// beta = rho alpha / (omega rho_old), p = r + beta (p - omega v), brkdn(rho).
+template <typename TV, int BS>
+__global__ void __launch_bounds__(BS) k_bs_p(int n, const TV *r, const TV *v, TV *p, KCtl *C,
+                                            const double *part, int np, int par)
+{
+  __shared__ double s_beta;
+  __shared__ int s_stop;
+  if(C->done) return;
+  double rr = sum_partials<double, BS>(part, np);
+  double rho = sum_partials<double, BS>(part + np, np);
+  if(threadIdx.x == 0) {
+    int stop = 0;
+    double fpar5 = 0., beta = 0.;
+    if(C->maxits > 0 && C->its >= C->maxits) stop = 1;
+    else {
+      fpar5 = sqrt(rr);
+      if(fpar5 > C->fpar4) beta = rho * C->alpha / (C->omega * C->rr[par]);
+      else stop = 2;
+    }
+    s_stop = stop;
+    s_beta = beta;
+    if(blockIdx.x == 0) {
+      if(stop == 1) {
+        C->status = -1;
+        C->done = 1;
+      }
+      else {
+        C->fpar5 = fpar5;
+        C->fpar6 = fpar5;
+        if(stop == 2) {
+          C->ipar11 = 1;
+          C->done = 1;
+        }
+        else {
+          C->rr[1 - par] = rho;
+          C->fpar7 = rho;
+          C->beta = beta;
+          const int bk = brkdn_code(rho);
+          if(bk) {  // tested after p is formed; the solve ends here
+            C->status = bk;
+            C->done = 1;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if(s_stop) return;
+  const double beta = s_beta, omega = C->omega;
+  for(long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS)
+    p[i] = (TV)((double)r[i] + beta * ((double)p[i] - omega * (double)v[i]));
+}
+
+// ---- TFQMR (iters.f:1307-1584) ------------------------------------------------------------
+// Work columns of the reference: r0 = w2, w = w3, y = w4, z = w5, ay = w6,
+// az = w7, v = w8, d = w9, dx = w11 (w10, the preconditioner's output, is z
+// of the driver).  Scalars: rho in parity slots rr[], beta (the routine's
+// first `alpha`), alpha, tau and te after update I (tao1, te1) and II (tao, te).
+// Update II both updates dx and may end the solve, so its block 0 only
+// raises pend_flag; the next kernel turns it into `done` (all its blocks
+// skip), and no block of update II can miss its share of the update.
+
+// labels 10/20: r0 = w = rhs - A sol (A sol in az); partial <r0, w> for
+// rho of the first iteration; the scalars bisinit clears.
+template <typename TV, int BS>
+__global__ void __launch_bounds__(BS) k_tq_init(int n, const TV *rhs, const TV *az, TV *r0, TV *w, KCtl *C,
+                                               double *part)
+{
+  double s = 0;
+  for(long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) {
+    TV v = rhs[i] - az[i];
+    r0[i] = v;
+    w[i] = v;
+    s += (double)v * (double)v;
+  }
+  s = block_sum<double, BS>(s);
+  if(threadIdx.x == 0) {
+    part[blockIdx.x] = s;
+    if(blockIdx.x == 0) {
+      C->its = 1;
+      C->status = 1;
+      C->te = 0.;
+      C->rr[0] = 0.;
+      C->fpar6 = 0.;
+      C->pend_flag = 0;
+    }
+  }
+}
+
+// label 30: rho = <r0, w>, brkdn(rho), beta = rho / rho_old (0 on the first
+// iteration), y = w + beta z.  The first iteration also sets tau = |r0| and
+// the target fpar(4) (label 20).
+template <typename TV, int BS>
+__global__ void __launch_bounds__(BS) k_tq_y(int n, const TV *w, const TV *z, TV *y, KCtl *C,
+                                            const double *part_rho, int np, int par)
+{
+  __shared__ double s_beta;
+  __shared__ int s_skip;
+  if(C->done) return;
+  if(C->pend_flag) {
+    if(blockIdx.x == 0 && threadIdx.x == 0) C->done = 1;
+    return;
+  }
+  double rho = sum_partials<double, BS>(part_rho, np);
+  if(threadIdx.x == 0) {
+    const int bk = brkdn_code(rho);
+    const bool first = C->its == 1;
+    s_skip = bk;
+    s_beta = (bk || first) ? 0. : rho / C->rr[par];
+    if(blockIdx.x == 0) {
+      if(first) {
+        C->fpar5 = sqrt(rho);
+        C->fpar3 = C->fpar5;
+        C->tao = C->fpar5;
+        C->fpar4 = C->rtol * C->tao + 0.0;
+      }
+      if(bk) {
+        C->status = bk;
+        C->done = 1;
+      }
+      else {
+        C->rr[1 - par] = rho;
+        C->beta = s_beta;
+      }
+    }
+  }
+  __syncthreads();
+  if(s_skip) return;
+  const double beta = s_beta;
+  for(long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS)
+    y[i] = (TV)((double)w[i] + beta * (double)z[i]);
+}
+
+// ay = A x, v = ay + beta (az + beta v); partial <r0, v>.
+template <typename TA, typename TV>
+__global__ void __launch_bounds__(256) k_tq_op_v(SellOp<TA> A, const TV *__restrict__ x,
+                                                 const TV *__restrict__ az, const TV *__restrict__ r0,
+                                                 TV *__restrict__ ay, TV *__restrict__ v, const KCtl *C,
+                                                 double *part)
+{
+  if(C->done) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  double t = 0.;
+  if(row < A.n) {
+    double sum;
+    sell_row<TA, TV>(row, A, x, sum);
+    TV ayv = (TV)sum;
+    ay[row] = ayv;
+    const double beta = C->beta;
+    TV vv = (TV)((double)ayv + beta * ((double)az[row] + beta * (double)v[row]));
+    v[row] = vv;
+    t = (double)r0[row] * (double)vv;
+  }
+  t = block_sum<double, 256>(t);
+  if(threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// label 60: sigma = <r0, v>, brkdn, alpha = rho / sigma, z = y - alpha v.
+template <typename TV, int BS>
+__global__ void __launch_bounds__(BS) k_tq_z(int n, const TV *y, const TV *v, TV *z, KCtl *C,
+                                            const double *part_sig, int np, int par)
+{
+  __shared__ double s_alpha;
+  __shared__ int s_skip;
+  if(C->done) return;
+  double sigma = sum_partials<double, BS>(part_sig, np);
+  if(threadIdx.x == 0) {
+    const int bk = brkdn_code(sigma);
+    s_skip = bk;
+    s_alpha = bk ? 0. : C->rr[1 - par] / sigma;
+    if(blockIdx.x == 0) {
+      C->its++;
+      if(bk) {
+        C->status = bk;
+        C->done = 1;
+      }
+      else
+        C->alpha = s_alpha;
+    }
+  }
+  __syncthreads();
+  if(s_skip) return;
+  const double alpha = s_alpha;
+  for(long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS)
+    z[i] = (TV)((double)y[i] - alpha * (double)v[i]);
+}
+
+// az = A x, w = w - alpha ay (label 90); partial |w|^2.
+template <typename TA, typename TV>
+__global__ void __launch_bounds__(256) k_tq_op_w(SellOp<TA> A, const TV *__restrict__ x,
+                                                 const TV *__restrict__ ay, TV *__restrict__ az,
+                                                 TV *__restrict__ w, const KCtl *C, double *part)
+{
+  if(C->done) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  double t = 0.;
+  if(row < A.n) {
+    double sum;
+    sell_row<TA, TV>(row, A, x, sum);
+    az[row] = (TV)sum;
+    TV wv = (TV)((double)w[row] - C->alpha * (double)ay[row]);
+    w[row] = wv;
+    t = (double)wv * (double)wv;
+  }
+  t = block_sum<double, 256>(t);
+  if(threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// tau update shared by updates I and II: theta = |w|^2 / tau^2,
+// sigma = 1 / (1 + theta), tau' = tau sqrt(sigma theta), eta = sigma alpha,
+// the coefficient of d te / alpha, te' = theta eta.
+__device__ inline void tq_tau(double ww, double tao, double te, double alpha, double &tao_new,
+                              double &eta, double &dcoef, double &te_new)
+{
+  const double theta = ww / (tao * tao);
+  const double sigma = 1.0 / (1.0 + theta);
+  tao_new = tao * sqrt(sigma * theta);
+  eta = sigma * alpha;
+  dcoef = te / alpha;
+  te_new = theta * eta;
+}
+
+// update I: d = y + (te/alpha) d, dx += eta d, w = w - alpha az; partial |w|^2
+// (part) and <r0, w> (part + np, rho of the next iteration).
+template <typename TV, int BS>
+__global__ void __launch_bounds__(BS) k_tq_upd1(int n, const TV *y, const TV *az, const TV *r0, TV *d,
+                                               TV *dx, TV *w, KCtl *C, const double *part_w, int npw,
+                                               double *part, int np)
+{
+  __shared__ double s_eta, s_dc;
+  __shared__ int s_skip;
+  if(C->done) return;
+  double ww = sum_partials<double, BS>(part_w, npw);
+  if(threadIdx.x == 0) {
+    double tao, eta, dc, te;
+    tq_tau(ww, C->tao, C->te, C->alpha, tao, eta, dc, te);
+    const int bk = brkdn_code(tao);
+    s_skip = bk;
+    s_eta = eta;
+    s_dc = dc;
+    if(blockIdx.x == 0) {
+      C->its++;
+      if(bk) {
+        C->status = bk;
+        C->done = 1;
+      }
+      else {
+        C->tao1 = tao;
+        C->te1 = te;
+      }
+    }
+  }
+  __syncthreads();
+  if(s_skip) return;
+  const double eta = s_eta, dc = s_dc, alpha = C->alpha;
+  double s1 = 0., s2 = 0.;
+  for(long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) {
+    const double di = (double)y[i] + dc * (double)d[i];
+    const TV dv = (TV)di;
+    d[i] = dv;
+    dx[i] = (TV)((double)dx[i] + eta * (double)dv);
+    const TV wv = (TV)((double)w[i] - alpha * (double)az[i]);
+    w[i] = wv;
+    s1 += (double)wv * (double)wv;
+    s2 += (double)r0[i] * (double)wv;
+  }
+  s1 = block_sum<double, BS>(s1);
+  s2 = block_sum<double, BS>(s2);
+  if(threadIdx.x == 0) {
+    part[blockIdx.x] = s1;
+    part[np + blockIdx.x] = s2;
+  }
+}
+
+// update II: d = z + (te/alpha) d, dx += eta d; fpar(5) = fpar(6) = tau and
+// the loop test of label 90 (tolerance or iteration limit).
+template <typename TV, int BS>
+__global__ void __launch_bounds__(BS) k_tq_upd2(int n, const TV *z, TV *d, TV *dx, KCtl *C,
+                                               const double *part_w, int npw)
+{
+  __shared__ double s_eta, s_dc;
+  __shared__ int s_skip;
+  if(C->done) return;
+  double ww = sum_partials<double, BS>(part_w, npw);
+  if(threadIdx.x == 0) {
+    double tao, eta, dc, te;
+    tq_tau(ww, C->tao1, C->te1, C->alpha, tao, eta, dc, te);
+    const int bk = brkdn_code(tao);
+    s_skip = bk;
+    s_eta = eta;
+    s_dc = dc;
+    if(blockIdx.x == 0) {
+      if(bk) {
+        C->status = bk;
+        C->pend_flag = 1;
+      }
+      else {
+        C->tao = tao;
+        C->te = te;
+        C->fpar5 = tao;
+        C->fpar6 = tao;
+        if(!(tao > C->fpar4 && (C->its < C->maxits || C->maxits <= 0))) C->pend_flag = 1;
+      }
+    }
+  }
+  __syncthreads();
+  if(s_skip) return;
+  const double eta = s_eta, dc = s_dc;
+  for(long i = (long)blockIdx.x * BS + threadIdx.x; i < n; i += (long)gridDim.x * BS) {
+    const TV dv = (TV)((double)z[i] + dc * (double)d[i]);
+    d[i] = dv;
+    dx[i] = (TV)((double)dx[i] + eta * (double)dv);
+  }
+}
+
 // ---- finalize kernels (one block) -------------------------------------------------------
 
 enum FinOp {
@@ -540,7 +998,8 @@ enum FinOp {
   FIN_MGS_NORM,   // nrm1 = |V_k|, Hessenberg column
   FIN_CG_INIT,
   FIN_CG_PAP,     // brkdn + alpha = <r,r>/<p,Ap>
-  FIN_CG_UPDATE   // stopbis (iters.f:3275-3330)
+  FIN_CG_UPDATE,  // stopbis (iters.f:3275-3330)
+  FIN_BS_INIT     // bcgstab label 20: rho, tolerance, early exit (iters.f:1140-1154)
 };
 
 template <int OP>
@@ -553,7 +1012,7 @@ __global__ void __launch_bounds__(B) k_fin(KCtl *C, const double *part, int npar
   else if(OP == FIN_NRM0) {
     if(gated(C, 2)) return;
   }
-  else if(OP != FIN_CG_INIT) {
+  else if(OP != FIN_CG_INIT && OP != FIN_BS_INIT) {
     if(gated(C, 2) || C->ierr12) return;
   }
   const double tot = sum_partials<double, B>(part, nparts);
@@ -613,6 +1072,18 @@ __global__ void __launch_bounds__(B) k_fin(KCtl *C, const double *part, int npar
     C->fpar6 = 0.0;
     C->ipar11 = 0;
     C->status = 1;
+  }
+  else if(OP == FIN_BS_INIT) {
+    C->its = 1;
+    C->fpar7 = tot;
+    C->rr[0] = tot;
+    C->fpar5 = sqrt(tot);
+    C->fpar3 = C->fpar5;
+    C->fpar4 = C->rtol * C->fpar3 + 0.0;
+    C->fpar6 = C->fpar5;
+    C->ipar11 = 0;
+    C->status = 1;
+    C->done = C->fpar5 <= C->fpar4 ? 1 : 0;
   }
   else if(OP == FIN_CG_PAP) {
     const double alpha = tot;
@@ -990,6 +1461,38 @@ int read_ctl(KCtl *d, int *done, int *status, int *its)
   return GDP_OK;
 }
 
+// tidycg (iters.f:3333-3365) after label 900 of cg / bcgstab / tfqmr: the final
+// status, then sol += M^{-1} dx (right preconditioning) or sol += dx.
+template <typename TV>
+int tidy_cg(const PrecOp<TV> &M, KCtl *C, TV *dx, TV *z, TV *sol, int n, KrylovResult &res)
+{
+  hipStream_t s = ctx().stream;
+  KCtl tmp;
+  GDP_TRY(d2h(&tmp, C, 1));
+  int final_status = tmp.status;
+  if(final_status > 0) {
+    if(tmp.fpar6 <= tmp.fpar4) final_status = 0;
+    else if(tmp.its >= tmp.maxits && tmp.maxits > 0) final_status = -1;
+    else final_status = -10;
+  }
+  int zero = 0;
+  GDP_TRY(h2d(&C->done, &zero, 1));
+  dim3 g(reduce_blocks(n)), blk(B);
+  if(M.kind != PREC_NONE) {
+    GDP_TRY(prec_apply(M, dx, z, C, 0, n));
+    hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, z, C);
+  }
+  else
+    hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, dx, C);
+  GDP_HIP(hipGetLastError());
+  GDP_TRY(sync());
+  if(M.kind == PREC_ILU) GDP_TRY(M.ilu->check_timeout());
+  res.its = tmp.its;
+  res.status = final_status;
+  res.res0 = tmp.fpar3;
+  res.res_est = tmp.fpar6;
+  return GDP_OK;
+}
 
 }  // namespace
 
@@ -1002,10 +1505,12 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
   const int m = std::max(2, std::min(opt.m <= 1 ? 15 : opt.m, kMaxKrylov));
   const bool flexible = opt.algorithm == 9;
   const bool cg = opt.algorithm == 1;
-  if(!cg && opt.algorithm != 8 && !flexible)
+  const bool bcg = opt.algorithm == 5;
+  const bool tfq = opt.algorithm == 6;
+  if(!cg && !bcg && !tfq && opt.algorithm != 8 && !flexible)
     return set_error(GDP_E_UNSUPPORTED, "Unknown algorithm for sparse matrix solver (%d)", opt.algorithm);
   const bool two_stage = flexible && M.kind == PREC_USER && M.ilu != nullptr;
-  size_t nvec = cg ? 6 : (size_t)(m + 2) + (flexible ? m : 1) + (two_stage ? 1 : 0);
+  size_t nvec = cg ? 6 : bcg ? 8 : tfq ? 10 : (size_t)(m + 2) + (flexible ? m : 1) + (two_stage ? 1 : 0);
   GDP_TRY(ensure_ws(nvec * (size_t)std::max(1, n) * sizeof(TV) + 256, n));
   Workspace &W = ws();
   TV *base = (TV *)W.vec;
@@ -1062,31 +1567,113 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
       GDP_TRY(read_ctl(C, &done, &status, &its));
       if(done) break;
     }
-    // tidycg (iters.f:3333-3365): sol += M^{-1} dx (right preconditioning) or dx
-    KCtl tmp;
-    GDP_TRY(d2h(&tmp, C, 1));
-    int final_status = tmp.status;
-    if(final_status > 0) {
-      if(tmp.fpar6 <= tmp.fpar4) final_status = 0;
-      else if(tmp.its >= tmp.maxits && tmp.maxits > 0) final_status = -1;
-      else final_status = -10;
+    return tidy_cg(M, C, dx, z, sol, n, res);
+  }
+
+  if(bcg) {
+    TV *r0 = base, *r = base + n, *sv = base + 2 * (size_t)n, *t = base + 3 * (size_t)n,
+       *v = base + 4 * (size_t)n, *p = base + 5 * (size_t)n, *z = base + 6 * (size_t)n,
+       *dx = base + 7 * (size_t)n;
+    GDP_HIP(hipMemsetAsync(dx, 0, sizeof(TV) * n, s));  // bisinit zeroes the work array
+    GDP_TRY(op_apply(A, sol, t, C, 0));
+    hipLaunchKernelGGL(k_bs_init<TV>, g, blk, 0, s, n, rhs, t, r0, r, p, part);
+    fin<FIN_BS_INIT>(C, part, nb, 0);
+    const bool rp = M.kind != PREC_NONE;
+    GraphCache gc;
+    const int batch = 8;
+    // five kernels per iteration (plus two preconditioner applications):
+    // SpMV + <r0,v>, s, SpMV + <t,t>/<t,s>, update + |r|^2/<r,r0>, p
+    constexpr int CGB = 1024;
+    const int ncg = std::max(1, std::min(ctx().cu_count, (n + CGB - 1) / CGB));
+    // partial-sum slots: <r0,v> (nrow), <t,t> and <t,s> (2 x nrow),
+    // |r|^2 and <r,r0> (2 x ncg)
+    const long rows = (long)part_rows(n);
+    double *part_t = part + rows, *part_r = part + 3 * rows;
+    auto body = [&]() -> int {
+      for(int it = 0; it < batch; it++) {
+        const TV *src = p;
+        if(rp) {
+          GDP_TRY(prec_apply(M, p, z, C, 1, n));
+          src = z;
+        }
+        hipLaunchKernelGGL((k_op_dot<TA, TV, true>), grow, dim3(256), 0, s, A, src, r0, v, C, 1, part);
+        hipLaunchKernelGGL((k_bs_s<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, r, v, sv, C, part, nrow,
+                           it & 1);
+        src = sv;
+        if(rp) {
+          GDP_TRY(prec_apply(M, sv, z, C, 1, n));
+          src = z;
+        }
+        hipLaunchKernelGGL((k_op_dot2<TA, TV>), grow, dim3(256), 0, s, A, src, sv, t, C, part_t, nrow);
+        hipLaunchKernelGGL((k_bs_update<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, p, sv, t, r0, r, dx, C,
+                           part_t, nrow, part_r, ncg);
+        hipLaunchKernelGGL((k_bs_p<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, r, v, p, C, part_r, ncg,
+                           it & 1);
+      }
+      GDP_HIP(hipGetLastError());
+      return GDP_OK;
+    };
+    GDP_TRY(read_ctl(C, &done, &status, &its));
+    while(!done) {
+      GDP_TRY(run_maybe_graph(opt.use_graphs, gc, body));
+      GDP_TRY(read_ctl(C, &done, &status, &its));
     }
-    int zero = 0;
-    GDP_TRY(h2d(&C->done, &zero, 1));
-    if(rp) {
-      GDP_TRY(prec_apply(M, dx, z, C, 0, n));
-      hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, z, C);
+    return tidy_cg(M, C, dx, z, sol, n, res);
+  }
+
+  if(tfq) {
+    TV *r0 = base, *w = base + n, *y = base + 2 * (size_t)n, *z = base + 3 * (size_t)n,
+       *ay = base + 4 * (size_t)n, *az = base + 5 * (size_t)n, *v = base + 6 * (size_t)n,
+       *d = base + 7 * (size_t)n, *tmp = base + 8 * (size_t)n, *dx = base + 9 * (size_t)n;
+    // bisinit zeroes the work array: z, v, d and dx are read before written
+    GDP_HIP(hipMemsetAsync(z, 0, sizeof(TV) * n, s));
+    GDP_HIP(hipMemsetAsync(v, 0, sizeof(TV) * n, s));
+    GDP_HIP(hipMemsetAsync(d, 0, sizeof(TV) * n, s));
+    GDP_HIP(hipMemsetAsync(dx, 0, sizeof(TV) * n, s));
+    GDP_TRY(op_apply(A, sol, az, C, 0));
+    constexpr int CGB = 1024;
+    const int ncg = std::max(1, std::min(ctx().cu_count, (n + CGB - 1) / CGB));
+    // partial-sum slots: <r0,v> (nrow), |w|^2 after the second product
+    // (nrow), |w|^2 and <r0,w> after update I (2 x ncg)
+    const long rows = (long)part_rows(n);
+    double *part_v = part, *part_w = part + rows, *part_u = part + 2 * rows;
+    hipLaunchKernelGGL((k_tq_init<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, rhs, az, r0, w, C, part_u + ncg);
+    const bool rp = M.kind != PREC_NONE;
+    GraphCache gc;
+    const int batch = 8;
+    // per iteration: y (+ rho), SpMV + v + <r0,v>, z (+ alpha),
+    // SpMV + w + |w|^2, update I (+ |w|^2, <r0,w>), update II
+    auto body = [&]() -> int {
+      for(int it = 0; it < batch; it++) {
+        const int par = it & 1;
+        hipLaunchKernelGGL((k_tq_y<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, w, z, y, C, part_u + ncg, ncg,
+                           par);
+        const TV *src = y;
+        if(rp) {
+          GDP_TRY(prec_apply(M, y, tmp, C, 1, n));
+          src = tmp;
+        }
+        hipLaunchKernelGGL((k_tq_op_v<TA, TV>), grow, dim3(256), 0, s, A, src, az, r0, ay, v, C, part_v);
+        hipLaunchKernelGGL((k_tq_z<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, y, v, z, C, part_v, nrow, par);
+        src = z;
+        if(rp) {
+          GDP_TRY(prec_apply(M, z, tmp, C, 1, n));
+          src = tmp;
+        }
+        hipLaunchKernelGGL((k_tq_op_w<TA, TV>), grow, dim3(256), 0, s, A, src, ay, az, w, C, part_w);
+        hipLaunchKernelGGL((k_tq_upd1<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, y, az, r0, d, dx, w, C, part_w,
+                           nrow, part_u, ncg);
+        hipLaunchKernelGGL((k_tq_upd2<TV, CGB>), dim3(ncg), dim3(CGB), 0, s, n, z, d, dx, C, part_u, ncg);
+      }
+      GDP_HIP(hipGetLastError());
+      return GDP_OK;
+    };
+    for(;;) {
+      GDP_TRY(run_maybe_graph(opt.use_graphs, gc, body));
+      GDP_TRY(read_ctl(C, &done, &status, &its));
+      if(done) break;
     }
-    else
-      hipLaunchKernelGGL(k_axpy1<TV>, g, blk, 0, s, n, sol, dx, C);
-    GDP_HIP(hipGetLastError());
-    GDP_TRY(sync());
-    if(M.kind == PREC_ILU) GDP_TRY(M.ilu->check_timeout());
-    res.its = tmp.its;
-    res.status = final_status;
-    res.res0 = tmp.fpar3;
-    res.res_est = tmp.fpar6;
-    return GDP_OK;
+    return tidy_cg(M, C, dx, tmp, sol, n, res);
   }
 
   // ---- (F)GMRES ----

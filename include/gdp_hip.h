@@ -142,7 +142,8 @@ int gdp_solver_bench_spmv(gdp_solver *s, int reps, double *avg_ms, int *compress
 
 /* ---- solve ----------------------------------------------------------------- */
 /* LinAlg_Solve (LinAlg_SPARSKIT.cpp:572 -> solve_matrix, Sparskit.cpp:108):
- * scaling, RCMK renumbering, ILU(0) and CG / GMRES / FGMRES on the device,
+ * scaling, RCMK renumbering, ILU(0) and CG (Algorithm 1, iters.f:379-562) /
+ * BCGSTAB (5, iters.f:1032-1306) / GMRES (8) / FGMRES (9) on the device,
  * x0 = 0, stop on the estimated residual <= Stopping_Test * |r0|. */
 int gdp_solve(gdp_matrix *A, gdp_vector *b, gdp_solver *s, gdp_vector *x);
 /* LinAlg_SolveAgain (:578): reuses permutation and factors of the last solve

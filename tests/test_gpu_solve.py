@@ -25,6 +25,10 @@ SOLVERS = {
     "cg_jacobi": dict(Algorithm=1, Preconditioner=0, Scaling=1),
     "gmres_ilu0_multicolour": dict(Algorithm=8, Preconditioner=6, Krylov_Size=30, ILU_Ordering=1),
     "gmres_ilu0_f64factors": dict(Algorithm=8, Preconditioner=6, Krylov_Size=30, ILU_Storage=1),
+    "bcgstab_ilu0": dict(Algorithm=5, Preconditioner=6),
+    "bcgstab_jacobi": dict(Algorithm=5, Preconditioner=0, Scaling=1),
+    "tfqmr_ilu0": dict(Algorithm=6, Preconditioner=6),
+    "tfqmr_jacobi": dict(Algorithm=6, Preconditioner=0, Scaling=1),
 }
 ORACLE_KW = {
     "gmres_ilu0": dict(algorithm=8, preconditioner=6, krylov=30),
@@ -33,8 +37,25 @@ ORACLE_KW = {
     "cg_jacobi": dict(algorithm=1, preconditioner=0, scaling=1),
     "gmres_ilu0_multicolour": dict(algorithm=8, preconditioner=6, krylov=30),
     "gmres_ilu0_f64factors": dict(algorithm=8, preconditioner=6, krylov=30, ilu_double=True),
+    "bcgstab_ilu0": dict(algorithm=5, preconditioner=6),
+    "bcgstab_jacobi": dict(algorithm=5, preconditioner=0, scaling=1),
+    "tfqmr_ilu0": dict(algorithm=6, preconditioner=6),
+    "tfqmr_jacobi": dict(algorithm=6, preconditioner=0, scaling=1),
 }
+# BCGSTAB does not converge on the curl-curl system (nearly singular: 1e-6
+# mass on the gradient modes); the reference breaks down there too, which
+# test_bcgstab_fails_like_reference_on_curlcurl checks.
+CASES = [(sy, so) for sy in SYSTEMS for so in SOLVERS
+         if not (sy == "curlcurl3d_4" and so.startswith("bcgstab"))]
 _cache = {}
+
+
+def _dense(ia, ja, a):
+    n = ia.shape[0] - 1
+    D = np.zeros((n, n))
+    for i in range(n):
+        D[i, ja[ia[i]:ia[i + 1]]] = a[ia[i]:ia[i + 1]]
+    return D
 
 
 def system(gpu, oracle, name):
@@ -46,8 +67,7 @@ def system(gpu, oracle, name):
     return _cache[name]
 
 
-@pytest.mark.parametrize("sys_name", list(SYSTEMS))
-@pytest.mark.parametrize("solver", list(SOLVERS))
+@pytest.mark.parametrize("sys_name,solver", CASES)
 def test_solve_parity(gpu, oracle, sys_name, solver):
     print(f"case {sys_name} {solver}", flush=True)
     pr, ia, ja, a, b, fe, A, bv = system(gpu, oracle, sys_name)
@@ -57,21 +77,42 @@ def test_solve_parity(gpu, oracle, sys_name, solver):
     info = gpu.solve(A, bv, s, x)
     xs = x.get()
     assert info["status"] == 0
-    assert info["rel_res"] <= 1e-10 * 1.5
+    # TFQMR stops on its quasi-residual tau (iters.f:1549-1552), which bounds
+    # the true residual only up to sqrt(k + 1), k the number of products
+    rbar = 1e-10 * (np.sqrt(info["its"] + 1) if solver.startswith("tfqmr") else 1.5)
+    assert info["rel_res"] <= rbar
     r = oracle.amux(ia, ja, a, xs) - b
     if SOLVERS[solver].get("Scaling") == 1:
         # the stopping test applies to the scaled system D^-1/2 A D^-1/2
         d = np.array([a[ia[i]:ia[i + 1]][ja[ia[i]:ia[i + 1]] == i][0] for i in range(fe.N)])
         w = 1.0 / np.sqrt(np.abs(d))
-        assert np.linalg.norm(w * r) <= 2e-10 * np.linalg.norm(w * b)
+        assert np.linalg.norm(w * r) <= 1.4 * rbar * np.linalg.norm(w * b)
     else:
-        assert np.linalg.norm(r) <= 2e-10 * np.linalg.norm(b)
+        assert np.linalg.norm(r) <= 1.4 * rbar * np.linalg.norm(b)
     err = np.abs(xs - xo).max() / np.abs(xo).max()
-    tol = 1e-9 if sys_name != "curlcurl3d_4" else 1e-4 if "multicolour" in solver else 1e-8
-    assert err <= tol, err
-    if solver in ("gmres_ilu0", "fgmres_ilu0", "cg_jacobi", "gmres_ilu0_f64factors"):
-        # same algorithm, same ordering, same factors: the reference's count
-        assert abs(info["its"] - io["its"]) <= max(2, io["its"] // 50), (info["its"], io["its"])
+    if sys_name == "curlcurl3d_4" and solver.startswith("tfqmr"):
+        # curl-curl with a 1e-6 mass has cond(A) ~ 1e9: a 1e-10 residual fixes
+        # the solution only to ~1e-3 (the oracle's own TFQMR and GMRES
+        # solutions differ by 1.1e-3).  Bound instead: x - xo = A^-1 (r - ro),
+        # so |x - xo| <= |r - ro| / sigma_min(A), sigma_min from a dense SVD.
+        smin = np.linalg.svd(_dense(ia, ja, a), compute_uv=False).min()
+        ro = oracle.amux(ia, ja, a, xo) - b
+        assert np.linalg.norm(xs - xo) <= 1.01 * np.linalg.norm(r - ro) / smin
+    else:
+        tol = 1e-9 if sys_name != "curlcurl3d_4" else 1e-4 if "multicolour" in solver else 1e-8
+        assert err <= tol, err
+    if solver in ("gmres_ilu0", "fgmres_ilu0", "cg_jacobi", "gmres_ilu0_f64factors", "bcgstab_ilu0",
+                  "bcgstab_jacobi", "tfqmr_ilu0", "tfqmr_jacobi"):
+        # same algorithm, same ordering, same factors: the reference's count.
+        # BCGSTAB and TFQMR count 2 products per iteration and their residual
+        # is not monotone, so rounding (the device's dot-product order) moves
+        # the crossing of the tolerance by an iteration or two.
+        # On curl-curl (cond ~1e9) TFQMR's count follows rounding: the oracle's
+        # own count moves from 269 to 243 with FP64 instead of FP32 ILU factors.
+        slack = max(4, io["its"] // 25) if solver[:5] in ("bcgst", "tfqmr") else max(2, io["its"] // 50)
+        if sys_name == "curlcurl3d_4" and solver.startswith("tfqmr"):
+            slack = io["its"] // 5
+        assert abs(info["its"] - io["its"]) <= slack, (info["its"], io["its"])
 
 
 @pytest.mark.parametrize("sys_name", ["poisson2d_32", "elasticity3d_4"])
@@ -134,6 +175,47 @@ def test_max_iterations_status(gpu, oracle):
     xo, io, _ = oracle.solve(ia, ja, a, b, algorithm=8, preconditioner=6, krylov=5, maxits=4)
     assert info["status"] == -1 == io["status"]
     assert info["its"] == io["its"]
+
+
+@pytest.mark.parametrize("algorithm,maxits", [(5, 4), (5, 7), (6, 4), (6, 9)])
+def test_bcg_tfqmr_max_iterations_status(gpu, oracle, algorithm, maxits):
+    """Sparskit counts matrix-vector products (1 + 2 per iteration) and tests
+    the limit after the second product of an iteration: 4 -> 5, 7 -> 7, 9 -> 9."""
+    pr, ia, ja, a, b, fe, A, bv = system(gpu, oracle, "poisson2d_32")
+    s = gpu.Solver(Algorithm=algorithm, Preconditioner=6, Nb_Iter_Max=maxits)
+    x = gpu.Vector(fe.N)
+    info = gpu.solve(A, bv, s, x)
+    xo, io, _ = oracle.solve(ia, ja, a, b, algorithm=algorithm, preconditioner=6, maxits=maxits)
+    assert info["status"] == -1 == io["status"]
+    assert info["its"] == io["its"]
+    # the same few steps: the partial solutions agree to rounding
+    assert np.abs(x.get() - xo).max() <= 1e-12 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("algorithm,status", [(5, 0), (6, -3)])
+def test_bcg_tfqmr_zero_rhs(gpu, oracle, algorithm, status):
+    """BCGSTAB stops before its first iteration; TFQMR reports the breakdown
+    rho = 0 (-3), as the reference does."""
+    pr, ia, ja, a, b, fe, A, bv = system(gpu, oracle, "poisson2d_32")
+    s = gpu.Solver(Algorithm=algorithm, Preconditioner=6)
+    x = gpu.Vector(fe.N)
+    info = gpu.solve(A, gpu.Vector(fe.N, np.zeros_like(b)), s, x)
+    xo, io, _ = oracle.solve(ia, ja, a, np.zeros_like(b), algorithm=algorithm, preconditioner=6)
+    assert info["status"] == status == io["status"] and info["its"] == 1 == io["its"]
+    assert not np.any(x.get())
+
+
+def test_bcgstab_fails_like_reference_on_curlcurl(gpu, oracle):
+    pr, ia, ja, a, b, fe, A, bv = system(gpu, oracle, "curlcurl3d_4")
+    xo, io, _ = oracle.solve(ia, ja, a, b, algorithm=5, preconditioner=6, maxits=2000)
+    s = gpu.Solver(Algorithm=5, Preconditioner=6, Nb_Iter_Max=2000)
+    x = gpu.Vector(fe.N)
+    info = gpu.solve(A, bv, s, x)
+    print(f"curl-curl BCGSTAB: device status {info['status']} its {info['its']}, "
+          f"reference status {io['status']} its {io['its']}")
+    # no convergence on either side; where and how it stops (breakdown -3/-9
+    # or the iteration limit) depends on rounding
+    assert io["status"] < 0 and info["status"] < 0
 
 
 def test_zero_pivot_error(gpu):
