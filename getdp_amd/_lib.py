@@ -82,6 +82,7 @@ SIGNATURES = [
     ("gdp_bench_spmv", C.c_int, [_vp, _vp, _vp, C.c_int, _dp]),
     ("gdp_matrix_sell_info", C.c_int, [_vp, C.POINTER(C.c_long), _ip]),
     ("gdp_solver_bench_spmv", C.c_int, [_vp, C.c_int, _dp, _ip]),
+    ("gdp_solver_prec_apply", C.c_int, [_vp, _vp, _vp, C.c_int, _dp]),
     ("gdp_solve", C.c_int, [_vp, _vp, _vp, _vp]),
     ("gdp_solve_again", C.c_int, [_vp, _vp, _vp, _vp]),
     ("gdp_fe_create", C.c_int, [C.POINTER(MeshDesc), C.c_int, C.POINTER(SpaceDesc), C.c_int,

@@ -14,6 +14,7 @@ struct IluFactor {
   double *lu64 = nullptr;
   bool is_f64 = false;
   int nlevels = 0, upper_nlevels = 0;
+  int g_lower = 1, g_upper = 1;             // lanes per row in the two sweeps
   std::vector<int> level_ptr;               // host: rows of level l
   int *level_rows = nullptr;
   double *tmp64 = nullptr;                  // forward-sweep output (sentinel-filled)

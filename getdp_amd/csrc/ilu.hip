@@ -13,13 +13,16 @@
 // exactly the reference's sequence of operations (compiled without
 // contraction), so the factors are bit-identical to the reference's.
 //
-// Sweeps (lusol, ilut.F:2297-2347): one thread per row, rows taken in
+// Sweeps (lusol, ilut.F:2297-2347): one thread (or a group of 2-32 lanes,
+// chosen from the mean number of entries per row) per row, rows taken in
 // dependency order; a row reads the solution entries it depends on as soon as
 // they are published.  Outputs are pre-filled with a signalling-NaN sentinel
 // that arithmetic never produces; a published entry is any non-sentinel
 // value, read and written with agent-scope atomics (one 64-bit access carries
 // both the value and its readiness).  Every wait is bounded by a wall-clock
 // limit: on expiry the kernel records an error flag and exits.
+#include <cstdlib>
+
 #include "ilu.h"
 
 namespace gdp {
@@ -134,97 +137,119 @@ __device__ inline bool wait_expired(unsigned long long &t0, unsigned long long t
   return false;
 }
 
-// Forward sweep: y = L^{-1} r (unit lower), rows ascending.  Also fills
-// `next` (the backward sweep's output) with the sentinel.
-template <typename TF, typename TV>
-__global__ void __launch_bounds__(256) k_fwd(int n, const int *__restrict__ ia,
-                                             const int *__restrict__ ja,
-                                             const int *__restrict__ diag,
-                                             const TF *__restrict__ lu, const TV *__restrict__ r,
-                                             TV scale, TV *y, TV *next, int *err,
-                                             unsigned long long tlimit, const int *gate,
-                                             int gmode)
-{
-  if(is_gated(gate, gmode)) return;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if(i >= n) return;
-  TV acc = r[i] * scale;
-  int k = ia[i];
-  const int kend = diag[i];
-  st_val(next + i, as_val<TV>(Sentinel<TV>::bits));
-  bool done = false;
-  unsigned long long t0 = 0;
-  // The loop runs until every lane of the wave is done (uniform exit), and a
-  // lane publishes its entry inside the iteration in which it finishes: a
-  // lane waiting for a row of the same wave then sees it on a later
-  // iteration.  The compiler barrier keeps the store from being moved to the
-  // loop exit, where it would wait for the whole wave.
-  for(;;) {
-    if(!done) {
-      if(k < kend) {
-        auto b = ld_bits(y + ja[k]);
-        if(b != Sentinel<TV>::bits) {
-          acc = acc - (TV)lu[k] * as_val<TV>(b);
-          k++;
-        }
-        else {
-          if(wait_expired(t0, tlimit, err)) k = kend;
-          __builtin_amdgcn_s_sleep(1);
-        }
-      }
-      if(k >= kend) {
-        st_val(y + i, acc);
-        done = true;
-      }
-    }
-    __asm__ volatile("" ::: "memory");
-    if(__all(done)) break;
-  }
-}
+// Sweeps, one kernel for both directions: rows in dependency order
+// (ascending for L, descending for U) over consecutive waves and blocks, so a
+// row only waits on rows of earlier waves, which the in-order dispatch of
+// workgroups has already made resident.  G lanes serve one row (G = 1 .. 32,
+// chosen from the mean number of entries per row): they wait for G of its
+// dependencies at once (one memory round trip instead of G on the critical
+// path), then every lane of the row subtracts the G products in ascending
+// column position -- lusol's order and rounding, whatever G.  The products
+// travel by cross-lane reads executed by the whole wave, outside divergent
+// branches.  The forward sweep fills `x` (the backward sweep's output) with
+// the sentinel; the backward sweep resets y to the sentinel as it consumes it.
+template <typename TV> __device__ inline TV lane_read(TV v, int src) { return __shfl(v, src, kWave); }
 
-// Backward sweep: x = U^{-1} y with inverted stored pivots, rows descending.
-// Resets y[i] to the sentinel once consumed (ready for the next forward sweep).
-template <typename TF, typename TV>
-__global__ void __launch_bounds__(256) k_bwd(int n, const int *__restrict__ ia,
-                                             const int *__restrict__ ja,
-                                             const int *__restrict__ diag,
-                                             const TF *__restrict__ lu, TV *y, TV *x, int *err,
-                                             unsigned long long tlimit, const int *gate,
-                                             int gmode, TV *addto)
+template <typename TF, typename TV, int G, bool FWD>
+__global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                               const int *__restrict__ diag, const TF *__restrict__ lu,
+                                               const TV *__restrict__ r, TV scale, TV *y, TV *x, int *err,
+                                               unsigned long long tlimit, const int *gate, int gmode,
+                                               TV *addto)
 {
-  if(is_gated(gate, gmode)) return;
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if(t >= n) return;
-  const int i = n - 1 - t;
-  TV acc = y[i];
-  st_val(y + i, as_val<TV>(Sentinel<TV>::bits));
-  const int di = diag[i];
-  int k = di + 1;
-  const int kend = ia[i + 1];
-  bool done = false;
-  unsigned long long t0 = 0;
-  for(;;) {  // same structure as the forward sweep
-    if(!done) {
-      if(k < kend) {
-        auto b = ld_bits(x + ja[k]);
-        if(b != Sentinel<TV>::bits) {
-          acc = acc - (TV)lu[k] * as_val<TV>(b);
-          k++;
-        }
-        else {
-          if(wait_expired(t0, tlimit, err)) k = kend;
-          __builtin_amdgcn_s_sleep(1);
-        }
+  if(is_gated(gate, gmode)) return;  // uniform across the grid
+  constexpr int R = kWave / G;  // rows per wave
+  constexpr unsigned long long GM = (G == 64) ? ~0ull : ((1ull << G) - 1);
+  const int lane = threadIdx.x & (kWave - 1), q = lane & (G - 1), gbase = lane - q;
+  TV *out = FWD ? y : x;
+  {
+    // rows t = base .. base + R - 1 of this wave, in dependency order
+    const int base = (blockIdx.x * (256 / kWave) + (int)(threadIdx.x / kWave)) * R;
+    const int t = base + lane / G;
+    const bool valid = t < n;
+    const int i = FWD ? t : n - 1 - t;
+    TV acc = 0;
+    int cb = 0, kend = 0, di = 0;
+    if(valid) {
+      di = diag[i];
+      if(FWD) {
+        acc = r[i] * scale;
+        cb = ia[i];
+        kend = di;
+        if(q == 0) st_val(x + i, as_val<TV>(Sentinel<TV>::bits));
       }
-      if(k >= kend) {
-        const TV xi = (TV)lu[di] * acc;
-        st_val(x + i, xi);
-        if(addto) addto[i] = addto[i] + xi;  // fused z += x of the caller
-        done = true;
+      else {
+        acc = y[i];
+        cb = di + 1;
+        kend = ia[i + 1];
       }
     }
-    __asm__ volatile("" ::: "memory");
-    if(__all(done)) break;
+    // every lane of the row has read y[i] (one instruction) before lane 0
+    // resets it for the next forward sweep
+    if(!FWD && valid && q == 0) st_val(y + i, as_val<TV>(Sentinel<TV>::bits));
+    auto publish = [&]() {
+      if(q == 0) {
+        if(FWD) st_val(y + i, acc);
+        else {
+          const TV xi = (TV)lu[di] * acc;
+          st_val(x + i, xi);
+          if(addto) addto[i] = addto[i] + xi;  // fused z += x of the caller
+        }
+      }
+    };
+    bool done = !valid, have = false;
+    if(valid && cb >= kend) {
+      publish();
+      done = true;
+    }
+    TV prod = 0;
+    unsigned long long t0 = 0;
+    for(;;) {
+      if(!done && !have) {
+        const int k = cb + q;
+        if(k < kend) {
+          auto b = ld_bits(out + ja[k]);
+          if(b != Sentinel<TV>::bits) {
+            prod = (TV)lu[k] * as_val<TV>(b);
+            have = true;
+          }
+          else if(wait_expired(t0, tlimit, err)) {
+            prod = 0;
+            have = true;
+          }
+          else
+            __builtin_amdgcn_s_sleep(1);
+        }
+        else {
+          prod = 0;
+          have = true;
+        }
+      }
+      const unsigned long long m = __ballot(have || done);
+      const bool ready = !done && ((m >> gbase) & GM) == GM;
+      if(G == 1) {
+        if(ready && cb < kend) acc = acc - prod;
+      }
+      else {
+#pragma unroll
+        for(int j = 0; j < G; j++) {
+          const TV pj = lane_read(prod, gbase + j);
+          if(ready && cb + j < kend) acc = acc - pj;
+        }
+      }
+      if(ready) {
+        cb += G;
+        have = false;
+        if(cb >= kend) {
+          publish();
+          done = true;
+        }
+      }
+      // keeps the publishing store inside the iteration (a lane waiting on a
+      // row of the same wave sees it on a later iteration)
+      __asm__ volatile("" ::: "memory");
+      if(__all(done)) break;
+    }
   }
 }
 
@@ -286,6 +311,27 @@ int IluFactor::analyse(const DevCSR &B, const std::vector<int> &hia, const std::
   for(int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
   std::vector<int> ulev;
   upper_nlevels = upper_levels(n, hia.data(), hja.data(), ulev);
+  // lanes per row of each sweep: the power of two at or above the mean count
+  // of strictly lower (upper) entries, at most 8 (wider groups spend more on
+  // cross-lane reads than they save in waits: elasticity 32^3, 20 entries a
+  // side, takes 4.0 ms per application with 8 lanes, 5.1 with 16, 11.1 with
+  // 32, 13.0 with 1), 1 below 1.5 entries
+  long nl = 0, nu = 0;
+  for(int i = 0; i < n; i++)
+    for(int k = hia[i]; k < hia[i + 1]; k++) {
+      if(hja[k] < i) nl++;
+      else if(hja[k] > i) nu++;
+    }
+  auto lanes = [&](long cnt) {
+    const double avg = n ? (double)cnt / n : 0.;
+    if(avg < 1.5) return 1;
+    int g = 2;
+    while(g < avg && g < 8) g *= 2;
+    return g;
+  };
+  g_lower = lanes(nl);
+  g_upper = lanes(nu);
+  if(const char *e = std::getenv("GDP_ILU_LANES")) g_lower = g_upper = std::max(1, std::min(32, atoi(e)));
   GDP_TRY(dalloc(&level_rows, (size_t)std::max(1, n)));
   GDP_TRY(h2d(level_rows, rows.data(), (size_t)n));
   GDP_TRY(dalloc(&diag, (size_t)std::max(1, n)));
@@ -402,6 +448,30 @@ unsigned long long IluFactor::wait_limit() const
   return 100000000ull;  // wall_clock64 runs at 100 MHz: 1 s per dependency wait
 }
 
+namespace {
+
+template <typename TF, typename TV, bool FWD>
+void sweep(int G, int n, const int *ia, const int *ja, const int *diag, const TF *lu, const TV *r, TV scale,
+           TV *y, TV *x, int *err, unsigned long long tl, const int *gate, int gmode, TV *addto)
+{
+  hipStream_t s = ctx().stream;
+  const dim3 grid((unsigned)(((long)n * G + 255) / 256)), blk(256);
+#define GDP_SWEEP(GG)                                                                                  \
+  hipLaunchKernelGGL((k_sweep<TF, TV, GG, FWD>), grid, blk, 0, s, n, ia, ja, diag, lu, r, scale, y, x, err, \
+                     tl, gate, gmode, addto)
+  switch(G) {
+  case 2: GDP_SWEEP(2); break;
+  case 4: GDP_SWEEP(4); break;
+  case 8: GDP_SWEEP(8); break;
+  case 16: GDP_SWEEP(16); break;
+  case 32: GDP_SWEEP(32); break;
+  default: GDP_SWEEP(1); break;
+  }
+#undef GDP_SWEEP
+}
+
+}  // namespace
+
 template <typename TV>
 int IluFactor::apply(const TV *r, TV scale, TV *x, const int *gate, int gmode, TV *addto)
 {
@@ -410,18 +480,17 @@ int IluFactor::apply(const TV *r, TV scale, TV *x, const int *gate, int gmode, T
   TV *t;
   if constexpr(sizeof(TV) == 8) t = tmp64;
   else t = tmp32;
-  dim3 g((n + 255) / 256);
   if(is_f64) {
-    hipLaunchKernelGGL((k_fwd<double, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu64, r, scale, t,
-                       x, err + 1, wait_limit(), gate, gmode);
-    hipLaunchKernelGGL((k_bwd<double, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu64, t, x,
-                       err + 1, wait_limit(), gate, gmode, addto);
+    sweep<double, TV, true>(g_lower, n, ia, ja, diag, lu64, r, scale, t, x, err + 1, wait_limit(), gate, gmode,
+                            nullptr);
+    sweep<double, TV, false>(g_upper, n, ia, ja, diag, lu64, nullptr, scale, t, x, err + 1, wait_limit(), gate,
+                             gmode, addto);
   }
   else {
-    hipLaunchKernelGGL((k_fwd<float, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu32, r, scale, t, x,
-                       err + 1, wait_limit(), gate, gmode);
-    hipLaunchKernelGGL((k_bwd<float, TV>), g, dim3(256), 0, s, n, ia, ja, diag, lu32, t, x, err + 1,
-                       wait_limit(), gate, gmode, addto);
+    sweep<float, TV, true>(g_lower, n, ia, ja, diag, lu32, r, scale, t, x, err + 1, wait_limit(), gate, gmode,
+                           nullptr);
+    sweep<float, TV, false>(g_upper, n, ia, ja, diag, lu32, nullptr, scale, t, x, err + 1, wait_limit(), gate,
+                            gmode, addto);
   }
   GDP_HIP(hipGetLastError());
   return GDP_OK;

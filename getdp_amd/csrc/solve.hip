@@ -572,3 +572,33 @@ extern "C" int gdp_solver_bench_spmv(gdp_solver *s, int reps, double *avg_ms, in
   *compressed = S.c16 != nullptr;
   return GDP_OK;
 }
+
+// out = (LU)^{-1} in with the ILU(0) factors of the solver's last solve, in
+// its internal numbering (renumbered, scaled system), `reps` times in a row;
+// *avg_ms (optional) = average time per application (HIP events).
+extern "C" int gdp_solver_prec_apply(gdp_solver *s, const gdp_vector *in, gdp_vector *out, int reps,
+                                     double *avg_ms)
+{
+  if(!s || !in || !out || reps < 1) return set_error(GDP_E_ARG, "gdp_solver_prec_apply: bad arguments");
+  SolveCache &c = s->cache;
+  if(!c.have_factor || c.precond != 6 || !c.n)
+    return set_error(GDP_E_STATE, "the solver holds no ILU(0) factors (no solve with Preconditioner 6)");
+  if(in->n != c.n || out->n != c.n || in == out)
+    return set_error(GDP_E_ARG, "gdp_solver_prec_apply: vectors must be distinct and of the system's size");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0.f;
+  int rc = sync();
+  if(!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = set_error(GDP_E_HIP, "hipEventCreate");
+  if(!rc && hipEventRecord(e0, ctx().stream) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventRecord");
+  for(int r = 0; !rc && r < reps; r++) rc = c.ilu.apply<double>(in->d, 1.0, out->d);
+  if(!rc && hipEventRecord(e1, ctx().stream) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventRecord");
+  if(!rc && hipEventSynchronize(e1) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventSynchronize");
+  if(!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventElapsedTime");
+  if(e0) (void)hipEventDestroy(e0);
+  if(e1) (void)hipEventDestroy(e1);
+  GDP_TRY(rc);
+  GDP_TRY(c.ilu.check_timeout());
+  if(avg_ms) *avg_ms = ms / reps;
+  return GDP_OK;
+}

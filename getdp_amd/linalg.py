@@ -169,6 +169,14 @@ def bench_solver_spmv(s: Solver, reps: int = 20) -> dict:
     return {"ms": ms.value, "compressed": bool(comp.value)}
 
 
+def solver_prec_apply(s: Solver, v: Vector, out: Vector, reps: int = 1) -> float:
+    """out = (LU)^{-1} v with the ILU(0) factors of the solver's last solve, in
+    its internal (renumbered, scaled) numbering; returns ms per application."""
+    ms = C.c_double()
+    check(lib().gdp_solver_prec_apply(s.h, v.h, out.h, int(reps), C.byref(ms)), "gdp_solver_prec_apply")
+    return ms.value
+
+
 def bench_spmv(A: Matrix, x: Vector, y: Vector, reps: int = 20) -> float:
     """Average device time (ms) of one CSR SpMV launch over `reps` launches."""
     v = C.c_double()

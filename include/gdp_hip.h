@@ -139,6 +139,13 @@ int gdp_matrix_sell_info(gdp_matrix *A, long *nnz_pad, int *compressed);
  * the last matrix it solved -- the operator of its Krylov iteration --
  * bracketed by HIP events; *avg_ms per launch, *compressed as above. */
 int gdp_solver_bench_spmv(gdp_solver *s, int reps, double *avg_ms, int *compressed);
+/* The lusol step of solve_matrix (ilut.F:2297-2347, called from the
+ * reverse-communication loop of Sparskit.cpp:556-567): out = (LU)^{-1} in with
+ * the ILU(0) factors of the solver's last solve, in its internal numbering
+ * (RCMK-renumbered, scaled system), `reps` times in a row; *avg_ms (may be
+ * null) = average time per application.  For tests and measurements. */
+int gdp_solver_prec_apply(gdp_solver *s, const gdp_vector *in, gdp_vector *out, int reps,
+                          double *avg_ms);
 
 /* ---- solve ----------------------------------------------------------------- */
 /* LinAlg_Solve (LinAlg_SPARSKIT.cpp:572 -> solve_matrix, Sparskit.cpp:108):
