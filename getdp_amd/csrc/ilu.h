@@ -15,6 +15,7 @@ struct IluFactor {
   bool is_f64 = false;
   int nlevels = 0, upper_nlevels = 0;
   int g_lower = 1, g_upper = 1;             // lanes per row in the two sweeps
+  int maxrow = 1;                           // longest row of the pattern
   std::vector<int> level_ptr;               // host: rows of level l
   int *level_rows = nullptr;
   double *tmp64 = nullptr;                  // forward-sweep output (sentinel-filled)
@@ -42,6 +43,7 @@ struct IluFactor {
   int reset_sweep_state();
   int check_timeout();
   unsigned long long wait_limit() const;
+  template <typename T> void enqueue_levels(T *lu);
 };
 
 }  // namespace gdp

@@ -9,11 +9,11 @@
 // HAVE_ILU_FLOAT, CMakeLists.txt:552) or double.
 //
 // Factorisation: rows are grouped into dependency levels (host analysis);
-// one launch per level processes its rows in parallel, each row performing
-// exactly the reference's sequence of operations (compiled without
-// contraction), so the factors are bit-identical to the reference's.
+// one launch per level processes its rows in parallel (a wave per row), each
+// row receiving exactly the reference's sequence of operations (compiled
+// without contraction), so the factors are bit-identical to the reference's.
 //
-// Sweeps (lusol, ilut.F:2297-2347): one thread (or a group of 2-32 lanes,
+// Sweeps (lusol, ilut.F:2297-2347): one lane (or a group of 2-8 lanes,
 // chosen from the mean number of entries per row) per row, rows taken in
 // dependency order; a row reads the solution entries it depends on as soon as
 // they are published.  Outputs are pre-filled with a signalling-NaN sentinel
@@ -78,6 +78,64 @@ __global__ void __launch_bounds__(128) k_ilu_level(int nrows, const int *__restr
     return;
   }
   lu[di] = (T)(1.0 / (double)piv);
+}
+
+// The same row operation with one wave per row: the row (columns and values)
+// is staged in LDS; for each strictly-lower entry k in order, lane 0's
+// multiplier tl is formed as above, then the lanes take the entries of
+// U(jrow) in parallel, each locating its column in the row by binary search
+// and updating that entry.  Distinct U(jrow) entries hit distinct columns, and
+// a barrier separates consecutive k, so every entry receives exactly the
+// reference's updates in the reference's order.  Rows up to `cap` entries.
+template <typename T>
+__global__ void __launch_bounds__(64) k_ilu_row(int nrows, const int *__restrict__ rows,
+                                                const int *__restrict__ ia, const int *__restrict__ ja,
+                                                const int *__restrict__ diag, T *__restrict__ lu,
+                                                int *__restrict__ pivot_err, int cap)
+{
+  extern __shared__ unsigned char smem[];
+  int *cols = (int *)smem;
+  T *vals = (T *)(smem + ((sizeof(int) * cap + 7) & ~(size_t)7));
+  __shared__ double s_tl;
+  const int lane = threadIdx.x;
+  const int i = rows[blockIdx.x];
+  const int rs = ia[i], re = ia[i + 1], di = diag[i], len = re - rs;
+  for(int p = lane; p < len; p += 64) {
+    cols[p] = ja[rs + p];
+    vals[p] = lu[rs + p];
+  }
+  __syncthreads();
+  for(int k = 0; k < di - rs; k++) {
+    const int jrow = cols[k];
+    const int dj = diag[jrow];
+    if(lane == 0) {
+      const double tl = prod_ll<T>(vals[k], lu[dj]);
+      vals[k] = (T)tl;
+      s_tl = tl;
+    }
+    __syncthreads();
+    const double tl = s_tl;
+    const int je = ia[jrow + 1];
+    for(int jj = dj + 1 + lane; jj < je; jj += 64) {
+      const int c = ja[jj];
+      int lo = k + 1, hi = len;  // first position with cols >= c
+      while(lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if(cols[mid] < c) lo = mid + 1;
+        else hi = mid;
+      }
+      if(lo < len && cols[lo] == c) vals[lo] = (T)((double)vals[lo] - tl * (double)lu[jj]);
+    }
+    __syncthreads();
+  }
+  const int dp = di - rs;
+  if(lane == 0) {
+    const T piv = vals[dp];
+    if(piv == (T)0) atomicMin(pivot_err, i + 1);
+    else vals[dp] = (T)(1.0 / (double)piv);
+  }
+  __syncthreads();
+  for(int p = lane; p < len; p += 64) lu[rs + p] = vals[p];
 }
 
 // ---- sweeps -----------------------------------------------------------------------
@@ -329,6 +387,8 @@ int IluFactor::analyse(const DevCSR &B, const std::vector<int> &hia, const std::
     while(g < avg && g < 8) g *= 2;
     return g;
   };
+  maxrow = 1;
+  for(int i = 0; i < n; i++) maxrow = std::max(maxrow, hia[i + 1] - hia[i]);
   g_lower = lanes(nl);
   g_upper = lanes(nu);
   if(const char *e = std::getenv("GDP_ILU_LANES")) g_lower = g_upper = std::max(1, std::min(32, atoi(e)));
@@ -363,6 +423,27 @@ int IluFactor::reset_sweep_state()
   return GDP_OK;
 }
 
+// One launch per dependency level: a wave per row (row staged in LDS) when
+// the longest row fits kRowCap entries, else a thread per row.
+constexpr int kRowCap = 1024;
+
+template <typename T> void IluFactor::enqueue_levels(T *lu)
+{
+  hipStream_t s = ctx().stream;
+  const bool wave_rows = maxrow <= kRowCap && !std::getenv("GDP_ILU_THREAD_ROWS");
+  const size_t lds = (((size_t)maxrow * sizeof(int) + 7) & ~(size_t)7) + (size_t)maxrow * sizeof(T);
+  for(int l = 0; l < nlevels; l++) {
+    const int cnt = level_ptr[l + 1] - level_ptr[l];
+    if(!cnt) continue;
+    if(wave_rows)
+      hipLaunchKernelGGL(k_ilu_row<T>, dim3(cnt), dim3(64), lds, s, cnt, level_rows + level_ptr[l], ia, ja,
+                         diag, lu, err, maxrow);
+    else
+      hipLaunchKernelGGL(k_ilu_level<T>, dim3((cnt + 127) / 128), dim3(128), 0, s, cnt,
+                         level_rows + level_ptr[l], ia, ja, diag, lu, err);
+  }
+}
+
 int IluFactor::factor(const double *a, bool f64, int *zero_pivot_row)
 {
   hipStream_t s = ctx().stream;
@@ -378,16 +459,8 @@ int IluFactor::factor(const double *a, bool f64, int *zero_pivot_row)
     if(f64) hipLaunchKernelGGL(k_ilu_copy<double>, g, dim3(256), 0, s, nnz, a, lu64);
     else hipLaunchKernelGGL(k_ilu_copy<float>, g, dim3(256), 0, s, nnz, a, lu32);
   }
-  for(int l = 0; l < nlevels; l++) {
-    int cnt = level_ptr[l + 1] - level_ptr[l];
-    dim3 gl((cnt + 127) / 128);
-    if(f64)
-      hipLaunchKernelGGL(k_ilu_level<double>, gl, dim3(128), 0, s, cnt, level_rows + level_ptr[l],
-                         ia, ja, diag, lu64, err);
-    else
-      hipLaunchKernelGGL(k_ilu_level<float>, gl, dim3(128), 0, s, cnt, level_rows + level_ptr[l], ia,
-                         ja, diag, lu32, err);
-  }
+  if(f64) enqueue_levels(lu64);
+  else enqueue_levels(lu32);
   GDP_HIP(hipGetLastError());
   int h;
   GDP_TRY(d2h(&h, err, 1));
@@ -421,16 +494,8 @@ int IluFactor::factor_enqueue(const double *a)
     if(is_f64) hipLaunchKernelGGL(k_ilu_copy<double>, g, dim3(256), 0, s, nnz, a, lu64);
     else hipLaunchKernelGGL(k_ilu_copy<float>, g, dim3(256), 0, s, nnz, a, lu32);
   }
-  for(int l = 0; l < nlevels; l++) {
-    int cnt = level_ptr[l + 1] - level_ptr[l];
-    dim3 gl((cnt + 127) / 128);
-    if(is_f64)
-      hipLaunchKernelGGL(k_ilu_level<double>, gl, dim3(128), 0, s, cnt, level_rows + level_ptr[l], ia, ja,
-                         diag, lu64, err);
-    else
-      hipLaunchKernelGGL(k_ilu_level<float>, gl, dim3(128), 0, s, cnt, level_rows + level_ptr[l], ia, ja,
-                         diag, lu32, err);
-  }
+  if(is_f64) enqueue_levels(lu64);
+  else enqueue_levels(lu32);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }

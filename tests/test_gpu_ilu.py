@@ -76,11 +76,15 @@ def test_ilu_apply_one_lane_rows(gpu, oracle, f64):
         assert np.allclose(Bv, v, rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("lanes", ["1", "2", "16", "32"])
+@pytest.mark.parametrize("lanes", ["1", "2", "16", "32", "thread-rows"])
 def test_ilu_apply_lane_counts(gpu, oracle, monkeypatch, lanes):
-    """Every lanes-per-row variant (GDP_ILU_LANES overrides the choice made
-    from the pattern) gives lusol's bits."""
-    monkeypatch.setenv("GDP_ILU_LANES", lanes)
+    """Every lanes-per-row variant of the sweeps (GDP_ILU_LANES overrides the
+    choice made from the pattern) and the thread-per-row factorisation (used
+    for rows beyond 1024 entries; GDP_ILU_THREAD_ROWS) give lusol's bits."""
+    if lanes == "thread-rows":
+        monkeypatch.setenv("GDP_ILU_THREAD_ROWS", "1")
+    else:
+        monkeypatch.setenv("GDP_ILU_LANES", lanes)
     ia, ja, a, b, tab = oracle.assemble(SYSTEMS["elasticity3d_8"]())
     perm, rperm, ib, jb, bb, nlev = oracle.rcmk(ia, ja, a)
     alu, jlu, ju, ierr = oracle.ilu0(ib, jb, bb)
