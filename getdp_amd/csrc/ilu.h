@@ -15,6 +15,7 @@ struct IluFactor {
   bool is_f64 = false;
   int nlevels = 0, upper_nlevels = 0;
   int g_lower = 1, g_upper = 1;             // lanes per row in the two sweeps
+  int c_lower = 1, c_upper = 1;             // entries per lane in flight
   int maxrow = 1;                           // longest row of the pattern
   std::vector<int> level_ptr;               // host: rows of level l
   int *level_rows = nullptr;

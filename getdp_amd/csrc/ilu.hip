@@ -208,7 +208,7 @@ __device__ inline bool wait_expired(unsigned long long &t0, unsigned long long t
 // the sentinel; the backward sweep resets y to the sentinel as it consumes it.
 template <typename TV> __device__ inline TV lane_read(TV v, int src) { return __shfl(v, src, kWave); }
 
-template <typename TF, typename TV, int G, bool FWD>
+template <typename TF, typename TV, int G, int C, bool FWD>
 __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                                                const int *__restrict__ diag, const TF *__restrict__ lu,
                                                const TV *__restrict__ r, TV scale, TV *y, TV *x, int *err,
@@ -217,6 +217,7 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
 {
   if(is_gated(gate, gmode)) return;  // uniform across the grid
   constexpr int R = kWave / G;  // rows per wave
+  constexpr int S = G * C;      // entries of a row in flight at once
   constexpr unsigned long long GM = (G == 64) ? ~0ull : ((1ull << G) - 1);
   const int lane = threadIdx.x & (kWave - 1), q = lane & (G - 1), gbase = lane - q;
   TV *out = FWD ? y : x;
@@ -245,63 +246,88 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
     // every lane of the row has read y[i] (one instruction) before lane 0
     // resets it for the next forward sweep
     if(!FWD && valid && q == 0) st_val(y + i, as_val<TV>(Sentinel<TV>::bits));
+    // a lane holds positions cb + q + G*c, c < C, of the row's current span of
+    // S entries; their columns and factor entries are loaded when the span
+    // starts, off the dependency chain of the polls
+    const TV ldi = (!FWD && valid) ? (TV)lu[di] : (TV)0;
+    int jk[C];
+    TV lk[C], prod[C];
+    unsigned pend = 0;  // bit c: position c still waits for its dependency
+    auto fetch = [&]() {
+      pend = 0;
+#pragma unroll
+      for(int c = 0; c < C; c++) {
+        const int k = cb + q + G * c;
+        prod[c] = 0;
+        if(k < kend) {
+          jk[c] = ja[k];
+          lk[c] = (TV)lu[k];
+          pend |= 1u << c;
+        }
+      }
+    };
+    if(valid) fetch();
     auto publish = [&]() {
       if(q == 0) {
         if(FWD) st_val(y + i, acc);
         else {
-          const TV xi = (TV)lu[di] * acc;
+          const TV xi = ldi * acc;
           st_val(x + i, xi);
           if(addto) addto[i] = addto[i] + xi;  // fused z += x of the caller
         }
       }
     };
-    bool done = !valid, have = false;
+    bool done = !valid;
     if(valid && cb >= kend) {
       publish();
       done = true;
     }
-    TV prod = 0;
     unsigned long long t0 = 0;
     for(;;) {
-      if(!done && !have) {
-        const int k = cb + q;
-        if(k < kend) {
-          auto b = ld_bits(out + ja[k]);
-          if(b != Sentinel<TV>::bits) {
-            prod = (TV)lu[k] * as_val<TV>(b);
-            have = true;
+      if(!done && pend) {
+        bool any_wait = false;
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+          if(pend & (1u << c)) {
+            auto b = ld_bits(out + jk[c]);
+            if(b != Sentinel<TV>::bits) {
+              prod[c] = lk[c] * as_val<TV>(b);
+              pend &= ~(1u << c);
+            }
+            else
+              any_wait = true;
           }
-          else if(wait_expired(t0, tlimit, err)) {
-            prod = 0;
-            have = true;
-          }
-          else
-            __builtin_amdgcn_s_sleep(1);
         }
-        else {
-          prod = 0;
-          have = true;
+        if(any_wait) {
+          if(wait_expired(t0, tlimit, err)) pend = 0;
+          else __builtin_amdgcn_s_sleep(1);
         }
       }
-      const unsigned long long m = __ballot(have || done);
+      const unsigned long long m = __ballot(!pend || done);
       const bool ready = !done && ((m >> gbase) & GM) == GM;
       if(G == 1) {
-        if(ready && cb < kend) acc = acc - prod;
-      }
-      else {
 #pragma unroll
-        for(int j = 0; j < G; j++) {
-          const TV pj = lane_read(prod, gbase + j);
-          if(ready && cb + j < kend) acc = acc - pj;
+        for(int c = 0; c < C; c++)
+          if(ready && cb + c < kend) acc = acc - prod[c];
+      }
+      else if(__any(ready)) {  // wave-uniform: cross-lane reads only when a row can use them
+#pragma unroll
+        for(int c = 0; c < C; c++) {
+#pragma unroll
+          for(int j = 0; j < G; j++) {
+            const TV pj = lane_read(prod[c], gbase + j);
+            if(ready && cb + G * c + j < kend) acc = acc - pj;
+          }
         }
       }
       if(ready) {
-        cb += G;
-        have = false;
+        cb += S;
         if(cb >= kend) {
           publish();
           done = true;
         }
+        else
+          fetch();
       }
       // keeps the publishing store inside the iteration (a lane waiting on a
       // row of the same wave sees it on a later iteration)
@@ -392,6 +418,7 @@ int IluFactor::analyse(const DevCSR &B, const std::vector<int> &hia, const std::
   g_lower = lanes(nl);
   g_upper = lanes(nu);
   if(const char *e = std::getenv("GDP_ILU_LANES")) g_lower = g_upper = std::max(1, std::min(32, atoi(e)));
+  c_lower = c_upper = 1;
   GDP_TRY(dalloc(&level_rows, (size_t)std::max(1, n)));
   GDP_TRY(h2d(level_rows, rows.data(), (size_t)n));
   GDP_TRY(dalloc(&diag, (size_t)std::max(1, n)));
@@ -516,14 +543,18 @@ unsigned long long IluFactor::wait_limit() const
 namespace {
 
 template <typename TF, typename TV, bool FWD>
-void sweep(int G, int n, const int *ia, const int *ja, const int *diag, const TF *lu, const TV *r, TV scale,
-           TV *y, TV *x, int *err, unsigned long long tl, const int *gate, int gmode, TV *addto)
+void sweep(int G, int C, int n, const int *ia, const int *ja, const int *diag, const TF *lu, const TV *r,
+           TV scale, TV *y, TV *x, int *err, unsigned long long tl, const int *gate, int gmode, TV *addto)
 {
   hipStream_t s = ctx().stream;
-  const dim3 grid((unsigned)(((long)n * G + 255) / 256)), blk(256);
+  // one entry per lane in flight (C = 1): spans of 2-4 entries per lane were
+  // measured no faster (the chain's cost is the publication latency, not the
+  // number of polls)
+  (void)C;
+  if(G != 2 && G != 4 && G != 8 && G != 16 && G != 32) G = 1;
 #define GDP_SWEEP(GG)                                                                                  \
-  hipLaunchKernelGGL((k_sweep<TF, TV, GG, FWD>), grid, blk, 0, s, n, ia, ja, diag, lu, r, scale, y, x, err, \
-                     tl, gate, gmode, addto)
+  hipLaunchKernelGGL((k_sweep<TF, TV, GG, 1, FWD>), dim3((unsigned)(((long)n * GG + 255) / 256)), dim3(256), 0, \
+                     s, n, ia, ja, diag, lu, r, scale, y, x, err, tl, gate, gmode, addto)
   switch(G) {
   case 2: GDP_SWEEP(2); break;
   case 4: GDP_SWEEP(4); break;
@@ -546,16 +577,16 @@ int IluFactor::apply(const TV *r, TV scale, TV *x, const int *gate, int gmode, T
   if constexpr(sizeof(TV) == 8) t = tmp64;
   else t = tmp32;
   if(is_f64) {
-    sweep<double, TV, true>(g_lower, n, ia, ja, diag, lu64, r, scale, t, x, err + 1, wait_limit(), gate, gmode,
-                            nullptr);
-    sweep<double, TV, false>(g_upper, n, ia, ja, diag, lu64, nullptr, scale, t, x, err + 1, wait_limit(), gate,
-                             gmode, addto);
+    sweep<double, TV, true>(g_lower, c_lower, n, ia, ja, diag, lu64, r, scale, t, x, err + 1, wait_limit(), gate,
+                            gmode, nullptr);
+    sweep<double, TV, false>(g_upper, c_upper, n, ia, ja, diag, lu64, nullptr, scale, t, x, err + 1,
+                             wait_limit(), gate, gmode, addto);
   }
   else {
-    sweep<float, TV, true>(g_lower, n, ia, ja, diag, lu32, r, scale, t, x, err + 1, wait_limit(), gate, gmode,
-                           nullptr);
-    sweep<float, TV, false>(g_upper, n, ia, ja, diag, lu32, nullptr, scale, t, x, err + 1, wait_limit(), gate,
-                            gmode, addto);
+    sweep<float, TV, true>(g_lower, c_lower, n, ia, ja, diag, lu32, r, scale, t, x, err + 1, wait_limit(), gate,
+                           gmode, nullptr);
+    sweep<float, TV, false>(g_upper, c_upper, n, ia, ja, diag, lu32, nullptr, scale, t, x, err + 1,
+                            wait_limit(), gate, gmode, addto);
   }
   GDP_HIP(hipGetLastError());
   return GDP_OK;
