@@ -38,6 +38,11 @@ class TermDesc(C.Structure):
                 ("ncoef", C.c_int), ("coef_nreg", C.c_int), ("coef", _dp)]
 
 
+class PostDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("space", C.c_int), ("nregions", C.c_int), ("regions", _ip),
+                ("ncoef", C.c_int), ("coef_nreg", C.c_int), ("coef", _dp), ("npoints", C.c_int)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("its", C.c_int), ("status", C.c_int), ("ilu_row", C.c_int),
                 ("ilu_levels", C.c_int), ("outer_its", C.c_int), ("res0", C.c_double),
@@ -93,6 +98,7 @@ SIGNATURES = [
     ("gdp_fe_pattern", C.c_int, [_vp, _ip, _ip]),
     ("gdp_fe_generate", C.c_int, [_vp, _vp, _vp]),
     ("gdp_fe_last_timing", C.c_int, [_vp, _dp, _dp, _dp]),
+    ("gdp_fe_post", C.c_int, [_vp, _vp, C.POINTER(PostDesc), _ip, _dp, C.c_long, _dp, _dp]),
     ("gdp_number_edges", C.c_int, [C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip, C.c_int]),
     ("gdp_rcmk", C.c_int, [C.c_int, _ip, _ip, _ip, _ip]),
     ("gdp_batch_create", C.c_int, [_vp, C.c_int, _ip, _ip, _pp]),

@@ -203,21 +203,15 @@ __device__ inline double dot3(const double a[3], const double b[3])
 
 // ---- kernel 1: per-element Jacobian, determinant, inverse ----------------------
 // JacobianVol2D / JacobianVol3D / JacobianSur3D and Get_InverseMatrix
-// (Get_Geometry.cpp:1099-1124, 1435-1469, 1629-1680, 1755-1808).
-__global__ void __launch_bounds__(256) k_geometry(int nelem, const int *__restrict__ geokind,
-                                                  const int *__restrict__ conn,
-                                                  const double *__restrict__ xyz,
-                                                  double *__restrict__ geo)
+// (Get_Geometry.cpp:1099-1124, 1435-1469, 1629-1680, 1755-1808), for the
+// element with nodes cn[0..3], into G (the layout above).
+__device__ inline void geo_of(int gk, const int *__restrict__ cn, const double *__restrict__ xyz, double *G)
 {
-  int e = blockIdx.x * 256 + threadIdx.x;
-  if(e >= nelem) return;
-  int gk = geokind[e];
-  if(gk == GEO_NONE) return;
   int tri = (gk != GEO_TET);
   int nn = tri ? 3 : 4;
   double x[4], y[4], z[4];
   for(int i = 0; i < nn; i++) {
-    long nd = conn[(long)e * 4 + i] - 1;
+    long nd = cn[i] - 1;
     x[i] = xyz[3 * nd];
     y[i] = xyz[3 * nd + 1];
     z[i] = xyz[3 * nd + 2];
@@ -284,10 +278,21 @@ __global__ void __launch_bounds__(256) k_geometry(int nelem, const int *__restri
     I[5] = -(J[0] * J[5] - J[2] * J[3]) / det;
     I[8] = (J[0] * J[4] - J[1] * J[3]) / det;
   }
-  double *G = geo + (long)e * kGeoStride;
   for(int k = 0; k < 9; k++) G[k] = J[k];
   G[9] = det;
   for(int k = 0; k < 9; k++) G[10 + k] = I[k];
+}
+
+__global__ void __launch_bounds__(256) k_geometry(int nelem, const int *__restrict__ geokind,
+                                                  const int *__restrict__ conn,
+                                                  const double *__restrict__ xyz,
+                                                  double *__restrict__ geo)
+{
+  int e = blockIdx.x * 256 + threadIdx.x;
+  if(e >= nelem) return;
+  int gk = geokind[e];
+  if(gk == GEO_NONE) return;
+  geo_of(gk, conn + (long)e * 4, xyz, geo + (long)e * kGeoStride);
 }
 
 
@@ -885,6 +890,7 @@ struct gdp_fe {
   std::vector<double> h_xyz;
   std::vector<std::vector<double>> h_fixval, h_coef;
   std::vector<std::vector<unsigned char>> h_active;
+  std::vector<std::vector<unsigned char>> h_support;  // per space, per region
   std::vector<gdp_term_desc> h_terms;
   bool uploaded = false;
   bool prepped_on_device = false;
@@ -1233,6 +1239,10 @@ extern "C" int gdp_fe_create(const gdp_mesh_desc *mesh, int nspaces,
     F->h_numtype[s] = spaces[s].numtype;
   }
   F->h_active.swap(active);
+  F->h_support.assign(nspaces, std::vector<unsigned char>(F->nreg, 0));
+  for(int sp = 0; sp < nspaces; sp++)
+    for(int q = 0; q < spaces[sp].nsupport; q++)
+      if(spaces[sp].support[q] >= 0 && spaces[sp].support[q] < F->nreg) F->h_support[sp][spaces[sp].support[q]] = 1;
   F->h_coef.resize(nterms);
   F->h_terms.assign(terms, terms + nterms);
   for(int ti = 0; ti < nterms; ti++) {
@@ -1792,4 +1802,278 @@ extern "C" int gdp_fe_pattern(const gdp_fe *fe, int *ia, int *ja)
   if(ia) std::copy(fe->h_ia.begin(), fe->h_ia.end(), ia);
   if(ja) std::copy(fe->h_ja.begin(), fe->h_ja.end(), ja);
   return GDP_OK;
+}
+
+// =============================================================================
+// Post-processing: local and integral quantities per element
+// (Pos_LocalOrIntegralQuantity, Cal_PostQuantity.cpp:29-247, with the
+// interpolation of Pos_FemInterpolation, Pos_FemInterpolation.cpp:226-330)
+// =============================================================================
+namespace {
+
+struct PostArgs {
+  int kind, bf, nsel, ncoef, npoints;
+  const int *sel;  // selected elements, ascending
+  const int *etype, *region, *conn, *edges;
+  const double *xyz;
+  const int *dofnum;
+  const double *fixval, *x;
+  const unsigned char *insupport;  // per region: in the space's support
+  const double *coef;              // coef[region * ncoef + c]
+  double *out;
+};
+
+// reference coordinates of vertex i (tri: (0,0) (1,0) (0,1); tet adds (0,0,1))
+__device__ inline void vertex_uvw(int i, double &u, double &v, double &w)
+{
+  u = i == 1 ? 1. : 0.;
+  v = i == 2 ? 1. : 0.;
+  w = i == 3 ? 1. : 0.;
+}
+
+// One element: {u} at each vertex, {d u} (gradient, curl or 2D perpendicular
+// curl), coef * {d u}, or Integral of coef * SquNorm[{d u}].  The sum over the
+// element's basis functions runs in their order with Val += BF * DofValue
+// (Pos_FemInterpolation.cpp:253-308), the DOF value being the unknown's or the
+// fixed value (Dof_GetRealDofValue); outside the space's support the element
+// has no basis functions and the value stays 0.
+__device__ void post_element(const PostArgs &a, int s, double o[12], double &Tout)
+{
+  const int e = a.sel[s];
+  const int et = a.etype[e];
+  const int tri = et != GDP_TET;
+  const int nn = tri ? 3 : 4;
+  double G[20];
+  geo_of(tri ? GEO_TRI2D : GEO_TET, a.conn + (long)e * 4, a.xyz, G);
+  const int reg = a.region[e];
+  const bool edge = a.bf == GDP_BF_EDGE, perp = a.bf == GDP_BF_PERP_EDGE;
+  const int nb = !a.insupport[reg] ? 0 : (edge ? 6 : nn);
+  double dv[6];
+  bool has[6];
+  for(int j = 0; j < nb; j++) {
+    const int ent = edge ? abs(a.edges[(long)e * 6 + j]) : a.conn[(long)e * 4 + j];
+    const int c = a.dofnum[ent];
+    has[j] = c != 0;
+    dv[j] = c > 0 ? a.x[c - 1] : (c < 0 ? a.fixval[-c - 1] : 0.);
+  }
+  const double *I = G + 10;
+  const double d = G[9];
+  // {d u}: constant on a P1 element
+  double D[3] = {0., 0., 0.};
+  if(a.kind != GDP_PQ_VALUE) {
+    for(int j = 0; j < nb; j++) {
+      if(!has[j]) continue;
+      double r[3], b[3];
+      if(edge) {
+        bf_curledge(j, a.edges[(long)e * 6 + j] < 0, r);
+        form2(G, d, r, b);
+      }
+      else if(perp) {
+        double g[3];
+        bf_gradnode(1, j, g);
+        b[0] = (g[1] * G[0] + -g[0] * G[3]) / d;
+        b[1] = (g[1] * G[1] + -g[0] * G[4]) / d;
+        b[2] = 0.;
+      }
+      else {
+        bf_gradnode(tri, j, r);
+        form1(I, r, b);
+      }
+      D[0] += b[0] * dv[j];
+      D[1] += b[1] * dv[j];
+      D[2] += b[2] * dv[j];
+    }
+  }
+  const double *cf = a.coef ? a.coef + (long)reg * a.ncoef : nullptr;
+  if(a.kind == GDP_PQ_INT_COEF_D2) {
+    const double adet = fabs(d);
+    const double tmp = cf[0] * (D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+    double T = 0.;
+    for(int q = 0; q < a.npoints; q++) {
+      double u, v, w, wt;
+      gauss_pt(tri, a.npoints, q, u, v, w, wt);
+      T = T + tmp * (wt * adet);
+    }
+    Tout = T;
+    return;
+  }
+  if(a.kind == GDP_PQ_D || a.kind == GDP_PQ_COEF_D) {
+    double R[3] = {D[0], D[1], D[2]};
+    if(a.kind == GDP_PQ_COEF_D) coef_times(a.ncoef, cf, D, R);
+    for(int i = 0; i < nn; i++)
+      for(int c = 0; c < 3; c++) o[i * 3 + c] = R[c];
+    return;
+  }
+  for(int i = 0; i < nn; i++) {  // GDP_PQ_VALUE at vertex i
+    double u, v, w;
+    vertex_uvw(i, u, v, w);
+    double V[3] = {0., 0., 0.};
+    for(int j = 0; j < nb; j++) {
+      if(!has[j]) continue;
+      if(edge) {
+        double r[3], b[3];
+        bf_edge(j, u, v, w, a.edges[(long)e * 6 + j] < 0, r);
+        form1(I, r, b);
+        V[0] += b[0] * dv[j];
+        V[1] += b[1] * dv[j];
+        V[2] += b[2] * dv[j];
+      }
+      else if(perp) {  // (0, 0, BF_Node) through ChangeOfCoord_Form1P
+        const double b2 = bf_node(1, j, u, v, w) / G[8];
+        V[0] += 0. * dv[j];
+        V[1] += 0. * dv[j];
+        V[2] += b2 * dv[j];
+      }
+      else
+        V[0] += bf_node(tri, j, u, v, w) * dv[j];
+    }
+    o[i * 3 + 0] = V[0];
+    o[i * 3 + 1] = V[1];
+    o[i * 3 + 2] = V[2];
+  }
+}
+
+// One thread per selected element; the 12 values per element leave through
+// LDS so that the block's stores are contiguous.
+__global__ void __launch_bounds__(128) k_post(PostArgs a)
+{
+  __shared__ double st[128 * 12];
+  const int s = blockIdx.x * 128 + threadIdx.x;
+  double o[12];
+  for(int i = 0; i < 12; i++) o[i] = 0.;
+  double T = 0.;
+  if(s < a.nsel) post_element(a, s, o, T);
+  if(a.kind == GDP_PQ_INT_COEF_D2) {  // uniform across the grid
+    if(s < a.nsel) a.out[s] = T;
+    return;
+  }
+  for(int i = 0; i < 12; i++) st[threadIdx.x * 12 + i] = o[i];
+  __syncthreads();
+  const long base = (long)blockIdx.x * 128 * 12, lim = (long)a.nsel * 12;
+  for(int k = threadIdx.x; k < 128 * 12; k += 128)
+    if(base + k < lim) a.out[base + k] = st[k];
+}
+
+// Sum of the per-element integrals in two deterministic stages: block b adds
+// elements b*256*kPostPer ... in a fixed order (each thread kPostPer
+// consecutive values, then the block's tree), one block adds the partials.
+constexpr int kPostPer = 16;
+__global__ void __launch_bounds__(256) k_post_part(int n, const double *v, double *part)
+{
+  const long b = ((long)blockIdx.x * 256 + threadIdx.x) * kPostPer;
+  double t = 0.;
+  for(int i = 0; i < kPostPer; i++)
+    if(b + i < n) t += v[b + i];
+  t = block_sum<double, 256>(t);
+  if(threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256) k_post_total(int np, const double *part, double *total)
+{
+  const double t = sum_partials<double, 256>(part, np);
+  if(threadIdx.x == 0) *total = t;
+}
+
+}  // namespace
+
+extern "C" int gdp_fe_post(gdp_fe *fe, const gdp_vector *x, const gdp_post_desc *q, int *nsel,
+                           double *out, long out_len, double *total, double *kernel_ms)
+{
+  if(!fe || !x || !q || !nsel) return set_error(GDP_E_ARG, "gdp_fe_post: null argument");
+  if(q->kind < GDP_PQ_VALUE || q->kind > GDP_PQ_INT_COEF_D2)
+    return set_error(GDP_E_UNSUPPORTED, "gdp_fe_post: quantity kind %d", q->kind);
+  if(q->space < 0 || q->space >= fe->nspaces) return set_error(GDP_E_ARG, "gdp_fe_post: space %d", q->space);
+  if(x->n != fe->N) return set_error(GDP_E_ARG, "gdp_fe_post: vector size %d, system %d", x->n, fe->N);
+  const bool needc = q->kind == GDP_PQ_COEF_D || q->kind == GDP_PQ_INT_COEF_D2;
+  if(needc) {
+    if(!q->coef || q->coef_nreg < fe->nreg) return set_error(GDP_E_ARG, "gdp_fe_post: coefficient table too short");
+    if(q->kind == GDP_PQ_INT_COEF_D2 ? q->ncoef != 1 : (q->ncoef != 1 && q->ncoef != 9))
+      return set_error(GDP_E_UNSUPPORTED, "gdp_fe_post: %d coefficient values", q->ncoef);
+  }
+  if(q->kind == GDP_PQ_INT_COEF_D2 && q->npoints < 1) return set_error(GDP_E_ARG, "gdp_fe_post: Gauss points");
+  const int bf = fe->h_bf[q->space];
+  // OnElementsOf / In: the elements of the listed regions, in element order
+  std::vector<unsigned char> inreg(fe->nreg, 0);
+  for(int r = 0; r < q->nregions; r++)
+    if(q->regions && q->regions[r] >= 0 && q->regions[r] < fe->nreg) inreg[q->regions[r]] = 1;
+  std::vector<int> sel;
+  for(int e = 0; e < fe->nelem; e++) {
+    if(!inreg[fe->h_region[e]]) continue;
+    const int et = fe->h_etype[e];
+    if(et != GDP_TET && et != GDP_TRI) return set_error(GDP_E_UNSUPPORTED, "gdp_fe_post: element type %d", et);
+    if(bf == GDP_BF_EDGE && et != GDP_TET) return set_error(GDP_E_UNSUPPORTED, "gdp_fe_post: edges on triangles");
+    if(bf == GDP_BF_PERP_EDGE && et != GDP_TRI)
+      return set_error(GDP_E_UNSUPPORTED, "gdp_fe_post: perpendicular edges on tetrahedra");
+    if(q->kind == GDP_PQ_INT_COEF_D2 &&
+       (et == GDP_TET ? (q->npoints != 1 && q->npoints != 4) : (q->npoints != 1 && q->npoints != 3 && q->npoints != 4)))
+      return set_error(GDP_E_UNSUPPORTED, "gdp_fe_post: %d Gauss points", q->npoints);
+    sel.push_back(e);
+  }
+  const int ns = (int)sel.size();
+  const long per = q->kind == GDP_PQ_INT_COEF_D2 ? 1 : 12;
+  *nsel = ns;
+  if(out && out_len < per * ns) return set_error(GDP_E_ARG, "gdp_fe_post: output holds %ld values, %ld needed", out_len, per * ns);
+  GDP_TRY(fe_upload(fe));
+  hipStream_t s = ctx().stream;
+  int *d_sel = nullptr;
+  double *d_out = nullptr, *d_tot = nullptr, *d_coef = nullptr;
+  unsigned char *d_sup = nullptr;
+  const int nparts = (int)std::max(1L, ((long)ns + 256L * kPostPer - 1) / (256L * kPostPer));
+  int rc = dalloc(&d_sel, (size_t)std::max(1, ns));
+  if(!rc) rc = dalloc(&d_out, (size_t)std::max(1L, per * ns) + 1 + nparts);
+  if(!rc) rc = dalloc(&d_sup, (size_t)fe->nreg);
+  if(!rc && needc) rc = dalloc(&d_coef, (size_t)fe->nreg * q->ncoef);
+  if(!rc && ns) rc = h2d(d_sel, sel.data(), (size_t)ns);
+  if(!rc) rc = h2d(d_sup, fe->h_support[q->space].data(), (size_t)fe->nreg);
+  if(!rc && needc) rc = h2d(d_coef, q->coef, (size_t)fe->nreg * q->ncoef);
+  d_tot = d_out + per * ns;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if(!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = set_error(GDP_E_HIP, "hipEventCreate");
+  if(!rc) {
+    PostArgs a;
+    a.kind = q->kind;
+    a.bf = bf;
+    a.nsel = ns;
+    a.ncoef = needc ? q->ncoef : 0;
+    a.npoints = q->npoints;
+    a.sel = d_sel;
+    a.etype = fe->d_etype;
+    a.region = fe->d_region;
+    a.conn = fe->d_conn;
+    a.edges = fe->d_edges;
+    a.xyz = fe->d_xyz;
+    a.dofnum = fe->d_dofnum[q->space];
+    a.fixval = fe->d_fixval[q->space];
+    a.x = x->d;
+    a.insupport = d_sup;
+    a.coef = d_coef;
+    a.out = d_out;
+    (void)hipEventRecord(e0, s);
+    if(ns) hipLaunchKernelGGL(k_post, dim3((ns + 127) / 128), dim3(128), 0, s, a);
+    if(q->kind == GDP_PQ_INT_COEF_D2) {
+      hipLaunchKernelGGL(k_post_part, dim3(nparts), dim3(256), 0, s, ns, d_out, d_tot + 1);
+      hipLaunchKernelGGL(k_post_total, dim3(1), dim3(256), 0, s, nparts, d_tot + 1, d_tot);
+    }
+    (void)hipEventRecord(e1, s);
+    if(hipGetLastError() != hipSuccess) rc = set_error(GDP_E_HIP, "k_post launch");
+  }
+  if(!rc) rc = sync();
+  if(!rc && kernel_ms) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *kernel_ms = ms;
+  }
+  if(!rc && out && ns) rc = d2h(out, d_out, (size_t)(per * ns));
+  if(!rc && total) {
+    *total = 0.;
+    if(q->kind == GDP_PQ_INT_COEF_D2) rc = d2h(total, d_tot, 1);
+  }
+  if(e0) (void)hipEventDestroy(e0);
+  if(e1) (void)hipEventDestroy(e1);
+  dfree(d_sel);
+  dfree(d_out);
+  dfree(d_sup);
+  dfree(d_coef);
+  return rc;
 }

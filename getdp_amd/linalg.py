@@ -15,7 +15,7 @@ from typing import Optional
 
 import numpy as np
 
-from ._lib import (GdpError, MeshDesc, SolveInfo, SpaceDesc, TermDesc, check, dptr, iptr,
+from ._lib import (GdpError, MeshDesc, PostDesc, SolveInfo, SpaceDesc, TermDesc, check, dptr, iptr,
                    lib)
 
 __all__ = ["init", "Solver", "Vector", "Matrix", "FESystem", "solve", "solve_again",
@@ -194,6 +194,10 @@ def solve_again(A: Matrix, b: Vector, s: Solver, x: Vector) -> dict:
     return s.info()
 
 
+# post-processing quantity kinds (gdp_hip.h GDP_PQ_*)
+PQ_VALUE, PQ_D, PQ_COEF_D, PQ_INT_COEF_D2 = 1, 2, 3, 4
+
+
 class FESystem(_Handle):
     """One formulation's device assembly (PRE pass at construction,
     ``generate`` = Generate[A])."""
@@ -261,6 +265,28 @@ class FESystem(_Handle):
         t = [C.c_double() for _ in range(3)]
         check(lib().gdp_fe_last_timing(self.h, *[C.byref(x) for x in t]), "timing")
         return dict(ms_total=t[0].value, ms_geometry=t[1].value, ms_gather=t[2].value)
+
+    def post(self, x: Vector, kind: int, regions, space: int = 0, coef=None, npoints: int = 1) -> dict:
+        """PostOperation Print[...] of a local or integral quantity of the
+        solution x on the elements of `regions` (element order): kind
+        PQ_VALUE / PQ_D / PQ_COEF_D -> values (nsel, 4, 3); PQ_INT_COEF_D2 ->
+        values (nsel,) and their total.  coef: per-region table (nreg, ncoef)."""
+        regs = np.ascontiguousarray(regions, np.int32)
+        cf = None if coef is None else np.ascontiguousarray(np.atleast_2d(np.asarray(coef, np.float64)))
+        d = PostDesc(kind, space, regs.shape[0], iptr(regs), 0 if cf is None else cf.shape[1],
+                     0 if cf is None else cf.shape[0], None if cf is None else dptr(cf), npoints)
+        ns = C.c_int()
+        check(lib().gdp_fe_post(self.h, x.h, C.byref(d), C.byref(ns), None, 0, None, None), "gdp_fe_post")
+        per = 1 if kind == PQ_INT_COEF_D2 else 12
+        out = np.zeros(max(1, per * ns.value))
+        tot = C.c_double()
+        ms = C.c_double()
+        check(lib().gdp_fe_post(self.h, x.h, C.byref(d), C.byref(ns), dptr(out), out.shape[0],
+                                C.byref(tot), C.byref(ms)), "gdp_fe_post")
+        vals = out[: per * ns.value]
+        if per == 12:
+            vals = vals.reshape(ns.value, 4, 3)
+        return {"values": vals, "total": tot.value, "kernel_ms": ms.value, "nsel": ns.value}
 
 
 def assemble(problem, fe: Optional[FESystem] = None):

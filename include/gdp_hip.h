@@ -235,6 +235,33 @@ int gdp_fe_pattern(const gdp_fe *fe, int *ia, int *ja);
  * ascending element order as the reference does.  A takes the formulation's
  * pattern on first use. */
 int gdp_fe_generate(gdp_fe *fe, gdp_matrix *A, gdp_vector *b);
+/* ---- post-processing (PostOperation Print[...] of local and integral
+ * quantities, Cal_PostQuantity.cpp:29-247 -> Pos_FemInterpolation,
+ * Pos_FemInterpolation.cpp:226-330) -------------------------------------- */
+#define GDP_PQ_VALUE 1        /* {u} at each vertex: nodal scalar, Whitney     */
+                              /* vector, or (0,0,phi) on perpendicular edges  */
+#define GDP_PQ_D 2            /* {d u}: gradient / curl / 2D curl, per element */
+#define GDP_PQ_COEF_D 3       /* coef[] * {d u} (scalar or 3x3 tensor)         */
+#define GDP_PQ_INT_COEF_D2 4  /* Integral { [ coef[] * SquNorm[{d u}] ] }      */
+typedef struct {
+  int kind;                   /* GDP_PQ_*                                      */
+  int space;                  /* function space index                          */
+  int nregions;
+  const int *regions;         /* OnElementsOf / In                             */
+  int ncoef;                  /* COEF_D: 1 or 9; INT_COEF_D2: 1                */
+  int coef_nreg;              /* table rows = region ids 0..coef_nreg-1        */
+  const double *coef;         /* coef[r*ncoef + c]                             */
+  int npoints;                /* INT_COEF_D2: Gauss points (tri 1/3/4, tet 1/4)*/
+} gdp_post_desc;
+/* Evaluates quantity q of the solution x (original DOF numbering, fixed DOFs
+ * taking their constraint values) on the elements of q->regions, in element
+ * order; *nsel = their count.  out (may be NULL): for VALUE, D and COEF_D,
+ * 12 values per element (4 vertices x 3 components, zero-padded; vertices in
+ * connectivity order); for INT_COEF_D2 the element integral (Gauss rule as in
+ * the integral quantity's Integration) and *total their sum (Print[q[Region],
+ * OnGlobal]).  *kernel_ms (may be NULL): device time of the evaluation. */
+int gdp_fe_post(gdp_fe *fe, const gdp_vector *x, const gdp_post_desc *q, int *nsel,
+                double *out, long out_len, double *total, double *kernel_ms);
 /* Device time of the last gdp_fe_generate (ms) and of its kernels. */
 int gdp_fe_last_timing(const gdp_fe *fe, double *ms_total, double *ms_geometry,
                        double *ms_gather);

@@ -1613,3 +1613,125 @@ long orc_assemble(const orc_mesh *m, int nspaces, const orc_space *sp,
   free(rows);
   return nnz;
 }
+
+/* ---- Post-processing ---------------------------------------------------- */
+
+/* Pos_LocalOrIntegralQuantity (Cal_PostQuantity.cpp:29-247) for one quantity
+ * term on P1 elements, with Pos_FemInterpolation's sum (Pos_FemInterpolation.cpp
+ * :253-308: Val += BF * DofValue over the element's basis functions in order,
+ * the DOF value being the unknown's or the constraint's, Dof_GetRealDofValue).
+ * Local quantities are evaluated at each vertex (OnElementsOf); the integral
+ * one sums Factor = weight * |detJ| times coef * SquNorm[{d u}] over the
+ * Gauss points (Cal_AddMultValue, Cal_Value.cpp:357-437; F_SquNorm,
+ * F_ExtMath.cpp:183-218). */
+int orc_post(const orc_mesh *m, const orc_space *sp, const orc_post_q *q, const int *dofnum,
+             const double *x, double *out, double *total)
+{
+  const orc_space *s = sp + q->space;
+  int ns = 0;
+  double tot = 0.0;
+  for(int e = 0; e < m->nelem; e++) {
+    if(!in_list(m->region[e], q->nregions, q->regions)) continue;
+    int t = m->etype[e], tri = (t == ORC_TRI), nn = nnodes_of(t);
+    double xs[4], ys[4], zs[4];
+    for(int i = 0; i < nn; i++) {
+      int nd = m->conn[(size_t)e * 4 + i] - 1;
+      xs[i] = m->xyz[3 * nd];
+      ys[i] = m->xyz[3 * nd + 1];
+      zs[i] = m->xyz[3 * nd + 2];
+    }
+    m3 J, I;
+    double d = jacobian(t, 0, xs, ys, zs, &J);
+    inverse(!tri, d, &J, &I);
+    int edge = s->bf == ORC_BF_EDGE, perp = s->bf == ORC_BF_PERP_EDGE;
+    int nb = in_list(m->region[e], s->nsupport, s->support) ? (edge ? 6 : nn) : 0;
+    double dv[6];
+    int has[6], neg[6];
+    for(int j = 0; j < nb; j++) {
+      int sign;
+      int ent = space_entity(m, s, e, j, &sign);
+      int c = dofnum[ent];
+      neg[j] = sign < 0;
+      has[j] = c != 0;
+      dv[j] = c > 0 ? x[c - 1] : (c < 0 ? s->fixed_value[-c - 1] : 0.0);
+    }
+    double D[3] = {0.0, 0.0, 0.0};
+    if(q->kind != ORC_PQ_VALUE) {
+      for(int j = 0; j < nb; j++) {
+        if(!has[j]) continue;
+        double r[3], b[3];
+        if(edge) {
+          bf_curledge_tet(j, r);
+          if(neg[j]) { r[0] = -r[0]; r[1] = -r[1]; r[2] = -r[2]; }
+          form2(&J, d, r, b);
+        }
+        else if(perp) {
+          bf_curlperp(t, j, r);
+          form2p(&J, d, r, b);
+        }
+        else {
+          bf_gradnode(t, j, r);
+          form1(&I, r, b);
+        }
+        D[0] += b[0] * dv[j];
+        D[1] += b[1] * dv[j];
+        D[2] += b[2] * dv[j];
+      }
+    }
+    const double *cf = q->coef ? q->coef + (size_t)m->region[e] * q->ncoef : NULL;
+    if(q->kind == ORC_PQ_INT_COEF_D2) {
+      double tmp = cf[0] * (D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+      double T = 0.0;
+      for(int g = 0; g < q->npoints; g++) {
+        double u, v, w, wt;
+        gauss_point(t, q->npoints, g, &u, &v, &w, &wt);
+        T = T + tmp * (wt * fabs(d));
+      }
+      if(out) out[ns] = T;
+      tot = tot + T;
+      ns++;
+      continue;
+    }
+    double *o = out ? out + (size_t)ns * 12 : NULL;
+    ns++;
+    if(!o) continue;
+    for(int i = 0; i < 12; i++) o[i] = 0.0;
+    if(q->kind != ORC_PQ_VALUE) {
+      double R[3] = {D[0], D[1], D[2]};
+      if(q->kind == ORC_PQ_COEF_D) coef_product(q->ncoef, cf, D, R);
+      for(int i = 0; i < nn; i++)
+        for(int c = 0; c < 3; c++) o[i * 3 + c] = R[c];
+      continue;
+    }
+    for(int i = 0; i < nn; i++) {
+      double u = i == 1 ? 1.0 : 0.0, v = i == 2 ? 1.0 : 0.0, w = i == 3 ? 1.0 : 0.0;
+      double V[3] = {0.0, 0.0, 0.0};
+      for(int j = 0; j < nb; j++) {
+        if(!has[j]) continue;
+        if(edge) {
+          double r[3], b[3];
+          bf_edge_tet(j, u, v, w, r);
+          if(neg[j]) { r[0] = -r[0]; r[1] = -r[1]; r[2] = -r[2]; }
+          form1(&I, r, b);
+          V[0] += b[0] * dv[j];
+          V[1] += b[1] * dv[j];
+          V[2] += b[2] * dv[j];
+        }
+        else if(perp) {
+          double r[3] = {0.0, 0.0, bf_node(t, j, u, v, w)}, b[3];
+          form1p(&J, r, b);
+          V[0] += b[0] * dv[j];
+          V[1] += b[1] * dv[j];
+          V[2] += b[2] * dv[j];
+        }
+        else
+          V[0] += bf_node(t, j, u, v, w) * dv[j];
+      }
+      o[i * 3 + 0] = V[0];
+      o[i * 3 + 1] = V[1];
+      o[i * 3 + 2] = V[2];
+    }
+  }
+  if(total) *total = tot;
+  return ns;
+}

@@ -168,6 +168,24 @@ long orc_assemble(const orc_mesh *m, int nspaces, const orc_space *sp,
                   const int *dofnum, int N, int *ia, int *ja, double *a,
                   long max_nnz, double *b);
 
+/* Post-processing (Pos_LocalOrIntegralQuantity, Cal_PostQuantity.cpp:29-247,
+ * interpolation as Pos_FemInterpolation.cpp:226-330) of the solution x on the
+ * elements of q->regions, in element order.  dofnum: the table of q->space
+ * (orc_number_dofs).  out: 12 values per element (vertex-major, 3 components)
+ * for VALUE, D, COEF_D; one per element for INT_COEF_D2, *total = their sum in
+ * element order (Cal_PostCumulativeQuantity).  Returns the element count. */
+enum { ORC_PQ_VALUE = 1, ORC_PQ_D = 2, ORC_PQ_COEF_D = 3, ORC_PQ_INT_COEF_D2 = 4 };
+typedef struct {
+  int kind, space;
+  int nregions;
+  const int *regions;
+  int ncoef, coef_nreg;
+  const double *coef;
+  int npoints;
+} orc_post_q;
+int orc_post(const orc_mesh *m, const orc_space *sp, const orc_post_q *q, const int *dofnum,
+             const double *x, double *out, double *total);
+
 /* Element matrix of one term on one element (for closed-form tests). */
 int orc_element_matrix(const orc_mesh *m, const orc_space *sp, const orc_term *t, int e,
                        double *Ek /* k*k row-major */, int *k_out);

@@ -324,3 +324,33 @@ def number_edges(etype, conn, visit):
     n = lib().orc_number_edges(ne, _i(etype), _i(conn), visit.shape[0], _i(visit), _i(edges),
                                _i(en), mx)
     return edges, en[:n]
+
+
+class OPost(C.Structure):
+    _fields_ = [("kind", C.c_int), ("space", C.c_int), ("nregions", C.c_int),
+                ("regions", C.POINTER(C.c_int)), ("ncoef", C.c_int), ("coef_nreg", C.c_int),
+                ("coef", C.POINTER(C.c_double)), ("npoints", C.c_int)]
+
+
+def post(problem, x, kind, regions, space=0, coef=None, npoints=1):
+    """Oracle post-processing of the solution x (original numbering): values
+    (nsel, 4, 3) for local quantities, (nsel,) + total for the integral."""
+    keep, m, sps, ts = _describe(problem)
+    nmax = problem.nmax_entity()
+    tab = np.zeros((len(problem.spaces), nmax + 1), np.int32)
+    lib().orc_number_dofs(C.byref(m), len(sps), sps, len(ts), ts, nmax, _i(tab))
+    regs = np.ascontiguousarray(regions, np.int32)
+    cf = None if coef is None else np.ascontiguousarray(np.atleast_2d(np.asarray(coef, np.float64)))
+    q = OPost(kind, space, regs.shape[0], _i(regs), 0 if cf is None else cf.shape[1],
+              0 if cf is None else cf.shape[0], None if cf is None else _d(cf), npoints)
+    x = np.ascontiguousarray(x, np.float64)
+    dn = np.ascontiguousarray(tab[space])
+    ns = lib().orc_post(C.byref(m), sps, C.byref(q), _i(dn), _d(x), None, None)
+    per = 1 if kind == 4 else 12
+    out = np.zeros(max(1, per * ns))
+    tot = np.zeros(1)
+    lib().orc_post(C.byref(m), sps, C.byref(q), _i(dn), _d(x), _d(out), _d(tot))
+    vals = out[: per * ns]
+    if per == 12:
+        vals = vals.reshape(ns, 4, 3)
+    return {"values": vals, "total": float(tot[0]), "nsel": ns}
