@@ -43,6 +43,7 @@ SOURCES = [
     ("solve.hip", []),
     ("unet.hip", []),
     ("batch.hip", []),
+    ("edges.hip", []),
     ("host_prep.cpp", []),
     ("solver_params.cpp", []),
 ]

@@ -27,6 +27,11 @@ struct Incidence {
 
 int number_edges(int nelem, const int *etype, const int *conn, int nvisit,
                  const int *visit, int *edges_out, int *edge_nodes, int max_edges);
+// The same numbering computed on the device (edges.hip); used once the
+// library is initialised unless GDP_HOST_PREP is set.
+bool edges_on_device();
+int number_edges_device(int nelem, const int *etype, const int *conn, int nvisit,
+                        const int *visit, int *edges_out, int *edge_nodes, int max_edges);
 int rcmk(int n, const int *ia, const int *ja, std::vector<int> &perm,
          std::vector<int> &rperm);
 // Dependency depth (levels) of the strictly-lower part of a CSR matrix with

@@ -227,6 +227,9 @@ extern "C" int gdp_number_edges(int nelem, const int *etype, const int *conn,
 {
   if(nelem < 0 || !etype || !conn || !visit || !edges_out)
     return gdp::set_error(GDP_E_ARG, "gdp_number_edges: bad arguments");
+  if(gdp::edges_on_device())
+    return gdp::number_edges_device(nelem, etype, conn, nvisit, visit, edges_out, edge_nodes,
+                                    max_edges);
   return gdp::number_edges(nelem, etype, conn, nvisit, visit, edges_out, edge_nodes,
                            max_edges);
 }

@@ -266,11 +266,14 @@ int gdp_fe_post(gdp_fe *fe, const gdp_vector *x, const gdp_post_desc *q, int *ns
 int gdp_fe_last_timing(const gdp_fe *fe, double *ms_total, double *ms_geometry,
                        double *ms_gather);
 
-/* ---- host-side pre-processing (no device work) ----------------------------- */
+/* ---- pre-processing of mesh entities and patterns -------------------------- */
 /* Edge creation (GeoEntity.cpp:437-490): visiting elements in `visit` order,
  * each new edge (sorted node pair) gets the next number; signed by local
  * orientation (+ when it runs from the lower to the higher node number).
- * edges_out: 6 per element, edge_nodes: 2 per edge.  Returns #edges or < 0. */
+ * edges_out: 6 per element, edge_nodes: 2 per edge.  Returns #edges or < 0.
+ * Computed on the device (sort-based, same numbers) once the library is
+ * initialised unless GDP_HOST_PREP is set; on the host otherwise.  The other
+ * functions of this section run on the host. */
 int gdp_number_edges(int nelem, const int *etype, const int *conn, int nvisit,
                      const int *visit, int *edges_out, int *edge_nodes,
                      int max_edges);
