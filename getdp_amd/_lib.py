@@ -98,6 +98,11 @@ SIGNATURES = [
     ("gdp_fe_pattern", C.c_int, [_vp, _ip, _ip]),
     ("gdp_fe_generate", C.c_int, [_vp, _vp, _vp]),
     ("gdp_fe_last_timing", C.c_int, [_vp, _dp, _dp, _dp]),
+    ("gdp_mesh_read", C.c_int, [C.c_char_p, _pp]),
+    ("gdp_mesh_info", C.c_int, [_vp, _ip, _ip, _ip, _ip]),
+    ("gdp_mesh_get", C.c_int, [_vp, _dp, _ip, _ip, _ip, _ip, _ip, _ip]),
+    ("gdp_mesh_free", C.c_int, [_vp]),
+    ("gdp_mesh_write", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, C.c_int, _ip, _ip, _ip]),
     ("gdp_fe_post", C.c_int, [_vp, _vp, C.POINTER(PostDesc), _ip, _dp, C.c_long, _dp, _dp]),
     ("gdp_number_edges", C.c_int, [C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip, C.c_int]),
     ("gdp_rcmk", C.c_int, [C.c_int, _ip, _ip, _ip, _ip]),

@@ -266,6 +266,25 @@ int gdp_fe_post(gdp_fe *fe, const gdp_vector *x, const gdp_post_desc *q, int *ns
 int gdp_fe_last_timing(const gdp_fe *fe, double *ms_total, double *ms_geometry,
                        double *ms_gather);
 
+/* ---- mesh files ------------------------------------------------------------ */
+/* Gmsh MSH 2.x, ASCII or binary (Geo_ReadFile, GeoData.cpp:573-1028): nodes
+ * and elements as the reference keeps them -- elements sorted by number,
+ * region = first tag (1 without tags in ASCII files, 0 in binary ones).
+ * Node numbers are mapped to 1..nnodes in ascending order (node_num gives the
+ * originals); conn holds those indices, 4 per element (zero-padded).  Point
+ * elements are counted (npoints) and not kept; types other than points,
+ * lines, triangles and tetrahedra are rejected.  Host code, no device. */
+typedef struct gdp_mesh_data gdp_mesh_data;
+int gdp_mesh_read(const char *path, gdp_mesh_data **mesh);
+int gdp_mesh_info(const gdp_mesh_data *mesh, int *nnodes, int *nelem, int *npoints, int *binary);
+int gdp_mesh_get(const gdp_mesh_data *mesh, double *xyz, int *node_num, int *elem_num, int *etype,
+                 int *region, int *elementary, int *conn);
+int gdp_mesh_free(gdp_mesh_data *mesh);
+/* Writes MSH 2.2 (ASCII with %.17g coordinates, or binary): nodes 1..nnodes,
+ * elements numbered 1..nelem with tags (region, region). */
+int gdp_mesh_write(const char *path, int binary, int nnodes, const double *xyz, int nelem,
+                   const int *etype, const int *region, const int *conn);
+
 /* ---- pre-processing of mesh entities and patterns -------------------------- */
 /* Edge creation (GeoEntity.cpp:437-490): visiting elements in `visit` order,
  * each new edge (sorted node pair) gets the next number; signed by local
