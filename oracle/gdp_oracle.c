@@ -114,6 +114,252 @@ int orc_ilu0_f64(int n, const int *ia, const int *ja, const double *a,
   ILU0_BODY(double, mul_f64)
 }
 
+/* qsplit, ilut.F:2403-2460: partial quick-split so that the ncut entries of
+ * largest magnitude come first (their order is the algorithm's). 1-based. */
+static void qsplit1(double *a, int *ind, int n, int ncut)
+{
+  int first = 1, last = n;
+  if(ncut < first || ncut > last) return;
+  for(;;) {
+    int mid = first;
+    double abskey = fabs(a[mid]);
+    for(int j = first + 1; j <= last; j++) {
+      if(fabs(a[j]) > abskey) {
+        mid = mid + 1;
+        double tmp = a[mid];
+        int itmp = ind[mid];
+        a[mid] = a[j];
+        ind[mid] = ind[j];
+        a[j] = tmp;
+        ind[j] = itmp;
+      }
+    }
+    double tmp = a[mid];
+    a[mid] = a[first];
+    a[first] = tmp;
+    int itmp = ind[mid];
+    ind[mid] = ind[first];
+    ind[first] = itmp;
+    if(mid == ncut) return;
+    if(mid > ncut) last = mid - 1;
+    else first = mid + 1;
+  }
+}
+
+/* ilut (ilut.F:52-396, pivot = 0) and ilutp (:397-806, pivot = 1, mbloc = n
+ * as solve_matrix passes it), with `scalar` = T for the factor storage: the
+ * products read alu in T and promote, stores round to T.  Written with the
+ * Fortran's 1-based indices; the MSR outputs are converted to 0-based.
+ * ilut's zero-pivot replacement uses the single-precision literal 0.0001,
+ * ilutp's the double 1.0D-4.  ilutp leaves ja (0-based here) permuted and
+ * iperm (2n, 0-based values) with iperm[0..n) = old numbers, iperm[n..2n) =
+ * new numbers. */
+#define ILUT_BODY(T)                                                                          \
+  if(lfil < 0) return -4;                                                                     \
+  int *IA = (int *)malloc(sizeof(int) * (size_t)(n + 2));                                     \
+  int *JW = (int *)calloc((size_t)2 * n + 2, sizeof(int));                                    \
+  int *IP = (int *)malloc(sizeof(int) * ((size_t)2 * n + 2));                                 \
+  double *W = (double *)calloc((size_t)n + 2, sizeof(double));                                \
+  T *ALU = (T *)calloc((size_t)iwk + 2, sizeof(T));                                           \
+  int *JLU = (int *)calloc((size_t)iwk + 2, sizeof(int));                                     \
+  int *JU = (int *)calloc((size_t)n + 2, sizeof(int));                                        \
+  int ierr = 0;                                                                               \
+  for(int i = 0; i <= n; i++) IA[i + 1] = ia[i] + 1;                                          \
+  int ju0 = n + 2;                                                                            \
+  JLU[1] = ju0;                                                                               \
+  for(int j = 1; j <= n; j++) {                                                               \
+    JW[n + j] = 0;                                                                            \
+    IP[j] = j;                                                                                \
+    IP[n + j] = j;                                                                            \
+  }                                                                                           \
+  for(int ii = 1; ii <= n && !ierr; ii++) {                                                   \
+    int j1 = IA[ii], j2 = IA[ii + 1] - 1;                                                     \
+    double tnorm = 0.0;                                                                       \
+    for(int k = j1; k <= j2; k++) tnorm = tnorm + fabs(a[k - 1]);                             \
+    if(tnorm == 0.0) {                                                                        \
+      ierr = -5;                                                                              \
+      break;                                                                                  \
+    }                                                                                         \
+    tnorm = tnorm / (double)(j2 - j1 + 1);                                                    \
+    int lenu = 1, lenl = 0;                                                                   \
+    JW[ii] = ii;                                                                              \
+    W[ii] = 0.0;                                                                              \
+    JW[n + ii] = ii;                                                                          \
+    for(int j = j1; j <= j2; j++) {                                                           \
+      int k = IP[n + (ja[j - 1] + 1)];                                                        \
+      double t = a[j - 1];                                                                    \
+      if(k < ii) {                                                                            \
+        lenl = lenl + 1;                                                                      \
+        JW[lenl] = k;                                                                         \
+        W[lenl] = t;                                                                          \
+        JW[n + k] = lenl;                                                                     \
+      }                                                                                       \
+      else if(k == ii)                                                                        \
+        W[ii] = t;                                                                            \
+      else {                                                                                  \
+        lenu = lenu + 1;                                                                      \
+        int jpos = ii + lenu - 1;                                                             \
+        JW[jpos] = k;                                                                         \
+        W[jpos] = t;                                                                          \
+        JW[n + k] = jpos;                                                                     \
+      }                                                                                       \
+    }                                                                                         \
+    int jj = 0, len = 0;                                                                      \
+    for(;;) {                                                                                 \
+      jj = jj + 1;                                                                            \
+      if(jj > lenl) break;                                                                    \
+      int jrow = JW[jj], k = jj;                                                              \
+      for(int j = jj + 1; j <= lenl; j++)                                                     \
+        if(JW[j] < jrow) {                                                                    \
+          jrow = JW[j];                                                                       \
+          k = j;                                                                              \
+        }                                                                                     \
+      if(k != jj) {                                                                           \
+        int j = JW[jj];                                                                       \
+        JW[jj] = JW[k];                                                                       \
+        JW[k] = j;                                                                            \
+        JW[n + jrow] = jj;                                                                    \
+        JW[n + j] = k;                                                                        \
+        double sw = W[jj];                                                                    \
+        W[jj] = W[k];                                                                         \
+        W[k] = sw;                                                                            \
+      }                                                                                       \
+      JW[n + jrow] = 0;                                                                       \
+      double fact = W[jj] * (double)ALU[jrow];                                                \
+      if(fabs(fact) <= droptol) continue;                                                     \
+      for(int kk = JU[jrow]; kk <= JLU[jrow + 1] - 1; kk++) {                                 \
+        double sv = fact * (double)ALU[kk];                                                   \
+        int j = IP[n + JLU[kk]];                                                              \
+        int jpos = JW[n + j];                                                                 \
+        if(j >= ii) {                                                                         \
+          if(jpos == 0) {                                                                     \
+            lenu = lenu + 1;                                                                  \
+            if(lenu > n) {                                                                    \
+              ierr = -1;                                                                      \
+              break;                                                                          \
+            }                                                                                 \
+            int i = ii + lenu - 1;                                                            \
+            JW[i] = j;                                                                        \
+            JW[n + j] = i;                                                                    \
+            W[i] = -sv;                                                                       \
+          }                                                                                   \
+          else                                                                                \
+            W[jpos] = W[jpos] - sv;                                                           \
+        }                                                                                     \
+        else {                                                                                \
+          if(jpos == 0) {                                                                     \
+            lenl = lenl + 1;                                                                  \
+            if(lenl > n) {                                                                    \
+              ierr = -1;                                                                      \
+              break;                                                                          \
+            }                                                                                 \
+            JW[lenl] = j;                                                                     \
+            JW[n + j] = lenl;                                                                 \
+            W[lenl] = -sv;                                                                    \
+          }                                                                                   \
+          else                                                                                \
+            W[jpos] = W[jpos] - sv;                                                           \
+        }                                                                                     \
+      }                                                                                       \
+      if(ierr) break;                                                                         \
+      len = len + 1;                                                                          \
+      W[len] = fact;                                                                          \
+      JW[len] = jrow;                                                                         \
+    }                                                                                         \
+    if(ierr) break;                                                                           \
+    for(int k = 1; k <= lenu; k++) JW[n + JW[ii + k - 1]] = 0;                                \
+    lenl = len;                                                                               \
+    len = lenl < lfil ? lenl : lfil;                                                          \
+    qsplit1(W, JW, lenl, len);                                                                \
+    for(int k = 1; k <= len; k++) {                                                           \
+      if(ju0 > iwk) {                                                                         \
+        ierr = -2;                                                                            \
+        break;                                                                                \
+      }                                                                                       \
+      ALU[ju0] = (T)W[k];                                                                     \
+      JLU[ju0] = IP[JW[k]];                                                                   \
+      ju0 = ju0 + 1;                                                                          \
+    }                                                                                         \
+    if(ierr) break;                                                                           \
+    JU[ii] = ju0;                                                                             \
+    len = 0;                                                                                  \
+    for(int k = 1; k <= lenu - 1; k++)                                                        \
+      if(fabs(W[ii + k]) > droptol * tnorm) {                                                 \
+        len = len + 1;                                                                        \
+        W[ii + len] = W[ii + k];                                                              \
+        JW[ii + len] = JW[ii + k];                                                            \
+      }                                                                                       \
+    lenu = len + 1;                                                                           \
+    len = lenu < lfil ? lenu : lfil;                                                          \
+    qsplit1(W + ii, JW + ii, lenu - 1, len);                                                  \
+    if(pivot) {                                                                               \
+      int imax = ii;                                                                          \
+      double xmax = fabs(W[imax]), xmax0 = xmax;                                              \
+      int icut = ii - 1 + n - (ii - 1) % n;                                                   \
+      for(int k = ii + 1; k <= ii + len - 1; k++) {                                           \
+        double t = fabs(W[k]);                                                                \
+        if(t > xmax && t * permtol > xmax0 && JW[k] <= icut) {                                \
+          imax = k;                                                                           \
+          xmax = t;                                                                           \
+        }                                                                                     \
+      }                                                                                       \
+      double tmp = W[ii];                                                                     \
+      W[ii] = W[imax];                                                                        \
+      W[imax] = tmp;                                                                          \
+      int j = JW[imax];                                                                       \
+      int i = IP[ii];                                                                         \
+      IP[ii] = IP[j];                                                                         \
+      IP[j] = i;                                                                              \
+      IP[n + IP[ii]] = ii;                                                                    \
+      IP[n + IP[j]] = j;                                                                      \
+    }                                                                                         \
+    if(len + ju0 > iwk) {                                                                     \
+      ierr = -3;                                                                              \
+      break;                                                                                  \
+    }                                                                                         \
+    for(int k = ii + 1; k <= ii + len - 1; k++) {                                             \
+      JLU[ju0] = IP[JW[k]];                                                                   \
+      ALU[ju0] = (T)W[k];                                                                     \
+      ju0 = ju0 + 1;                                                                          \
+    }                                                                                         \
+    if(W[ii] == 0.0) W[ii] = (pivot ? (1.0e-4 + droptol) : ((double)0.0001f + droptol)) * tnorm; \
+    ALU[ii] = (T)(1.0 / W[ii]);                                                               \
+    JLU[ii + 1] = ju0;                                                                        \
+  }                                                                                           \
+  if(!ierr) {                                                                                 \
+    if(pivot) {                                                                               \
+      for(int k = JLU[1]; k <= JLU[n + 1] - 1; k++) JLU[k] = IP[n + JLU[k]];                  \
+      for(int k = 0; k < ia[n]; k++) ja[k] = IP[n + ja[k] + 1] - 1;                           \
+    }                                                                                         \
+    for(int k = 1; k <= ju0; k++) {                                                           \
+      alu[k - 1] = ALU[k];                                                                    \
+      jlu[k - 1] = JLU[k] - 1;                                                                \
+    }                                                                                         \
+    for(int i = 1; i <= n; i++) ju[i - 1] = JU[i] - 1;                                        \
+    if(iperm)                                                                                 \
+      for(int i = 1; i <= 2 * n; i++) iperm[i - 1] = IP[i] - 1;                               \
+  }                                                                                           \
+  free(IA);                                                                                   \
+  free(JW);                                                                                   \
+  free(IP);                                                                                   \
+  free(W);                                                                                    \
+  free(ALU);                                                                                  \
+  free(JLU);                                                                                  \
+  free(JU);                                                                                   \
+  return ierr;
+
+int orc_ilut_f32(int n, const int *ia, int *ja, const double *a, int lfil, double droptol,
+                 double permtol, int pivot, int iwk, float *alu, int *jlu, int *ju, int *iperm)
+{
+  ILUT_BODY(float)
+}
+
+int orc_ilut_f64(int n, const int *ia, int *ja, const double *a, int lfil, double droptol,
+                 double permtol, int pivot, int iwk, double *alu, int *jlu, int *ju, int *iperm)
+{
+  ILUT_BODY(double)
+}
+
 /* lusol, contrib/Sparskit/ilut.F:2297-2347: unit-L forward sweep, then U
  * backward sweep with the stored inverted diagonal.  In place allowed. */
 void orc_lusol_f32(int n, const double *y, double *x, const float *alu,
@@ -916,7 +1162,53 @@ int orc_solve(int n, const int *ia, const int *ja, double *a, double *b,
   double *alu64 = NULL;
   int *jlu = NULL, *ju = NULL;
   int ret = 0;
-  if(p->preconditioner == 6) {
+  int *ja_p = NULL, *iperm = NULL; /* ILUTP: permuted columns, permutation */
+  if(p->preconditioner == 1 || p->preconditioner == 2) {
+    /* Sparskit.cpp:312-317, 375-390, 435-446: first guess of the factor size,
+     * grown by half and factored again on -2/-3; ilutp permutes the columns of
+     * the matrix the iteration then multiplies with */
+    int nnz_ilu = 2 * (n + 1) * (p->nb_fill + 1);
+    ja_p = (int *)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+    iperm = (int *)malloc(sizeof(int) * 2 * (size_t)(n > 0 ? n : 1));
+    for(;;) {
+      memcpy(ja_p, JA, sizeof(int) * (size_t)nnz);
+      jlu = (int *)malloc(sizeof(int) * ((size_t)nnz_ilu + 2));
+      ju = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+      int ierr;
+      if(p->ilu_double) {
+        alu64 = (double *)malloc(sizeof(double) * ((size_t)nnz_ilu + 2));
+        ierr = orc_ilut_f64(n, IA, ja_p, A, p->nb_fill, p->droptol, p->permtol,
+                            p->preconditioner == 2, nnz_ilu, alu64, jlu, ju, iperm);
+      }
+      else {
+        alu32 = (float *)malloc(sizeof(float) * ((size_t)nnz_ilu + 2));
+        ierr = orc_ilut_f32(n, IA, ja_p, A, p->nb_fill, p->droptol, p->permtol,
+                            p->preconditioner == 2, nnz_ilu, alu32, jlu, ju, iperm);
+      }
+      if(ierr == -2 || ierr == -3) {
+        nnz_ilu += nnz_ilu / 2;
+        free(alu32);
+        free(alu64);
+        free(jlu);
+        free(ju);
+        alu32 = NULL;
+        alu64 = NULL;
+        continue;
+      }
+      info->ilu_ierr = ierr;
+      break;
+    }
+    if(p->preconditioner == 1)
+      for(int i = 0; i < n; i++) iperm[n + i] = i;
+    JA = ja_p;
+    sc.ja = JA;
+    sc.alu32 = alu32;
+    sc.alu64 = alu64;
+    sc.jlu = jlu;
+    sc.ju = ju;
+    ret = info->ilu_ierr;
+  }
+  else if(p->preconditioner == 6) {
     size_t nz = (size_t)nnz + n + 1;
     jlu = (int *)malloc(sizeof(int) * nz);
     ju = (int *)malloc(sizeof(int) * (size_t)(n + 1));
@@ -941,7 +1233,7 @@ int orc_solve(int n, const int *ia, const int *ja, double *a, double *b,
     /* DIAGONAL (=8) is the identity (Sparskit.cpp:338-354): applying it is a
      * copy, which leaves every value unchanged. */
     orc_op prec = NULL;
-    if(p->preconditioner == 6) prec = op_lusol;
+    if(p->preconditioner == 6 || p->preconditioner == 1 || p->preconditioner == 2) prec = op_lusol;
     int pos = p->precond_position ? p->precond_position : 2;
     if(pos != 2 && prec) prec = NULL; /* left preconditioning: not restated */
     static const int dummy = 0;
@@ -974,7 +1266,8 @@ int orc_solve(int n, const int *ia, const int *ja, double *a, double *b,
       s2 += w[i] * w[i];
     }
     info->true_res = sqrt(s2);
-    for(int i = 0; i < n; i++) x[i] = sol[perm[i]];
+    for(int i = 0; i < n; i++) /* Sparskit.cpp:597-602 */
+      x[i] = iperm ? sol[iperm[n + perm[i]]] : sol[perm[i]];
     if(rowscal)
       for(int i = 0; i < n; i++) x[i] *= rowscal[i];
     free(w);
@@ -990,6 +1283,8 @@ int orc_solve(int n, const int *ia, const int *ja, double *a, double *b,
   free(alu64);
   free(jlu);
   free(ju);
+  free(ja_p);
+  free(iperm);
   free(rowscal);
   return ret;
 }

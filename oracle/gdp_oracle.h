@@ -27,6 +27,16 @@ void orc_amux(int n, const double *x, double *y, const int *ia, const int *ja,
               const double *a);
 int orc_ilu0_f32(int n, const int *ia, const int *ja, const double *a,
                  float *alu, int *jlu, int *ju);
+/* ilut (ilut.F:52-396, pivot 0) / ilutp (:397-806, pivot 1, mbloc = n):
+ * threshold ILU with lfil entries per L/U row, dropping below droptol
+ * (relative to the row norm for U), column pivoting with permtol.  0-based
+ * MSR output (alu[0..n) inverted diagonal, jlu[0..n] row starts, ju row
+ * starts of U) with room for iwk entries; ilutp permutes ja in place and
+ * fills iperm (2n).  Returns 0 or Sparskit's ierr (-1..-5). */
+int orc_ilut_f32(int n, const int *ia, int *ja, const double *a, int lfil, double droptol,
+                 double permtol, int pivot, int iwk, float *alu, int *jlu, int *ju, int *iperm);
+int orc_ilut_f64(int n, const int *ia, int *ja, const double *a, int lfil, double droptol,
+                 double permtol, int pivot, int iwk, double *alu, int *jlu, int *ju, int *iperm);
 int orc_ilu0_f64(int n, const int *ia, const int *ja, const double *a,
                  double *alu, int *jlu, int *ju);
 void orc_lusol_f32(int n, const double *y, double *x, const float *alu,
@@ -41,7 +51,7 @@ int orc_rcmk(int n, const int *ia, const int *ja, const double *a, int *perm,
 /* solver.par subset (contrib/Sparskit/Sparskit.cpp:1847-1868). */
 typedef struct {
   int algorithm;        /* 1 CG, 8 GMRES, 9 FGMRES */
-  int preconditioner;   /* 0 NONE, 6 ILU0, 8 DIAGONAL */
+  int preconditioner;   /* 0 NONE, 1 ILUT, 2 ILUTP, 6 ILU0, 8 DIAGONAL */
   int precond_position; /* 1 left, 2 right (default 2) */
   int krylov;           /* Krylov_Size */
   int maxits;           /* Nb_Iter_Max */
@@ -49,6 +59,9 @@ typedef struct {
   int scaling;          /* 0 none, 1 diagonal */
   int ilu_double;       /* 0: FP32 factor storage (reference default) */
   double rtol;          /* Stopping_Test */
+  int nb_fill;          /* ILUT/ILUTP: Nb_Fill */
+  double droptol;       /* Dropping_Tolerance */
+  double permtol;       /* ILUTP: Permutation_Tolerance */
 } orc_params;
 
 typedef struct {

@@ -27,7 +27,8 @@ class Params(C.Structure):
     _fields_ = [("algorithm", C.c_int), ("preconditioner", C.c_int),
                 ("precond_position", C.c_int), ("krylov", C.c_int), ("maxits", C.c_int),
                 ("renumbering", C.c_int), ("scaling", C.c_int), ("ilu_double", C.c_int),
-                ("rtol", C.c_double)]
+                ("rtol", C.c_double), ("nb_fill", C.c_int), ("droptol", C.c_double),
+                ("permtol", C.c_double)]
 
 
 class Info(C.Structure):
@@ -41,7 +42,8 @@ class Info(C.Structure):
 class RefParams(C.Structure):
     _fields_ = [("algorithm", C.c_int), ("preconditioner", C.c_int),
                 ("precond_position", C.c_int), ("krylov", C.c_int), ("maxits", C.c_int),
-                ("renumbering", C.c_int), ("scaling", C.c_int), ("rtol", C.c_double)]
+                ("renumbering", C.c_int), ("scaling", C.c_int), ("rtol", C.c_double),
+                ("nb_fill", C.c_int), ("droptol", C.c_double), ("permtol", C.c_double)]
 
 
 class RefInfo(C.Structure):
@@ -125,6 +127,22 @@ def ilu0(ia, ja, a, double=False):
     return alu, jlu, ju, ierr
 
 
+def ilut(ia, ja, a, lfil=20, droptol=0.0, permtol=0.05, pivot=False, double=False, iwk=None):
+    """Oracle ilut / ilutp: (alu, jlu, ju, iperm, ja_after, ierr), 0-based MSR."""
+    n = ia.shape[0] - 1
+    iwk = iwk or 2 * (n + 1) * (lfil + 1)
+    alu = np.zeros(iwk + 2, np.float64 if double else np.float32)
+    jlu = np.zeros(iwk + 2, np.int32)
+    ju = np.zeros(n + 1, np.int32)
+    iperm = np.zeros(2 * n, np.int32)
+    ja2 = np.array(ja, np.int32)
+    f = lib().orc_ilut_f64 if double else lib().orc_ilut_f32
+    ierr = f(n, _i(ia), _i(ja2), _d(np.ascontiguousarray(a, np.float64)), int(lfil), C.c_double(droptol),
+             C.c_double(permtol), int(pivot), int(iwk), alu.ctypes.data_as(_dp if double else _fp),
+             _i(jlu), _i(ju), _i(iperm))
+    return alu, jlu, ju, iperm, ja2, ierr
+
+
 def lusol(alu, jlu, ju, y):
     n = ju.shape[0] - 1
     x = np.zeros(n)
@@ -150,13 +168,14 @@ def rcmk(ia, ja, a=None):
 
 
 def solve(ia, ja, a, b, algorithm=8, preconditioner=6, krylov=30, maxits=100000,
-          renumbering=1, scaling=0, ilu_double=False, rtol=1e-10, precond_position=2):
+          renumbering=1, scaling=0, ilu_double=False, rtol=1e-10, precond_position=2,
+          nb_fill=20, droptol=0.0, permtol=0.05):
     n = ia.shape[0] - 1
     a = np.array(a, np.float64)
     b = np.array(b, np.float64)
     x = np.zeros(n)
     p = Params(algorithm, preconditioner, precond_position, krylov, maxits, renumbering,
-               scaling, int(ilu_double), rtol)
+               scaling, int(ilu_double), rtol, nb_fill, droptol, permtol)
     info = Info()
     r = lib().orc_solve(n, _i(ia), _i(ja), _d(a), _d(b), _d(x), C.byref(p), C.byref(info))
     return x, info.as_dict(), r
@@ -169,7 +188,8 @@ def to1(ia, ja):
 
 
 def ref_solve(ia, ja, a, b, algorithm=8, preconditioner=6, krylov=30, maxits=100000,
-              renumbering=1, scaling=0, ilu_double=False, rtol=1e-10, precond_position=2):
+              renumbering=1, scaling=0, ilu_double=False, rtol=1e-10, precond_position=2,
+              nb_fill=20, droptol=0.0, permtol=0.05):
     R = ref(ilu_double)
     if R is None:
         return None
@@ -179,7 +199,7 @@ def ref_solve(ia, ja, a, b, algorithm=8, preconditioner=6, krylov=30, maxits=100
     b = np.array(b, np.float64)
     x = np.zeros(n)
     p = RefParams(algorithm, preconditioner, precond_position, krylov, maxits, renumbering,
-                  scaling, rtol)
+                  scaling, rtol, nb_fill, droptol, permtol)
     info = RefInfo()
     r = R.ref_solve(n, _d(a), _i(ja1), _i(ia1), _d(b), _d(x), C.byref(p), C.byref(info))
     return x, info.as_dict(), r
@@ -199,6 +219,27 @@ def ref_ilu0(ia, ja, a, double=False):
     ierr = R.ref_ilu0(n, _d(a), _i(ja1), _i(ia1), alu.ctypes.data_as(_dp if double else _fp),
                       _i(jlu), _i(ju))
     return alu, jlu, ju, ierr
+
+
+def ref_ilut(ia, ja, a, lfil=20, droptol=0.0, permtol=0.05, pivot=False, double=False, iwk=None):
+    """The reference's ilut / ilutp, converted to the oracle's 0-based MSR."""
+    R = ref(double)
+    if R is None:
+        return None
+    n = ia.shape[0] - 1
+    iwk = iwk or 2 * (n + 1) * (lfil + 1)
+    ia1, ja1 = to1(ia, ja)
+    alu = np.zeros(iwk + 2, np.float64 if double else np.float32)
+    jlu = np.zeros(iwk + 2, np.int32)
+    ju = np.zeros(n + 1, np.int32)
+    iperm = np.zeros(2 * n, np.int32)
+    ierr = R.ref_ilut(n, _d(np.ascontiguousarray(a, np.float64)), _i(ja1), _i(ia1), int(lfil),
+                      C.c_double(droptol), C.c_double(permtol), int(pivot), int(iwk),
+                      alu.ctypes.data_as(_dp if double else _fp), _i(jlu), _i(ju), _i(iperm))
+    used = int(jlu[n]) - 1 if ierr == 0 else 0
+    jlu0 = jlu.copy()
+    jlu0[:max(used, 0)] -= 1
+    return alu, jlu0, ju - 1, iperm - 1, ja1 - 1, ierr
 
 
 def ref_lusol(alu, jlu, ju, y):

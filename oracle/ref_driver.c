@@ -26,6 +26,11 @@ extern void atmux_(int *n, double *x, double *y, double *a, int *ja, int *ia);
 extern void ilu0_(int *n, double *a, int *ja, int *ia, sscalar *alu, int *jlu,
                   int *ju, int *iw, int *ierr);
 extern void lusol_(int *n, double *y, double *x, sscalar *alu, int *jlu, int *ju);
+extern void ilut_(int *n, double *a, int *ja, int *ia, int *lfil, double *droptol,
+                  sscalar *alu, int *jlu, int *ju, int *iwk, double *w, int *jw, int *ierr);
+extern void ilutp_(int *n, double *a, int *ja, int *ia, int *lfil, double *droptol,
+                   double *permtol, int *mbloc, sscalar *alu, int *jlu, int *ju, int *iwk,
+                   double *w, int *jw, int *iperm, int *ierr);
 extern void lutsol_(int *n, double *y, double *x, sscalar *alu, int *jlu,
                     int *ju);
 extern void cg_(int *n, double *rhs, double *sol, int *ipar, double *fpar,
@@ -55,12 +60,14 @@ int ref_sscalar_size(void) { return (int)sizeof(sscalar); }
 
 /* Sparskit.h algorithm / preconditioner codes (contrib/Sparskit/Sparskit.h). */
 enum { A_CG = 1, A_BCGSTAB = 5, A_TFQMR = 6, A_GMRES = 8, A_FGMRES = 9 };
-enum { P_NONE = 0, P_ILU0 = 6, P_DIAGONAL = 8 };
+enum { P_NONE = 0, P_ILUT = 1, P_ILUTP = 2, P_ILU0 = 6, P_DIAGONAL = 8 };
 
 typedef struct {
   int algorithm, preconditioner, precond_position, krylov, maxits;
   int renumbering, scaling;
   double rtol;
+  int nb_fill;
+  double droptol, permtol;
 } ref_params;
 
 typedef struct {
@@ -97,6 +104,26 @@ int ref_ilu0(int n, double *a, int *ja, int *ia, sscalar *alu, int *jlu, int *ju
   int *iw = (int *)malloc(sizeof(int) * (size_t)(n + 1));
   ilu0_(&n, a, ja, ia, alu, jlu, ju, iw, &ierr);
   free(iw);
+  return ierr;
+}
+
+/* ilut / ilutp on a 1-based CSR matrix (ja permuted in place by ilutp);
+ * iwk entries of room; iperm (2n) for ilutp. */
+int ref_ilut(int n, double *a, int *ja, int *ia, int lfil, double droptol, double permtol,
+             int pivot, int iwk, sscalar *alu, int *jlu, int *ju, int *iperm)
+{
+  int ierr = 0;
+  double *w = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+  int *jw = (int *)malloc(sizeof(int) * 2 * (size_t)(n + 1));
+  if(pivot) {
+    int mbloc = n;
+    ilutp_(&n, a, ja, ia, &lfil, &droptol, &permtol, &mbloc, alu, jlu, ju, &iwk, w, jw, iperm,
+           &ierr);
+  }
+  else
+    ilut_(&n, a, ja, ia, &lfil, &droptol, alu, jlu, ju, &iwk, w, jw, &ierr);
+  free(w);
+  free(jw);
   return ierr;
 }
 
@@ -162,8 +189,37 @@ int ref_solve(int n, double *a, int *ja, int *ia, double *b, double *x,
   }
 
   sscalar *alu = NULL;
-  int *jlu = NULL, *ju = NULL;
-  if(p->preconditioner == P_ILU0) {
+  int *jlu = NULL, *ju = NULL, *permp = NULL;
+  if(p->preconditioner == P_ILUT || p->preconditioner == P_ILUTP) {
+    /* Sparskit.cpp:312-317, 375-390, 435-446: first guess, grow by half on
+     * -2/-3 and factor again */
+    int nnz_ilu = 2 * (n + 1) * (p->nb_fill + 1);
+    permp = (int *)malloc(sizeof(int) * 2 * (size_t)n);
+    int *ja_keep = (int *)malloc(sizeof(int) * (size_t)nnz);
+    memcpy(ja_keep, JA, sizeof(int) * (size_t)nnz);
+    for(;;) {
+      alu = (sscalar *)malloc(sizeof(sscalar) * (size_t)nnz_ilu);
+      jlu = (int *)malloc(sizeof(int) * (size_t)nnz_ilu);
+      ju = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+      memcpy(JA, ja_keep, sizeof(int) * (size_t)nnz);
+      int ierr = ref_ilut(n, A, JA, IA, p->nb_fill, p->droptol, p->permtol,
+                          p->preconditioner == P_ILUTP, nnz_ilu, alu, jlu, ju, permp);
+      if(ierr == -2 || ierr == -3) {
+        nnz_ilu += nnz_ilu / 2;
+        free(alu);
+        free(jlu);
+        free(ju);
+        continue;
+      }
+      info->ilu_ierr = ierr;
+      break;
+    }
+    free(ja_keep);
+    if(p->preconditioner == P_ILUT)
+      for(int i = 0; i < n; i++) permp[n + i] = i + 1;
+    if(info->ilu_ierr) goto done;
+  }
+  else if(p->preconditioner == P_ILU0) {
     int nnz_ilu = nnz + n + 1;
     alu = (sscalar *)malloc(sizeof(sscalar) * (size_t)nnz_ilu);
     jlu = (int *)malloc(sizeof(int) * (size_t)nnz_ilu);
@@ -247,7 +303,8 @@ int ref_solve(int n, double *a, int *ja, int *ia, double *b, double *x,
     free(w2);
     free(w);
   }
-  for(int i = 0; i < n; i++) x[i] = sol[permr[i] - 1];
+  for(int i = 0; i < n; i++) /* Sparskit.cpp:597-602 */
+    x[i] = permp ? sol[permp[n + permr[i] - 1] - 1] : sol[permr[i] - 1];
   if(rowscal)
     for(int i = 0; i < n; i++) x[i] *= rowscal[i];
 
@@ -262,6 +319,7 @@ done:
   free(alu);
   free(jlu);
   free(ju);
+  free(permp);
   free(rowscal);
   return info->ilu_ierr ? info->ilu_ierr : 0;
 }

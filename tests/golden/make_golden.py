@@ -9,8 +9,8 @@ the outputs are what the reference's kernels return for them: the RCMK
 permutation (cmkreord + sortcol), the FP32 ILU(0) factors in MSR form (ilu0),
 and the solutions / iteration counts / status of solve_matrix's sequence for
 GMRES(30), FGMRES(30), CG with diagonal scaling, GMRES(10) without
-renumbering, BCGSTAB + ILU(0), BCGSTAB with diagonal scaling, TFQMR + ILU(0) and
-unpreconditioned TFQMR.
+renumbering, BCGSTAB + ILU(0), BCGSTAB with diagonal scaling, TFQMR + ILU(0),
+unpreconditioned TFQMR, GetDP's default ILUTP(20)-GMRES(40) and ILUT(5, 1e-3).
 """
 import sys
 from pathlib import Path
@@ -36,6 +36,8 @@ SOLVES = {
     "bcgstab_jacobi": dict(algorithm=5, preconditioner=0, scaling=1),
     "tfqmr_ilu0": dict(algorithm=6, preconditioner=6),
     "tfqmr_none": dict(algorithm=6, preconditioner=0),
+    "gmres40_ilutp_default": dict(algorithm=8, preconditioner=2, krylov=40),
+    "gmres40_ilut_fill5": dict(algorithm=8, preconditioner=1, krylov=40, nb_fill=5, droptol=1e-3),
 }
 
 

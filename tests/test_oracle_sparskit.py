@@ -19,6 +19,8 @@ SOLVES = {
     "bcgstab_jacobi": dict(algorithm=5, preconditioner=0, scaling=1),
     "tfqmr_ilu0": dict(algorithm=6, preconditioner=6),
     "tfqmr_none": dict(algorithm=6, preconditioner=0),
+    "gmres40_ilutp_default": dict(algorithm=8, preconditioner=2, krylov=40),
+    "gmres40_ilut_fill5": dict(algorithm=8, preconditioner=1, krylov=40, nb_fill=5, droptol=1e-3),
 }
 
 
@@ -182,3 +184,55 @@ def test_zero_rhs_matches_reference(oracle, golden, algorithm, status):
     if oracle.ref() is not None:
         xr, ir, _ = oracle.ref_solve(ia, ja, a, np.zeros_like(b), algorithm=algorithm, preconditioner=6)
         assert ir["status"] == status and ir["its"] == 1
+
+
+def _ilut_cases(oracle, golden):
+    rng = np.random.default_rng(21)
+    out = []
+    for boost in (1.0, 0.05):  # weak diagonals make ilutp permute columns
+        ia, ja, a = _random_csr(rng, 180, 600, diag_boost=boost)
+        out.append((ia, ja, a))
+    for name in CASES:
+        ia, ja, a, b = _sys(golden, name)
+        perm, rperm, ib, jb, bb, nlev = oracle.rcmk(ia, ja, a)
+        out.append((ib, jb, bb))
+    return out
+
+
+@pytest.mark.parametrize("pivot", [False, True])
+@pytest.mark.parametrize("double", [False, True])
+def test_ilut_ilutp_match_reference(oracle, golden, pivot, double):
+    """ilut / ilutp factors (values, columns, row pointers, permutation and
+    the in-place column permutation of the matrix) bit for bit."""
+    if oracle.ref() is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    permuted = 0
+    for ia, ja, a in _ilut_cases(oracle, golden):
+        n = ia.shape[0] - 1
+        for lfil, droptol, permtol in ((20, 0.0, 0.05), (5, 1e-3, 0.05), (3, 0.0, 0.5)):
+            o = oracle.ilut(ia, ja, a, lfil, droptol, permtol, pivot, double)
+            r = oracle.ref_ilut(ia, ja, a, lfil, droptol, permtol, pivot, double)
+            assert o[5] == r[5] == 0
+            used = int(o[1][n])
+            v = np.uint64 if double else np.uint32
+            assert np.array_equal(o[0][:used].view(v), r[0][:used].view(v))
+            assert np.array_equal(o[1][:used], r[1][:used])
+            assert np.array_equal(o[2][:n], r[2][:n])
+            if pivot:
+                assert np.array_equal(o[3], r[3]) and np.array_equal(o[4], r[4])
+                permuted += int(np.any(o[3][:n] != np.arange(n)))
+    if pivot:
+        assert permuted > 0  # the pivoting path was exercised
+
+
+def test_ilutp_solve_with_pivoting_matches_reference(oracle):
+    if oracle.ref() is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(5)
+    ia, ja, a = _random_csr(rng, 150, 300, diag_boost=0.3)
+    b = rng.standard_normal(150)
+    for kw in (dict(preconditioner=2, permtol=0.5), dict(preconditioner=1)):
+        xo, io, _ = oracle.solve(ia, ja, a, b, algorithm=8, krylov=40, maxits=400, renumbering=0, **kw)
+        xr, ir, _ = oracle.ref_solve(ia, ja, a, b, algorithm=8, krylov=40, maxits=400, renumbering=0, **kw)
+        assert io["its"] == ir["its"] and io["status"] == ir["status"]
+        assert np.array_equal(xo, xr)
