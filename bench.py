@@ -176,6 +176,45 @@ def c2_section(n: int = 48, its64: int = 1500):
     }
 
 
+def default_solver_section(n: int = 512):
+    """GetDP's default linear solver (Preconditioner 2 = ILUTP, Nb_Fill 20,
+    Permutation_Tolerance 0.05, no dropping; GMRES(40); RCMK) on the 2D
+    Poisson problem poisson2d_n, once, outside the timed steps: set-up (RCMK,
+    host ILUTP factorisation, upload of the factors) and the device GMRES."""
+    from getdp_amd import linalg as LA
+    from getdp_amd import problems as P
+    pr = P.poisson2d(n)
+    fe = LA.FESystem(pr)
+    A, b, x = LA.Matrix(fe.N), LA.Vector(fe.N), LA.Vector(fe.N)
+    fe.generate(A, b)
+    s = LA.Solver()  # every parameter at its default
+    info = LA.solve(A, b, s, x)
+    again = LA.solve_again(A, b, s, x)  # factors kept: the GMRES phase alone
+    apply_ms = LA.solver_prec_apply(s, b, LA.Vector(fe.N), reps=5)
+    res = {"workload": f"poisson2d_{n}: {fe.N} DOFs, nnz {fe.nnz}", "its": info["its"],
+           "status": info["status"], "rel_res": info["rel_res"],
+           "setup_ms": round(info["t_setup_ms"], 1), "solve_ms": round(info["t_solve_ms"], 1),
+           "solve_again_ms": round(again["t_solve_ms"], 1), "ilu_levels": info["ilu_levels"],
+           "ilu_apply_ms": round(apply_ms, 3),
+           "total_ms": round(info["t_setup_ms"] + info["t_solve_ms"], 1)}
+    log(f"default solver (ILUTP(20)-GMRES(40)) poisson2d_{n}: its={info['its']} status={info['status']} "
+        f"setup {info['t_setup_ms']:.0f} ms solve {info['t_solve_ms']:.0f} ms")
+    s.close()
+    return res, pr
+
+
+def default_solver_cpu(pr):
+    """The oracle's solve_matrix with the same defaults on the same system
+    (single thread): the reference's CPU path for this solver."""
+    from oracle import pyoracle as O
+    ia, ja, a, b, tab = O.assemble(pr)
+    t0 = time.perf_counter()
+    x, info, _ = O.solve(ia, ja, a, b, algorithm=8, preconditioner=2, krylov=40, maxits=1000,
+                         renumbering=1)
+    return {"value_ms": round(1e3 * (time.perf_counter() - t0), 1), "its": info["its"],
+            "status": info["status"], "cores": 1, "kind": "port"}
+
+
 def batch_section(count: int = 512):
     from getdp_amd import batch as BT
     return BT.bench(count, reps=1, log=log)
@@ -219,6 +258,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain line")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-instance batch line")
     ap.add_argument("--no-c2", action="store_true", help="skip the curl-curl mixed-precision line")
+    ap.add_argument("--no-default-solver", action="store_true",
+                    help="skip the ILUTP(20)-GMRES(40) line (GetDP's default solver)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the SpMV launches and a calibration copy (for PMC runs)")
     args = ap.parse_args()
@@ -374,9 +415,14 @@ def main():
         out["batch"] = batch_section()
     if not args.no_c2:
         out["c2_mixed_precision"] = c2_section()
+    dpr = None
+    if not args.no_default_solver:
+        out["default_solver"], dpr = default_solver_section()
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(pr, out["cg_its"], args.cg_sample)
+            if dpr is not None:
+                out["default_solver"]["cpu"] = default_solver_cpu(dpr)
         except Exception as e:  # the oracle is a checker; report, do not fail
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out))

@@ -45,6 +45,7 @@ SOURCES = [
     ("batch.hip", []),
     ("edges.hip", []),
     ("host_prep.cpp", []),
+    ("ilut.cpp", ["-ffp-contract=off"]),
     ("mesh_io.cpp", []),
     ("solver_params.cpp", []),
 ]

@@ -45,6 +45,19 @@ void permute_pattern(int n, const int *ia, const int *ja, const int *perm,
                      std::vector<int> &ib, std::vector<int> &jb,
                      std::vector<int> &map);
 
+// ---- ILUT / ILUTP factorisation (ilut.cpp) ------------------------------------
+// Row i occupies [ptr[i], ptr[i+1]): its L entries, then at dpos[i] the
+// inverted pivot (column i), then its U entries, each part in the order the
+// triangular sweeps accumulate it; columns in the permuted numbering.
+template <typename T> struct IlutRows {
+  std::vector<int> ptr, dpos, col;
+  std::vector<T> val;
+  std::vector<int> newpos;  // ILUTP: permuted number of each column (empty: none)
+};
+template <typename T>
+int ilut_factor(int n, const int *ia, const int *ja, const double *a, int lfil, double droptol,
+                double permtol, bool pivot, IlutRows<T> &f);
+
 // ---- solver parameters (solver_params.cpp) ---------------------------------
 struct Params {
   int Matrix_Format = 1, Matrix_Printing = 0, Matrix_Storage = 0;

@@ -15,18 +15,27 @@ struct IluFactor {
   bool is_f64 = false;
   int nlevels = 0, upper_nlevels = 0;
   int g_lower = 1, g_upper = 1;             // lanes per row in the two sweeps
-  int c_lower = 1, c_upper = 1;             // entries per lane in flight
   int maxrow = 1;                           // longest row of the pattern
   std::vector<int> level_ptr;               // host: rows of level l
   int *level_rows = nullptr;
   double *tmp64 = nullptr;                  // forward-sweep output (sentinel-filled)
   float *tmp32 = nullptr;
   int *err = nullptr;                       // [0] zero pivot, [1] timeout, [2] missing diag
+  // factors computed on the host (ILUT / ILUTP): the object owns their pattern
+  int *own_ia = nullptr, *own_ja = nullptr;
+  // ILUTP: entry j of an application's output is entry colmap[j] of the
+  // backward sweep's result, which is staged in pbuf64 / pbuf32
+  int *colmap = nullptr;
+  double *pbuf64 = nullptr;
+  float *pbuf32 = nullptr;
 
   ~IluFactor() { release(); }
   void release();
   // Dependency levels and diagonal positions for pattern B (kept by pointer).
   int analyse(const DevCSR &B, const std::vector<int> &hia, const std::vector<int> &hja);
+  // Take factors computed on the host, in their row layout; storage FP64
+  // when f64, else FP32 (the values are converted once).
+  template <typename T> int load(const IlutRows<T> &f, bool f64);
   // Factor values a (B's CSR layout); FP32 or FP64 storage.
   int factor(const double *a, bool f64, int *zero_pivot_row);
   // The same factorisation, enqueued only (no allocation, copy or host
@@ -45,6 +54,9 @@ struct IluFactor {
   int check_timeout();
   unsigned long long wait_limit() const;
   template <typename T> void enqueue_levels(T *lu);
+
+ private:
+  int setup_sweeps(const int *hia, const int *hja);
 };
 
 }  // namespace gdp

@@ -149,17 +149,17 @@ template <> struct Sentinel<float> {
   static constexpr U bits = 0x7F80BEEFu;  // signalling NaN
 };
 
-template <typename T> __device__ inline typename Sentinel<T>::U ld_bits(const T *p)
+template <typename T, int SCOPE = __HIP_MEMORY_SCOPE_AGENT> __device__ inline typename Sentinel<T>::U ld_bits(const T *p)
 {
   using U = typename Sentinel<T>::U;
-  return __hip_atomic_load((const U *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load((const U *)p, __ATOMIC_RELAXED, SCOPE);
 }
-template <typename T> __device__ inline void st_val(T *p, T v)
+template <typename T, int SCOPE = __HIP_MEMORY_SCOPE_AGENT> __device__ inline void st_val(T *p, T v)
 {
   using U = typename Sentinel<T>::U;
   U b;
   __builtin_memcpy(&b, &v, sizeof(T));
-  __hip_atomic_store((U *)p, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((U *)p, b, __ATOMIC_RELAXED, SCOPE);
 }
 template <typename T> __device__ inline T as_val(typename Sentinel<T>::U b)
 {
@@ -208,14 +208,14 @@ __device__ inline bool wait_expired(unsigned long long &t0, unsigned long long t
 // the sentinel; the backward sweep resets y to the sentinel as it consumes it.
 template <typename TV> __device__ inline TV lane_read(TV v, int src) { return __shfl(v, src, kWave); }
 
-template <typename TF, typename TV, int G, int C, bool FWD>
-__global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                                               const int *__restrict__ diag, const TF *__restrict__ lu,
-                                               const TV *__restrict__ r, TV scale, TV *y, TV *x, int *err,
-                                               unsigned long long tlimit, const int *gate, int gmode,
-                                               TV *addto)
+// Rows base .. base + 64/G - 1 (in dependency order) by one wave; SCOPE is
+// the memory scope of the publications and polls.
+template <typename TF, typename TV, int G, int C, bool FWD, int SCOPE>
+__device__ inline void sweep_rows(int base, int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                  const int *__restrict__ diag, const TF *__restrict__ lu,
+                                  const TV *__restrict__ r, TV scale, TV *y, TV *x, int *err,
+                                  unsigned long long tlimit, TV *addto)
 {
-  if(is_gated(gate, gmode)) return;  // uniform across the grid
   constexpr int R = kWave / G;  // rows per wave
   constexpr int S = G * C;      // entries of a row in flight at once
   constexpr unsigned long long GM = (G == 64) ? ~0ull : ((1ull << G) - 1);
@@ -223,7 +223,6 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
   TV *out = FWD ? y : x;
   {
     // rows t = base .. base + R - 1 of this wave, in dependency order
-    const int base = (blockIdx.x * (256 / kWave) + (int)(threadIdx.x / kWave)) * R;
     const int t = base + lane / G;
     const bool valid = t < n;
     const int i = FWD ? t : n - 1 - t;
@@ -235,7 +234,7 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
         acc = r[i] * scale;
         cb = ia[i];
         kend = di;
-        if(q == 0) st_val(x + i, as_val<TV>(Sentinel<TV>::bits));
+        if(q == 0) st_val<TV, SCOPE>(x + i, as_val<TV>(Sentinel<TV>::bits));
       }
       else {
         acc = y[i];
@@ -245,7 +244,7 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
     }
     // every lane of the row has read y[i] (one instruction) before lane 0
     // resets it for the next forward sweep
-    if(!FWD && valid && q == 0) st_val(y + i, as_val<TV>(Sentinel<TV>::bits));
+    if(!FWD && valid && q == 0) st_val<TV, SCOPE>(y + i, as_val<TV>(Sentinel<TV>::bits));
     // a lane holds positions cb + q + G*c, c < C, of the row's current span of
     // S entries; their columns and factor entries are loaded when the span
     // starts, off the dependency chain of the polls
@@ -269,10 +268,10 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
     if(valid) fetch();
     auto publish = [&]() {
       if(q == 0) {
-        if(FWD) st_val(y + i, acc);
+        if(FWD) st_val<TV, SCOPE>(y + i, acc);
         else {
           const TV xi = ldi * acc;
-          st_val(x + i, xi);
+          st_val<TV, SCOPE>(x + i, xi);
           if(addto) addto[i] = addto[i] + xi;  // fused z += x of the caller
         }
       }
@@ -289,7 +288,7 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
 #pragma unroll
         for(int c = 0; c < C; c++) {
           if(pend & (1u << c)) {
-            auto b = ld_bits(out + jk[c]);
+            auto b = ld_bits<TV, SCOPE>(out + jk[c]);
             if(b != Sentinel<TV>::bits) {
               prod[c] = lk[c] * as_val<TV>(b);
               pend &= ~(1u << c);
@@ -337,6 +336,19 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
   }
 }
 
+template <typename TF, typename TV, int G, int C, bool FWD>
+__global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                               const int *__restrict__ diag, const TF *__restrict__ lu,
+                                               const TV *__restrict__ r, TV scale, TV *y, TV *x, int *err,
+                                               unsigned long long tlimit, const int *gate, int gmode,
+                                               TV *addto)
+{
+  if(is_gated(gate, gmode)) return;  // uniform across the grid
+  const int base = (blockIdx.x * (256 / kWave) + (int)(threadIdx.x / kWave)) * (kWave / G);
+  sweep_rows<TF, TV, G, C, FWD, __HIP_MEMORY_SCOPE_AGENT>(base, n, ia, ja, diag, lu, r, scale, y, x, err, tlimit,
+                                                          addto);
+}
+
 template <typename T> __global__ void k_fill_bits(long n, T *p)
 {
   long i = (long)blockIdx.x * 256 + threadIdx.x;
@@ -360,6 +372,20 @@ __global__ void k_diag_pos(int n, const int *ia, const int *ja, int *diag, int *
   }
 }
 
+// ILUTP's column permutation of the factors: x[j] = t[colmap[j]] (and
+// addto[j] += x[j] when given).
+template <typename TV>
+__global__ void __launch_bounds__(256) k_colperm(int n, const TV *__restrict__ t, const int *__restrict__ colmap,
+                                                 TV *__restrict__ x, TV *addto, const int *gate, int gmode)
+{
+  if(is_gated(gate, gmode)) return;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if(j >= n) return;
+  const TV v = t[colmap[j]];
+  x[j] = v;
+  if(addto) addto[j] = addto[j] + v;
+}
+
 }  // namespace
 
 // ---- host side ------------------------------------------------------------------------
@@ -373,8 +399,69 @@ void IluFactor::release()
   dfree(tmp64);
   dfree(tmp32);
   dfree(err);
+  dfree(own_ia);
+  dfree(own_ja);
+  dfree(colmap);
+  dfree(pbuf64);
+  dfree(pbuf32);
+  ia = ja = nullptr;
   level_ptr.clear();
   n = 0;
+}
+
+// Levels, lanes per row and the sweep buffers for the host pattern hia/hja
+// (strictly lower entries first in each row, strictly upper entries last).
+int IluFactor::setup_sweeps(const int *hia, const int *hja)
+{
+  std::vector<int> lev;
+  nlevels = lower_levels(n, hia, hja, lev);
+  std::vector<int> cnt(nlevels + 1, 0);
+  for(int i = 0; i < n; i++) cnt[lev[i] + 1]++;
+  for(int l = 0; l < nlevels; l++) cnt[l + 1] += cnt[l];
+  level_ptr = cnt;
+  std::vector<int> rows(n);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for(int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
+  std::vector<int> ulev;
+  upper_nlevels = upper_levels(n, hia, hja, ulev);
+  // lanes per row of each sweep: the power of two at or above the mean count
+  // of strictly lower (upper) entries, 1 below 1.5 entries.  When the levels
+  // hold many rows (short dependency chains: ILU(0) in RCMK order) the sweep
+  // is throughput-bound and the groups stop at 8 lanes (elasticity 32^3, 20
+  // entries a side: 3.4 ms per application with 8 lanes, 3.2 with 16, 3.9
+  // with 32; Poisson 256^2: 2.0 ms with 8, 3.4 with 32).  When the chain is
+  // long (under 64 rows a level: ILUT/ILUTP with fill) its latency dominates
+  // and a group as wide as the row takes all its entries in one span
+  // (ILUTP(20) on Poisson 256^2, 20 entries a side, 10363 levels: 58.8 ms
+  // with 8 lanes, 52.8 with 16, 40.8 with 32).
+  long nl = 0, nu = 0;
+  for(int i = 0; i < n; i++)
+    for(int k = hia[i]; k < hia[i + 1]; k++) {
+      if(hja[k] < i) nl++;
+      else if(hja[k] > i) nu++;
+    }
+  const int cap = (long)nlevels * 64 > n ? 32 : 8;
+  auto lanes = [&](long c) {
+    const double avg = n ? (double)c / n : 0.;
+    if(avg < 1.5) return 1;
+    int g = 2;
+    while(g < avg && g < cap) g *= 2;
+    return g;
+  };
+  maxrow = 1;
+  for(int i = 0; i < n; i++) maxrow = std::max(maxrow, hia[i + 1] - hia[i]);
+  g_lower = lanes(nl);
+  g_upper = lanes(nu);
+  if(const char *e = std::getenv("GDP_ILU_LANES")) g_lower = g_upper = std::max(1, std::min(32, atoi(e)));
+  GDP_TRY(dalloc(&level_rows, (size_t)std::max(1, n)));
+  GDP_TRY(h2d(level_rows, rows.data(), (size_t)n));
+  GDP_TRY(dalloc(&diag, (size_t)std::max(1, n)));
+  GDP_TRY(dalloc(&err, 4));
+  int init[4] = {0x7fffffff, 0, 0x7fffffff, 0};
+  GDP_TRY(h2d(err, init, 4));
+  GDP_TRY(dalloc(&tmp64, (size_t)std::max(1, n)));
+  GDP_TRY(dalloc(&tmp32, (size_t)std::max(1, n)));
+  return GDP_OK;
 }
 
 int IluFactor::analyse(const DevCSR &B, const std::vector<int> &hia, const std::vector<int> &hja)
@@ -384,47 +471,7 @@ int IluFactor::analyse(const DevCSR &B, const std::vector<int> &hia, const std::
   nnz = B.nnz;
   ia = B.ia;
   ja = B.ja;
-  std::vector<int> lev;
-  nlevels = lower_levels(n, hia.data(), hja.data(), lev);
-  std::vector<int> cnt(nlevels + 1, 0);
-  for(int i = 0; i < n; i++) cnt[lev[i] + 1]++;
-  for(int l = 0; l < nlevels; l++) cnt[l + 1] += cnt[l];
-  level_ptr = cnt;
-  std::vector<int> rows(n);
-  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-  for(int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
-  std::vector<int> ulev;
-  upper_nlevels = upper_levels(n, hia.data(), hja.data(), ulev);
-  // lanes per row of each sweep: the power of two at or above the mean count
-  // of strictly lower (upper) entries, at most 8 (wider groups spend more on
-  // cross-lane reads than they save in waits: elasticity 32^3, 20 entries a
-  // side, takes 4.0 ms per application with 8 lanes, 5.1 with 16, 11.1 with
-  // 32, 13.0 with 1), 1 below 1.5 entries
-  long nl = 0, nu = 0;
-  for(int i = 0; i < n; i++)
-    for(int k = hia[i]; k < hia[i + 1]; k++) {
-      if(hja[k] < i) nl++;
-      else if(hja[k] > i) nu++;
-    }
-  auto lanes = [&](long cnt) {
-    const double avg = n ? (double)cnt / n : 0.;
-    if(avg < 1.5) return 1;
-    int g = 2;
-    while(g < avg && g < 8) g *= 2;
-    return g;
-  };
-  maxrow = 1;
-  for(int i = 0; i < n; i++) maxrow = std::max(maxrow, hia[i + 1] - hia[i]);
-  g_lower = lanes(nl);
-  g_upper = lanes(nu);
-  if(const char *e = std::getenv("GDP_ILU_LANES")) g_lower = g_upper = std::max(1, std::min(32, atoi(e)));
-  c_lower = c_upper = 1;
-  GDP_TRY(dalloc(&level_rows, (size_t)std::max(1, n)));
-  GDP_TRY(h2d(level_rows, rows.data(), (size_t)n));
-  GDP_TRY(dalloc(&diag, (size_t)std::max(1, n)));
-  GDP_TRY(dalloc(&err, 4));
-  int init[4] = {0x7fffffff, 0, 0x7fffffff, 0};
-  GDP_TRY(h2d(err, init, 4));
+  GDP_TRY(setup_sweeps(hia.data(), hja.data()));
   if(n)
     hipLaunchKernelGGL(k_diag_pos, dim3((n + 255) / 256), dim3(256), 0, ctx().stream, n, ia, ja,
                        diag, err + 2);
@@ -433,11 +480,45 @@ int IluFactor::analyse(const DevCSR &B, const std::vector<int> &hia, const std::
   GDP_TRY(d2h(h, err, 4));
   if(h[2] != 0x7fffffff)
     return set_error(GDP_E_ZERO_PIVOT, "Zero pivot on line %d in ILU (missing diagonal)", h[2]);
-  GDP_TRY(dalloc(&tmp64, (size_t)std::max(1, n)));
-  GDP_TRY(dalloc(&tmp32, (size_t)std::max(1, n)));
   GDP_TRY(reset_sweep_state());
   return GDP_OK;
 }
+
+template <typename T> int IluFactor::load(const IlutRows<T> &f, bool f64)
+{
+  release();
+  n = (int)f.dpos.size();
+  nnz = (long)f.col.size();
+  GDP_TRY(setup_sweeps(f.ptr.data(), f.col.data()));
+  GDP_TRY(dalloc(&own_ia, (size_t)n + 1));
+  GDP_TRY(dalloc(&own_ja, (size_t)std::max(1L, nnz)));
+  GDP_TRY(h2d(own_ia, f.ptr.data(), (size_t)n + 1));
+  GDP_TRY(h2d(own_ja, f.col.data(), (size_t)nnz));
+  GDP_TRY(h2d(diag, f.dpos.data(), (size_t)n));
+  ia = own_ia;
+  ja = own_ja;
+  is_f64 = f64;
+  if(f64) {
+    std::vector<double> v(f.val.begin(), f.val.end());
+    GDP_TRY(dalloc(&lu64, (size_t)std::max(1L, nnz)));
+    GDP_TRY(h2d(lu64, v.data(), (size_t)nnz));
+  }
+  else {
+    std::vector<float> v(f.val.begin(), f.val.end());
+    GDP_TRY(dalloc(&lu32, (size_t)std::max(1L, nnz)));
+    GDP_TRY(h2d(lu32, v.data(), (size_t)nnz));
+  }
+  if(!f.newpos.empty()) {
+    GDP_TRY(dalloc(&colmap, (size_t)std::max(1, n)));
+    GDP_TRY(h2d(colmap, f.newpos.data(), (size_t)n));
+    GDP_TRY(dalloc(&pbuf64, (size_t)std::max(1, n)));
+    GDP_TRY(dalloc(&pbuf32, (size_t)std::max(1, n)));
+  }
+  return reset_sweep_state();
+}
+
+template int IluFactor::load<float>(const IlutRows<float> &, bool);
+template int IluFactor::load<double>(const IlutRows<double> &, bool);
 
 int IluFactor::reset_sweep_state()
 {
@@ -543,14 +624,13 @@ unsigned long long IluFactor::wait_limit() const
 namespace {
 
 template <typename TF, typename TV, bool FWD>
-void sweep(int G, int C, int n, const int *ia, const int *ja, const int *diag, const TF *lu, const TV *r,
+void sweep(int G, int n, const int *ia, const int *ja, const int *diag, const TF *lu, const TV *r,
            TV scale, TV *y, TV *x, int *err, unsigned long long tl, const int *gate, int gmode, TV *addto)
 {
   hipStream_t s = ctx().stream;
   // one entry per lane in flight (C = 1): spans of 2-4 entries per lane were
   // measured no faster (the chain's cost is the publication latency, not the
   // number of polls)
-  (void)C;
   if(G != 2 && G != 4 && G != 8 && G != 16 && G != 32) G = 1;
 #define GDP_SWEEP(GG)                                                                                  \
   hipLaunchKernelGGL((k_sweep<TF, TV, GG, 1, FWD>), dim3((unsigned)(((long)n * GG + 255) / 256)), dim3(256), 0, \
@@ -573,21 +653,31 @@ int IluFactor::apply(const TV *r, TV scale, TV *x, const int *gate, int gmode, T
 {
   hipStream_t s = ctx().stream;
   if(!n) return GDP_OK;
-  TV *t;
-  if constexpr(sizeof(TV) == 8) t = tmp64;
-  else t = tmp32;
-  if(is_f64) {
-    sweep<double, TV, true>(g_lower, c_lower, n, ia, ja, diag, lu64, r, scale, t, x, err + 1, wait_limit(), gate,
-                            gmode, nullptr);
-    sweep<double, TV, false>(g_upper, c_upper, n, ia, ja, diag, lu64, nullptr, scale, t, x, err + 1,
-                             wait_limit(), gate, gmode, addto);
+  TV *t, *out = x, *add = addto;
+  if constexpr(sizeof(TV) == 8) {
+    t = tmp64;
+    if(colmap) out = pbuf64;
   }
   else {
-    sweep<float, TV, true>(g_lower, c_lower, n, ia, ja, diag, lu32, r, scale, t, x, err + 1, wait_limit(), gate,
-                           gmode, nullptr);
-    sweep<float, TV, false>(g_upper, c_upper, n, ia, ja, diag, lu32, nullptr, scale, t, x, err + 1,
-                            wait_limit(), gate, gmode, addto);
+    t = tmp32;
+    if(colmap) out = pbuf32;
   }
+  if(colmap) add = nullptr;
+  if(is_f64) {
+    sweep<double, TV, true>(g_lower, n, ia, ja, diag, lu64, r, scale, t, out, err + 1, wait_limit(),
+                            gate, gmode, nullptr);
+    sweep<double, TV, false>(g_upper, n, ia, ja, diag, lu64, nullptr, scale, t, out, err + 1,
+                             wait_limit(), gate, gmode, add);
+  }
+  else {
+    sweep<float, TV, true>(g_lower, n, ia, ja, diag, lu32, r, scale, t, out, err + 1, wait_limit(),
+                           gate, gmode, nullptr);
+    sweep<float, TV, false>(g_upper, n, ia, ja, diag, lu32, nullptr, scale, t, out, err + 1,
+                            wait_limit(), gate, gmode, add);
+  }
+  if(colmap)
+    hipLaunchKernelGGL(k_colperm<TV>, dim3((n + 255) / 256), dim3(256), 0, s, n, out, colmap, x, addto, gate,
+                       gmode);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }

@@ -156,10 +156,21 @@ struct SolveCache {
   double *rs = nullptr;   // scaling factors (null when not scaling)
   double *rhs = nullptr, *sol = nullptr, *work = nullptr;
   float *rhs32 = nullptr, *sol32 = nullptr;
-  IluFactor ilu;
+  IluFactor ilu;   // ILU(0) on Bm's pattern
+  IluFactor ilut;  // ILUT / ILUTP factors (own pattern)
+  int ilut_fill = -1, ilut_storage = -1;
+  double ilut_drop = -1., ilut_perm = -1.;
   std::unique_ptr<InnerSolver> inner;  // Precision 2
   int ilu_row = 0;
   bool have_factor = false;
+
+  // the factors Preconditioner `prec` applies (null: none)
+  IluFactor *factors(int prec)
+  {
+    if(prec == 6) return &ilu;
+    if(prec == 1 || prec == 2) return &ilut;
+    return nullptr;
+  }
 
   void release()
   {
@@ -177,6 +188,8 @@ struct SolveCache {
     dfree(rhs32);
     dfree(sol32);
     ilu.release();
+    ilut.release();
+    ilut_fill = -1;
     have_factor = false;
     A = nullptr;
   }
@@ -217,7 +230,10 @@ int prepare(gdp_solver *S, const gdp_matrix *A, bool reuse_factor)
                       c.renumbering == p.Renumbering_Technique && c.ordering == p.ILU_Ordering;
   bool same_values = same_pattern && c.value_version == A->value_version &&
                      c.scaling == p.Scaling;
-  if(reuse_factor && same_values && c.precond == prec && c.have_factor) return GDP_OK;
+  const bool same_ilut = (prec != 1 && prec != 2) ||
+                         (c.ilut_fill == p.Nb_Fill && c.ilut_drop == p.Dropping_Tolerance &&
+                          c.ilut_perm == p.Permutation_Tolerance && c.ilut_storage == p.ILU_Storage);
+  if(reuse_factor && same_values && c.precond == prec && c.have_factor && same_ilut) return GDP_OK;
   const int n = A->n;
   const long nnz = A->csr.nnz;
   hipStream_t s = ctx().stream;
@@ -277,7 +293,6 @@ int prepare(gdp_solver *S, const gdp_matrix *A, bool reuse_factor)
     GDP_TRY(dalloc(&c.rhs, (size_t)std::max(1, n)));
     GDP_TRY(dalloc(&c.sol, (size_t)std::max(1, n)));
     GDP_TRY(dalloc(&c.work, (size_t)std::max(1, n)));
-    GDP_TRY(c.ilu.analyse(c.Bm, c.ib, c.jb));
     c.A = A;
     c.pattern_version = A->pattern_version;
     c.renumbering = p.Renumbering_Technique;
@@ -324,6 +339,7 @@ int prepare(gdp_solver *S, const gdp_matrix *A, bool reuse_factor)
   // factorisation
   c.precond = prec;
   if(prec == 6) {
+    if(!c.ilu.diag) GDP_TRY(c.ilu.analyse(c.Bm, c.ib, c.jb));  // first ILU(0) on this pattern
     GDP_TRY(c.ilu.factor(c.Bm.a, p.ILU_Storage == 1, &c.ilu_row));
     if(c.ilu_row) {
       c.have_factor = false;
@@ -331,10 +347,40 @@ int prepare(gdp_solver *S, const gdp_matrix *A, bool reuse_factor)
     }
     c.have_factor = true;
   }
+  else if(prec == 1 || prec == 2) {
+    // ILUT / ILUTP (Sparskit.cpp:375-390) of the renumbered, scaled matrix,
+    // factored on the host; the sweeps run on the device.  ILUTP's column
+    // permutation stays inside the preconditioner: Krylov on A with
+    // M^-1 = Q (LU)^-1, (Q v)[j] = v[newpos[j]], builds the same iterates as
+    // the reference's Krylov on the column-permuted A Q with (LU)^-1, and its
+    // solution is already in A's column numbering.
+    c.have_factor = false;
+    c.ilut_fill = -1;
+    std::vector<double> hv(nnz);
+    GDP_TRY(d2h(hv.data(), c.Bm.a, (size_t)nnz));
+    const bool pivot = prec == 2;
+    if(p.ILU_Storage == 1) {
+      IlutRows<double> f;
+      GDP_TRY(ilut_factor(n, c.ib.data(), c.jb.data(), hv.data(), p.Nb_Fill, p.Dropping_Tolerance,
+                          p.Permutation_Tolerance, pivot, f));
+      GDP_TRY(c.ilut.load(f, true));
+    }
+    else {
+      IlutRows<float> f;
+      GDP_TRY(ilut_factor(n, c.ib.data(), c.jb.data(), hv.data(), p.Nb_Fill, p.Dropping_Tolerance,
+                          p.Permutation_Tolerance, pivot, f));
+      GDP_TRY(c.ilut.load(f, false));
+    }
+    c.ilut_fill = p.Nb_Fill;
+    c.ilut_drop = p.Dropping_Tolerance;
+    c.ilut_perm = p.Permutation_Tolerance;
+    c.ilut_storage = p.ILU_Storage;
+    c.have_factor = true;
+  }
   else if(prec == 0 || prec == 8)
     c.have_factor = true;  // NONE, or DIAGONAL = identity (Sparskit.cpp:338-354)
   else
-    return set_error(GDP_E_UNSUPPORTED, "Preconditioner %d not supported (0, 6, 8)", prec);
+    return set_error(GDP_E_UNSUPPORTED, "Preconditioner %d not supported (0, 1, 2, 6, 8)", prec);
   return GDP_OK;
 }
 
@@ -361,7 +407,8 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
   GDP_TRY(sync());
   double t1 = now_ms();
   info.t_setup_ms = t1 - t0;
-  info.ilu_levels = c.ilu.nlevels;
+  IluFactor *F = c.factors(p.Preconditioner);
+  info.ilu_levels = F ? F->nlevels : 0;
   hipLaunchKernelGGL(k_permute_rhs, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, b->d, c.rs, c.d_rperm, c.rhs);
   GDP_HIP(hipGetLastError());
   double bnorm;
@@ -373,7 +420,7 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
   opt.maxits = p.Nb_Iter_Max;
   opt.rtol = p.Stopping_Test;
   opt.orth = p.Orthogonalization;
-  if(p.Preconditioner_Position != 2 && p.Preconditioner == 6)
+  if(p.Preconditioner_Position != 2 && F)
     return set_error(GDP_E_UNSUPPORTED, "only right preconditioning (Preconditioner_Position 2)");
   KrylovResult kr;
   UserPrec *up = S->user;
@@ -386,9 +433,9 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
     InnerSolver &in = *c.inner;
     in.A = sell_op<float>(c.sell);
     in.M = PrecOp<float>();
-    if(p.Preconditioner == 6) {
+    if(F) {
       in.M.kind = PREC_ILU;
-      in.M.ilu = &c.ilu;
+      in.M.ilu = F;
     }
     if(in.k != p.Inner_Krylov_Size) {
       dfree(in.vec);
@@ -401,7 +448,7 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
     KrylovOptions o2 = opt;
     o2.algorithm = 9;
     GDP_TRY((krylov_solve<double, double>(op, M, c.rhs, c.sol, o2, kr)));
-    if(p.Preconditioner == 6) GDP_TRY(c.ilu.check_timeout());
+    if(F) GDP_TRY(F->check_timeout());
     info.its = kr.its;
     info.status = kr.status;
     info.res0 = kr.res0;
@@ -416,9 +463,9 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
       M.user = up;
       if(p.Preconditioner == 6) M.ilu = &c.ilu;  // network, then ILU(0) correction
     }
-    else if(p.Preconditioner == 6) {
+    else if(F) {
       M.kind = PREC_ILU;
-      M.ilu = &c.ilu;
+      M.ilu = F;
     }
     GDP_TRY((krylov_solve<double, double>(op, M, c.rhs, c.sol, opt, kr)));
     info.its = kr.its;
@@ -439,9 +486,9 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
       M.user = up;
       if(p.Preconditioner == 6) M.ilu = &c.ilu;
     }
-    else if(p.Preconditioner == 6) {
+    else if(F) {
       M.kind = PREC_ILU;
-      M.ilu = &c.ilu;
+      M.ilu = F;
     }
     KrylovOptions in = opt;
     in.rtol = p.Inner_Tolerance;
@@ -581,8 +628,9 @@ extern "C" int gdp_solver_prec_apply(gdp_solver *s, const gdp_vector *in, gdp_ve
 {
   if(!s || !in || !out || reps < 1) return set_error(GDP_E_ARG, "gdp_solver_prec_apply: bad arguments");
   SolveCache &c = s->cache;
-  if(!c.have_factor || c.precond != 6 || !c.n)
-    return set_error(GDP_E_STATE, "the solver holds no ILU(0) factors (no solve with Preconditioner 6)");
+  IluFactor *F = c.factors(c.precond);
+  if(!c.have_factor || !F || !c.n)
+    return set_error(GDP_E_STATE, "the solver holds no ILU factors (no solve with Preconditioner 1, 2 or 6)");
   if(in->n != c.n || out->n != c.n || in == out)
     return set_error(GDP_E_ARG, "gdp_solver_prec_apply: vectors must be distinct and of the system's size");
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -591,14 +639,14 @@ extern "C" int gdp_solver_prec_apply(gdp_solver *s, const gdp_vector *in, gdp_ve
   if(!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     rc = set_error(GDP_E_HIP, "hipEventCreate");
   if(!rc && hipEventRecord(e0, ctx().stream) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventRecord");
-  for(int r = 0; !rc && r < reps; r++) rc = c.ilu.apply<double>(in->d, 1.0, out->d);
+  for(int r = 0; !rc && r < reps; r++) rc = F->apply<double>(in->d, 1.0, out->d);
   if(!rc && hipEventRecord(e1, ctx().stream) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventRecord");
   if(!rc && hipEventSynchronize(e1) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventSynchronize");
   if(!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = set_error(GDP_E_HIP, "hipEventElapsedTime");
   if(e0) (void)hipEventDestroy(e0);
   if(e1) (void)hipEventDestroy(e1);
   GDP_TRY(rc);
-  GDP_TRY(c.ilu.check_timeout());
+  GDP_TRY(F->check_timeout());
   if(avg_ms) *avg_ms = ms / reps;
   return GDP_OK;
 }

@@ -299,6 +299,19 @@ int gdp_number_edges(int nelem, const int *etype, const int *conn, int nvisit,
 /* RCMK renumbering (cmkreord.f:3-62 with perphn/BFS of reordering.f):
  * perm[old] = new, rperm[new] = old (0-based).  Returns #levels or < 0. */
 int gdp_rcmk(int n, const int *ia, const int *ja, int *perm, int *rperm);
+/* ILUT (pivot 0, ilut.F:52-396) or ILUTP (pivot 1, ilut.F:397-806 with
+ * mbloc = n) factors of the 0-based CSR matrix (ia, ja, a), on the host, as
+ * gdp_solve computes them for Preconditioner 1 / 2; factor storage FP32
+ * unless f64 (the values are returned in double, exactly).  Row i occupies
+ * [ptr[i], ptr[i+1]): its L entries, then at dpos[i] the inverted pivot, then
+ * its U entries, each part in the order the triangular sweeps accumulate it;
+ * columns in the permuted numbering, newpos[c] = permuted number of column c
+ * (identity for ILUT).  *nnz_out = number of entries; when it exceeds
+ * capacity only ptr is filled and GDP_E_NOMEM is returned (the reference's
+ * iwk overflow, ierr -2/-3): call again with capacity >= *nnz_out. */
+int gdp_ilut(int n, const int *ia, const int *ja, const double *a, int lfil, double droptol,
+             double permtol, int pivot, int f64, long capacity, int *ptr, int *dpos, int *col,
+             double *val, int *newpos, long *nnz_out);
 
 /* ---- batched solves (no single counterpart in the reference) ----------------
  * Many independent systems solved as the reference solves one (Algorithm 8,

@@ -92,3 +92,32 @@ def test_ilu_apply_lane_counts(gpu, oracle, monkeypatch, lanes):
     want = oracle.lusol(alu, jlu, ju, v)
     got = _device_apply(gpu, ia, ja, a, v, False)
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+
+
+@pytest.mark.parametrize("lanes", ["auto", "8"])
+@pytest.mark.parametrize("pivot", [False, True])
+@pytest.mark.parametrize("f64", [False, True])
+@pytest.mark.parametrize("name", ["poisson2d_32", "elasticity3d_8", "curlcurl3d_4"])
+def test_ilut_apply_bit_exact(gpu, oracle, name, f64, pivot, lanes, monkeypatch):
+    """ILUT / ILUTP (fill 20, the reference's defaults) factored by the
+    library on the host and applied by the device sweeps, against the oracle's
+    ilut/ilutp + lusol; ILUTP's column permutation applied to the output."""
+    if lanes != "auto":
+        monkeypatch.setenv("GDP_ILU_LANES", lanes)
+    print(f"case {name} f64={f64} pivot={pivot} lanes={lanes}", flush=True)
+    ia, ja, a, b, tab = oracle.assemble(SYSTEMS[name]())
+    perm, rperm, ib, jb, bb, nlev = oracle.rcmk(ia, ja, a)
+    n = ib.shape[0] - 1
+    alu, jlu, ju, iperm, _, ierr = oracle.ilut(ib, jb, bb, 20, 0.0, 0.05, pivot, f64)
+    assert ierr == 0
+    v = np.random.default_rng(5).standard_normal(n)
+    want = oracle.lusol(alu, jlu, ju, v)[iperm[n:]] if pivot else oracle.lusol(alu, jlu, ju, v)
+    A = gpu.Matrix(n)
+    A.set_csr(ia, ja, a)
+    s = gpu.Solver(Algorithm=8, Preconditioner=2 if pivot else 1, Krylov_Size=40, ILU_Storage=1 if f64 else 0,
+                   Nb_Iter_Max=3)
+    gpu.solve(A, gpu.Vector(n, np.ones(n)), s, gpu.Vector(n))
+    out = gpu.Vector(n)
+    gpu.solver_prec_apply(s, gpu.Vector(n, v), out, reps=3)
+    got = out.get()
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), np.abs(got - want).max()
