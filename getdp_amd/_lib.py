@@ -106,6 +106,12 @@ SIGNATURES = [
     ("gdp_fe_post", C.c_int, [_vp, _vp, C.POINTER(PostDesc), _ip, _dp, C.c_long, _dp, _dp]),
     ("gdp_number_edges", C.c_int, [C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip, C.c_int]),
     ("gdp_rcmk", C.c_int, [C.c_int, _ip, _ip, _ip, _ip]),
+    ("gdp_vector_print", C.c_int, [_vp, _vp]),
+    ("gdp_vector_write", C.c_int, [_vp, _vp]),
+    ("gdp_vector_scan", C.c_int, [_vp, _vp]),
+    ("gdp_vector_read", C.c_int, [_vp, _vp]),
+    ("gdp_res_write", C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _vp]),
+    ("gdp_res_read", C.c_int, [C.c_char_p, C.c_int, C.c_int, _vp, _dp, _dp, _ip, _ip]),
     ("gdp_ilut", C.c_int, [C.c_int, _ip, _ip, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                            C.c_long, _ip, _ip, _ip, _dp, _ip, _lp]),
     ("gdp_batch_create", C.c_int, [_vp, C.c_int, _ip, _ip, _pp]),

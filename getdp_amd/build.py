@@ -47,6 +47,7 @@ SOURCES = [
     ("host_prep.cpp", []),
     ("ilut.cpp", ["-ffp-contract=off"]),
     ("mesh_io.cpp", []),
+    ("res_io.cpp", []),
     ("solver_params.cpp", []),
 ]
 

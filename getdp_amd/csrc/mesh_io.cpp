@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "gdp_internal.h"
+#include "host_par.h"
 
 struct gdp_mesh_data {
   std::vector<double> xyz;     // 3 per node, ascending node number
@@ -37,6 +38,7 @@ struct gdp_mesh_data {
 
 namespace {
 
+using gdp::parallel_ranges;
 using gdp::set_error;
 
 int nodes_of_type(int t)
@@ -48,29 +50,6 @@ int nodes_of_type(int t)
   case 4: return 4;
   default: return -1;
   }
-}
-
-int worker_count(size_t items)
-{
-  unsigned hw = std::thread::hardware_concurrency();
-  int n = (int)std::min<size_t>(hw ? hw : 4, 16);
-  if(items < 20000) n = 1;
-  return std::max(1, n);
-}
-
-template <typename Fn> void parallel_ranges(size_t n, Fn fn)
-{
-  const int nt = worker_count(n);
-  if(nt == 1) {
-    fn(0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  for(int t = 0; t < nt; t++) {
-    size_t a = n * t / nt, b = n * (t + 1) / nt;
-    th.emplace_back([=] { fn(a, b); });
-  }
-  for(auto &x : th) x.join();
 }
 
 struct Cursor {

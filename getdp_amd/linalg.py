@@ -194,6 +194,24 @@ def solve_again(A: Matrix, b: Vector, s: Solver, x: Vector) -> dict:
     return s.info()
 
 
+def res_write(path: str, x: Vector, create: bool = True, binary: bool = False, num: int = 0,
+              time: float = 0.0, time_imag: float = 0.0, step: int = 0):
+    """SaveSolution: one $Solution block (Dof_WriteFileRES, DofData.cpp:741-767),
+    after the $ResFormat block of a new file when `create`."""
+    check(lib().gdp_res_write(path.encode(), int(create), int(binary), num, time, time_imag, step, x.h),
+          "gdp_res_write")
+
+
+def res_read(path: str, x: Vector, num: int = 0, which: int = -1) -> dict:
+    """Solution `which` (the last when < 0) of DofData `num` from a .res file
+    (Dof_ReadFileRES, DofData.cpp:922-1010) into x."""
+    t, ti = C.c_double(), C.c_double()
+    st, cnt = C.c_int(), C.c_int()
+    check(lib().gdp_res_read(path.encode(), num, which, x.h, C.byref(t), C.byref(ti), C.byref(st),
+                             C.byref(cnt)), "gdp_res_read")
+    return {"time": t.value, "time_imag": ti.value, "step": st.value, "count": cnt.value}
+
+
 # post-processing quantity kinds (gdp_hip.h GDP_PQ_*)
 PQ_VALUE, PQ_D, PQ_COEF_D, PQ_INT_COEF_D2 = 1, 2, 3, 4
 

@@ -22,6 +22,7 @@
 #define GDP_HIP_H
 
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -102,6 +103,27 @@ int gdp_vector_norm2(const gdp_vector *v, double *norm);      /* LinAlg_VectorNo
 int gdp_vector_copy(const gdp_vector *src, gdp_vector *dst);  /* LinAlg_CopyVector    */
 /* Device address of the data (FP64, n entries) for zero-copy interop. */
 int gdp_vector_device_ptr(gdp_vector *v, double **dptr);
+
+/* Vector text and binary I/O (LinAlg_PrintVector / LinAlg_WriteVector /
+ * LinAlg_ScanVector / LinAlg_ReadVector, LinAlg_SPARSKIT.cpp:174-229): print
+ * writes one "%.16g\n" per entry, write the n doubles in native byte order
+ * and a newline; scan reads n numbers as fscanf("%lf"), read n raw doubles. */
+int gdp_vector_print(const gdp_vector *v, FILE *f);
+int gdp_vector_write(const gdp_vector *v, FILE *f);
+int gdp_vector_scan(gdp_vector *v, FILE *f);
+int gdp_vector_read(gdp_vector *v, FILE *f);
+/* Result files (.res).  gdp_res_write appends one solution block
+ * (Dof_WriteFileRES, DofData.cpp:741-767: "$Solution  /" "* DofData #num *" "/",
+ * "num time time_imag step", the vector, "$EndSolution"), after writing the
+ * $ResFormat block to a new file when create != 0 (Dof_WriteFileRES0,
+ * :631-642).  gdp_res_read follows Dof_ReadFileRES (:922-1010) for DofData
+ * `num` (any when < 0): solution `which` of those found (the last when < 0)
+ * goes into x; *count = solutions found.  ASCII or binary per the file's
+ * $ResFormat; format version 1.0 files carry no imaginary time. */
+int gdp_res_write(const char *path, int create, int binary, int num, double time, double time_imag,
+                  int step, const gdp_vector *x);
+int gdp_res_read(const char *path, int num, int which, gdp_vector *x, double *time, double *time_imag,
+                 int *step, int *count);
 
 /* ---- matrices (gMatrix) -------------------------------------------------- */
 typedef struct gdp_matrix gdp_matrix;
