@@ -260,6 +260,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true", help="skip the curl-curl mixed-precision line")
     ap.add_argument("--no-default-solver", action="store_true",
                     help="skip the ILUTP(20)-GMRES(40) line (GetDP's default solver)")
+    ap.add_argument("--default-solver-only", type=int, default=0, metavar="N",
+                    help="only GetDP's default solver (ILUTP(20)-GMRES(40)) on poisson2d_N")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the SpMV launches and a calibration copy (for PMC runs)")
     args = ap.parse_args()
@@ -270,6 +272,12 @@ def main():
 
     if args.roofline_only:
         roofline_only(args.n, 20)
+        return
+    if args.default_solver_only:
+        from getdp_amd import linalg as LA
+        LA.init(0)
+        res, _ = default_solver_section(args.default_solver_only)
+        print(json.dumps(res))
         return
 
     from getdp_amd import linalg as LA
