@@ -106,6 +106,19 @@ SIGNATURES = [
     ("gdp_fe_post", C.c_int, [_vp, _vp, C.POINTER(PostDesc), _ip, _dp, C.c_long, _dp, _dp]),
     ("gdp_number_edges", C.c_int, [C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip, C.c_int]),
     ("gdp_rcmk", C.c_int, [C.c_int, _ip, _ip, _ip, _ip]),
+    ("gdp_vector_add", C.c_int, [_vp, _vp, _vp]),
+    ("gdp_vector_add_scaled", C.c_int, [_vp, _vp, C.c_double, _vp]),
+    ("gdp_vector_sub", C.c_int, [_vp, _vp, _vp]),
+    ("gdp_vector_scale", C.c_int, [_vp, C.c_double, _vp]),
+    ("gdp_vector_swap", C.c_int, [_vp, _vp]),
+    ("gdp_vector_set_all", C.c_int, [_vp, C.c_double]),
+    ("gdp_vector_dot", C.c_int, [_vp, _vp, _dp]),
+    ("gdp_vector_norm_inf", C.c_int, [_vp, _dp]),
+    ("gdp_vector_dummy", C.c_int, [_vp, _ip]),
+    ("gdp_matrix_add", C.c_int, [_vp, _vp, _vp]),
+    ("gdp_matrix_add_scaled", C.c_int, [_vp, _vp, C.c_double, _vp]),
+    ("gdp_matrix_scale", C.c_int, [_vp, C.c_double, _vp]),
+    ("gdp_matrix_size", C.c_int, [_vp, _ip, _ip]),
     ("gdp_vector_print", C.c_int, [_vp, _vp]),
     ("gdp_vector_write", C.c_int, [_vp, _vp]),
     ("gdp_vector_scan", C.c_int, [_vp, _vp]),

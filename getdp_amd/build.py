@@ -44,6 +44,7 @@ SOURCES = [
     ("unet.hip", []),
     ("batch.hip", []),
     ("edges.hip", []),
+    ("linalg_ops.hip", ["-ffp-contract=off"]),
     ("host_prep.cpp", []),
     ("ilut.cpp", ["-ffp-contract=off"]),
     ("mesh_io.cpp", []),

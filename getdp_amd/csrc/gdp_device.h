@@ -69,6 +69,8 @@ template <typename T> int d2h(T *h, const T *d, size_t n)
   return GDP_OK;
 }
 int sync();
+// A fresh pattern / value version number for gdp_matrix.
+uint64_t next_version();
 
 // ---- reductions -----------------------------------------------------------
 template <typename T> __device__ inline T wave_sum(T v)

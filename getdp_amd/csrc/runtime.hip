@@ -326,7 +326,7 @@ extern "C" int gdp_vector_device_ptr(gdp_vector *v, double **dptr)
 // =============================================================================
 // C-ABI: matrices
 // =============================================================================
-static uint64_t next_version()
+uint64_t gdp::next_version()
 {
   static uint64_t v = 0;
   return ++v;

@@ -104,6 +104,28 @@ int gdp_vector_copy(const gdp_vector *src, gdp_vector *dst);  /* LinAlg_CopyVect
 /* Device address of the data (FP64, n entries) for zero-copy interop. */
 int gdp_vector_device_ptr(gdp_vector *v, double **dptr);
 
+/* Whole-vector arithmetic (LinAlg_SPARSKIT.cpp over Sparskit.cpp:797-1215),
+ * on the device, with the reference's argument rules (the result must alias
+ * the operand named, else GDP_E_ARG "Wrong arguments in ..."):
+ *   gdp_vector_add        v3 == v1: v1 += v2          (LinAlg_AddVectorVector)
+ *   gdp_vector_add_scaled v3 == v1: v1 += d * v2      (LinAlg_AddVectorProdVectorDouble)
+ *   gdp_vector_sub        v3 == v1: v1 -= v2; v3 == v2: v2 = v1 - v2 (LinAlg_SubVectorVector)
+ *   gdp_vector_scale      v2 == v1: v1 *= d           (LinAlg_ProdVectorDouble)
+ *   gdp_vector_swap, gdp_vector_set_all (every entry = d)  (LinAlg_SwapVector, LinAlg_SetVector)
+ *   gdp_vector_dot        (LinAlg_ProdVectorVector; fixed two-stage reduction)
+ *   gdp_vector_norm_inf   (LinAlg_VectorNormInf; NaN entries ignored, as there)
+ *   gdp_vector_dummy      entries with dummy[i] == 1 set to 0 (LinAlg_DummyVector;
+ *                         dummy = Current.DofData->DummyDof, NULL: none)
+ * Elementwise results equal the sequential loops bit for bit. */
+int gdp_vector_add(gdp_vector *v1, gdp_vector *v2, gdp_vector *v3);
+int gdp_vector_add_scaled(gdp_vector *v1, gdp_vector *v2, double d, gdp_vector *v3);
+int gdp_vector_sub(gdp_vector *v1, gdp_vector *v2, gdp_vector *v3);
+int gdp_vector_scale(gdp_vector *v1, double d, gdp_vector *v2);
+int gdp_vector_swap(gdp_vector *v1, gdp_vector *v2);
+int gdp_vector_set_all(gdp_vector *v, double d);
+int gdp_vector_dot(const gdp_vector *v1, const gdp_vector *v2, double *d);
+int gdp_vector_norm_inf(const gdp_vector *v, double *norm);
+int gdp_vector_dummy(gdp_vector *v, const int *dummy);
 /* Vector text and binary I/O (LinAlg_PrintVector / LinAlg_WriteVector /
  * LinAlg_ScanVector / LinAlg_ReadVector, LinAlg_SPARSKIT.cpp:174-229): print
  * writes one "%.16g\n" per entry, write the n doubles in native byte order
@@ -127,6 +149,17 @@ int gdp_res_read(const char *path, int num, int which, gdp_vector *x, double *ti
 
 /* ---- matrices (gMatrix) -------------------------------------------------- */
 typedef struct gdp_matrix gdp_matrix;
+/* Whole-matrix arithmetic (add_matrix_matrix, add_matrix_prod_matrix_double,
+ * prod_matrix_double, Sparskit.cpp:865-1135), same argument rules:
+ *   gdp_matrix_add        m3 == m1: m1 += m2; m3 == m2: m2 += m1 (LinAlg_AddMatrixMatrix)
+ *   gdp_matrix_add_scaled m3 == m1: m1 += d * m2      (LinAlg_AddMatrixProdMatrixDouble)
+ *   gdp_matrix_scale      m2 == m1: m1 *= d           (LinAlg_ProdMatrixDouble)
+ * Patterns may differ: the result's pattern is their union, an entry of m2
+ * alone holds (d *) its value.  gdp_matrix_size: LinAlg_GetMatrixSize. */
+int gdp_matrix_add(gdp_matrix *m1, gdp_matrix *m2, gdp_matrix *m3);
+int gdp_matrix_add_scaled(gdp_matrix *m1, gdp_matrix *m2, double d, gdp_matrix *m3);
+int gdp_matrix_scale(gdp_matrix *m1, double d, gdp_matrix *m2);
+int gdp_matrix_size(const gdp_matrix *A, int *n, int *m);
 int gdp_matrix_create(int n, int m, gdp_matrix **A);          /* LinAlg_CreateMatrix  */
 int gdp_matrix_destroy(gdp_matrix *A);                        /* LinAlg_DestroyMatrix */
 /* LinAlg_ZeroMatrix: keeps a device pattern (values zeroed) when one exists;
