@@ -28,6 +28,23 @@ struct IluFactor {
   int *colmap = nullptr;
   double *pbuf64 = nullptr;
   float *pbuf32 = nullptr;
+  // One-wave sweeps, used when the dependency chain is long and the levels
+  // are narrow (threshold ILU with fill): each sweep's rows in level order,
+  // in chunks of 64/G rows of one level with G lanes per row.  Entry slot w
+  // of lane l in chunk c sits at (c * wg + w) * 64 + l; per-lane fields at
+  // c * 64 + l, set on the first lane of each row's group only.
+  struct WaveSweep {
+    int G = 1, wg = 0, nch = 0;
+    int *col = nullptr;   // LDS ring slot of the entry's column (the zero slot: padding)
+    int *kpos = nullptr;  // position of the entry in the factors (-1: padding)
+    int *row = nullptr;   // row of the lane's group (-1: none)
+    int *aux = nullptr;   // forward: the row's position in the backward stream; backward: its pivot's position
+    float *val32 = nullptr, *dinv32 = nullptr;  // entries (and inverted pivots) packed from the factors
+    double *val64 = nullptr, *dinv64 = nullptr;
+    double *pre64 = nullptr;  // forward: scale * r; backward: the forward result; in stream order
+    float *pre32 = nullptr;
+  } wave[2];  // [0] forward (L), [1] backward (U)
+  bool wave_mode = false;
 
   ~IluFactor() { release(); }
   void release();
@@ -57,6 +74,11 @@ struct IluFactor {
 
  private:
   int setup_sweeps(const int *hia, const int *hja);
+  int setup_wave(const int *hia, const int *hja, const std::vector<int> &lrows,
+                 const std::vector<int> &lptr);
+  int wave_alloc_values();
+  void wave_pack();
+  void wave_release();
 };
 
 }  // namespace gdp

@@ -22,6 +22,7 @@
 // both the value and its readiness).  Every wait is bounded by a wall-clock
 // limit: on expiry the kernel records an error flag and exits.
 #include <cstdlib>
+#include <cstring>
 
 #include "ilu.h"
 
@@ -349,6 +350,137 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
                                                           addto);
 }
 
+// ---- one-wave sweeps -----------------------------------------------------------
+//
+// With a long dependency chain of narrow levels (threshold ILU with fill:
+// ILUTP(20) on Poisson 256^2 has about 10,000 levels of 6 rows per triangle)
+// the sweeps above pay one publication round trip through the memory
+// hierarchy per level.  Here one wave walks the levels in order instead.  A
+// chunk holds up to 64/G rows of one level, G lanes per row, lane q of a row
+// holding its entries q, q + G, ...; the solution entries of the last kRing
+// rows stay in an LDS ring indexed by row mod kRing (the host checks that
+// every dependency is still there when it is read); the chunks are loaded
+// a few chunks ahead.  A level then costs LDS reads, the row's subtraction
+// chain and an LDS write, with no memory round trip.  Every row subtracts its
+// products in ascending position (lusol's order; padding subtracts +0 last).
+constexpr int kRing = 16384;
+// Chunks in flight: a chunk is 2 * WG + 3 loads per lane, and the loads in
+// flight must stay within what the wave's memory counter tracks (63), or the
+// compiler waits for all of them at the top of every round.
+__host__ __device__ constexpr int wave_depth(int wg) { return 60 / (2 * wg + 3); }
+// (lanes per row, entry slots per lane) of the kernel instances, by
+// increasing capacity; a sweep takes the first that holds its longest row
+constexpr int kWaveShapes[][2] = {{1, 4}, {2, 4}, {4, 3}, {4, 6}, {8, 8}};
+constexpr int kNumWaveShapes = 5;
+
+template <typename TF, typename TV, int WG> struct WaveRec {
+  int c[WG];
+  TF v[WG];
+  int r, a;
+  TF d;
+  TV p;
+};
+
+// nch (a multiple of the depth) chunks, followed by depth padding chunks that
+// the last prefetches read; padding rows are -1 and never stored.
+template <typename TF, typename TV, int G, int WG, bool FWD>
+__global__ void __launch_bounds__(64) k_wave_sweep(int nch, const int *__restrict__ col, const TF *__restrict__ val,
+                                                   const int *__restrict__ row, const int *__restrict__ aux,
+                                                   const TF *__restrict__ dinv, const TV *__restrict__ pre, TV *out,
+                                                   const int *gate, int gmode)
+{
+  if(is_gated(gate, gmode)) return;
+  constexpr int kDepth = wave_depth(WG);
+  __shared__ TV ring[kRing + 1];
+  const int lane = threadIdx.x, q = lane & (G - 1), gbase = lane - q;
+  if(lane == 0) ring[kRing] = 0;  // the padding slot
+  __syncthreads();
+  // whole rounds double-buffered: the next round's loads are issued before
+  // the current round is computed, so the one wait per round (the output
+  // stores share the loads' counter, which makes the compiler wait for all
+  // loads at a round's end) comes a round of work after the loads
+  WaveRec<TF, TV, WG> cur[kDepth], nxt[kDepth];
+  auto load = [&](WaveRec<TF, TV, WG> &b, int c) {
+#pragma unroll
+    for(int w = 0; w < WG; w++) {
+      const size_t i = ((size_t)c * WG + w) * kWave + lane;
+      b.c[w] = col[i];
+      b.v[w] = val[i];
+    }
+    const size_t li = (size_t)c * kWave + lane;
+    b.r = row[li];
+    if constexpr(FWD) b.a = aux[li];
+    else b.d = dinv[li];
+    b.p = pre[li];
+  };
+#pragma unroll
+  for(int d = 0; d < kDepth; d++) load(cur[d], d);
+  for(int c0 = 0; c0 < nch; c0 += kDepth) {
+#pragma unroll
+    for(int d = 0; d < kDepth; d++) load(nxt[d], c0 + kDepth + d);
+#pragma unroll
+    for(int d = 0; d < kDepth; d++) {
+      const WaveRec<TF, TV, WG> &rc = cur[d];
+      TV prod[WG];
+#pragma unroll
+      for(int w = 0; w < WG; w++) prod[w] = (TV)rc.v[w] * ring[rc.c[w]];
+      TV acc = rc.p;
+#pragma unroll
+      for(int w = 0; w < WG; w++) {
+        if constexpr(G == 1) acc = acc - prod[w];
+        else {
+#pragma unroll
+          for(int j = 0; j < G; j++) acc = acc - lane_read(prod[w], gbase + j);
+        }
+      }
+      if(q == 0 && rc.r >= 0) {
+        const TV xv = FWD ? acc : (TV)rc.d * acc;
+        ring[rc.r & (kRing - 1)] = xv;
+        if constexpr(FWD) out[rc.a] = xv;  // the backward sweep's input, in its order
+        else out[rc.r] = xv;
+      }
+    }
+#pragma unroll
+    for(int d = 0; d < kDepth; d++) cur[d] = nxt[d];
+  }
+}
+
+// addto += x after the one-wave backward sweep (the same single addition the
+// sync-free sweep fuses; inside the sweep its load would sit on the chain)
+template <typename TV>
+__global__ void __launch_bounds__(256) k_wave_addto(int n, const TV *__restrict__ x, TV *__restrict__ addto,
+                                                    const int *gate, int gmode)
+{
+  if(is_gated(gate, gmode)) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if(i < n) addto[i] = addto[i] + x[i];
+}
+
+// pre[p] = scale * r[row[p]] in the forward stream's order
+template <typename TV>
+__global__ void __launch_bounds__(256) k_wave_gather(long cnt, const int *__restrict__ row, const TV *__restrict__ r,
+                                                     TV scale, TV *__restrict__ pre, const int *gate, int gmode)
+{
+  if(is_gated(gate, gmode)) return;
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if(p < cnt) {
+    const int i = row[p];
+    pre[p] = i >= 0 ? r[i] * scale : (TV)0;
+  }
+}
+
+// val[p] = lu[kpos[p]] (0 for padding), after each factorisation
+template <typename TF>
+__global__ void __launch_bounds__(256) k_wave_pack(long cnt, const int *__restrict__ kpos, const TF *__restrict__ lu,
+                                                   TF *__restrict__ val)
+{
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if(p < cnt) {
+    const int k = kpos[p];
+    val[p] = k >= 0 ? lu[k] : (TF)0;
+  }
+}
+
 template <typename T> __global__ void k_fill_bits(long n, T *p)
 {
   long i = (long)blockIdx.x * 256 + threadIdx.x;
@@ -404,9 +536,199 @@ void IluFactor::release()
   dfree(colmap);
   dfree(pbuf64);
   dfree(pbuf32);
+  wave_release();
   ia = ja = nullptr;
   level_ptr.clear();
   n = 0;
+}
+
+void IluFactor::wave_release()
+{
+  for(auto &w : wave) {
+    dfree(w.col);
+    dfree(w.kpos);
+    dfree(w.row);
+    dfree(w.aux);
+    dfree(w.val32);
+    dfree(w.dinv32);
+    dfree(w.val64);
+    dfree(w.dinv64);
+    dfree(w.pre64);
+    dfree(w.pre32);
+    w = WaveSweep();
+  }
+  wave_mode = false;
+}
+
+// The one-wave sweep streams for the host pattern (lower rows lrows grouped by
+// level through lptr).  They are built when every row's entries fit one of the
+// kernel's (lanes, slots) shapes, every dependency is still in the LDS ring when read,
+// and the estimated time per application beats the sync-free sweeps': about
+// 2 us per dependency level there (one publication round trip) against about
+// 0.25 us per chunk here.  GDP_ILU_SWEEP=wave / free forces either choice.
+int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int> &lrows,
+                          const std::vector<int> &lptr)
+{
+  wave_release();
+  const char *mode = std::getenv("GDP_ILU_SWEEP");
+  const bool force = mode && !std::strcmp(mode, "wave");
+  // GDP_ILU_LANES selects a lane count of the sync-free sweeps
+  if(!n || (mode && !std::strcmp(mode, "free")) || (!force && std::getenv("GDP_ILU_LANES"))) return GDP_OK;
+  std::vector<int> dpos(n);
+  for(int i = 0; i < n; i++) {
+    int k = hia[i];
+    while(k < hia[i + 1] && hja[k] < i) k++;
+    if(k == hia[i + 1] || hja[k] != i) return GDP_OK;  // missing diagonal: reported by the caller
+    dpos[i] = k;
+  }
+  std::vector<int> ulev;
+  const int nul = upper_levels(n, hia, hja, ulev);
+  std::vector<int> uptr(nul + 1, 0), urows(n);
+  for(int i = 0; i < n; i++) uptr[ulev[i] + 1]++;
+  for(int l = 0; l < nul; l++) uptr[l + 1] += uptr[l];
+  {
+    std::vector<int> f(uptr.begin(), uptr.end() - 1);
+    for(int i = n - 1; i >= 0; i--) urows[f[ulev[i]]++] = i;
+  }
+  // entries of row i on side s: [b, e)
+  auto range = [&](int s, int i, int &b, int &e) {
+    if(s == 0) { b = hia[i]; e = dpos[i]; }
+    else { b = dpos[i] + 1; e = hia[i + 1]; }
+  };
+  struct Side {
+    int G, wg, rc;
+    std::vector<int> cstart, ccount;  // chunk -> rows (index into the ordered rows)
+  } sd[2];
+  const std::vector<int> *rows[2] = {&lrows, &urows};
+  const std::vector<int> *ptrs[2] = {&lptr, &uptr};
+  for(int s = 0; s < 2; s++) {
+    int maxc = 0;
+    for(int i = 0; i < n; i++) {
+      int b, e;
+      range(s, i, b, e);
+      maxc = std::max(maxc, e - b);
+    }
+    int shape = 0;
+    while(shape < kNumWaveShapes && kWaveShapes[shape][0] * kWaveShapes[shape][1] < maxc) shape++;
+    if(shape == kNumWaveShapes) return GDP_OK;
+    sd[s].G = kWaveShapes[shape][0];
+    sd[s].wg = kWaveShapes[shape][1];
+    sd[s].rc = kWave / sd[s].G;
+    const std::vector<int> &p = *ptrs[s];
+    for(size_t l = 0; l + 1 < p.size(); l++)
+      for(int a = p[l]; a < p[l + 1]; a += sd[s].rc) {
+        sd[s].cstart.push_back(a);
+        sd[s].ccount.push_back(std::min(sd[s].rc, p[l + 1] - a));
+      }
+    // every dependency must still be in the ring when its chunk reads it
+    std::vector<int> owner(kRing, -1), stamp(kRing, -1);
+    const std::vector<int> &ord = *rows[s];
+    for(size_t c = 0; c < sd[s].cstart.size(); c++) {
+      for(int t = 0; t < sd[s].ccount[c]; t++) {
+        const int i = ord[sd[s].cstart[c] + t];
+        int b, e;
+        range(s, i, b, e);
+        for(int k = b; k < e; k++)
+          if(owner[hja[k] & (kRing - 1)] != hja[k]) return GDP_OK;
+      }
+      for(int t = 0; t < sd[s].ccount[c]; t++) {  // two rows of one chunk on one slot: neither is kept
+        const int i = ord[sd[s].cstart[c] + t], sl = i & (kRing - 1);
+        owner[sl] = stamp[sl] == (int)c ? -2 : i;
+        stamp[sl] = (int)c;
+      }
+    }
+  }
+  const long nch_all = (long)sd[0].cstart.size() + (long)sd[1].cstart.size();
+  const double bytes = 8. * kWave * ((double)sd[0].cstart.size() * sd[0].wg + (double)sd[1].cstart.size() * sd[1].wg);
+  const double t_wave = 0.25 * nch_all + bytes / 1e5, t_free = 2.0 * ((double)nlevels + upper_nlevels);
+  if(!force && t_wave >= t_free) return GDP_OK;
+  // positions of the rows in the backward stream (the forward sweep's output)
+  std::vector<int> bpos(n);
+  for(size_t c = 0; c < sd[1].cstart.size(); c++)
+    for(int t = 0; t < sd[1].ccount[c]; t++) bpos[urows[sd[1].cstart[c] + t]] = (int)c * kWave + t * sd[1].G;
+  for(int s = 0; s < 2; s++) {
+    const Side &S = sd[s];
+    const long nch = (long)S.cstart.size();
+    // steps rounded up to whole prefetch rounds, then one round of padding chunks
+    const int depth = wave_depth(S.wg);
+    const long nsteps = (nch + depth - 1) / depth * depth, nalloc = nsteps + depth;
+    const size_t ne = (size_t)nalloc * S.wg * kWave, nl = (size_t)nalloc * kWave;
+    std::vector<int> col(ne, kRing), kpos(ne, -1), row(nl, -1), aux(nl, -1);
+    const std::vector<int> &ord = *rows[s];
+    for(long c = 0; c < nch; c++)
+      for(int t = 0; t < S.ccount[c]; t++) {
+        const int i = ord[S.cstart[c] + t], gl = t * S.G;
+        int b, e;
+        range(s, i, b, e);
+        for(int k = b; k < e; k++) {
+          const int x = k - b, lane = gl + x % S.G, w = x / S.G;
+          const size_t idx = ((size_t)c * S.wg + w) * kWave + lane;
+          col[idx] = hja[k] & (kRing - 1);
+          kpos[idx] = k;
+        }
+        row[(size_t)c * kWave + gl] = i;
+        aux[(size_t)c * kWave + gl] = s == 0 ? bpos[i] : dpos[i];
+      }
+    WaveSweep &W = wave[s];
+    W.G = S.G;
+    W.wg = S.wg;
+    W.nch = (int)nsteps;
+    GDP_TRY(dalloc(&W.col, ne));
+    GDP_TRY(dalloc(&W.kpos, ne));
+    GDP_TRY(dalloc(&W.row, nl));
+    GDP_TRY(dalloc(&W.aux, nl));
+    GDP_TRY(dalloc(&W.pre64, nl));
+    GDP_TRY(dalloc(&W.pre32, nl));
+    GDP_TRY(h2d(W.col, col.data(), ne));
+    GDP_TRY(h2d(W.kpos, kpos.data(), ne));
+    GDP_TRY(h2d(W.row, row.data(), nl));
+    GDP_TRY(h2d(W.aux, aux.data(), nl));
+  }
+  wave_mode = true;
+  return GDP_OK;
+}
+
+// Storage for the packed factor entries, in the factors' precision.
+int IluFactor::wave_alloc_values()
+{
+  if(!wave_mode) return GDP_OK;
+  for(int s = 0; s < 2; s++) {
+    WaveSweep &W = wave[s];
+    const size_t nalloc = (size_t)W.nch + wave_depth(W.wg), ne = nalloc * W.wg * kWave, nl = nalloc * kWave;
+    dfree(W.val32);
+    dfree(W.dinv32);
+    dfree(W.val64);
+    dfree(W.dinv64);
+    if(is_f64) {
+      GDP_TRY(dalloc(&W.val64, ne));
+      if(s == 1) GDP_TRY(dalloc(&W.dinv64, nl));
+    }
+    else {
+      GDP_TRY(dalloc(&W.val32, ne));
+      if(s == 1) GDP_TRY(dalloc(&W.dinv32, nl));
+    }
+  }
+  return GDP_OK;
+}
+
+// Copy the factor entries into the streams (enqueued: usable under capture).
+void IluFactor::wave_pack()
+{
+  if(!wave_mode) return;
+  hipStream_t st = ctx().stream;
+  for(int s = 0; s < 2; s++) {
+    WaveSweep &W = wave[s];
+    const long nalloc = (long)W.nch + wave_depth(W.wg), ne = nalloc * W.wg * kWave, nl = nalloc * kWave;
+    const dim3 ge((unsigned)((ne + 255) / 256)), gl((unsigned)((nl + 255) / 256));
+    if(is_f64) {
+      hipLaunchKernelGGL(k_wave_pack<double>, ge, dim3(256), 0, st, ne, W.kpos, lu64, W.val64);
+      if(s == 1) hipLaunchKernelGGL(k_wave_pack<double>, gl, dim3(256), 0, st, nl, W.aux, lu64, W.dinv64);
+    }
+    else {
+      hipLaunchKernelGGL(k_wave_pack<float>, ge, dim3(256), 0, st, ne, W.kpos, lu32, W.val32);
+      if(s == 1) hipLaunchKernelGGL(k_wave_pack<float>, gl, dim3(256), 0, st, nl, W.aux, lu32, W.dinv32);
+    }
+  }
 }
 
 // Levels, lanes per row and the sweep buffers for the host pattern hia/hja
@@ -461,7 +783,7 @@ int IluFactor::setup_sweeps(const int *hia, const int *hja)
   GDP_TRY(h2d(err, init, 4));
   GDP_TRY(dalloc(&tmp64, (size_t)std::max(1, n)));
   GDP_TRY(dalloc(&tmp32, (size_t)std::max(1, n)));
-  return GDP_OK;
+  return setup_wave(hia, hja, rows, level_ptr);
 }
 
 int IluFactor::analyse(const DevCSR &B, const std::vector<int> &hia, const std::vector<int> &hja)
@@ -508,6 +830,8 @@ template <typename T> int IluFactor::load(const IlutRows<T> &f, bool f64)
     GDP_TRY(dalloc(&lu32, (size_t)std::max(1L, nnz)));
     GDP_TRY(h2d(lu32, v.data(), (size_t)nnz));
   }
+  GDP_TRY(wave_alloc_values());
+  wave_pack();
   if(!f.newpos.empty()) {
     GDP_TRY(dalloc(&colmap, (size_t)std::max(1, n)));
     GDP_TRY(h2d(colmap, f.newpos.data(), (size_t)n));
@@ -567,8 +891,10 @@ int IluFactor::factor(const double *a, bool f64, int *zero_pivot_row)
     if(f64) hipLaunchKernelGGL(k_ilu_copy<double>, g, dim3(256), 0, s, nnz, a, lu64);
     else hipLaunchKernelGGL(k_ilu_copy<float>, g, dim3(256), 0, s, nnz, a, lu32);
   }
+  GDP_TRY(wave_alloc_values());
   if(f64) enqueue_levels(lu64);
   else enqueue_levels(lu32);
+  wave_pack();
   GDP_HIP(hipGetLastError());
   int h;
   GDP_TRY(d2h(&h, err, 1));
@@ -589,7 +915,7 @@ int IluFactor::reserve(bool f64)
   dfree(lu64);
   if(f64) GDP_TRY(dalloc(&lu64, (size_t)std::max<long>(1, nnz)));
   else GDP_TRY(dalloc(&lu32, (size_t)std::max<long>(1, nnz)));
-  return GDP_OK;
+  return wave_alloc_values();
 }
 
 int IluFactor::factor_enqueue(const double *a)
@@ -604,6 +930,7 @@ int IluFactor::factor_enqueue(const double *a)
   }
   if(is_f64) enqueue_levels(lu64);
   else enqueue_levels(lu32);
+  wave_pack();
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
@@ -646,6 +973,44 @@ void sweep(int G, int n, const int *ia, const int *ja, const int *diag, const TF
 #undef GDP_SWEEP
 }
 
+template <typename TF> const TF *wave_val(const IluFactor::WaveSweep &W);
+template <> const float *wave_val<float>(const IluFactor::WaveSweep &W) { return W.val32; }
+template <> const double *wave_val<double>(const IluFactor::WaveSweep &W) { return W.val64; }
+template <typename TF> const TF *wave_dinv(const IluFactor::WaveSweep &W);
+template <> const float *wave_dinv<float>(const IluFactor::WaveSweep &W) { return W.dinv32; }
+template <> const double *wave_dinv<double>(const IluFactor::WaveSweep &W) { return W.dinv64; }
+template <typename TV> TV *wave_pre(const IluFactor::WaveSweep &W);
+template <> float *wave_pre<float>(const IluFactor::WaveSweep &W) { return W.pre32; }
+template <> double *wave_pre<double>(const IluFactor::WaveSweep &W) { return W.pre64; }
+
+// x = (LU)^{-1} (scale * r) by the one-wave sweeps: gather scale * r into the
+// forward stream's order, forward sweep (its output in the backward stream's
+// order), backward sweep into x (and addto).
+template <typename TF, typename TV>
+void wave_apply(int n, const IluFactor::WaveSweep *W, const TV *r, TV scale, TV *x, TV *addto, const int *gate,
+                int gmode)
+{
+  hipStream_t s = ctx().stream;
+  const IluFactor::WaveSweep &F = W[0], &B = W[1];
+  const long nl = ((long)F.nch + wave_depth(F.wg)) * kWave;
+  hipLaunchKernelGGL(k_wave_gather<TV>, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, s, nl, F.row, r, scale,
+                     wave_pre<TV>(F), gate, gmode);
+#define GDP_WAVE(GG, WW, FWD, S, OUT, ADD)                                                                \
+  hipLaunchKernelGGL((k_wave_sweep<TF, TV, GG, WW, FWD>), dim3(1), dim3(kWave), 0, s, S.nch, S.col,       \
+                     wave_val<TF>(S), S.row, S.aux, wave_dinv<TF>(S), wave_pre<TV>(S), OUT, gate, gmode)
+#define GDP_WAVE_SHAPE(FWD, S, OUT, ADD)                                        \
+  if(S.G == 1) GDP_WAVE(1, 4, FWD, S, OUT, ADD);                                \
+  else if(S.G == 2) GDP_WAVE(2, 4, FWD, S, OUT, ADD);                           \
+  else if(S.G == 4 && S.wg == 3) GDP_WAVE(4, 3, FWD, S, OUT, ADD);              \
+  else if(S.G == 4) GDP_WAVE(4, 6, FWD, S, OUT, ADD);                           \
+  else GDP_WAVE(8, 8, FWD, S, OUT, ADD)
+  GDP_WAVE_SHAPE(true, F, wave_pre<TV>(B), (TV *)nullptr);
+  GDP_WAVE_SHAPE(false, B, x, addto);
+#undef GDP_WAVE_SHAPE
+#undef GDP_WAVE
+  if(addto) hipLaunchKernelGGL(k_wave_addto<TV>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, addto, gate, gmode);
+}
+
 }  // namespace
 
 template <typename TV>
@@ -663,7 +1028,11 @@ int IluFactor::apply(const TV *r, TV scale, TV *x, const int *gate, int gmode, T
     if(colmap) out = pbuf32;
   }
   if(colmap) add = nullptr;
-  if(is_f64) {
+  if(wave_mode) {
+    if(is_f64) wave_apply<double, TV>(n, wave, r, scale, out, add, gate, gmode);
+    else wave_apply<float, TV>(n, wave, r, scale, out, add, gate, gmode);
+  }
+  else if(is_f64) {
     sweep<double, TV, true>(g_lower, n, ia, ja, diag, lu64, r, scale, t, out, err + 1, wait_limit(),
                             gate, gmode, nullptr);
     sweep<double, TV, false>(g_upper, n, ia, ja, diag, lu64, nullptr, scale, t, out, err + 1,

@@ -18,6 +18,7 @@ SYSTEMS = {
     "elasticity3d_4": lambda: P.elasticity3d(4, nbins=16),
     "elasticity3d_8": lambda: P.elasticity3d(8),    # ~18 lower entries: 8 lanes, 2-5 chunks a row
     "curlcurl3d_4": lambda: P.curlcurl3d(4),
+    "poisson2d_192": lambda: P.poisson2d(192),  # 36,481 rows: wraps the one-wave sweeps' LDS ring
 }
 
 
@@ -35,10 +36,15 @@ def _device_apply(gpu, ia, ja, a, v, f64, reps=1):
     return out.get()
 
 
+@pytest.mark.parametrize("sweep", ["free", "wave"])
 @pytest.mark.parametrize("f64", [False, True])
 @pytest.mark.parametrize("name", list(SYSTEMS))
-def test_ilu_apply_bit_exact(gpu, oracle, name, f64):
-    print(f"case {name} f64={f64}", flush=True)
+def test_ilu_apply_bit_exact(gpu, oracle, name, f64, sweep, monkeypatch):
+    """Both sweep kinds: the sync-free sweeps and the one-wave sweeps (levels
+    walked by one wave, solution entries in an LDS ring of 16384 rows; the
+    192^2 system has 36,481 rows, so the ring wraps)."""
+    monkeypatch.setenv("GDP_ILU_SWEEP", sweep)
+    print(f"case {name} f64={f64} sweep={sweep}", flush=True)
     ia, ja, a, b, tab = oracle.assemble(SYSTEMS[name]())
     perm, rperm, ib, jb, bb, nlev = oracle.rcmk(ia, ja, a)
     alu, jlu, ju, ierr = oracle.ilu0(ib, jb, bb, double=f64)
@@ -94,7 +100,7 @@ def test_ilu_apply_lane_counts(gpu, oracle, monkeypatch, lanes):
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
 
 
-@pytest.mark.parametrize("lanes", ["auto", "8"])
+@pytest.mark.parametrize("lanes", ["auto", "8", "wave"])
 @pytest.mark.parametrize("pivot", [False, True])
 @pytest.mark.parametrize("f64", [False, True])
 @pytest.mark.parametrize("name", ["poisson2d_32", "elasticity3d_8", "curlcurl3d_4"])
@@ -102,7 +108,9 @@ def test_ilut_apply_bit_exact(gpu, oracle, name, f64, pivot, lanes, monkeypatch)
     """ILUT / ILUTP (fill 20, the reference's defaults) factored by the
     library on the host and applied by the device sweeps, against the oracle's
     ilut/ilutp + lusol; ILUTP's column permutation applied to the output."""
-    if lanes != "auto":
+    if lanes == "wave":
+        monkeypatch.setenv("GDP_ILU_SWEEP", "wave")
+    elif lanes != "auto":
         monkeypatch.setenv("GDP_ILU_LANES", lanes)
     print(f"case {name} f64={f64} pivot={pivot} lanes={lanes}", flush=True)
     ia, ja, a, b, tab = oracle.assemble(SYSTEMS[name]())
@@ -119,5 +127,29 @@ def test_ilut_apply_bit_exact(gpu, oracle, name, f64, pivot, lanes, monkeypatch)
     gpu.solve(A, gpu.Vector(n, np.ones(n)), s, gpu.Vector(n))
     out = gpu.Vector(n)
     gpu.solver_prec_apply(s, gpu.Vector(n, v), out, reps=3)
+    got = out.get()
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), np.abs(got - want).max()
+
+
+@pytest.mark.parametrize("sweep", ["free", "wave"])
+@pytest.mark.parametrize("pivot", [False, True])
+def test_ilut_apply_long_chain(gpu, oracle, sweep, pivot, monkeypatch):
+    """ILUT / ILUTP(20) on Poisson 160^2 (25,281 rows, more than the one-wave
+    sweeps' LDS ring holds, and thousands of narrow levels): lusol's bits with
+    either sweep kind."""
+    monkeypatch.setenv("GDP_ILU_SWEEP", sweep)
+    ia, ja, a, b, tab = oracle.assemble(P.poisson2d(160))
+    perm, rperm, ib, jb, bb, nlev = oracle.rcmk(ia, ja, a)
+    n = ib.shape[0] - 1
+    alu, jlu, ju, iperm, _, ierr = oracle.ilut(ib, jb, bb, 20, 0.0, 0.05, pivot, False)
+    assert ierr == 0
+    v = np.random.default_rng(9).standard_normal(n)
+    want = oracle.lusol(alu, jlu, ju, v)[iperm[n:]] if pivot else oracle.lusol(alu, jlu, ju, v)
+    A = gpu.Matrix(n)
+    A.set_csr(ia, ja, a)
+    s = gpu.Solver(Algorithm=8, Preconditioner=2 if pivot else 1, Krylov_Size=40, Nb_Iter_Max=3)
+    gpu.solve(A, gpu.Vector(n, np.ones(n)), s, gpu.Vector(n))
+    out = gpu.Vector(n)
+    gpu.solver_prec_apply(s, gpu.Vector(n, v), out, reps=2)
     got = out.get()
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), np.abs(got - want).max()
