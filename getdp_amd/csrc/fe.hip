@@ -615,18 +615,21 @@ __global__ void __launch_bounds__(256) k_prep_mark(DevPlan P, PrepOut O, int *er
   }
 }
 
-// fixed entities of one space: the last occurrence in the list wins, as in
-// the serial loop (marks become f + 2)
+// fixed entities of one space: the first occurrence in the list wins, as the
+// reference's constraint loop stops at the first match and never re-assigns
+// a fixed DOF (Get_ConstraintOfElement.cpp:51-54, DofData.cpp:1137-1151).
+// Marks become kFixMark - f, so the largest mark is the smallest f.
+constexpr int kFixMark = 0x7fffffff;
 __global__ void __launch_bounds__(256) k_prep_fix(int nfix, const int *fixent, int nmax, int *dofnum)
 {
   const int f = blockIdx.x * 256 + threadIdx.x;
   if(f >= nfix) return;
   const int ent = fixent[f];
-  if(ent >= 1 && ent <= nmax && dofnum[ent] >= 1) atomicMax(dofnum + ent, f + 2);
+  if(ent >= 1 && ent <= nmax && dofnum[ent] >= 1) atomicMax(dofnum + ent, kFixMark - f);
 }
 
 // unknown flags (1 for a mark that stayed 1) for the numbering scan; fixed
-// marks f + 2 become -(f + 1)
+// marks kFixMark - f become -(f + 1)
 __global__ void __launch_bounds__(256) k_prep_flags(int nmax, int *dofnum, int *flag)
 {
   const int ent = blockIdx.x * 256 + threadIdx.x;
@@ -637,7 +640,7 @@ __global__ void __launch_bounds__(256) k_prep_flags(int nmax, int *dofnum, int *
   }
   const int v = dofnum[ent];
   flag[ent] = v == 1;
-  if(v >= 2) dofnum[ent] = -(v - 1);
+  if(v >= 2) dofnum[ent] = -(kFixMark - v + 1);
 }
 
 // numbers in entity order after an exclusive scan of the flags; base = the
@@ -1021,7 +1024,9 @@ static int fe_prep_host(gdp_fe *F)
   for(int s = 0; s < nspaces; s++)
     for(size_t f = 0; f < F->h_fixent[s].size(); f++) {
       int ent = F->h_fixent[s][f];
-      if(ent >= 1 && ent <= nmax && F->h_dofnum[s][ent]) F->h_dofnum[s][ent] = -(f + 1);
+      // first occurrence wins: only an unknown mark becomes fixed
+      // (Get_ConstraintOfElement.cpp:51-54, DofData.cpp:1137-1151)
+      if(ent >= 1 && ent <= nmax && F->h_dofnum[s][ent] > 0) F->h_dofnum[s][ent] = -(f + 1);
     }
   std::vector<int> order(nspaces);
   std::iota(order.begin(), order.end(), 0);
@@ -1984,6 +1989,7 @@ extern "C" int gdp_fe_post(gdp_fe *fe, const gdp_vector *x, const gdp_post_desc 
     return set_error(GDP_E_UNSUPPORTED, "gdp_fe_post: quantity kind %d", q->kind);
   if(q->space < 0 || q->space >= fe->nspaces) return set_error(GDP_E_ARG, "gdp_fe_post: space %d", q->space);
   if(x->n != fe->N) return set_error(GDP_E_ARG, "gdp_fe_post: vector size %d, system %d", x->n, fe->N);
+  GDP_TRY(vec_flush(x));
   const bool needc = q->kind == GDP_PQ_COEF_D || q->kind == GDP_PQ_INT_COEF_D2;
   if(needc) {
     if(!q->coef || q->coef_nreg < fe->nreg) return set_error(GDP_E_ARG, "gdp_fe_post: coefficient table too short");

@@ -159,8 +159,17 @@ int spmv_lanes(long nnz, int n);
 struct gdp_vector {
   int n = 0;
   double *d = nullptr;
-  std::vector<std::pair<int, double>> staged;
+  // LinAlg_AddDoubleInVector calls not yet applied; every entry point that
+  // reads the vector applies them first (vec_flush), every one that
+  // overwrites it drops them, so the vector behaves as if each addition
+  // had gone straight into it (LinAlg_SPARSKIT.cpp:356-364)
+  mutable std::vector<std::pair<int, double>> staged;
 };
+
+namespace gdp {
+// Apply the vector's staged additions in call order (a no-op without any).
+int vec_flush(const gdp_vector *v);
+}  // namespace gdp
 
 struct gdp_matrix {
   int n = 0, m = 0;

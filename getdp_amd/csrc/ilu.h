@@ -45,6 +45,8 @@ struct IluFactor {
     float *pre32 = nullptr;
   } wave[2];  // [0] forward (L), [1] backward (U)
   bool wave_mode = false;
+  // which sweep kernels apply the factors (gdp_solve_info.ilu_sweep)
+  int sweep_kind() const { return wave_mode ? GDP_SWEEP_WAVE : GDP_SWEEP_FREE; }
 
   ~IluFactor() { release(); }
   void release();

@@ -565,12 +565,14 @@ void IluFactor::wave_release()
 // kernel's (lanes, slots) shapes, every dependency is still in the LDS ring when read,
 // and the estimated time per application beats the sync-free sweeps': about
 // 2 us per dependency level there (one publication round trip) against about
-// 0.25 us per chunk here.  GDP_ILU_SWEEP=wave / free forces either choice.
+// 0.25 us per chunk here.  GDP_ILU_SWEEP=wave / free forces either choice; a
+// forced `wave` on a system the kernel cannot take is GDP_E_UNSUPPORTED.
 int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int> &lrows,
                           const std::vector<int> &lptr)
 {
   wave_release();
   const char *mode = std::getenv("GDP_ILU_SWEEP");
+  // forced: a system the kernel cannot take is an error, not a silent fallback
   const bool force = mode && !std::strcmp(mode, "wave");
   // GDP_ILU_LANES selects a lane count of the sync-free sweeps
   if(!n || (mode && !std::strcmp(mode, "free")) || (!force && std::getenv("GDP_ILU_LANES"))) return GDP_OK;
@@ -578,7 +580,8 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
   for(int i = 0; i < n; i++) {
     int k = hia[i];
     while(k < hia[i + 1] && hja[k] < i) k++;
-    if(k == hia[i + 1] || hja[k] != i) return GDP_OK;  // missing diagonal: reported by the caller
+    if(k == hia[i + 1] || hja[k] != i)  // missing diagonal: reported by the caller
+      return force ? set_error(GDP_E_UNSUPPORTED, "one-wave sweeps: row %d has no diagonal entry", i) : GDP_OK;
     dpos[i] = k;
   }
   std::vector<int> ulev;
@@ -610,7 +613,10 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
     }
     int shape = 0;
     while(shape < kNumWaveShapes && kWaveShapes[shape][0] * kWaveShapes[shape][1] < maxc) shape++;
-    if(shape == kNumWaveShapes) return GDP_OK;
+    if(shape == kNumWaveShapes)
+      return force ? set_error(GDP_E_UNSUPPORTED, "one-wave sweeps: a row holds %d entries on one side (at most %d)",
+                               maxc, kWaveShapes[kNumWaveShapes - 1][0] * kWaveShapes[kNumWaveShapes - 1][1])
+                   : GDP_OK;
     sd[s].G = kWaveShapes[shape][0];
     sd[s].wg = kWaveShapes[shape][1];
     sd[s].rc = kWave / sd[s].G;
@@ -629,7 +635,11 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
         int b, e;
         range(s, i, b, e);
         for(int k = b; k < e; k++)
-          if(owner[hja[k] & (kRing - 1)] != hja[k]) return GDP_OK;
+          if(owner[hja[k] & (kRing - 1)] != hja[k])
+            return force ? set_error(GDP_E_UNSUPPORTED,
+                                     "one-wave sweeps: row %d depends on row %d, which has left the %d-row ring", i,
+                                     hja[k], kRing)
+                         : GDP_OK;
       }
       for(int t = 0; t < sd[s].ccount[c]; t++) {  // two rows of one chunk on one slot: neither is kept
         const int i = ord[sd[s].cstart[c] + t], sl = i & (kRing - 1);

@@ -80,7 +80,11 @@ __global__ void __launch_bounds__(kB) k_zero_masked(long n, const int *__restric
 
 int vec_op(int op, double d, gdp_vector *u, gdp_vector *v)
 {
-  if(u->n && !u->staged.empty()) return set_error(GDP_E_STATE, "vector has unassembled additions");
+  // staged additions count as already made (LinAlg_AddDoubleInVector adds
+  // straight into the vector in the reference); u = d drops u's
+  if(op == 6) u->staged.clear();
+  else GDP_TRY(vec_flush(u));
+  if(v) GDP_TRY(vec_flush(v));
   if(u->n)
     hipLaunchKernelGGL(k_vec_op, grid_for(u->n), dim3(kB), 0, ctx().stream, (long)u->n, op, d, u->d,
                        v ? v->d : nullptr);
@@ -227,7 +231,6 @@ extern "C" int gdp_vector_swap(gdp_vector *v1, gdp_vector *v2)
 extern "C" int gdp_vector_set_all(gdp_vector *v, double d)
 {
   if(!v) return set_error(GDP_E_ARG, "gdp_vector_set_all: null argument");
-  v->staged.clear();
   return vec_op(6, d, v, nullptr);
 }
 
@@ -235,18 +238,22 @@ extern "C" int gdp_vector_dot(const gdp_vector *v1, const gdp_vector *v2, double
 {
   if(!v1 || !v2 || !d) return set_error(GDP_E_ARG, "gdp_vector_dot: null argument");
   if(v1->n != v2->n) return set_error(GDP_E_ARG, "gdp_vector_dot: size mismatch");
+  GDP_TRY(vec_flush(v1));
+  GDP_TRY(vec_flush(v2));
   return reduce(v1->n, v1->d, v2->d, false, d);
 }
 
 extern "C" int gdp_vector_norm_inf(const gdp_vector *v, double *norm)
 {
   if(!v || !norm) return set_error(GDP_E_ARG, "gdp_vector_norm_inf: null argument");
+  GDP_TRY(vec_flush(v));
   return reduce(v->n, v->d, nullptr, true, norm);
 }
 
 extern "C" int gdp_vector_dummy(gdp_vector *v, const int *dummy)
 {
   if(!v) return set_error(GDP_E_ARG, "gdp_vector_dummy: null argument");
+  GDP_TRY(vec_flush(v));
   if(!dummy || !v->n) return GDP_OK;  // no dummy DOFs
   int *dm = nullptr;
   GDP_TRY(dalloc(&dm, (size_t)v->n));

@@ -242,6 +242,7 @@ extern "C" int gdp_vector_size(const gdp_vector *v, int *n)
 extern "C" int gdp_vector_set(gdp_vector *v, const double *host, int n)
 {
   if(!v || (!host && n) || n != v->n) return set_error(GDP_E_ARG, "gdp_vector_set: size %d != %d", n, v ? v->n : -1);
+  v->staged.clear();  // setting every entry overwrites earlier additions
   GDP_TRY(h2d(v->d, host, (size_t)n));
   return sync();
 }
@@ -249,6 +250,7 @@ extern "C" int gdp_vector_set(gdp_vector *v, const double *host, int n)
 extern "C" int gdp_vector_get(const gdp_vector *v, double *host, int n)
 {
   if(!v || (!host && n) || n != v->n) return set_error(GDP_E_ARG, "gdp_vector_get: size %d != %d", n, v ? v->n : -1);
+  GDP_TRY(vec_flush(v));
   return d2h(host, v->d, (size_t)n);
 }
 
@@ -259,10 +261,9 @@ extern "C" int gdp_vector_add_double(gdp_vector *v, int i, double d)
   return GDP_OK;
 }
 
-extern "C" int gdp_vector_assemble(gdp_vector *v)
+int gdp::vec_flush(const gdp_vector *v)
 {
-  if(!v) return set_error(GDP_E_ARG, "null vector");
-  if(v->staged.empty()) return GDP_OK;
+  if(!v || v->staged.empty()) return GDP_OK;
   size_t m = v->staged.size();
   std::vector<int> idx(m);
   std::vector<double> val(m);
@@ -286,9 +287,16 @@ extern "C" int gdp_vector_assemble(gdp_vector *v)
   return r;
 }
 
+extern "C" int gdp_vector_assemble(gdp_vector *v)
+{
+  if(!v) return set_error(GDP_E_ARG, "null vector");
+  return vec_flush(v);
+}
+
 extern "C" int gdp_vector_norm2(const gdp_vector *v, double *norm)
 {
   if(!v || !norm) return set_error(GDP_E_ARG, "null argument");
+  GDP_TRY(vec_flush(v));
   if(v->n == 0) {
     *norm = 0;
     return GDP_OK;
@@ -310,6 +318,8 @@ extern "C" int gdp_vector_norm2(const gdp_vector *v, double *norm)
 extern "C" int gdp_vector_copy(const gdp_vector *src, gdp_vector *dst)
 {
   if(!src || !dst || src->n != dst->n) return set_error(GDP_E_ARG, "gdp_vector_copy: size mismatch");
+  GDP_TRY(vec_flush(src));
+  if(src != dst) dst->staged.clear();
   if(src->n)
     GDP_HIP(hipMemcpyAsync(dst->d, src->d, sizeof(double) * src->n, hipMemcpyDeviceToDevice,
                            ctx().stream));
@@ -498,6 +508,8 @@ extern "C" int gdp_prod_matrix_vector(gdp_matrix *A, const gdp_vector *x, gdp_ve
   if(!A || !x || !y || x->n != A->m || y->n != A->n)
     return set_error(GDP_E_ARG, "gdp_prod_matrix_vector: size mismatch");
   if(!A->csr.ia) return set_error(GDP_E_STATE, "matrix has no assembled pattern");
+  GDP_TRY(vec_flush(x));
+  if(y != x) y->staged.clear();  // y is overwritten
   GDP_TRY(matrix_sell(A));
   GDP_TRY(sell_spmv(A->sell, x->d, y->d));
   return sync();
@@ -509,6 +521,8 @@ extern "C" int gdp_bench_spmv(gdp_matrix *A, const gdp_vector *x, gdp_vector *y,
   if(!A || !x || !y || reps < 1 || !avg_ms || x->n != A->m || y->n != A->n)
     return set_error(GDP_E_ARG, "gdp_bench_spmv: bad arguments");
   if(!A->csr.ia) return set_error(GDP_E_STATE, "matrix has no assembled pattern");
+  GDP_TRY(vec_flush(x));
+  if(y != x) y->staged.clear();
   hipEvent_t e0, e1;
   GDP_HIP(hipEventCreate(&e0));
   GDP_HIP(hipEventCreate(&e1));

@@ -390,6 +390,8 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
   if(A->n != b->n || A->n != x->n) return set_error(GDP_E_ARG, "gdp_solve: size mismatch");
   if(!A->csr.ia) return set_error(GDP_E_STATE, "gdp_solve: matrix has no assembled pattern");
   GDP_TRY(ensure_ready());
+  GDP_TRY(vec_flush(b));
+  if(x != b) x->staged.clear();  // the solve starts from x = 0 and overwrites x
   const Params &p = S->p;
   gdp_solve_info &info = S->info;
   std::memset(&info, 0, sizeof(info));
@@ -409,6 +411,7 @@ int do_solve(gdp_solver *S, gdp_matrix *A, gdp_vector *b, gdp_vector *x, bool ag
   info.t_setup_ms = t1 - t0;
   IluFactor *F = c.factors(p.Preconditioner);
   info.ilu_levels = F ? F->nlevels : 0;
+  info.ilu_sweep = F ? F->sweep_kind() : 0;
   hipLaunchKernelGGL(k_permute_rhs, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, b->d, c.rs, c.d_rperm, c.rhs);
   GDP_HIP(hipGetLastError());
   double bnorm;
@@ -633,6 +636,8 @@ extern "C" int gdp_solver_prec_apply(gdp_solver *s, const gdp_vector *in, gdp_ve
     return set_error(GDP_E_STATE, "the solver holds no ILU factors (no solve with Preconditioner 1, 2 or 6)");
   if(in->n != c.n || out->n != c.n || in == out)
     return set_error(GDP_E_ARG, "gdp_solver_prec_apply: vectors must be distinct and of the system's size");
+  GDP_TRY(vec_flush(in));
+  out->staged.clear();
   hipEvent_t e0 = nullptr, e1 = nullptr;
   float ms = 0.f;
   int rc = sync();

@@ -84,7 +84,11 @@ typedef struct {
   double rel_res;     /* true_res / |b|_2                                  */
   double t_setup_ms;  /* renumbering + factorisation                       */
   double t_solve_ms;  /* Krylov iterations                                 */
+  int ilu_sweep;      /* kernels applying the ILU factors: 0 none,
+                         GDP_SWEEP_FREE sync-free multi-wave sweeps,
+                         GDP_SWEEP_WAVE one-wave level walk            */
 } gdp_solve_info;
+enum { GDP_SWEEP_FREE = 1, GDP_SWEEP_WAVE = 2 };
 int gdp_solver_info(const gdp_solver *s, gdp_solve_info *info);
 
 /* ---- vectors (gVector) --------------------------------------------------- */
@@ -96,7 +100,12 @@ int gdp_vector_size(const gdp_vector *v, int *n);             /* LinAlg_GetVecto
 int gdp_vector_set(gdp_vector *v, const double *host, int n); /* LinAlg_SetDoubleInVector, bulk */
 int gdp_vector_get(const gdp_vector *v, double *host, int n); /* LinAlg_GetDoubleInVector, bulk */
 /* LinAlg_AddDoubleInVector (LinAlg_SPARSKIT.cpp:356-364): staged on the host
- * and applied in call order by gdp_vector_assemble. */
+ * and applied in call order by gdp_vector_assemble, or by the first entry
+ * point that reads the vector (arithmetic, norms, dot, get, copy, products,
+ * solves, output), so that every read sees the additions as the reference's
+ * direct `V[i] += d` would.  Entry points that overwrite every entry
+ * (gdp_vector_set, _set_all, _zero, _scan, _read, and outputs of products
+ * and solves) drop pending additions, as the reference's overwrite does. */
 int gdp_vector_add_double(gdp_vector *v, int i, double d);
 int gdp_vector_assemble(gdp_vector *v);                       /* LinAlg_AssembleVector */
 int gdp_vector_norm2(const gdp_vector *v, double *norm);      /* LinAlg_VectorNorm2   */

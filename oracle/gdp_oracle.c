@@ -1780,11 +1780,15 @@ int orc_number_dofs(const orc_mesh *m, int nspaces, const orc_space *sp,
       }
     }
   }
-  /* fixed DOFs (Dof_DefineAssignFixedDof) */
+  /* fixed DOFs: the first constraint that holds the entity wins
+   * (Treatment_ConstraintForElement stops at the first match,
+   * Get_ConstraintOfElement.cpp:51-54), and Dof_DefineAssignFixedDof only
+   * turns an unknown DOF into a fixed one, never re-assigns a fixed one
+   * (DofData.cpp:1137-1151) */
   for(int s = 0; s < nspaces; s++)
     for(int f = 0; f < sp[s].nfixed; f++) {
       int ent = sp[s].fixed_entity[f];
-      if(ent >= 1 && ent <= nmax && dofnum[(size_t)s * (nmax + 1) + ent])
+      if(ent >= 1 && ent <= nmax && dofnum[(size_t)s * (nmax + 1) + ent] > 0)
         dofnum[(size_t)s * (nmax + 1) + ent] = -(f + 1);
     }
   /* number unknowns in (NumType, Entity) order: AVL in-order traversal,

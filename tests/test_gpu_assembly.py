@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from getdp_amd import problems as P
+from dirichlet import with_dirichlet
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +40,17 @@ CASES = {
     "poisson2d_az_32": lambda: P.poisson2d(32, basis="perp"),
     "poisson2d_az_general": az_general,
 }
+
+
+# non-zero fixed values, some entities listed twice with different values
+# (the first listed value must win), on every family
+DIRICHLET_CASES = {
+    "poisson2d_24_dirichlet": lambda: with_dirichlet(P.poisson2d(24), seed=11),
+    "elasticity3d_4_dirichlet": lambda: with_dirichlet(P.elasticity3d(4, nbins=16), seed=12),
+    "curlcurl3d_4_dirichlet": lambda: with_dirichlet(P.curlcurl3d(4), seed=13),
+    "poisson2d_az_16_dirichlet": lambda: with_dirichlet(P.poisson2d(16, basis="perp"), seed=14),
+}
+CASES.update(DIRICHLET_CASES)
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -156,3 +168,21 @@ def test_device_pre_pass_matches_host(gpu, monkeypatch, name):
         assert np.array_equal(h[0][s], d[0][s])
     for k in range(1, 5):
         assert np.array_equal(h[k], d[k])
+
+
+@pytest.mark.parametrize("name", list(DIRICHLET_CASES))
+def test_solve_with_nonzero_dirichlet(gpu, oracle, name):
+    """ILU(0)-GMRES on the lifted systems: solution and iteration count
+    against the oracle's."""
+    pr = DIRICHLET_CASES[name]()
+    print(f"case {name}", flush=True)
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    fe, A, bv = gpu.assemble(pr)
+    s = gpu.Solver(Algorithm=8, Preconditioner=6, Krylov_Size=30)
+    x = gpu.Vector(fe.N)
+    info = gpu.solve(A, bv, s, x)
+    xo, io, _ = oracle.solve(ia, ja, a, b, algorithm=8, preconditioner=6, krylov=30)
+    assert info["status"] == 0 and info["rel_res"] <= 1.5e-10
+    assert abs(info["its"] - io["its"]) <= max(2, io["its"] // 50), (info["its"], io["its"])
+    tol = 1e-8 if name.startswith("curlcurl") else 1e-9
+    assert np.abs(x.get() - xo).max() <= tol * np.abs(xo).max()

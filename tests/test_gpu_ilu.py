@@ -30,10 +30,13 @@ def _device_apply(gpu, ia, ja, a, v, f64, reps=1):
     x = gpu.Vector(n)
     s = gpu.Solver(Algorithm=8, Preconditioner=6, Krylov_Size=30, Renumbering_Technique=1,
                    ILU_Storage=1 if f64 else 0, Nb_Iter_Max=3)
-    gpu.solve(A, b, s, x)  # builds the renumbering and the factors
+    info = gpu.solve(A, b, s, x)  # builds the renumbering and the factors
     out = gpu.Vector(n)
     gpu.solver_prec_apply(s, gpu.Vector(n, v), out, reps=reps)
-    return out.get()
+    return out.get(), info["ilu_sweep"]
+
+
+SWEEP_KIND = {"free": 1, "wave": 2}
 
 
 @pytest.mark.parametrize("sweep", ["free", "wave"])
@@ -51,7 +54,8 @@ def test_ilu_apply_bit_exact(gpu, oracle, name, f64, sweep, monkeypatch):
     assert ierr == 0
     v = np.random.default_rng(5).standard_normal(ib.shape[0] - 1)
     want = oracle.lusol(alu, jlu, ju, v)
-    got = _device_apply(gpu, ia, ja, a, v, f64, reps=3)  # repeated: sweep state reset each time
+    got, kind = _device_apply(gpu, ia, ja, a, v, f64, reps=3)  # repeated: sweep state reset each time
+    assert kind == SWEEP_KIND[sweep], kind  # the forced kernel really ran
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), np.abs(got - want).max()
 
 
@@ -75,7 +79,7 @@ def test_ilu_apply_one_lane_rows(gpu, oracle, f64):
     alu, jlu, ju, ierr = oracle.ilu0(ib, jb, bb, double=f64)
     v = rng.standard_normal(n)
     want = oracle.lusol(alu, jlu, ju, v)
-    got = _device_apply(gpu, ia, ja, a, v, f64)
+    got, _ = _device_apply(gpu, ia, ja, a, v, f64)
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
     if f64:
         Bv = oracle.amux(ib, jb, bb, got)
@@ -96,7 +100,9 @@ def test_ilu_apply_lane_counts(gpu, oracle, monkeypatch, lanes):
     alu, jlu, ju, ierr = oracle.ilu0(ib, jb, bb)
     v = np.random.default_rng(6).standard_normal(ib.shape[0] - 1)
     want = oracle.lusol(alu, jlu, ju, v)
-    got = _device_apply(gpu, ia, ja, a, v, False)
+    got, kind = _device_apply(gpu, ia, ja, a, v, False)
+    if lanes != "thread-rows":
+        assert kind == SWEEP_KIND["free"]  # a lane count selects the sync-free sweeps
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
 
 
@@ -124,7 +130,11 @@ def test_ilut_apply_bit_exact(gpu, oracle, name, f64, pivot, lanes, monkeypatch)
     A.set_csr(ia, ja, a)
     s = gpu.Solver(Algorithm=8, Preconditioner=2 if pivot else 1, Krylov_Size=40, ILU_Storage=1 if f64 else 0,
                    Nb_Iter_Max=3)
-    gpu.solve(A, gpu.Vector(n, np.ones(n)), s, gpu.Vector(n))
+    info = gpu.solve(A, gpu.Vector(n, np.ones(n)), s, gpu.Vector(n))
+    if lanes == "wave":
+        assert info["ilu_sweep"] == SWEEP_KIND["wave"]
+    elif lanes == "8":
+        assert info["ilu_sweep"] == SWEEP_KIND["free"]
     out = gpu.Vector(n)
     gpu.solver_prec_apply(s, gpu.Vector(n, v), out, reps=3)
     got = out.get()
@@ -148,8 +158,30 @@ def test_ilut_apply_long_chain(gpu, oracle, sweep, pivot, monkeypatch):
     A = gpu.Matrix(n)
     A.set_csr(ia, ja, a)
     s = gpu.Solver(Algorithm=8, Preconditioner=2 if pivot else 1, Krylov_Size=40, Nb_Iter_Max=3)
-    gpu.solve(A, gpu.Vector(n, np.ones(n)), s, gpu.Vector(n))
+    info = gpu.solve(A, gpu.Vector(n, np.ones(n)), s, gpu.Vector(n))
+    assert info["ilu_sweep"] == SWEEP_KIND[sweep]
     out = gpu.Vector(n)
     gpu.solver_prec_apply(s, gpu.Vector(n, v), out, reps=2)
     got = out.get()
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), np.abs(got - want).max()
+
+
+def test_forced_wave_sweep_rejects_what_it_cannot_take(gpu, oracle, monkeypatch):
+    """GDP_ILU_SWEEP=wave on a system whose dependencies leave the one-wave
+    sweeps' 16,384-row LDS ring (natural ordering of a 3-block elasticity
+    system, 26,460 rows: uz rows depend on ux rows ~2N/3 back) must fail with GDP_E_UNSUPPORTED,
+    not fall back silently; unforced, the same system solves."""
+    from getdp_amd._lib import GdpError
+    ia, ja, a, b, tab = oracle.assemble(P.elasticity3d(20, nbins=8))
+    n = ia.shape[0] - 1
+    assert n > 16384  # 26,460 rows
+    A = gpu.Matrix(n)
+    A.set_csr(ia, ja, a)
+    opts = dict(Algorithm=8, Preconditioner=6, Krylov_Size=30, Renumbering_Technique=0, Nb_Iter_Max=2)
+    monkeypatch.setenv("GDP_ILU_SWEEP", "wave")
+    with pytest.raises(GdpError) as e:
+        gpu.solve(A, gpu.Vector(n, np.ones(n)), gpu.Solver(**opts), gpu.Vector(n))
+    assert "ring" in str(e.value)
+    monkeypatch.delenv("GDP_ILU_SWEEP")
+    info = gpu.solve(A, gpu.Vector(n, np.ones(n)), gpu.Solver(**opts), gpu.Vector(n))
+    assert info["ilu_sweep"] == SWEEP_KIND["free"]

@@ -138,3 +138,76 @@ def test_matrix_ops_same_and_different_patterns(gpu, oracle):
     _ok(L.gdp_matrix_size(A2.h, C.byref(nn), C.byref(mm)))
     assert nn.value == n
     assert L.gdp_matrix_add(A2.h, Cm.h, _matrix(gpu, C_ia, C_ja, C_a).h) == -1
+
+
+def test_staged_additions_count_on_every_operand(gpu):
+    """LinAlg_AddDoubleInVector adds straight into the vector in the
+    reference (LinAlg_SPARSKIT.cpp:356-364): pending additions must be seen
+    by every operation that reads a vector, on either operand, and dropped by
+    every operation that overwrites all of it."""
+    import ctypes as C
+    L = lib()
+    n = 1000
+    rng = np.random.default_rng(9)
+    u0, v0 = rng.standard_normal(n), rng.standard_normal(n)
+    adds = [(3, 0.5), (17, -2.0), (3, 1.25), (999, 4.0)]
+
+    def staged(base):
+        V = gpu.Vector(n, base)
+        for i, d in adds:
+            V.add_double(i, d)
+        want = base.copy()
+        for i, d in adds:
+            want[i] = want[i] + d
+        return V, want
+
+    # second operand of add / scaled add / subtract (both forms) / swap
+    for op in ("add", "add_scaled", "sub", "sub_rev", "swap"):
+        U = gpu.Vector(n, u0)
+        V, vw = staged(v0)
+        if op == "add":
+            _ok(L.gdp_vector_add(U.h, V.h, U.h))
+            assert _eq(U.get(), u0 + vw)
+        elif op == "add_scaled":
+            _ok(L.gdp_vector_add_scaled(U.h, V.h, 0.3, U.h))
+            assert _eq(U.get(), u0 + 0.3 * vw)
+        elif op == "sub":
+            _ok(L.gdp_vector_sub(U.h, V.h, U.h))
+            assert _eq(U.get(), u0 - vw)
+        elif op == "sub_rev":
+            _ok(L.gdp_vector_sub(U.h, V.h, V.h))
+            assert _eq(V.get(), u0 - vw)
+        else:
+            _ok(L.gdp_vector_swap(U.h, V.h))
+            assert _eq(U.get(), vw) and _eq(V.get(), u0)
+    # first operand too
+    U, uw = staged(u0)
+    V = gpu.Vector(n, v0)
+    _ok(L.gdp_vector_add(U.h, V.h, U.h))
+    assert _eq(U.get(), uw + v0)
+    # dot, inf-norm, 2-norm, get
+    U, uw = staged(u0)
+    V, vw = staged(v0)
+    r = C.c_double()
+    _ok(L.gdp_vector_dot(U.h, V.h, C.byref(r)))
+    assert abs(r.value - math.fsum((uw * vw).tolist())) <= n * 2.2e-16 * np.abs(uw * vw).sum()
+    big = np.zeros(n)
+    B = gpu.Vector(n, big)
+    B.add_double(5, -7.0)
+    _ok(L.gdp_vector_norm_inf(B.h, C.byref(r)))
+    assert r.value == 7.0
+    B = gpu.Vector(n, big)
+    B.add_double(5, 3.0)
+    B.add_double(6, 4.0)
+    assert B.norm2() == 5.0
+    U, uw = staged(u0)
+    assert _eq(U.get(), uw)
+    # set overwrites earlier additions: a later assemble must not re-add them
+    U, uw = staged(u0)
+    U.set(v0)
+    U.assemble()
+    assert _eq(U.get(), v0)
+    U, uw = staged(u0)
+    _ok(L.gdp_vector_set_all(U.h, 1.5))
+    U.assemble()
+    assert np.all(U.get() == 1.5)

@@ -11,6 +11,7 @@ import pytest
 
 from getdp_amd import _host, problems as P
 from getdp_amd._lib import LIB_PATH, SIGNATURES, GdpError, check, lib
+from dirichlet import with_dirichlet
 
 ROOT = Path(__file__).resolve().parents[1]
 
@@ -68,6 +69,9 @@ PROBLEMS = {
     "elasticity3d_3": lambda: P.elasticity3d(3, nbins=8),
     "curlcurl3d_3": lambda: P.curlcurl3d(3),
     "poisson2d_az_12": lambda: P.poisson2d(12, basis="perp"),
+    "poisson2d_dirichlet_dup": lambda: with_dirichlet(P.poisson2d(10), seed=1),
+    "elasticity3d_dirichlet_dup": lambda: with_dirichlet(P.elasticity3d(3, nbins=8), seed=2),
+    "curlcurl3d_dirichlet_dup": lambda: with_dirichlet(P.curlcurl3d(3), seed=3),
 }
 
 

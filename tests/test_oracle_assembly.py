@@ -272,3 +272,133 @@ def test_perpendicular_edge_is_the_nodal_problem(oracle):
     assert np.array_equal(b, b2)
     assert np.abs(a - a2).max() <= 1e-13 * np.abs(a).max()
     assert not np.array_equal(a, a2)      # the two change-of-coordinate paths round differently
+
+
+# ---------------------------------------------------------------------------
+# Fixed DOFs with non-zero values: the first constraint holding an entity wins
+# (Get_ConstraintOfElement.cpp:51-54, DofData.cpp:1137-1151), and the
+# unknown x fixed branch of Dof_AssembleInMat lifts b -= value * Ek
+# (DofData.cpp:1489-1500).
+
+
+def test_first_constraint_wins_hand_worked(oracle):
+    """3x3 nodes, 8 right triangles, unit reluctivity, no source: node 5 (the
+    centre) is the only unknown.  Its row of the P1 stiffness is the 5-point
+    Laplacian scaled by W3/0.5 (the hypotenuse couplings to nodes 1 and 9 are
+    exact zeros), so b_5 = (W3/0.5) * (v2 + v4 + v6 + v8) with the FIRST value
+    listed for each node."""
+    pr = P.poisson2d(2, nu_vals=(1.0, 1.0), js=0.0)
+    ent = [2, 4, 6, 8, 1, 3, 7, 9, 2, 6, 4]
+    val = [1.0, 2.0, 3.0, 4.0, 7.0, 7.0, 7.0, 7.0, 100.0, -50.0, 9.0]
+    pr.spaces[0].fixed_entity = np.array(ent, np.int32)
+    pr.spaces[0].fixed_value = np.array(val)
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    assert b.shape == (1,) and tab[0][5] == 1
+    # each boundary node keeps the index of its first occurrence
+    for node, first in {2: 0, 4: 1, 6: 2, 8: 3, 1: 4, 3: 5, 7: 6, 9: 7}.items():
+        assert tab[0][node] == -(first + 1)
+    s = W3 / 0.5
+    assert a[0] == pytest.approx(4 * s, rel=1e-15)
+    assert b[0] == pytest.approx(s * (1 + 2 + 3 + 4), rel=1e-14)
+    # a last-wins rule would give s * (100 + 9 - 50 + 4)
+    assert abs(b[0] - s * 63) > 1.0
+
+
+def _solve_with_fixed(pr, oracle):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    N = b.shape[0]
+    A = sp.csr_matrix((a, ja, ia), shape=(N, N))
+    x = spla.spsolve(A.tocsc(), b)
+    return x, tab
+
+
+def test_dirichlet_patch_test_poisson(oracle):
+    """An affine field is the exact discrete solution of the Laplace problem
+    with its own boundary values (P1 reproduces linear functions), so every
+    node must carry it to rounding; the lifting of non-zero fixed values is
+    the only thing that drives the solution."""
+    n = 9
+    pr = P.poisson2d(n, nu_vals=(2.5, 2.5), js=0.0)
+    sp0 = pr.spaces[0]
+    u = lambda X: 1.0 + 2.0 * X[:, 0] - 3.0 * X[:, 1]
+    sp0.fixed_value = u(pr.xyz[sp0.fixed_entity - 1])
+    x, tab = _solve_with_fixed(pr, oracle)
+    t = tab[0]
+    inner = np.flatnonzero(t > 0)
+    got = x[t[inner] - 1]
+    assert np.abs(got - u(pr.xyz[inner - 1])).max() <= 1e-12
+    assert inner.size == (n - 1) ** 2
+
+
+def test_dirichlet_patch_test_elasticity(oracle):
+    """Constant strain: u = c + G x with uniform material is in equilibrium
+    with zero body force, so prescribing it on the whole boundary of the cube
+    must reproduce it at every interior node (all three block-numbered
+    components)."""
+    n = 3
+    pr = P.elasticity3d(n, sigma=0.0, traction=0.0)
+    X = pr.xyz
+    bnd = (np.flatnonzero(np.any((X == 0.0) | (X == 1.0), axis=1)) + 1).astype(np.int32)
+    G = np.array([[0.3, -0.2, 0.1], [0.05, 0.4, -0.3], [-0.1, 0.2, 0.25]])
+    c = np.array([1.0, -2.0, 0.5])
+    U = c + X @ G.T
+    for s in range(3):
+        pr.spaces[s].fixed_entity = bnd
+        pr.spaces[s].fixed_value = U[bnd - 1, s]
+    x, tab = _solve_with_fixed(pr, oracle)
+    for s in range(3):
+        inner = np.flatnonzero(tab[s] > 0)
+        assert inner.size == (n - 1) ** 3
+        assert np.abs(x[tab[s][inner] - 1] - U[inner - 1, s]).max() <= 1e-12
+
+
+@pytest.mark.parametrize("family", ["poisson", "elasticity", "curlcurl", "az"])
+def test_dirichlet_lifting_equals_unconstrained_system(oracle, family):
+    """The same problem assembled without constraints gives the full
+    operator K and load f over every entity; with fixed entities F (values
+    v, the first listed value of each) the constrained system must be
+    A = K[U, U] and b = f[U] - K[U, F] v.  Checks the lifting of non-zero
+    values through a different code path (unknown x unknown only) for every
+    family, with entities listed twice under different values."""
+    import copy
+    from dirichlet import first_values, with_dirichlet
+    make = {"poisson": lambda: P.poisson2d(6, field_kind="lognormal", nbins=4),
+            "elasticity": lambda: P.elasticity3d(2, nbins=4),
+            "curlcurl": lambda: P.curlcurl3d(3),
+            "az": lambda: P.poisson2d(6, basis="perp")}[family]
+    pr = with_dirichlet(make(), seed=5, ndup=8)
+    free = copy.deepcopy(pr)
+    for sp_ in free.spaces:
+        sp_.fixed_entity = np.zeros(0, np.int32)
+        sp_.fixed_value = np.zeros(0)
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    fia, fja, fa, fb, ftab = oracle.assemble(free)
+    Nf = fb.shape[0]
+    K = np.zeros((Nf, Nf))
+    for r in range(Nf):
+        K[r, fja[fia[r]:fia[r + 1]]] += fa[fia[r]:fia[r + 1]]
+    U, F, vF = [], [], []
+    for si, sp_ in enumerate(pr.spaces):
+        fv = first_values(sp_)
+        for ent in np.flatnonzero(ftab[si] > 0):
+            if tab[si][ent] > 0:
+                U.append((tab[si][ent] - 1, ftab[si][ent] - 1))
+            else:
+                assert tab[si][ent] < 0 and ent in fv
+                # the constrained table points at the first occurrence
+                assert sp_.fixed_value[-tab[si][ent] - 1] == fv[ent]
+                F.append(ftab[si][ent] - 1)
+                vF.append(fv[ent])
+    U.sort()
+    urow = np.array([u[1] for u in U])
+    assert len(U) == b.shape[0] and len(F) > 0
+    N = b.shape[0]
+    A = np.zeros((N, N))
+    for r in range(N):
+        A[r, ja[ia[r]:ia[r + 1]]] += a[ia[r]:ia[r + 1]]
+    assert np.array_equal(A, K[np.ix_(urow, urow)])
+    expect = fb[urow] - K[np.ix_(urow, F)] @ np.array(vF)
+    scale = np.abs(K[np.ix_(urow, F)]).max() * np.abs(vF).max() + np.abs(fb).max()
+    assert np.abs(b - expect).max() <= 1e-13 * scale
