@@ -113,6 +113,7 @@ SIGNATURES = [
     ("gdp_vector_swap", C.c_int, [_vp, _vp]),
     ("gdp_vector_set_all", C.c_int, [_vp, C.c_double]),
     ("gdp_vector_dot", C.c_int, [_vp, _vp, _dp]),
+    ("gdp_vector_dot_noalloc", C.c_int, [_vp, _vp, _dp]),
     ("gdp_vector_norm_inf", C.c_int, [_vp, _dp]),
     ("gdp_vector_dummy", C.c_int, [_vp, _ip]),
     ("gdp_matrix_add", C.c_int, [_vp, _vp, _vp]),

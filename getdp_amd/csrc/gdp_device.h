@@ -169,6 +169,8 @@ struct gdp_vector {
 namespace gdp {
 // Apply the vector's staged additions in call order (a no-op without any).
 int vec_flush(const gdp_vector *v);
+// Per-device set-up of gdp_vector_dot_noalloc's scratch address (gdp_init).
+int noalloc_setup(int device);
 }  // namespace gdp
 
 struct gdp_matrix {

@@ -92,21 +92,18 @@ int vec_op(int op, double d, gdp_vector *u, gdp_vector *v)
   return sync();
 }
 
-int reduce(int n, const double *u, const double *v, bool absmax, double *out)
+// Every reduction launches kParts blocks (grid-stride over the entries), each
+// writing one partial; the host combines the partials in block order.
+constexpr int kParts = 512;
+
+// The reduction over the partial buffer `part` (kParts doubles on the device).
+int reduce_into(int n, const double *u, const double *v, bool absmax, double *part, double *out)
 {
-  if(!n) {
-    *out = 0.;
-    return GDP_OK;
-  }
-  const int nb = 512;
-  double *part = nullptr;
-  GDP_TRY(dalloc(&part, nb));
-  if(absmax) hipLaunchKernelGGL(k_absmax_partial, dim3(nb), dim3(kB), 0, ctx().stream, (long)n, u, part);
-  else hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kB), 0, ctx().stream, (long)n, u, v, part);
-  std::vector<double> h(nb);
-  int r = d2h(h.data(), part, nb);
-  dfree(part);
-  GDP_TRY(r);
+  if(absmax) hipLaunchKernelGGL(k_absmax_partial, dim3(kParts), dim3(kB), 0, ctx().stream, (long)n, u, part);
+  else hipLaunchKernelGGL(k_dot_partial, dim3(kParts), dim3(kB), 0, ctx().stream, (long)n, u, v, part);
+  GDP_HIP(hipGetLastError());
+  double h[kParts];
+  GDP_TRY(d2h(h, part, kParts));
   double s = 0.;
   for(double x : h) {
     if(absmax) {
@@ -118,6 +115,30 @@ int reduce(int n, const double *u, const double *v, bool absmax, double *out)
   *out = s;
   return GDP_OK;
 }
+
+int reduce(int n, const double *u, const double *v, bool absmax, double *out)
+{
+  if(!n) {
+    *out = 0.;
+    return GDP_OK;
+  }
+  double *part = nullptr;
+  GDP_TRY(dalloc(&part, kParts));
+  int r = reduce_into(n, u, v, absmax, part, out);
+  dfree(part);
+  return r;
+}
+
+// ---- the allocation-free dot product -----------------------------------------
+// gdp_vector_dot_noalloc keeps its kParts partial sums (4 KiB) in this array
+// instead of a hipMalloc'd buffer.  Every GPU the library runs on holds its own
+// copy until the process ends.  In host code the array's name is not a device
+// address: the address hipGetSymbolAddress returns for each device, looked up
+// once per device at gdp_init with that device current, is what host code
+// passes to the kernels.
+static __device__ __align__(256) double scratch[kParts];
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {};
 
 __global__ void __launch_bounds__(kB) k_mat_axpy(long nnz, double d, int use_d, double *__restrict__ a,
                                                  const double *__restrict__ b)
@@ -181,6 +202,17 @@ int mat_add(gdp_matrix *dst, const gdp_matrix *src, double d, bool use_d)
 
 }  // namespace
 
+// Called by gdp_init with `device` current.
+int noalloc_setup(int device)
+{
+  if(device < 0 || device >= kMaxDevices) return set_error(GDP_E_ARG, "device %d: at most %d devices", device, kMaxDevices);
+  if(g_scratch[device]) return GDP_OK;
+  void *p = nullptr;
+  GDP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch[device] = static_cast<double *>(p);
+  return GDP_OK;
+}
+
 }  // namespace gdp
 
 using namespace gdp;
@@ -241,6 +273,24 @@ extern "C" int gdp_vector_dot(const gdp_vector *v1, const gdp_vector *v2, double
   GDP_TRY(vec_flush(v1));
   GDP_TRY(vec_flush(v2));
   return reduce(v1->n, v1->d, v2->d, false, d);
+}
+
+extern "C" int gdp_vector_dot_noalloc(const gdp_vector *v1, const gdp_vector *v2, double *d)
+{
+  if(!v1 || !v2 || !d) return set_error(GDP_E_ARG, "gdp_vector_dot_noalloc: null argument");
+  if(v1->n != v2->n) return set_error(GDP_E_ARG, "gdp_vector_dot_noalloc: size mismatch");
+  // initialising the library or applying staged additions would allocate
+  if(!ctx().ready) return set_error(GDP_E_STATE, "gdp_vector_dot_noalloc: call gdp_init first");
+  if(!v1->staged.empty() || !v2->staged.empty())
+    return set_error(GDP_E_STATE, "gdp_vector_dot_noalloc: a vector has pending additions (gdp_vector_assemble first)");
+  double *part = g_scratch[ctx().device];
+  if(!part) return set_error(GDP_E_STATE, "gdp_vector_dot_noalloc: no scratch address for device %d", ctx().device);
+  if(!v1->n) {
+    *d = 0.;
+    return GDP_OK;
+  }
+  // any length fits: the launch is always kParts blocks, one partial each
+  return reduce_into(v1->n, v1->d, v2->d, false, part, d);
 }
 
 extern "C" int gdp_vector_norm_inf(const gdp_vector *v, double *norm)

@@ -167,6 +167,7 @@ extern "C" int gdp_init(int device)
   GDP_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   c.pinned_bytes = 1 << 16;
   GDP_HIP(hipHostMalloc(&c.pinned, c.pinned_bytes, hipHostMallocDefault));
+  GDP_TRY(noalloc_setup(device));
   c.device = device;
   c.ready = true;
   return GDP_OK;

@@ -133,6 +133,16 @@ int gdp_vector_scale(gdp_vector *v1, double d, gdp_vector *v2);
 int gdp_vector_swap(gdp_vector *v1, gdp_vector *v2);
 int gdp_vector_set_all(gdp_vector *v, double d);
 int gdp_vector_dot(const gdp_vector *v1, const gdp_vector *v2, double *d);
+/* gdp_vector_dot without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, code that runs beside graph capture).  Same kernels and
+ * summation order as gdp_vector_dot, so the same bits.  Its 512 partial sums
+ * live in a fixed 4 KiB device array, one per GPU, held until the process
+ * ends; vectors of any length fit (the reduction always runs 512 blocks).
+ * Needs gdp_init first; refuses vectors with pending gdp_vector_add_double
+ * additions (GDP_E_STATE), since applying them needs device memory.  Calls
+ * that overlap on one GPU share the array and race. */
+int gdp_vector_dot_noalloc(const gdp_vector *v1, const gdp_vector *v2, double *d);
 int gdp_vector_norm_inf(const gdp_vector *v, double *norm);
 int gdp_vector_dummy(gdp_vector *v, const int *dummy);
 /* Vector text and binary I/O (LinAlg_PrintVector / LinAlg_WriteVector /
