@@ -35,6 +35,8 @@ struct IluFactor {
   // c * 64 + l, set on the first lane of each row's group only.
   struct WaveSweep {
     int G = 1, wg = 0, nch = 0;
+    int team = 1;         // waves sharing the walk (one workgroup)
+    int *need = nullptr;  // per chunk: every chunk below it must be done first (team > 1)
     int *col = nullptr;   // LDS ring slot of the entry's column (the zero slot: padding)
     int *kpos = nullptr;  // position of the entry in the factors (-1: padding)
     int *row = nullptr;   // row of the lane's group (-1: none)
@@ -78,6 +80,7 @@ struct IluFactor {
   int setup_sweeps(const int *hia, const int *hja);
   int setup_wave(const int *hia, const int *hja, const std::vector<int> &lrows,
                  const std::vector<int> &lptr);
+  long wave_chunks_alloc(const WaveSweep &W) const;
   int wave_alloc_values();
   void wave_pack();
   void wave_release();

@@ -350,51 +350,72 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
                                                           addto);
 }
 
-// ---- one-wave sweeps -----------------------------------------------------------
+// ---- level walks in one workgroup ----------------------------------------------
 //
 // With a long dependency chain of narrow levels (threshold ILU with fill:
 // ILUTP(20) on Poisson 256^2 has about 10,000 levels of 6 rows per triangle)
 // the sweeps above pay one publication round trip through the memory
-// hierarchy per level.  Here one wave walks the levels in order instead.  A
-// chunk holds up to 64/G rows of one level, G lanes per row, lane q of a row
-// holding its entries q, q + G, ...; the solution entries of the last kRing
-// rows stay in an LDS ring indexed by row mod kRing (the host checks that
-// every dependency is still there when it is read); the chunks are loaded
+// hierarchy per level.  Here one workgroup walks the levels in order instead.
+// A chunk holds up to 64/G rows of one level, G lanes per row, lane q of a
+// row holding its entries q, q + G, ...; row t of chunk c has the stream
+// position c * 64/G + t, and the solution entries of the last kRing stream
+// positions stay in an LDS ring indexed by position mod kRing (the host
+// checks that every dependency is still there when it is read: in level
+// order a dependency is a few levels back, 255 positions at most on ILU(0)
+// of Poisson 256^2 in RCMK order, about 6,000 on elasticity 32^3, although
+// the row numbers of one level span the whole matrix); the chunks are loaded
 // a few chunks ahead.  A level then costs LDS reads, the row's subtraction
 // chain and an LDS write, with no memory round trip.  Every row subtracts its
 // products in ascending position (lusol's order; padding subtracts +0 last).
 constexpr int kRing = 16384;
-// Chunks in flight: a chunk is 2 * WG + 3 loads per lane, and the loads in
+// Chunks in flight: a chunk is 2 * WG + 4 loads per lane, and the loads in
 // flight must stay within what the wave's memory counter tracks (63), or the
-// compiler waits for all of them at the top of every round.
-__host__ __device__ constexpr int wave_depth(int wg) { return 60 / (2 * wg + 3); }
+// compiler waits for all of them at the top of every round.  The widest
+// shape keeps two chunks in flight, so that a team of 8 waves (two per SIMD)
+// holds both rounds in registers.
+__host__ __device__ constexpr int wave_depth(int wg) { return wg >= 8 ? 2 : 60 / (2 * wg + 4); }
 // (lanes per row, entry slots per lane) of the kernel instances, by
 // increasing capacity; a sweep takes the first that holds its longest row
 constexpr int kWaveShapes[][2] = {{1, 4}, {2, 4}, {4, 3}, {4, 6}, {8, 8}};
 constexpr int kNumWaveShapes = 5;
 
+// Several waves of one workgroup may share the walk (a team): wave w takes
+// chunks w, w + T, w + 2T, ... (T waves).  Chunk c may start once every chunk
+// below need[c] is done: need[c] covers the chunks holding c's dependencies,
+// and, for each ring slot c writes, the chunk that last wrote it and the
+// last chunk that reads the value it holds (the host computes need from the
+// sequential order, in which the ring check holds).  Each wave publishes how
+// many of its chunks are done in LDS; chunks below s are all done when every
+// wave v has finished its ceil((s - v) / T) chunks below s.
+constexpr int kMaxTeam = 8;
+
 template <typename TF, typename TV, int WG> struct WaveRec {
   int c[WG];
   TF v[WG];
-  int r, a;
+  int r, a, nd;
   TF d;
   TV p;
 };
 
-// nch (a multiple of the depth) chunks, followed by depth padding chunks that
-// the last prefetches read; padding rows are -1 and never stored.
+// nch chunks (a multiple of team * depth), followed by team * depth padding
+// chunks that the last prefetches read; padding rows are -1 and never stored.
 template <typename TF, typename TV, int G, int WG, bool FWD>
-__global__ void __launch_bounds__(64) k_wave_sweep(int nch, const int *__restrict__ col, const TF *__restrict__ val,
+__global__ void __launch_bounds__(kWave *kMaxTeam) k_wave_sweep(int nch, const int *__restrict__ col, const TF *__restrict__ val,
                                                    const int *__restrict__ row, const int *__restrict__ aux,
+                                                   const int *__restrict__ need,
                                                    const TF *__restrict__ dinv, const TV *__restrict__ pre, TV *out,
-                                                   const int *gate, int gmode)
+                                                   int *err, unsigned long long tlimit, const int *gate, int gmode)
 {
   if(is_gated(gate, gmode)) return;
   constexpr int kDepth = wave_depth(WG);
   __shared__ TV ring[kRing + 1];
-  const int lane = threadIdx.x, q = lane & (G - 1), gbase = lane - q;
-  if(lane == 0) ring[kRing] = 0;  // the padding slot
+  __shared__ int prog[kMaxTeam];
+  const int lane = threadIdx.x & (kWave - 1), q = lane & (G - 1), gbase = lane - q;
+  const int team = blockDim.x / kWave, wv = threadIdx.x / kWave;
+  if(threadIdx.x == 0) ring[kRing] = 0;  // the padding slot
+  if(threadIdx.x < kMaxTeam) prog[threadIdx.x] = 0;
   __syncthreads();
+  bool expired = false;
   // whole rounds double-buffered: the next round's loads are issued before
   // the current round is computed, so the one wait per round (the output
   // stores share the loads' counter, which makes the compiler wait for all
@@ -412,15 +433,34 @@ __global__ void __launch_bounds__(64) k_wave_sweep(int nch, const int *__restric
     if constexpr(FWD) b.a = aux[li];
     else b.d = dinv[li];
     b.p = pre[li];
+    b.nd = team > 1 ? need[c] : 0;
   };
+  // this wave's k-th chunk
+  auto chunk = [&](int k) { return wv + team * k; };
+  const int nk = nch / team;
 #pragma unroll
-  for(int d = 0; d < kDepth; d++) load(cur[d], d);
-  for(int c0 = 0; c0 < nch; c0 += kDepth) {
+  for(int d = 0; d < kDepth; d++) load(cur[d], chunk(d));
+  for(int k0 = 0; k0 < nk; k0 += kDepth) {
 #pragma unroll
-    for(int d = 0; d < kDepth; d++) load(nxt[d], c0 + kDepth + d);
+    for(int d = 0; d < kDepth; d++) load(nxt[d], chunk(k0 + kDepth + d));
 #pragma unroll
     for(int d = 0; d < kDepth; d++) {
       const WaveRec<TF, TV, WG> &rc = cur[d];
+      if(team > 1 && rc.nd > 0) {
+        // wait until every chunk below rc.nd is done (bounded)
+        const int s = rc.nd;
+        const int want = lane < team && s > lane ? (s - lane + team - 1) / team : 0;
+        const unsigned long long t0 = wall_clock64();
+        while(true) {
+          const int have =
+            lane < team ? __hip_atomic_load(prog + lane, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
+          if(__all(have >= want) || expired) break;
+          if(wall_clock64() - t0 > tlimit) {
+            expired = true;
+            if(lane == 0) atomicOr(err, 1);
+          }
+        }
+      }
       TV prod[WG];
 #pragma unroll
       for(int w = 0; w < WG; w++) prod[w] = (TV)rc.v[w] * ring[rc.c[w]];
@@ -435,10 +475,12 @@ __global__ void __launch_bounds__(64) k_wave_sweep(int nch, const int *__restric
       }
       if(q == 0 && rc.r >= 0) {
         const TV xv = FWD ? acc : (TV)rc.d * acc;
-        ring[rc.r & (kRing - 1)] = xv;
+        ring[(chunk(k0 + d) * (kWave / G) + lane / G) & (kRing - 1)] = xv;  // its stream position
         if constexpr(FWD) out[rc.a] = xv;  // the backward sweep's input, in its order
         else out[rc.r] = xv;
       }
+      if(team > 1 && lane == 0)  // this wave's chunks up to k0 + d are done
+        __hip_atomic_store(prog + wv, k0 + d + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 #pragma unroll
     for(int d = 0; d < kDepth; d++) cur[d] = nxt[d];
@@ -555,6 +597,7 @@ void IluFactor::wave_release()
     dfree(w.dinv64);
     dfree(w.pre64);
     dfree(w.pre32);
+    dfree(w.need);
     w = WaveSweep();
   }
   wave_mode = false;
@@ -601,6 +644,7 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
   struct Side {
     int G, wg, rc;
     std::vector<int> cstart, ccount;  // chunk -> rows (index into the ordered rows)
+    std::vector<int> spos;            // row -> stream position c * rc + t (its ring slot mod kRing)
   } sd[2];
   const std::vector<int> *rows[2] = {&lrows, &urows};
   const std::vector<int> *ptrs[2] = {&lptr, &uptr};
@@ -626,32 +670,85 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
         sd[s].cstart.push_back(a);
         sd[s].ccount.push_back(std::min(sd[s].rc, p[l + 1] - a));
       }
-    // every dependency must still be in the ring when its chunk reads it
-    std::vector<int> owner(kRing, -1), stamp(kRing, -1);
+    // the ring holds the solved entries of the last kRing stream positions
+    // (slot = position mod kRing, so it empties in solve order); every
+    // dependency must still be there when its chunk reads it, i.e. solved
+    // fewer than kRing positions earlier
     const std::vector<int> &ord = *rows[s];
+    sd[s].spos.assign(n, -1);
+    for(size_t c = 0; c < sd[s].cstart.size(); c++)
+      for(int t = 0; t < sd[s].ccount[c]; t++) sd[s].spos[ord[sd[s].cstart[c] + t]] = (int)c * sd[s].rc + t;
     for(size_t c = 0; c < sd[s].cstart.size(); c++) {
       for(int t = 0; t < sd[s].ccount[c]; t++) {
         const int i = ord[sd[s].cstart[c] + t];
         int b, e;
         range(s, i, b, e);
         for(int k = b; k < e; k++)
-          if(owner[hja[k] & (kRing - 1)] != hja[k])
+          if((long)c * sd[s].rc - sd[s].spos[hja[k]] >= kRing - kWave)
             return force ? set_error(GDP_E_UNSUPPORTED,
-                                     "one-wave sweeps: row %d depends on row %d, which has left the %d-row ring", i,
+                                     "one-wave sweeps: row %d depends on row %d, which has left the %d-entry ring", i,
                                      hja[k], kRing)
                          : GDP_OK;
       }
-      for(int t = 0; t < sd[s].ccount[c]; t++) {  // two rows of one chunk on one slot: neither is kept
-        const int i = ord[sd[s].cstart[c] + t], sl = i & (kRing - 1);
-        owner[sl] = stamp[sl] == (int)c ? -2 : i;
-        stamp[sl] = (int)c;
-      }
     }
   }
-  const long nch_all = (long)sd[0].cstart.size() + (long)sd[1].cstart.size();
-  const double bytes = 8. * kWave * ((double)sd[0].cstart.size() * sd[0].wg + (double)sd[1].cstart.size() * sd[1].wg);
-  const double t_wave = 0.25 * nch_all + bytes / 1e5, t_free = 2.0 * ((double)nlevels + upper_nlevels);
+  // waves per walk: GDP_ILU_TEAM overrides.  Measured per application
+  // (1, 2, 4, 8 waves): ILU(0) of Poisson 256^2 in RCMK order 0.62, 0.59,
+  // 0.48, 0.49 ms; ILUTP(20) of Poisson 512^2 28.0, 23.6, 21.3, 22.7 ms.
+  int team = 4;
+  if(const char *e = std::getenv("GDP_ILU_TEAM")) team = std::max(1, std::min(kMaxTeam, atoi(e)));
+  // Estimated time per application.  A chunk of the walk costs its row
+  // chain (about 0.2 us with 4 entry slots a row, 0.55 us with 12-24, 0.9 us
+  // with 64), and the chunks of one level go about two at a time (elasticity
+  // 32^3, 9 chunks a level: 6.3 ms against 3.4 ms for the sync-free sweeps,
+  // which cost about 2 us per level).
+  double t_wave = 0.;
+  for(int s = 0; s < 2; s++) {
+    const int cap = sd[s].G * sd[s].wg;
+    const double per = cap <= 4 ? 0.2 : cap <= 8 ? 0.3 : cap <= 24 ? 0.55 : 0.9;
+    const std::vector<int> &p = *ptrs[s];
+    for(size_t l = 0; l + 1 < p.size(); l++) {
+      const int ch = (p[l + 1] - p[l] + sd[s].rc - 1) / sd[s].rc;
+      t_wave += per * std::max(1, (ch + 1) / 2);
+    }
+  }
+  const double t_free = 2.0 * ((double)nlevels + upper_nlevels);
   if(!force && t_wave >= t_free) return GDP_OK;
+  // need[c] for the team walk (sequential chunk order, in which the ring check
+  // above holds): the chunks of c's dependencies, and for every slot c writes
+  // its previous writer and the last reader of the value it holds
+  std::vector<int> need[2];
+  for(int s = 0; s < 2; s++) {
+    const Side &S = sd[s];
+    const std::vector<int> &ord = *rows[s];
+    const long nch = (long)S.cstart.size();
+    std::vector<int> chunk_of(n, -1), writer(kRing, -1), reader(kRing, -1);
+    need[s].assign(nch, 0);
+    for(long c = 0; c < nch; c++) {
+      int nd = 0;
+      for(int t = 0; t < S.ccount[c]; t++) {
+        const int i = ord[S.cstart[c] + t];
+        int b, e;
+        range(s, i, b, e);
+        for(int k = b; k < e; k++) nd = std::max(nd, chunk_of[hja[k]] + 1);
+        const int sl = S.spos[i] & (kRing - 1);
+        nd = std::max(nd, std::max(writer[sl], reader[sl]) + 1);
+      }
+      for(int t = 0; t < S.ccount[c]; t++) {
+        const int i = ord[S.cstart[c] + t];
+        int b, e;
+        range(s, i, b, e);
+        for(int k = b; k < e; k++) reader[S.spos[hja[k]] & (kRing - 1)] = (int)c;
+      }
+      for(int t = 0; t < S.ccount[c]; t++) {
+        const int i = ord[S.cstart[c] + t], sl = S.spos[i] & (kRing - 1);
+        writer[sl] = (int)c;
+        reader[sl] = -1;
+        chunk_of[i] = (int)c;
+      }
+      need[s][c] = nd;
+    }
+  }
   // positions of the rows in the backward stream (the forward sweep's output)
   std::vector<int> bpos(n);
   for(size_t c = 0; c < sd[1].cstart.size(); c++)
@@ -659,9 +756,11 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
   for(int s = 0; s < 2; s++) {
     const Side &S = sd[s];
     const long nch = (long)S.cstart.size();
-    // steps rounded up to whole prefetch rounds, then one round of padding chunks
+    // steps rounded up to whole prefetch rounds of every wave, then one round
+    // of padding chunks per wave
     const int depth = wave_depth(S.wg);
-    const long nsteps = (nch + depth - 1) / depth * depth, nalloc = nsteps + depth;
+    const long span = (long)team * depth;
+    const long nsteps = (nch + span - 1) / span * span, nalloc = nsteps + span;
     const size_t ne = (size_t)nalloc * S.wg * kWave, nl = (size_t)nalloc * kWave;
     std::vector<int> col(ne, kRing), kpos(ne, -1), row(nl, -1), aux(nl, -1);
     const std::vector<int> &ord = *rows[s];
@@ -673,7 +772,7 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
         for(int k = b; k < e; k++) {
           const int x = k - b, lane = gl + x % S.G, w = x / S.G;
           const size_t idx = ((size_t)c * S.wg + w) * kWave + lane;
-          col[idx] = hja[k] & (kRing - 1);
+          col[idx] = S.spos[hja[k]] & (kRing - 1);
           kpos[idx] = k;
         }
         row[(size_t)c * kWave + gl] = i;
@@ -683,6 +782,11 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
     W.G = S.G;
     W.wg = S.wg;
     W.nch = (int)nsteps;
+    W.team = team;
+    std::vector<int> nd((size_t)nalloc, 0);
+    std::copy(need[s].begin(), need[s].end(), nd.begin());
+    GDP_TRY(dalloc(&W.need, (size_t)nalloc));
+    GDP_TRY(h2d(W.need, nd.data(), (size_t)nalloc));
     GDP_TRY(dalloc(&W.col, ne));
     GDP_TRY(dalloc(&W.kpos, ne));
     GDP_TRY(dalloc(&W.row, nl));
@@ -698,13 +802,16 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
   return GDP_OK;
 }
 
+// Chunks stored per stream: the walk's chunks and one prefetch round per wave.
+long IluFactor::wave_chunks_alloc(const WaveSweep &W) const { return (long)W.nch + (long)W.team * wave_depth(W.wg); }
+
 // Storage for the packed factor entries, in the factors' precision.
 int IluFactor::wave_alloc_values()
 {
   if(!wave_mode) return GDP_OK;
   for(int s = 0; s < 2; s++) {
     WaveSweep &W = wave[s];
-    const size_t nalloc = (size_t)W.nch + wave_depth(W.wg), ne = nalloc * W.wg * kWave, nl = nalloc * kWave;
+    const size_t nalloc = (size_t)wave_chunks_alloc(W), ne = nalloc * W.wg * kWave, nl = nalloc * kWave;
     dfree(W.val32);
     dfree(W.dinv32);
     dfree(W.val64);
@@ -728,7 +835,7 @@ void IluFactor::wave_pack()
   hipStream_t st = ctx().stream;
   for(int s = 0; s < 2; s++) {
     WaveSweep &W = wave[s];
-    const long nalloc = (long)W.nch + wave_depth(W.wg), ne = nalloc * W.wg * kWave, nl = nalloc * kWave;
+    const long nalloc = wave_chunks_alloc(W), ne = nalloc * W.wg * kWave, nl = nalloc * kWave;
     const dim3 ge((unsigned)((ne + 255) / 256)), gl((unsigned)((nl + 255) / 256));
     if(is_f64) {
       hipLaunchKernelGGL(k_wave_pack<double>, ge, dim3(256), 0, st, ne, W.kpos, lu64, W.val64);
@@ -997,17 +1104,18 @@ template <> double *wave_pre<double>(const IluFactor::WaveSweep &W) { return W.p
 // forward stream's order, forward sweep (its output in the backward stream's
 // order), backward sweep into x (and addto).
 template <typename TF, typename TV>
-void wave_apply(int n, const IluFactor::WaveSweep *W, const TV *r, TV scale, TV *x, TV *addto, const int *gate,
-                int gmode)
+void wave_apply(int n, const IluFactor::WaveSweep *W, const TV *r, TV scale, TV *x, TV *addto, int *err,
+                unsigned long long tl, const int *gate, int gmode)
 {
   hipStream_t s = ctx().stream;
   const IluFactor::WaveSweep &F = W[0], &B = W[1];
-  const long nl = ((long)F.nch + wave_depth(F.wg)) * kWave;
+  const long nl = ((long)F.nch + (long)F.team * wave_depth(F.wg)) * kWave;
   hipLaunchKernelGGL(k_wave_gather<TV>, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, s, nl, F.row, r, scale,
                      wave_pre<TV>(F), gate, gmode);
 #define GDP_WAVE(GG, WW, FWD, S, OUT, ADD)                                                                \
-  hipLaunchKernelGGL((k_wave_sweep<TF, TV, GG, WW, FWD>), dim3(1), dim3(kWave), 0, s, S.nch, S.col,       \
-                     wave_val<TF>(S), S.row, S.aux, wave_dinv<TF>(S), wave_pre<TV>(S), OUT, gate, gmode)
+  hipLaunchKernelGGL((k_wave_sweep<TF, TV, GG, WW, FWD>), dim3(1), dim3(kWave * S.team), 0, s, S.nch, S.col, \
+                     wave_val<TF>(S), S.row, S.aux, S.need, wave_dinv<TF>(S), wave_pre<TV>(S), OUT, err, tl, gate, \
+                     gmode)
 #define GDP_WAVE_SHAPE(FWD, S, OUT, ADD)                                        \
   if(S.G == 1) GDP_WAVE(1, 4, FWD, S, OUT, ADD);                                \
   else if(S.G == 2) GDP_WAVE(2, 4, FWD, S, OUT, ADD);                           \
@@ -1039,8 +1147,8 @@ int IluFactor::apply(const TV *r, TV scale, TV *x, const int *gate, int gmode, T
   }
   if(colmap) add = nullptr;
   if(wave_mode) {
-    if(is_f64) wave_apply<double, TV>(n, wave, r, scale, out, add, gate, gmode);
-    else wave_apply<float, TV>(n, wave, r, scale, out, add, gate, gmode);
+    if(is_f64) wave_apply<double, TV>(n, wave, r, scale, out, add, err + 1, wait_limit(), gate, gmode);
+    else wave_apply<float, TV>(n, wave, r, scale, out, add, err + 1, wait_limit(), gate, gmode);
   }
   else if(is_f64) {
     sweep<double, TV, true>(g_lower, n, ia, ja, diag, lu64, r, scale, t, out, err + 1, wait_limit(),

@@ -22,10 +22,28 @@
 /* BLAS-1 ddot as vendored by the reference (contrib/Sparskit/blas1.f:52-100):
  * clean-up loop of n mod 5 terms, then a 5-way unrolled sum evaluated left to
  * right.  Sparskit's distdot (iters.f:3-10) is this ddot. */
+static int g_dot_blocks = 0;
+
+/* Tests only: with nblocks > 0 every dot product is summed in nblocks
+ * contiguous blocks whose sums are then added in order -- the reference's
+ * algorithm with a different rounding order (as a blocked device reduction
+ * has), used to measure how far rounding alone moves a result.  0 restores
+ * the reference's order. */
+void orc_set_dot_blocks(int nblocks) { g_dot_blocks = nblocks > 0 ? nblocks : 0; }
+
 double orc_ddot(int n, const double *x, const double *y)
 {
   double t = 0.0;
   if(n <= 0) return 0.0;
+  if(g_dot_blocks > 0) {
+    const int nb = g_dot_blocks, per = (n + nb - 1) / nb;
+    for(int b = 0; b < nb; b++) {
+      double s = 0.0;
+      for(int i = b * per; i < n && i < (b + 1) * per; i++) s = s + x[i] * y[i];
+      t = t + s;
+    }
+    return t;
+  }
   int m = n % 5;
   for(int i = 0; i < m; i++) t = t + x[i] * y[i];
   if(n < 5) return t;

@@ -95,6 +95,12 @@ def lib():
     return _O
 
 
+def set_dot_blocks(nblocks: int = 0):
+    """Tests only: sum every dot product in ``nblocks`` contiguous blocks
+    (0: the reference's order), to measure what rounding alone changes."""
+    lib().orc_set_dot_blocks(int(nblocks))
+
+
 def ref(f64_ilu: bool = False):
     """The reference's Fortran Sparskit kernels (None if not built here)."""
     key = "f64" if f64_ilu else "f32"

@@ -39,14 +39,19 @@ def _device_apply(gpu, ia, ja, a, v, f64, reps=1):
 SWEEP_KIND = {"free": 1, "wave": 2}
 
 
-@pytest.mark.parametrize("sweep", ["free", "wave"])
+@pytest.mark.parametrize("sweep", ["free", "wave", "wave-solo", "wave-team2"])
 @pytest.mark.parametrize("f64", [False, True])
 @pytest.mark.parametrize("name", list(SYSTEMS))
 def test_ilu_apply_bit_exact(gpu, oracle, name, f64, sweep, monkeypatch):
-    """Both sweep kinds: the sync-free sweeps and the one-wave sweeps (levels
-    walked by one wave, solution entries in an LDS ring of 16384 rows; the
-    192^2 system has 36,481 rows, so the ring wraps)."""
-    monkeypatch.setenv("GDP_ILU_SWEEP", sweep)
+    """Both sweep kinds: the sync-free sweeps and the level walk in one
+    workgroup (solution entries in an LDS ring of 16384 rows; the 192^2
+    system has 36,481 rows, so the ring wraps), walked by a team of waves
+    (the default 8, and 2) or by one wave."""
+    monkeypatch.setenv("GDP_ILU_SWEEP", sweep.split("-")[0])
+    if sweep == "wave-solo":
+        monkeypatch.setenv("GDP_ILU_TEAM", "1")
+    elif sweep == "wave-team2":
+        monkeypatch.setenv("GDP_ILU_TEAM", "2")
     print(f"case {name} f64={f64} sweep={sweep}", flush=True)
     ia, ja, a, b, tab = oracle.assemble(SYSTEMS[name]())
     perm, rperm, ib, jb, bb, nlev = oracle.rcmk(ia, ja, a)
@@ -55,7 +60,7 @@ def test_ilu_apply_bit_exact(gpu, oracle, name, f64, sweep, monkeypatch):
     v = np.random.default_rng(5).standard_normal(ib.shape[0] - 1)
     want = oracle.lusol(alu, jlu, ju, v)
     got, kind = _device_apply(gpu, ia, ja, a, v, f64, reps=3)  # repeated: sweep state reset each time
-    assert kind == SWEEP_KIND[sweep], kind  # the forced kernel really ran
+    assert kind == SWEEP_KIND[sweep.split("-")[0]], kind  # the forced kernel really ran
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), np.abs(got - want).max()
 
 
@@ -141,13 +146,15 @@ def test_ilut_apply_bit_exact(gpu, oracle, name, f64, pivot, lanes, monkeypatch)
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), np.abs(got - want).max()
 
 
-@pytest.mark.parametrize("sweep", ["free", "wave"])
+@pytest.mark.parametrize("sweep", ["free", "wave", "wave-solo"])
 @pytest.mark.parametrize("pivot", [False, True])
 def test_ilut_apply_long_chain(gpu, oracle, sweep, pivot, monkeypatch):
     """ILUT / ILUTP(20) on Poisson 160^2 (25,281 rows, more than the one-wave
     sweeps' LDS ring holds, and thousands of narrow levels): lusol's bits with
-    either sweep kind."""
-    monkeypatch.setenv("GDP_ILU_SWEEP", sweep)
+    either sweep kind, a team of waves or one wave."""
+    monkeypatch.setenv("GDP_ILU_SWEEP", sweep.split("-")[0])
+    if sweep == "wave-solo":
+        monkeypatch.setenv("GDP_ILU_TEAM", "1")
     ia, ja, a, b, tab = oracle.assemble(P.poisson2d(160))
     perm, rperm, ib, jb, bb, nlev = oracle.rcmk(ia, ja, a)
     n = ib.shape[0] - 1
@@ -159,22 +166,30 @@ def test_ilut_apply_long_chain(gpu, oracle, sweep, pivot, monkeypatch):
     A.set_csr(ia, ja, a)
     s = gpu.Solver(Algorithm=8, Preconditioner=2 if pivot else 1, Krylov_Size=40, Nb_Iter_Max=3)
     info = gpu.solve(A, gpu.Vector(n, np.ones(n)), s, gpu.Vector(n))
-    assert info["ilu_sweep"] == SWEEP_KIND[sweep]
+    assert info["ilu_sweep"] == SWEEP_KIND[sweep.split("-")[0]]
     out = gpu.Vector(n)
     gpu.solver_prec_apply(s, gpu.Vector(n, v), out, reps=2)
     got = out.get()
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), np.abs(got - want).max()
 
 
-def test_forced_wave_sweep_rejects_what_it_cannot_take(gpu, oracle, monkeypatch):
-    """GDP_ILU_SWEEP=wave on a system whose dependencies leave the one-wave
-    sweeps' 16,384-row LDS ring (natural ordering of a 3-block elasticity
-    system, 26,460 rows: uz rows depend on ux rows ~2N/3 back) must fail with GDP_E_UNSUPPORTED,
-    not fall back silently; unforced, the same system solves."""
+def test_forced_wave_sweep_rejects_what_it_cannot_take(gpu, monkeypatch):
+    """GDP_ILU_SWEEP=wave on a system whose dependencies leave the level
+    walk's 16,384-entry LDS ring must fail with GDP_E_UNSUPPORTED, not fall
+    back silently; unforced, the same system solves with the sync-free
+    sweeps.  The system: 40,000 rows, each coupled to its neighbours (one
+    row per level) and to the rows 20,000 away, so a dependency was solved
+    20,000 stream positions earlier."""
     from getdp_amd._lib import GdpError
-    ia, ja, a, b, tab = oracle.assemble(P.elasticity3d(20, nbins=8))
-    n = ia.shape[0] - 1
-    assert n > 16384  # 26,460 rows
+    n, far = 40_000, 20_000
+    ia, ja, a = [0], [], []
+    for i in range(n):
+        for j in (i - far, i - 1, i, i + 1, i + far):
+            if 0 <= j < n:
+                ja.append(j)
+                a.append(6.0 if j == i else -1.0)
+        ia.append(len(ja))
+    ia, ja, a = np.array(ia, np.int32), np.array(ja, np.int32), np.array(a)
     A = gpu.Matrix(n)
     A.set_csr(ia, ja, a)
     opts = dict(Algorithm=8, Preconditioner=6, Krylov_Size=30, Renumbering_Technique=0, Nb_Iter_Max=2)
