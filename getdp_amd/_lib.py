@@ -96,6 +96,7 @@ SIGNATURES = [
     ("gdp_fe_num_dofs", C.c_int, [_vp, _ip, _lp]),
     ("gdp_fe_dof_table", C.c_int, [_vp, C.c_int, _ip, C.c_int]),
     ("gdp_fe_pattern", C.c_int, [_vp, _ip, _ip]),
+    ("gdp_fe_write_pre", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _ip]),
     ("gdp_fe_generate", C.c_int, [_vp, _vp, _vp]),
     ("gdp_fe_last_timing", C.c_int, [_vp, _dp, _dp, _dp]),
     ("gdp_mesh_read", C.c_int, [C.c_char_p, _pp]),

@@ -1706,6 +1706,42 @@ extern "C" int gdp_fe_dof_table(const gdp_fe *fe, int space, int *dofnum, int nm
   return GDP_OK;
 }
 
+// The DofData block of a .pre file (Dof_WriteFilePRE, DofData.cpp:363-460):
+// header lines, then one line per DOF in (NumType, Entity, Harmonic) order --
+// the in-order walk of the DOF tree (Dof_WriteDofPRE, :412-456): unknowns
+// "NumType Entity 0 1 NumDof 1", fixed DOFs "NumType Entity 0 2 value 0"
+// (value as LinAlg_PrintScalar writes it; constraints without a time function
+// store time function index 0, which is also the DofData's only one).
+extern "C" int gdp_fe_write_pre(const gdp_fe *fe, FILE *f, int num, int resolution_index, int system_index,
+                                int nfs, const int *fs_index)
+{
+  if(!fe || !f || nfs < 0 || (nfs && !fs_index)) return set_error(GDP_E_ARG, "gdp_fe_write_pre: bad arguments");
+  std::vector<int> order(fe->nspaces);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return fe->h_numtype[x] < fe->h_numtype[y]; });
+  long nany = 0;
+  for(int s = 0; s < fe->nspaces; s++)
+    for(int ent = 1; ent <= fe->nmax; ent++) nany += fe->h_dofnum[s][ent] != 0;
+  fprintf(f, "$DofData /* #%d */\n", num);
+  fprintf(f, "%d %d\n", resolution_index, system_index);
+  fprintf(f, "%d", nfs);
+  for(int i = 0; i < nfs; i++) fprintf(f, " %d", fs_index[i]);
+  fprintf(f, "\n");
+  fprintf(f, "1 0\n");  // time function indices
+  fprintf(f, "1 0\n");  // harmonics
+  fprintf(f, "%ld %d\n", nany, fe->N);
+  for(int s : order)
+    for(int ent = 1; ent <= fe->nmax; ent++) {
+      const int d = fe->h_dofnum[s][ent];
+      if(d > 0) fprintf(f, "%d %d 0 1 %d 1\n", fe->h_numtype[s], ent, d);
+      else if(d < 0) fprintf(f, "%d %d 0 2 %.16g 0\n", fe->h_numtype[s], ent, fe->h_fixval[s][-d - 1]);
+    }
+  fprintf(f, "$EndDofData\n");
+  fflush(f);
+  return ferror(f) ? set_error(GDP_E_ARG, "gdp_fe_write_pre: write error") : GDP_OK;
+}
+
 // element pass, then the CSR-entry and RHS sums (enqueue only)
 static int fe_two_phase(gdp_fe *fe, double *a, double *b)
 {

@@ -264,11 +264,30 @@ class FESystem(_Handle):
         nnz = C.c_long()
         check(lib().gdp_fe_num_dofs(self.h, C.byref(n), C.byref(nnz)), "gdp_fe_num_dofs")
         self.N, self.nnz = n.value, nnz.value
+        self.nspaces = len(p.spaces)
 
     def dof_table(self, space: int = 0) -> np.ndarray:
         out = np.zeros(self.nmax + 1, np.int32)
         check(lib().gdp_fe_dof_table(self.h, space, iptr(out), self.nmax), "gdp_fe_dof_table")
         return out
+
+    def write_pre(self, path: str, num: int = 0, resolution_index: int = 0, system_index: int = 0,
+                  fs_index=None):
+        """The $DofData block of a .pre file (Dof_WriteFilePRE) for these DOFs."""
+        libc = C.CDLL(None)
+        libc.fopen.restype = C.c_void_p
+        libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+        libc.fclose.argtypes = [C.c_void_p]
+        fs = np.ascontiguousarray(fs_index if fs_index is not None else np.arange(self.nspaces),
+                                  np.int32)
+        f = libc.fopen(path.encode(), b"w")
+        if not f:
+            raise OSError(f"cannot open {path}")
+        try:
+            check(lib().gdp_fe_write_pre(self.h, f, num, resolution_index, system_index, fs.shape[0],
+                                         fs.ctypes.data_as(C.POINTER(C.c_int))), "gdp_fe_write_pre")
+        finally:
+            libc.fclose(f)
 
     def pattern(self):
         ia = np.zeros(self.N + 1, np.int32)

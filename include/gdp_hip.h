@@ -303,6 +303,13 @@ int gdp_fe_dof_table(const gdp_fe *fe, int space, int *dofnum, int nmax);
 /* The formulation's CSR pattern (rows = equation DOFs, sorted columns):
  * ia (N+1) and ja (nnz); host data, available without a device. */
 int gdp_fe_pattern(const gdp_fe *fe, int *ia, int *ja);
+/* Dof_WriteFilePRE (DofData.cpp:363-460): the $DofData block of a .pre file
+ * for the formulation's DOFs (DofData number num, its resolution and system
+ * indices and function space indices as GetDP holds them), one line per DOF
+ * in (NumType, Entity) order, in Dof_WriteDofPRE's formats.  Host data,
+ * available without a device. */
+int gdp_fe_write_pre(const gdp_fe *fe, FILE *f, int num, int resolution_index, int system_index, int nfs,
+                     const int *fs_index);
 /* Generate[A] (SolvingOperations.cpp:276-500 -> per element
  * Cal_GalerkinTermOfFemEquation + Dof_AssembleInMat): zeroes A and b, then
  * assembles every term of every element on the device, summing each entry in
