@@ -134,6 +134,7 @@ SIGNATURES = [
     ("gdp_batch_run", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("gdp_batch_info", C.c_int, [_vp, _ip, _ip, _dp, _dp, _dp, _dp, _dp]),
     ("gdp_unet_create", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint, _pp]),
+    ("gdp_unet_create_3d", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, _pp]),
     ("gdp_unet_destroy", C.c_int, [_vp]),
     ("gdp_unet_num_params", C.c_int, [_vp, _lp]),
     ("gdp_unet_get_params", C.c_int, [_vp, _fp]),

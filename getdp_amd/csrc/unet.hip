@@ -17,6 +17,10 @@
 // pool = 2x2 average, up = nearest neighbour, ++ = channel concatenation;
 // both are fused into the operand loaders of the next convolution.
 //
+// The same network runs on 3D grids (D x H x W voxels): 3x3x3 convolutions
+// (27 taps), 2x2x2 average pooling and nearest upsampling in all three
+// directions.  A 2D grid is the case D = 1 with 9 taps.
+//
 // Layout: activations NHWC in BF16 with channels padded to a multiple of 8,
 // so that one lane loads 8 consecutive channels (16 bytes) of one pixel.
 // A 3x3 convolution is an implicit GEMM: rows = pixels, columns = output
@@ -60,6 +64,7 @@ __device__ __host__ inline u16 f2bf(float f)
 
 struct ConvArgs {
   int B, H, W;        // output (= logical input) spatial size
+  int D, d3;          // depth (1 on 2D grids); d3: 3D grid (27 taps, 2x2x2 pooling)
   int cin_pad, cout;  // logical input channels (padded), output channels
   int K_pad;          // reduction length, multiple of 32
   int c1;             // UPCAT: channels of the coarse tensor
@@ -82,62 +87,31 @@ __device__ inline bool unet_gated(const int *gate, int gmode)
   return (gate[0] | gate[3]) != 0;
 }
 
-// 8 consecutive logical input channels ci0.. at pixel (b, y, x) of the
-// logical input of a convolution (zeros outside the image).
+// one logical input element at voxel (b, z, y, x) (zeros outside the grid)
 template <int MODE>
-__device__ inline uint4 load8(const ConvArgs &a, int b, int y, int x, int ci0)
+__device__ inline float load1(const ConvArgs &a, int b, int z, int y, int x, int ci)
 {
-  uint4 z = {0u, 0u, 0u, 0u};
-  if(y < 0 || x < 0 || y >= a.H || x >= a.W) return z;
-  if(MODE == M_PLAIN) {
-    const u16 *p = a.in + (((long)b * a.H + y) * a.W + x) * a.cin_pad + ci0;
-    return *(const uint4 *)p;
-  }
-  else if(MODE == M_POOL) {
-    const int W2 = 2 * a.W, H2 = 2 * a.H;
-    const u16 *p = a.in + (((long)b * H2 + 2 * y) * W2 + 2 * x) * a.cin_pad + ci0;
-    uint4 q[4];
-    q[0] = *(const uint4 *)p;
-    q[1] = *(const uint4 *)(p + a.cin_pad);
-    q[2] = *(const uint4 *)(p + (long)W2 * a.cin_pad);
-    q[3] = *(const uint4 *)(p + (long)W2 * a.cin_pad + a.cin_pad);
-    const u16 *h[4] = {(const u16 *)&q[0], (const u16 *)&q[1], (const u16 *)&q[2], (const u16 *)&q[3]};
-    uint4 r;
-    u16 *o = (u16 *)&r;
-#pragma unroll
-    for(int c = 0; c < 8; c++) o[c] = f2bf(0.25f * (bf2f(h[0][c]) + bf2f(h[1][c]) + bf2f(h[2][c]) + bf2f(h[3][c])));
-    return r;
-  }
-  else {
-    if(ci0 < a.c1) {
-      const int Wc = a.W >> 1, Hc = a.H >> 1;
-      const u16 *p = a.in + (((long)b * Hc + (y >> 1)) * Wc + (x >> 1)) * a.c1 + ci0;
-      return *(const uint4 *)p;
-    }
-    const int c2 = a.cin_pad - a.c1;
-    const u16 *p = a.in2 + (((long)b * a.H + y) * a.W + x) * c2 + (ci0 - a.c1);
-    return *(const uint4 *)p;
-  }
-}
-
-// one logical input element (scalar form of load8)
-template <int MODE>
-__device__ inline float load1(const ConvArgs &a, int b, int y, int x, int ci)
-{
-  if(y < 0 || x < 0 || y >= a.H || x >= a.W) return 0.f;
-  if(MODE == M_PLAIN) return bf2f(a.in[(((long)b * a.H + y) * a.W + x) * a.cin_pad + ci]);
+  if(z < 0 || y < 0 || x < 0 || z >= a.D || y >= a.H || x >= a.W) return 0.f;
+  if(MODE == M_PLAIN) return bf2f(a.in[((((long)b * a.D + z) * a.H + y) * a.W + x) * a.cin_pad + ci]);
   if(MODE == M_POOL) {
-    const int W2 = 2 * a.W, H2 = 2 * a.H;
-    const u16 *p = a.in + (((long)b * H2 + 2 * y) * W2 + 2 * x) * a.cin_pad + ci;
-    return bf2f(f2bf(0.25f * (bf2f(p[0]) + bf2f(p[a.cin_pad]) + bf2f(p[(long)W2 * a.cin_pad]) +
-                              bf2f(p[(long)W2 * a.cin_pad + a.cin_pad]))));
+    const long W2 = 2 * a.W, H2 = 2 * a.H, D2 = a.d3 ? 2 * a.D : 1;
+    const long cin = a.cin_pad;
+    const u16 *p = a.in + ((((long)b * D2 + (a.d3 ? 2 * z : 0)) * H2 + 2 * y) * W2 + 2 * x) * cin + ci;
+    float s = bf2f(p[0]) + bf2f(p[cin]) + bf2f(p[W2 * cin]) + bf2f(p[W2 * cin + cin]);
+    if(a.d3) {
+      const u16 *p1 = p + H2 * W2 * cin;
+      s += bf2f(p1[0]) + bf2f(p1[cin]) + bf2f(p1[W2 * cin]) + bf2f(p1[W2 * cin + cin]);
+      return bf2f(f2bf(0.125f * s));
+    }
+    return bf2f(f2bf(0.25f * s));
   }
   if(ci < a.c1) {
-    const int Wc = a.W >> 1, Hc = a.H >> 1;
-    return bf2f(a.in[(((long)b * Hc + (y >> 1)) * Wc + (x >> 1)) * a.c1 + ci]);
+    const int Wc = a.W >> 1, Hc = a.H >> 1, Dc = a.d3 ? a.D >> 1 : 1;
+    const int zc = a.d3 ? z >> 1 : 0;
+    return bf2f(a.in[((((long)b * Dc + zc) * Hc + (y >> 1)) * Wc + (x >> 1)) * a.c1 + ci]);
   }
   const int c2 = a.cin_pad - a.c1;
-  return bf2f(a.in2[(((long)b * a.H + y) * a.W + x) * c2 + (ci - a.c1)]);
+  return bf2f(a.in2[((((long)b * a.D + z) * a.H + y) * a.W + x) * c2 + (ci - a.c1)]);
 }
 
 __device__ inline bf16x8 as_bf8(uint4 v)
@@ -155,41 +129,58 @@ __device__ inline bf16x8 as_bf8(uint4 v)
 // The reduction position of a lane's 8-channel group (tap, channel) and the
 // pixel and weight addresses are carried from one 32-wide k-step to the next
 // by additions only: no division or address rebuild inside the loop.
-__device__ inline int tap_dy(int tap) { return ((tap * 43) >> 7) - 1; }  // tap / 3 - 1, tap < 9
-__device__ inline int tap_dx(int tap) { return tap - 3 * ((tap * 43) >> 7) - 1; }
+// tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1) (dz = 0 and tap < 9 on 2D grids)
+__device__ inline int tap_dz(int tap) { return ((tap * 57) >> 9) - 1; }  // tap / 9 - 1, tap < 27
+__device__ inline int tap_t9(int tap) { return tap - 9 * ((tap * 57) >> 9); }
+__device__ inline int tap_dy(int tap) { return ((tap_t9(tap) * 43) >> 7) - 1; }  // (tap % 9) / 3 - 1
+__device__ inline int tap_dx(int tap) { return tap_t9(tap) - 3 * ((tap_t9(tap) * 43) >> 7) - 1; }
 
 // 8 input channels (ci0..ci0+7, logical input of the convolution) of the
-// pixel at offset (dy, dx) from the lane's output pixel; zeros outside.
-// pix: PLAIN/UPCAT pixel index b*H*W + y*W + x, POOL fine pixel of (2y, 2x);
-// cbh: b * H/2 (UPCAT coarse rows).
+// voxel at offset (dz, dy, dx) from the lane's output voxel; zeros outside.
+// pix: PLAIN/UPCAT voxel index ((b*D + z)*H + y)*W + x, POOL fine voxel of
+// (2z, 2y, 2x) (z = 0 on 2D grids); cb: b * (D/2) (UPCAT coarse planes; b on
+// 2D grids).
 template <int MODE>
-__device__ inline uint4 conv_load_a(const ConvArgs &a, long pix, int py, int px, int cbh, int dy, int dx,
-                                    int ci0, bool ok)
+__device__ inline uint4 conv_load_a(const ConvArgs &a, long pix, int pz, int py, int px, int cb, int dz, int dy,
+                                    int dx, int ci0, bool ok)
 {
   uint4 v = {0u, 0u, 0u, 0u};
   const int H = a.H, W = a.W, cin = a.cin_pad;
-  const int y = py + dy, x = px + dx;
-  if(!(ok && (unsigned)y < (unsigned)H && (unsigned)x < (unsigned)W)) return v;
-  if(MODE == M_PLAIN) return *(const uint4 *)(a.in + (pix + (long)dy * W + dx) * cin + ci0);
+  const int z = pz + dz, y = py + dy, x = px + dx;
+  if(!(ok && (unsigned)z < (unsigned)a.D && (unsigned)y < (unsigned)H && (unsigned)x < (unsigned)W)) return v;
+  if(MODE == M_PLAIN) return *(const uint4 *)(a.in + (pix + ((long)dz * H + dy) * W + dx) * cin + ci0);
   if(MODE == M_POOL) {
-    const long W2 = 2 * W;
-    const u16 *p0 = a.in + (pix + 2L * dy * W2 + 2 * dx) * cin + ci0;
-    uint4 q[4];
+    const long W2 = 2 * W, H2 = 2 * H;
+    const u16 *p0 = a.in + (pix + (2L * dz * H2 + 2L * dy) * W2 + 2 * dx) * cin + ci0;
+    uint4 q[8];
     q[0] = *(const uint4 *)p0;
     q[1] = *(const uint4 *)(p0 + cin);
     q[2] = *(const uint4 *)(p0 + W2 * cin);
     q[3] = *(const uint4 *)(p0 + W2 * cin + cin);
-    const u16 *h0 = (const u16 *)&q[0], *h1 = (const u16 *)&q[1], *h2 = (const u16 *)&q[2],
-              *h3 = (const u16 *)&q[3];
+    if(a.d3) {
+      const u16 *p1 = p0 + H2 * W2 * cin;
+      q[4] = *(const uint4 *)p1;
+      q[5] = *(const uint4 *)(p1 + cin);
+      q[6] = *(const uint4 *)(p1 + W2 * cin);
+      q[7] = *(const uint4 *)(p1 + W2 * cin + cin);
+    }
     u16 *o = (u16 *)&v;
 #pragma unroll
-    for(int c = 0; c < 8; c++) o[c] = f2bf(0.25f * (bf2f(h0[c]) + bf2f(h1[c]) + bf2f(h2[c]) + bf2f(h3[c])));
+    for(int c = 0; c < 8; c++) {
+      float s = bf2f(((const u16 *)&q[0])[c]) + bf2f(((const u16 *)&q[1])[c]) + bf2f(((const u16 *)&q[2])[c]) +
+                bf2f(((const u16 *)&q[3])[c]);
+      if(a.d3)
+        s += bf2f(((const u16 *)&q[4])[c]) + bf2f(((const u16 *)&q[5])[c]) + bf2f(((const u16 *)&q[6])[c]) +
+             bf2f(((const u16 *)&q[7])[c]);
+      o[c] = f2bf((a.d3 ? 0.125f : 0.25f) * s);
+    }
     return v;
   }
   // upsampled coarse channels or skip channels: one load from a selected address
   const bool coarse = ci0 < a.c1;
-  const u16 *pc = a.in + ((long)(cbh + (y >> 1)) * (W >> 1) + (x >> 1)) * a.c1 + ci0;
-  const u16 *pf = a.in2 + (pix + (long)dy * W + dx) * (cin - a.c1) + (ci0 - a.c1);
+  const int zc = a.d3 ? z >> 1 : 0;
+  const u16 *pc = a.in + (((long)(cb + zc) * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1)) * a.c1 + ci0;
+  const u16 *pf = a.in2 + (pix + ((long)dz * H + dy) * W + dx) * (cin - a.c1) + (ci0 - a.c1);
   return *(const uint4 *)(coarse ? pc : pf);
 }
 
@@ -198,14 +189,15 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a)
 {
   if(unet_gated(a.gate, a.gmode)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long P = (long)a.B * a.H * a.W;
+  const long P = (long)a.B * a.D * a.H * a.W;
   const long pbase = ((long)blockIdx.x * 4 + wave) * (TM * 16);
   const int cobase = blockIdx.y * 16 * TN;
   const int r16 = lane & 15, kq = lane >> 4;
   const int H = a.H, W = a.W, cin = a.cin_pad;
-  int py[TM], px[TM];
-  long pix[TM];   // PLAIN/UPCAT: pixel index b*H*W + y*W + x; POOL: fine pixel of (2y, 2x)
-  int cbh[TM];    // UPCAT: b * (H/2) (coarse row base)
+  const int taps = a.d3 ? 27 : 9, t0 = a.d3 ? 0 : 9;  // t0: 2D taps are the dz = 0 plane
+  int pz[TM], py[TM], px[TM];
+  long pix[TM];   // PLAIN/UPCAT: voxel index; POOL: fine voxel of (2z, 2y, 2x)
+  int cb[TM];     // UPCAT: b * (D/2) (coarse plane base)
   bool pok[TM];
 #pragma unroll
   for(int t = 0; t < TM; t++) {
@@ -213,12 +205,16 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a)
     pok[t] = p < P;
     long q = pok[t] ? p : 0;
     px[t] = (int)(q % W);
-    long q2 = q / W;
-    py[t] = (int)(q2 % H);
-    const int b = (int)(q2 / H);
-    if(MODE == M_POOL) pix[t] = ((long)b * 2 * H + 2 * py[t]) * (2 * W) + 2 * px[t];
-    else pix[t] = q;
-    cbh[t] = b * (H >> 1);
+    q /= W;
+    py[t] = (int)(q % H);
+    q /= H;
+    pz[t] = (int)(q % a.D);
+    const int b = (int)(q / a.D);
+    if(MODE == M_POOL)
+      pix[t] = a.d3 ? (((long)b * 2 * a.D + 2 * pz[t]) * (2 * H) + 2 * py[t]) * (2 * W) + 2 * px[t]
+                    : ((long)b * 2 * H + 2 * py[t]) * (2 * W) + 2 * px[t];
+    else pix[t] = pok[t] ? p : 0;
+    cb[t] = a.d3 ? b * (a.D >> 1) : b;
   }
   f32x4 acc[TM][TN];
 #pragma unroll
@@ -245,11 +241,11 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a)
 #pragma unroll
     for(int j = 0; j < PF; j++) {
       const bool on = kk + 32 * j < kend;
-      const bool kvalid = on && tap < 9;
-      const int dy = tap_dy(tap), dx = tap_dx(tap);
+      const bool kvalid = on && tap < taps;
+      const int dz = tap_dz(tap + t0), dy = tap_dy(tap + t0), dx = tap_dx(tap + t0);
 #pragma unroll
       for(int t = 0; t < TM; t++)
-        av[j][t] = conv_load_a<MODE>(a, pix[t], py[t], px[t], cbh[t], dy, dx, ci0, kvalid && pok[t]);
+        av[j][t] = conv_load_a<MODE>(a, pix[t], pz[t], py[t], px[t], cb[t], dz, dy, dx, ci0, kvalid && pok[t]);
 #pragma unroll
       for(int n = 0; n < TN; n++) {
         bv[j][n] = (on && wok[n]) ? *(const uint4 *)wp[n] : zero4;
@@ -310,20 +306,25 @@ __global__ void __launch_bounds__(256) k_conv_ks(ConvArgs a)
   if(unet_gated(a.gate, a.gmode)) return;
   __shared__ f32x4 red[3][TN][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long P = (long)a.B * a.H * a.W;
+  const long P = (long)a.B * a.D * a.H * a.W;
   const long pbase = (long)blockIdx.x * 16;
   const int cobase = blockIdx.y * 16 * TN;
   const int r16 = lane & 15, kq = lane >> 4;
   const int H = a.H, W = a.W, cin = a.cin_pad;
+  const int taps = a.d3 ? 27 : 9, t0 = a.d3 ? 0 : 9;
   const long p = pbase + r16;
   const bool pok = p < P;
-  const long q = pok ? p : 0;
+  long q = pok ? p : 0;
   const int px = (int)(q % W);
-  const long q2 = q / W;
-  const int py = (int)(q2 % H);
-  const int b = (int)(q2 / H);
-  const long pix = MODE == M_POOL ? ((long)b * 2 * H + 2 * py) * (2 * W) + 2 * px : q;
-  const int cbh = b * (H >> 1);
+  q /= W;
+  const int py = (int)(q % H);
+  q /= H;
+  const int pz = (int)(q % a.D);
+  const int b = (int)(q / a.D);
+  const long pix = MODE == M_POOL ? (a.d3 ? (((long)b * 2 * a.D + 2 * pz) * (2 * H) + 2 * py) * (2 * W) + 2 * px
+                                          : ((long)b * 2 * H + 2 * py) * (2 * W) + 2 * px)
+                                  : (pok ? p : 0);
+  const int cb = a.d3 ? b * (a.D >> 1) : b;
   const int nsteps = a.K_pad / 32;
   const int s0 = wave * nsteps / 4, s1 = (wave + 1) * nsteps / 4;
   f32x4 acc[TN];
@@ -348,7 +349,8 @@ __global__ void __launch_bounds__(256) k_conv_ks(ConvArgs a)
 #pragma unroll
     for(int j = 0; j < PF; j++) {
       const bool on = c0 + j < s1;
-      av[j] = conv_load_a<MODE>(a, pix, py, px, cbh, tap_dy(tap), tap_dx(tap), ci0, on && tap < 9 && pok);
+      av[j] = conv_load_a<MODE>(a, pix, pz, py, px, cb, tap_dz(tap + t0), tap_dy(tap + t0), tap_dx(tap + t0), ci0,
+                                on && tap < taps && pok);
 #pragma unroll
       for(int n = 0; n < TN; n++) {
         bv[j][n] = (on && wok[n]) ? *(const uint4 *)wp[n] : zero4;
@@ -398,7 +400,7 @@ __global__ void __launch_bounds__(256) k_conv_ks(ConvArgs a)
 __global__ void __launch_bounds__(256) k_conv_split_sum(ConvArgs a, int nsplit)
 {
   if(unet_gated(a.gate, a.gmode)) return;
-  const long P = (long)a.B * a.H * a.W;
+  const long P = (long)a.B * a.D * a.H * a.W;
   const long n = P * a.cout;
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if(i >= n) return;
@@ -426,7 +428,7 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs w)
   const ConvArgs &a = w.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r16 = lane & 15, kq = lane >> 4;
-  const long P = (long)a.B * a.H * a.W;
+  const long P = (long)a.B * a.D * a.H * a.W;
   const long p_begin = (long)blockIdx.x * w.chunk;
   const long p_end = p_begin + w.chunk < P ? p_begin + w.chunk : P;
   const int co = blockIdx.y * 16 + r16;     // A row
@@ -435,31 +437,39 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs w)
   // decode the column's tap / channel once
   int tap = kcol / a.cin_pad;
   const int ci = kcol - tap * a.cin_pad;
-  const bool kvalid = tap < 9 && kcol < a.K_pad;
-  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  const bool kvalid = tap < (a.d3 ? 27 : 9) && kcol < a.K_pad;
+  const int t3 = a.d3 ? tap : tap + 9;
+  const int dz = t3 / 9 - 1, dy = (t3 % 9) / 3 - 1, dx = t3 % 3 - 1;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
   const int W = a.W, H = a.H;
   for(long p0 = p_begin; p0 < p_end; p0 += 32) {
     bf16x8 av, bv;
     u16 *ah = (u16 *)&av, *bh = (u16 *)&bv;
-    // coordinates of the lane's first pixel, then advanced one pixel at a time
-    const int pb = (int)(p0 + 8 * kq);
-    int x = pb % W, b = pb / W;
-    int y = b % H;
-    b /= H;
+    // coordinates of the lane's first voxel, then advanced one voxel at a time
+    const long pb = p0 + 8 * kq;
+    long q = pb;
+    int x = (int)(q % W);
+    q /= W;
+    int y = (int)(q % H);
+    q /= H;
+    int z = (int)(q % a.D);
+    int b = (int)(q / a.D);
 #pragma unroll
     for(int j = 0; j < 8; j++) {
-      const long p = (long)pb + j;
+      const long p = pb + j;
       const bool ok = p < p_end;
       ah[j] = (ok && co < a.cout) ? w.g[p * a.cout + co] : (u16)0;
       float xv = 0.f;
-      if(ok && kvalid) xv = load1<MODE>(a, b, y + dy, x + dx, ci);
+      if(ok && kvalid) xv = load1<MODE>(a, b, z + dz, y + dy, x + dx, ci);
       bh[j] = f2bf(xv);
       if(++x == W) {
         x = 0;
         if(++y == H) {
           y = 0;
-          b++;
+          if(++z == a.D) {
+            z = 0;
+            b++;
+          }
         }
       }
     }
@@ -539,41 +549,50 @@ __global__ void k_add(long n, const float *src, float *dst)
   if(i < n) dst[i] += src[i];
 }
 
-// pool^T: dFine[2y+a][2x+b][c] += dX[y][x][c] / 4
-__global__ void k_pool_adj(int B, int H, int W, int C, const float *dx, float *dfine)
+// pool^T: dFine[2z+c][2y+a][2x+b][ch] += dX[z][y][x][ch] / 8 (3D), or
+// dFine[2y+a][2x+b][ch] += dX[y][x][ch] / 4 (2D, D = 1)
+__global__ void k_pool_adj(int B, int D, int H, int W, int C, int d3, const float *dx, float *dfine)
 {
   long i = (long)blockIdx.x * 256 + threadIdx.x;
-  long n = (long)B * 2 * H * 2 * W * C;
+  const int Df = d3 ? 2 * D : 1;
+  long n = (long)B * Df * 2 * H * 2 * W * C;
   if(i >= n) return;
   int c = (int)(i % C);
   long q = i / C;
   int xf = (int)(q % (2 * W));
   q /= 2 * W;
   int yf = (int)(q % (2 * H));
-  int b = (int)(q / (2 * H));
-  dfine[i] += 0.25f * dx[(((long)b * H + (yf >> 1)) * W + (xf >> 1)) * C + c];
+  q /= 2 * H;
+  int zf = (int)(q % Df);
+  int b = (int)(q / Df);
+  const int zc = d3 ? zf >> 1 : 0;
+  dfine[i] += (d3 ? 0.125f : 0.25f) * dx[((((long)b * D + zc) * H + (yf >> 1)) * W + (xf >> 1)) * C + c];
 }
 
-// up^T (++ split): dCoarse[yc][xc][c] += sum of the 4 fine dX[.][.][c];
-// dSkip[y][x][c] += dX[y][x][c1 + c]
-__global__ void k_upcat_adj(int B, int H, int W, int c1, int c2, const float *dx, float *dcoarse,
+// up^T (++ split): dCoarse[zc][yc][xc][c] += sum of the 8 (2D: 4) fine
+// dX[.][.][.][c]; dSkip[z][y][x][c] += dX[z][y][x][c1 + c]
+__global__ void k_upcat_adj(int B, int D, int H, int W, int d3, int c1, int c2, const float *dx, float *dcoarse,
                             float *dskip)
 {
   long i = (long)blockIdx.x * 256 + threadIdx.x;
   const int C = c1 + c2;
-  long ncoarse = (long)B * (H / 2) * (W / 2) * c1;
-  long nskip = (long)B * H * W * c2;
+  const int Dc = d3 ? D / 2 : 1;
+  long ncoarse = (long)B * Dc * (H / 2) * (W / 2) * c1;
+  long nskip = (long)B * D * H * W * c2;
   if(i < ncoarse) {
     int c = (int)(i % c1);
     long q = i / c1;
     int xc = (int)(q % (W / 2));
     q /= W / 2;
     int yc = (int)(q % (H / 2));
-    int b = (int)(q / (H / 2));
+    q /= H / 2;
+    int zc = (int)(q % Dc);
+    int b = (int)(q / Dc);
     float s = 0.f;
-    for(int a2 = 0; a2 < 2; a2++)
-      for(int b2 = 0; b2 < 2; b2++)
-        s += dx[(((long)b * H + 2 * yc + a2) * W + 2 * xc + b2) * C + c];
+    for(int c2z = 0; c2z < (d3 ? 2 : 1); c2z++)
+      for(int a2 = 0; a2 < 2; a2++)
+        for(int b2 = 0; b2 < 2; b2++)
+          s += dx[((((long)b * D + (d3 ? 2 * zc + c2z : 0)) * H + 2 * yc + a2) * W + 2 * xc + b2) * C + c];
     dcoarse[i] += s;
   }
   else if(i < ncoarse + nskip) {
@@ -655,7 +674,7 @@ __global__ void k_adam(long n, float *w, const float *g, float *m, float *v, flo
 }
 
 // BF16 forward weights and the tap-mirrored, transposed adjoint weights
-__global__ void k_prep_weights(int cout, int cin_pad, int K_pad, int K2_pad, const float *w32,
+__global__ void k_prep_weights(int cout, int cin_pad, int K_pad, int K2_pad, int taps, const float *w32,
                                u16 *wt, u16 *wtT)
 {
   long i = (long)blockIdx.x * 256 + threadIdx.x;
@@ -666,7 +685,7 @@ __global__ void k_prep_weights(int cout, int cin_pad, int K_pad, int K2_pad, con
     int ci = (int)(i / K2_pad), k2 = (int)(i % K2_pad);
     int tap2 = k2 / cout, co = k2 - tap2 * cout;
     float v = 0.f;
-    if(tap2 < 9) v = w32[(long)co * K_pad + (8 - tap2) * cin_pad + ci];
+    if(tap2 < taps) v = w32[(long)co * K_pad + (taps - 1 - tap2) * cin_pad + ci];
     wtT[i] = f2bf(v);
   }
 }
@@ -784,6 +803,9 @@ struct ConvLayer {
 
 struct UserPrec {
   int H = 0, W = 0, Bmax = 0;  // grid and batch capacity
+  int D = 1, d3 = 0, taps = 9; // depth (1: 2D grid), 3D flag, taps per convolution
+  // voxels of one sample at level lvl (spatial size / 2^lvl in each direction)
+  long vox(int lvl) const { return (long)(d3 ? D >> lvl : 1) * (H >> lvl) * (W >> lvl); }
   std::vector<ConvLayer> L;    // 14 convolutions
   float *ow = nullptr, *ob = nullptr, *gow = nullptr, *gob = nullptr;  // 1x1 output (16 + 1)
   float *mow = nullptr, *vow = nullptr, *mob = nullptr, *vob = nullptr;
@@ -826,6 +848,8 @@ ConvArgs conv_args(const UserPrec &U, const ConvLayer &l, int B)
   a.B = B;
   a.H = U.H >> l.level;
   a.W = U.W >> l.level;
+  a.D = U.d3 ? U.D >> l.level : 1;
+  a.d3 = U.d3;
   a.cin_pad = l.cin_pad;
   a.cout = l.cout;
   a.K_pad = l.K_pad;
@@ -901,7 +925,7 @@ template <int MODE, int TM> int launch_conv_tm(const ConvArgs &a, int TN, dim3 g
 
 template <int MODE> int launch_conv_mode(ConvArgs a, const ConvPlan &c, float *split)
 {
-  const long P = (long)a.B * a.H * a.W;
+  const long P = (long)a.B * a.D * a.H * a.W;
   a.kchunk = c.kchunk;
   a.split = c.nsplit > 1 ? split : nullptr;
   if(c.ks) {
@@ -932,7 +956,7 @@ dim3 g1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 // path reserves it when the network is attached.
 int launch_conv(UserPrec &U, const ConvArgs &a, int mode, bool may_alloc)
 {
-  const ConvPlan c = plan_conv((long)a.B * a.H * a.W, a.cout, a.K_pad);
+  const ConvPlan c = plan_conv((long)a.B * a.D * a.H * a.W, a.cout, a.K_pad);
   if((size_t)c.split_floats > U.split_size) {
     if(!may_alloc)
       return set_error(GDP_E_STATE, "convolution scratch not reserved (%ld floats)", c.split_floats);
@@ -952,7 +976,7 @@ size_t forward_split_floats(const UserPrec &U, int B)
 {
   size_t m = 0;
   for(const auto &l : U.L) {
-    long P = (long)B * (U.H >> l.level) * (U.W >> l.level);
+    long P = (long)B * U.vox(l.level);
     m = std::max(m, (size_t)plan_conv(P, l.cout, l.K_pad).split_floats);
   }
   return m;
@@ -968,7 +992,7 @@ int forward(UserPrec &U, int B, const int *gate, int gmode, bool may_alloc, bool
     a.gmode = gmode;
     GDP_TRY(launch_conv(U, a, U.L[i].mode, may_alloc));
   }
-  long P = (long)B * U.H * U.W;
+  long P = (long)B * U.vox(0);
   if(out_layer)
     hipLaunchKernelGGL(k_out_fwd, g1(P), dim3(256), 0, ctx().stream, P, 16, U.act.back(), U.ow, U.ob,
                        U.z, gate, gmode);
@@ -996,9 +1020,9 @@ template <typename T> int colsum(UserPrec &U, const T *x, long P, int C, const f
 int backward(UserPrec &U, int B)
 {
   hipStream_t s = ctx().stream;
-  const long P0 = (long)B * U.H * U.W;
+  const long P0 = (long)B * U.vox(0);
   for(size_t t = 0; t < U.gact.size(); t++) {
-    long n = (long)B * (U.H >> U.act_lvl[t]) * (U.W >> U.act_lvl[t]) * U.act_ch[t];
+    long n = (long)B * U.vox(U.act_lvl[t]) * U.act_ch[t];
     GDP_HIP(hipMemsetAsync(U.gact[t], 0, sizeof(float) * n, s));
   }
   // output 1x1: weight gradient sum_p dz[p] x[p][c], bias gradient sum_p dz[p]
@@ -1007,8 +1031,8 @@ int backward(UserPrec &U, int B)
   hipLaunchKernelGGL(k_out_bwd_x, g1(P0 * 16), dim3(256), 0, s, P0, 16, U.dz, U.ow, U.gact.back());
   for(int i = (int)U.L.size() - 1; i >= 0; i--) {
     const ConvLayer &l = U.L[i];
-    const int H = U.H >> l.level, W = U.W >> l.level;
-    const long P = (long)B * H * W;
+    const int H = U.H >> l.level, W = U.W >> l.level, Dl = U.d3 ? U.D >> l.level : 1;
+    const long P = (long)B * Dl * H * W;
     // G = relu'(y) * dY (BF16)
     hipLaunchKernelGGL(k_relu_grad, g1(P * l.cout), dim3(256), 0, s, P * l.cout, U.gact[i + 1],
                        U.act[i + 1], U.g);
@@ -1039,6 +1063,8 @@ int backward(UserPrec &U, int B)
     d.B = B;
     d.H = H;
     d.W = W;
+    d.D = Dl;
+    d.d3 = U.d3;
     d.cin_pad = l.cout;
     d.cout = l.cin_pad;
     d.K_pad = l.K2_pad;
@@ -1051,11 +1077,11 @@ int backward(UserPrec &U, int B)
     if(l.mode == M_PLAIN)
       hipLaunchKernelGGL(k_add, g1(nx), dim3(256), 0, s, nx, U.dx, U.gact[l.in_tensor]);
     else if(l.mode == M_POOL)
-      hipLaunchKernelGGL(k_pool_adj, g1(nx * 4), dim3(256), 0, s, B, H, W, l.cin_pad, U.dx,
+      hipLaunchKernelGGL(k_pool_adj, g1(nx * (U.d3 ? 8 : 4)), dim3(256), 0, s, B, Dl, H, W, l.cin_pad, U.d3, U.dx,
                          U.gact[l.in_tensor]);
     else {
-      long nt = (long)B * (H / 2) * (W / 2) * l.c1 + P * (l.cin_pad - l.c1);
-      hipLaunchKernelGGL(k_upcat_adj, g1(nt), dim3(256), 0, s, B, H, W, l.c1, l.cin_pad - l.c1,
+      long nt = (long)B * (U.d3 ? Dl / 2 : 1) * (H / 2) * (W / 2) * l.c1 + P * (l.cin_pad - l.c1);
+      hipLaunchKernelGGL(k_upcat_adj, g1(nt), dim3(256), 0, s, B, Dl, H, W, U.d3, l.c1, l.cin_pad - l.c1,
                          U.dx, U.gact[l.in_tensor], U.gact[l.in2_tensor]);
     }
   }
@@ -1068,7 +1094,7 @@ int prep_weights(UserPrec &U)
   for(auto &l : U.L) {
     long n = std::max((long)l.cout * l.K_pad, (long)l.cin_pad * l.K2_pad);
     hipLaunchKernelGGL(k_prep_weights, g1(n), dim3(256), 0, ctx().stream, l.cout, l.cin_pad, l.K_pad,
-                       l.K2_pad, l.w32, l.wt, l.wtT);
+                       l.K2_pad, U.taps, l.w32, l.wt, l.wtT);
   }
   GDP_HIP(hipGetLastError());
   return GDP_OK;
@@ -1135,7 +1161,7 @@ int user_prec_apply_t(UserPrec *u, const TV *in, TV *out, KCtl *C, int gmode)
   hipStream_t s = ctx().stream;
   const int *gate = (const int *)C;
   const int nparts = u->pc_nparts;
-  const int npix = u->H * u->W;
+  const int npix = (int)u->vox(0);
   hipLaunchKernelGGL(k_pc_unscale<TV>, dim3(nparts), dim3(256), 0, s, npix, u->pix2dof, u->perm, u->rs,
                      u->dscale, in, u->r32, u->pc_part, gate, gmode);
   const int odd = u->odd && u->Bmax >= 2;
@@ -1173,16 +1199,20 @@ struct gdp_unet {
   std::unique_ptr<UserPrec> u;
 };
 
-extern "C" int gdp_unet_create(int H, int W, int batch, unsigned seed, gdp_unet **out)
+static int unet_create(int D, int H, int W, int batch, unsigned seed, gdp_unet **out)
 {
-  if(!out || H <= 0 || W <= 0 || H % 8 || W % 8 || batch < 1)
-    return set_error(GDP_E_ARG, "gdp_unet_create: grid must be a positive multiple of 8");
+  if(!out || H <= 0 || W <= 0 || D <= 0 || H % 8 || W % 8 || (D > 1 && D % 8) || batch < 1)
+    return set_error(GDP_E_ARG, "gdp_unet_create: grid sides must be positive multiples of 8");
   GDP_TRY(ensure_ready());
   auto *N = new gdp_unet;
   N->u.reset(new UserPrec);
   UserPrec &U = *N->u;
   U.H = H;
   U.W = W;
+  U.D = D;
+  U.d3 = D > 1;
+  U.taps = U.d3 ? 27 : 9;
+  const int taps = U.taps;
   U.Bmax = batch;
   // activations: 0 = input (8 channels)
   auto add_act = [&](int ch, int lvl) {
@@ -1221,15 +1251,15 @@ extern "C" int gdp_unet_create(int H, int W, int batch, unsigned seed, gdp_unet 
       l.cin_pad = U.act_ch[l.in_tensor];
       l.cin = (i == 0) ? 2 : l.cin_pad;
     }
-    l.K_pad = (int)rup(9L * l.cin_pad, 32);
-    l.K2_pad = (int)rup(9L * l.cout, 32);
+    l.K_pad = (int)rup((long)taps * l.cin_pad, 32);
+    l.K2_pad = (int)rup((long)taps * l.cout, 32);
     add_act(l.cout, l.level);
     long nw = (long)l.cout * l.K_pad;
     std::vector<float> w(nw, 0.f), b(l.cout, 0.f);
     // He initialisation over the true fan-in
-    std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (9.f * l.cin)));
+    std::normal_distribution<float> nd(0.f, std::sqrt(2.f / ((float)taps * l.cin)));
     for(int co = 0; co < l.cout; co++)
-      for(int tap = 0; tap < 9; tap++)
+      for(int tap = 0; tap < taps; tap++)
         for(int ci = 0; ci < l.cin; ci++) w[(long)co * l.K_pad + tap * l.cin_pad + ci] = nd(rng);
     GDP_TRY(dalloc(&l.w32, nw));
     GDP_TRY(dalloc(&l.gw, nw));
@@ -1262,7 +1292,7 @@ extern "C" int gdp_unet_create(int H, int W, int batch, unsigned seed, gdp_unet 
   }
   size_t maxx = 0;
   for(size_t t = 0; t < U.act_ch.size(); t++) {
-    size_t n = (size_t)batch * (H >> U.act_lvl[t]) * (W >> U.act_lvl[t]) * U.act_ch[t];
+    size_t n = (size_t)batch * U.vox(U.act_lvl[t]) * U.act_ch[t];
     u16 *a = nullptr;
     float *g = nullptr;
     GDP_TRY(dalloc(&a, n));
@@ -1273,19 +1303,30 @@ extern "C" int gdp_unet_create(int H, int W, int batch, unsigned seed, gdp_unet 
     maxx = std::max(maxx, n);
   }
   size_t maxin = 0;
-  for(auto &l : U.L) maxin = std::max(maxin, (size_t)batch * (H >> l.level) * (W >> l.level) * l.cin_pad);
+  for(auto &l : U.L) maxin = std::max(maxin, (size_t)batch * U.vox(l.level) * l.cin_pad);
   GDP_TRY(dalloc(&U.dx, std::max(maxx, maxin)));
   GDP_TRY(dalloc(&U.g, maxx));
-  size_t P = (size_t)batch * H * W;
+  size_t P = (size_t)batch * U.vox(0);
   GDP_TRY(dalloc(&U.z, P));
   GDP_TRY(dalloc(&U.dz, P));
   GDP_TRY(dalloc(&U.target, P));
-  GDP_TRY(dalloc(&U.mask, (size_t)H * W));
+  GDP_TRY(dalloc(&U.mask, (size_t)U.vox(0)));
   GDP_TRY(dalloc(&U.loss, (size_t)batch));
   GDP_TRY(prep_weights(U));
   GDP_TRY(sync());
   *out = N;
   return GDP_OK;
+}
+
+extern "C" int gdp_unet_create(int H, int W, int batch, unsigned seed, gdp_unet **out)
+{
+  return unet_create(1, H, W, batch, seed, out);
+}
+
+extern "C" int gdp_unet_create_3d(int D, int H, int W, int batch, unsigned seed, gdp_unet **out)
+{
+  if(D < 8) return set_error(GDP_E_ARG, "gdp_unet_create_3d: depth must be a positive multiple of 8");
+  return unet_create(D, H, W, batch, seed, out);
 }
 
 extern "C" int gdp_unet_destroy(gdp_unet *n)
@@ -1300,7 +1341,7 @@ extern "C" int gdp_unet_num_params(const gdp_unet *n, long *count)
 {
   if(!n || !count) return set_error(GDP_E_ARG, "null argument");
   long c = 17;
-  for(auto &l : n->u->L) c += (long)l.cout * 9 * l.cin + l.cout;
+  for(auto &l : n->u->L) c += (long)l.cout * n->u->taps * l.cin + l.cout;
   *count = c;
   return GDP_OK;
 }
@@ -1319,7 +1360,7 @@ static int params_io(gdp_unet *n, float *host, bool get, bool grads)
     GDP_TRY(d2h(w.data(), dw, nw));
     GDP_TRY(d2h(b.data(), db, l.cout));
     for(int co = 0; co < l.cout; co++)
-      for(int tap = 0; tap < 9; tap++)
+      for(int tap = 0; tap < U.taps; tap++)
         for(int ci = 0; ci < l.cin; ci++) {
           float &slot = w[(long)co * l.K_pad + tap * l.cin_pad + ci];
           if(get) host[off] = slot;
@@ -1373,7 +1414,7 @@ extern "C" int gdp_unet_forward(gdp_unet *n, int batch, const float *r, const fl
 {
   if(!n || !r || !feat || batch < 1 || batch > n->u->Bmax) return set_error(GDP_E_ARG, "gdp_unet_forward");
   UserPrec &U = *n->u;
-  const long HW = (long)U.H * U.W, P = batch * HW;
+  const long HW = U.vox(0), P = batch * HW;
   float *dr = nullptr, *df = nullptr;
   GDP_TRY(dalloc(&dr, P));
   GDP_TRY(dalloc(&df, P));
@@ -1399,7 +1440,7 @@ extern "C" int gdp_unet_train_step(gdp_unet *n, int batch, const float *r, const
     return set_error(GDP_E_ARG, "gdp_unet_train_step");
   UserPrec &U = *n->u;
   hipStream_t s = ctx().stream;
-  const long HW = (long)U.H * U.W, P = batch * HW;
+  const long HW = U.vox(0), P = batch * HW;
   float *dr = nullptr, *df = nullptr;
   GDP_TRY(dalloc(&dr, P));
   GDP_TRY(dalloc(&df, P));
@@ -1449,7 +1490,7 @@ extern "C" int gdp_unet_attach(gdp_unet *n, gdp_solver *s, int n_dof, const int 
 {
   if(!n || !s || !pix2dof || !feat || n_dof < 1) return set_error(GDP_E_ARG, "gdp_unet_attach");
   UserPrec &U = *n->u;
-  const long npix = (long)U.H * U.W;
+  const long npix = U.vox(0);
   std::vector<int> d2p(n_dof, -1);
   for(long p = 0; p < npix; p++) {
     int d = pix2dof[p];
@@ -1512,12 +1553,12 @@ extern "C" int gdp_unet_get_activation(gdp_unet *n, int t, int batch, float *hos
   const int nt = (int)U.act.size();
   if(t < 0 || t > nt) return set_error(GDP_E_ARG, "activation index %d out of range", t);
   if(t == nt) {
-    long c = (long)batch * U.H * U.W;
+    long c = (long)batch * U.vox(0);
     if(count) *count = c;
     if(host) GDP_TRY(d2h(host, U.z, (size_t)c));
     return GDP_OK;
   }
-  long c = (long)batch * (U.H >> U.act_lvl[t]) * (U.W >> U.act_lvl[t]) * U.act_ch[t];
+  long c = (long)batch * U.vox(U.act_lvl[t]) * U.act_ch[t];
   if(count) *count = c;
   if(!host) return GDP_OK;
   std::vector<u16> h(c);

@@ -31,13 +31,19 @@ FEAT_SCALE = 0.25  # log(nu) feature scaling (keeps the input O(1))
 
 
 class UNet:
-    """Device U-Net (16-32-64-128 channels) on an H x W grid."""
+    """Device U-Net (16-32-64-128 channels) on an H x W grid, or (D > 1) on a
+    D x H x W grid with 3x3x3 convolutions."""
 
-    def __init__(self, H: int, W: int, batch: int = 1, seed: int = 0):
-        self.H, self.W, self.batch = int(H), int(W), int(batch)
+    def __init__(self, H: int, W: int, batch: int = 1, seed: int = 0, D: int = 1):
+        self.H, self.W, self.batch, self.D = int(H), int(W), int(batch), int(D)
+        self.shape = (self.D, self.H, self.W) if self.D > 1 else (self.H, self.W)
         self.h = C.c_void_p()
-        check(lib().gdp_unet_create(self.H, self.W, self.batch, seed, C.byref(self.h)),
-              "gdp_unet_create")
+        if self.D > 1:
+            check(lib().gdp_unet_create_3d(self.D, self.H, self.W, self.batch, seed, C.byref(self.h)),
+                  "gdp_unet_create_3d")
+        else:
+            check(lib().gdp_unet_create(self.H, self.W, self.batch, seed, C.byref(self.h)),
+                  "gdp_unet_create")
 
     def close(self):
         if self.h and self.h.value:
@@ -72,17 +78,17 @@ class UNet:
         return g
 
     def forward(self, r: np.ndarray, feat: np.ndarray) -> np.ndarray:
-        r = np.ascontiguousarray(r, np.float32).reshape(-1, self.H, self.W)
+        r = np.ascontiguousarray(r, np.float32).reshape(-1, *self.shape)
         feat = np.ascontiguousarray(np.broadcast_to(feat, r.shape), np.float32)
         z = np.zeros_like(r)
         check(lib().gdp_unet_forward(self.h, r.shape[0], fptr(r), fptr(feat), fptr(z)), "forward")
         return z
 
     def train_step(self, r, feat, target, mask, lr: float) -> float:
-        r = np.ascontiguousarray(r, np.float32).reshape(-1, self.H, self.W)
+        r = np.ascontiguousarray(r, np.float32).reshape(-1, *self.shape)
         feat = np.ascontiguousarray(np.broadcast_to(feat, r.shape), np.float32)
         target = np.ascontiguousarray(target, np.float32).reshape(r.shape)
-        mask = np.ascontiguousarray(mask, np.float32).reshape(self.H, self.W)
+        mask = np.ascontiguousarray(mask, np.float32).reshape(self.shape)
         loss = C.c_float()
         check(lib().gdp_unet_train_step(self.h, r.shape[0], fptr(r), fptr(feat), fptr(target),
                                         fptr(mask), C.c_float(lr), C.byref(loss)), "train_step")
@@ -112,7 +118,7 @@ class UNet:
         check(lib().gdp_unet_detach(solver.h), "gdp_unet_detach")
 
     def save(self, path):
-        np.savez_compressed(path, params=self.get_params(), H=self.H, W=self.W)
+        np.savez_compressed(path, params=self.get_params(), H=self.H, W=self.W, D=self.D)
 
     def load(self, path):
         with np.load(path, allow_pickle=False) as z:
@@ -128,11 +134,15 @@ class GridMaps:
     H: int
     W: int
     ndof: int
-    pix2dof: np.ndarray   # (H*W,) int32, -1 where no DOF
+    pix2dof: np.ndarray   # (H*W,) or (D*H*W,) int32, -1 where no DOF
     dof2pix: np.ndarray   # (ndof,) int64
-    mask: np.ndarray      # (H, W) float32, 1 on DOF pixels
-    feat: np.ndarray      # (H, W) float32, scaled log(nu) per node
+    mask: np.ndarray      # (H, W) or (D, H, W) float32, 1 on DOF pixels
+    feat: np.ndarray      # like mask: scaled log(nu) per node
     dscale: Optional[np.ndarray] = None  # (ndof,) 1/sqrt(diag A): the network's space
+
+    @property
+    def shape(self):
+        return self.mask.shape
 
 
 def diag_scale(A) -> np.ndarray:
@@ -181,9 +191,9 @@ def grid_maps(pr, dof_table: np.ndarray, A=None) -> GridMaps:
 
 
 def to_grid(v: np.ndarray, maps: GridMaps) -> np.ndarray:
-    g = np.zeros(maps.H * maps.W, np.float64)
+    g = np.zeros(int(np.prod(maps.shape)), np.float64)
     g[maps.dof2pix] = v
-    return g.reshape(maps.H, maps.W)
+    return g.reshape(maps.shape)
 
 
 def from_grid(g: np.ndarray, maps: GridMaps) -> np.ndarray:
@@ -283,8 +293,9 @@ def normalise_pairs(r, e, mask):
     """r -> unit RMS over the DOF pixels; e -> unit 2-norm (the loss target)."""
     m = mask[None]
     nd = float(mask.sum())
-    rn = np.sqrt((r * r * m).sum(axis=(1, 2), keepdims=True) / nd)
-    en = np.sqrt((e * e * m).sum(axis=(1, 2), keepdims=True))
+    ax = tuple(range(1, r.ndim))
+    rn = np.sqrt((r * r * m).sum(axis=ax, keepdims=True) / nd)
+    en = np.sqrt((e * e * m).sum(axis=ax, keepdims=True))
     rn[rn == 0] = 1
     en[en == 0] = 1
     return (r * m / rn).astype(np.float32), (e * m / en).astype(np.float32)

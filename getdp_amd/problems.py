@@ -13,6 +13,8 @@ Problems built here:
                       (``[ nu[] * Dof{d u}, {d u} ] + [ -js[], {u} ]``)
   * elasticity3d(n)   3D linear elasticity, 6-tet Kuhn cubes, the tensor form
                       of templates/Lib_Elasticity_u.pro:151-219 (block DOFs)
+  * poisson3d(n)      3D scalar Poisson, P1 on 6-tet Kuhn cells, checkerboard
+                      or lognormal nu (the learned preconditioner's 3D problem)
   * curlcurl3d(n)     3D magnetostatics with Whitney edge elements
                       (``[ nu[] * Dof{d a}, {d a} ] + [ 1e-6 * Dof{a}, {a} ]
                       + [ -js[], {a} ]``), PEC boundary
@@ -292,6 +294,58 @@ def elasticity3d(n: int = 64, seed: int = 0, sigma: float = 1.0, poisson: float 
     terms.append(Term(T_SRC_NODE, 0, -1, [RTRAC], 3, f, jac_sur=1))
     return Problem(f"elasticity3d_{n}", xyz, etype, region, conn, spaces, terms,
                    meta=dict(n=n, seed=seed, E=Evals, ntet=nt))
+
+
+def poisson3d(n: int = 64, seed: int = 0, nsub: int = 8, nu_vals=(1.0, 1e-3), frac: float = 0.5,
+              field_kind: str = "checker", sigma: float = 1.0, nbins: int = 64, js: float = 1.0,
+              radius: float = 0.25, npoints: int = 4) -> Problem:
+    """3D scalar magnetostatics/Poisson on the unit cube, 6-tet Kuhn cells:
+    ``[ nu[] * Dof{d u}, {d u} ] + [ -js[], {u} ]`` with P1 BF_Node, u = 0 on
+    the whole boundary.  nu per element: ``checker`` -- a seeded random
+    ``frac`` of the nsub^3 subcubes take nu_vals[1], the rest nu_vals[0];
+    ``lognormal`` -- per-cell lognormal (sigma), quantised to nbins classes.
+    Source js inside a centred ball of the given radius.
+
+    Regions: tetrahedra ``1 + cls + K*in_src`` (K classes), boundary triangles 0."""
+    xyz, tets, ci, cj, ck, cell, nid = _cube_mesh(n)
+    nt = tets.shape[0]
+    rng = np.random.default_rng(seed)
+    if field_kind == "checker":
+        nsc = nsub ** 3
+        chosen = np.zeros(nsc, np.int32)
+        chosen[rng.choice(nsc, size=int(round(frac * nsc)), replace=False)] = 1
+        sc = ((ck * nsub // n) * nsub + (cj * nsub // n)) * nsub + (ci * nsub // n)
+        cls = np.repeat(chosen[sc], 6).astype(np.int32)
+        K = 2
+        cls_val = np.asarray(nu_vals, np.float64)
+    else:
+        z = rng.normal(0.0, sigma, size=n ** 3)
+        edges_ = np.linspace(-4 * sigma, 4 * sigma, nbins + 1)
+        cls = np.repeat(np.clip(np.searchsorted(edges_, z) - 1, 0, nbins - 1), 6).astype(np.int32)
+        K = nbins
+        cls_val = np.exp(0.5 * (edges_[:-1] + edges_[1:]))
+    cen = xyz[tets - 1].mean(axis=1)
+    src = (((cen - 0.5) ** 2).sum(axis=1) < radius ** 2).astype(np.int32)
+    reg_tet = 1 + cls + K * src
+    nreg = 1 + 2 * K
+    bnd = np.concatenate([_face_tris(n, nid, ax, sd) for ax in range(3) for sd in (0, 1)])
+    etype = np.concatenate([np.full(nt, TET), np.full(bnd.shape[0], TRI)]).astype(np.int32)
+    region = np.concatenate([reg_tet, np.zeros(bnd.shape[0], np.int32)]).astype(np.int32)
+    conn = np.concatenate([tets, bnd]).astype(np.int32)
+    dom = list(range(1, nreg))
+    srcreg = list(range(1 + K, nreg))
+    nu = np.zeros((nreg, 1))
+    for r in range(1, nreg):
+        nu[r, 0] = cls_val[(r - 1) % K]
+    f = np.zeros((nreg, 1))
+    f[srcreg, 0] = -js
+    bnodes = np.unique(bnd[:, :3].ravel()).astype(np.int32)
+    space = Space(BF_NODE, 1, dom, bnodes, np.zeros(bnodes.shape[0]))
+    terms = [Term(T_GRAD_GRAD, 0, 0, dom, npoints, nu),
+             Term(T_SRC_NODE, 0, -1, srcreg, npoints, f)]
+    return Problem(f"poisson3d_{n}", xyz, etype, region, conn, [space], terms,
+                   grid=(n + 1, n + 1, n + 1),
+                   meta=dict(n=n, seed=seed, ncls=K, cls_val=cls_val, ntet=nt))
 
 
 def curlcurl3d(n: int = 48, seed: int = 0, nsub: int = 8, frac: float = 0.2,

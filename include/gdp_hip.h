@@ -430,6 +430,11 @@ int gdp_batch_info(const gdp_batch *batch, int *its, int *status, double *res0, 
 typedef struct gdp_unet gdp_unet;
 /* H, W multiples of 8; batch = largest batch of train/forward calls. */
 int gdp_unet_create(int H, int W, int batch, unsigned seed, gdp_unet **out);
+/* The same network on a 3D grid of D x H x W voxels (each side a multiple of
+ * 8): 3x3x3 convolutions, 2x2x2 pooling and upsampling; parameters in the
+ * layout [cout][27][cin], taps ordered (dz, dy, dx) row-major.  Every other
+ * gdp_unet_* call works on either kind, with voxels in (z, y, x) order. */
+int gdp_unet_create_3d(int D, int H, int W, int batch, unsigned seed, gdp_unet **out);
 int gdp_unet_destroy(gdp_unet *net);
 int gdp_unet_num_params(const gdp_unet *net, long *count);
 /* Parameters as one FP32 array: per convolution weights [cout][3][3][cin]

@@ -1,5 +1,6 @@
 """Plain PyTorch (CPU, FP32) reference of the device U-Net, for the numerics
-tests of the BF16 MFMA kernels.
+tests of the BF16 MFMA kernels, on 2D grids (3x3 convolutions) and on 3D
+grids (3x3x3 convolutions, 2x2x2 pooling); ``dim`` selects which.
 
 ``bf16_points=True`` rounds to BF16 exactly where the device kernels do
 (network input, weights, every convolution output after bias+ReLU, every
@@ -32,18 +33,22 @@ def layer_cin():
     return cins
 
 
-def num_params():
-    return sum(c * 9 * ci + c for (m, c, t, t2), ci in zip(SPEC, layer_cin())) + 17
+def num_params(dim: int = 2):
+    taps = 3 ** dim
+    return sum(c * taps * ci + c for (m, c, t, t2), ci in zip(SPEC, layer_cin())) + 17
 
 
-def unpack(params: np.ndarray, dtype=torch.float32):
-    """Flat parameter array (device layout) -> list of (W[cout,cin,3,3], b), (ow, ob)."""
+def unpack(params: np.ndarray, dtype=torch.float32, dim: int = 2):
+    """Flat parameter array (device layout [cout][taps][cin], taps (dz,) dy, dx
+    row-major) -> list of (W[cout,cin,(3,)3,3], b), (ow, ob)."""
     p = torch.as_tensor(np.asarray(params), dtype=dtype)
     off = 0
     layers = []
+    taps = 3 ** dim
     for (mode, cout, t, t2), cin in zip(SPEC, layer_cin()):
-        n = cout * 9 * cin
-        w = p[off:off + n].reshape(cout, 3, 3, cin).permute(0, 3, 1, 2).contiguous()
+        n = cout * taps * cin
+        w = p[off:off + n].reshape(cout, *([3] * dim), cin)
+        w = w.permute(0, dim + 1, *range(1, dim + 1)).contiguous()
         off += n
         b = p[off:off + cout].clone()
         off += cout
@@ -57,7 +62,8 @@ def unpack(params: np.ndarray, dtype=torch.float32):
 def pack(layers, out) -> np.ndarray:
     parts = []
     for w, b in layers:
-        parts.append(w.permute(0, 2, 3, 1).reshape(-1))
+        dim = w.dim() - 2
+        parts.append(w.permute(0, *range(2, dim + 2), 1).reshape(-1))
         parts.append(b.reshape(-1))
     parts += [out[0].reshape(-1), out[1].reshape(-1)]
     return torch.cat(parts).detach().numpy()
@@ -90,24 +96,27 @@ class _QB(torch.autograd.Function):
 
 
 def forward(params, r, feat, bf16_points=True, keep=False):
-    """r, feat: (B, H, W).  Returns z (B, H, W) and, with keep, the list of
-    tensors (B, C, h, w) in the device's tensor numbering."""
+    """r, feat: (B, H, W) or (B, D, H, W).  Returns z of the same shape and,
+    with keep, the list of tensors (B, C, ...) in the device's numbering."""
     QF = _QF.apply if bf16_points else (lambda x: x)
     QB = _QB.apply if bf16_points else (lambda x: x)
     layers, (ow, ob) = params
     dt = ow.dtype
+    dim = np.ndim(r) - 1
+    conv = F.conv3d if dim == 3 else F.conv2d
+    pool = F.avg_pool3d if dim == 3 else F.avg_pool2d
     x0 = QF(torch.stack([torch.as_tensor(r, dtype=dt), torch.as_tensor(feat, dtype=dt)], 1))
     acts = [x0]
     for (mode, cout, t, t2), (w, b) in zip(SPEC, layers):
         x = acts[t]
         if mode == "pool":
-            x = QF(F.avg_pool2d(x, 2))
+            x = QF(pool(x, 2))
         elif mode == "upcat":
             x = torch.cat([F.interpolate(x, scale_factor=2, mode="nearest"), acts[t2]], 1)
-        y = F.conv2d(x, QF(w), b, padding=1)
+        y = conv(x, QF(w), b, padding=1)
         y = QB(y)
         acts.append(QF(F.relu(y)))
-    z = F.conv2d(acts[-1], ow.view(1, 16, 1, 1), ob)[:, 0]
+    z = conv(acts[-1], ow.view(1, 16, *([1] * dim)), ob)[:, 0]
     return (z, acts) if keep else z
 
 
@@ -116,13 +125,14 @@ def loss_fn(z, target, mask):
     m = torch.as_tensor(mask, dtype=z.dtype)
     t = torch.as_tensor(target, dtype=z.dtype)
     zm = z * m
-    nz = torch.sqrt((zm * zm).sum(dim=(1, 2)))
-    c = (zm * t).sum(dim=(1, 2)) / nz
+    ax = tuple(range(1, z.dim()))
+    nz = torch.sqrt((zm * zm).sum(dim=ax))
+    c = (zm * t).sum(dim=ax) / nz
     return (2 - 2 * c).sum(), 2 - 2 * c
 
 
 def grads(params_flat, r, feat, target, mask, bf16_points=True, dtype=torch.float32):
-    layers, out = unpack(params_flat, dtype)
+    layers, out = unpack(params_flat, dtype, dim=np.ndim(r) - 1)
     leaves = []
     for w, b in layers:
         w.requires_grad_(True)
