@@ -174,18 +174,61 @@ __global__ void __launch_bounds__(256) k_op_dot(SellOp<TA> A, const TV *__restri
   if(threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// t = v - A x (gated): the residual of the learned stage, one pass.
+// Scale of the learned stage (two-stage preconditioner): w = A x into t, and
+// per-block partial sums of <v, w> and <w, w> into part[2 b], part[2 b + 1].
 template <typename TA, typename TV>
-__global__ void __launch_bounds__(256) k_op_resid(SellOp<TA> A, const TV *__restrict__ x,
-                                                  const TV *__restrict__ v, TV *__restrict__ t,
-                                                  const KCtl *C, int gmode)
+__global__ void __launch_bounds__(256) k_op_dots(SellOp<TA> A, const TV *__restrict__ x,
+                                                 const TV *__restrict__ v, TV *__restrict__ t, double *part,
+                                                 const KCtl *C, int gmode)
 {
   if(gated(C, gmode)) return;
   const int row = blockIdx.x * 256 + threadIdx.x;
-  if(row >= A.n) return;
-  double sum;
-  sell_row<TA, TV>(row, A, x, sum);
-  t[row] = v[row] - (TV)sum;
+  double vw = 0., ww = 0.;
+  if(row < A.n) {
+    double sum;
+    sell_row<TA, TV>(row, A, x, sum);
+    const TV w = (TV)sum;
+    t[row] = w;
+    vw = (double)v[row] * (double)w;
+    ww = (double)w * (double)w;
+  }
+  vw = block_sum<double, 256>(vw);
+  __syncthreads();
+  ww = block_sum<double, 256>(ww);
+  if(threadIdx.x == 0) {
+    part[2 * blockIdx.x] = vw;
+    part[2 * blockIdx.x + 1] = ww;
+  }
+}
+
+// alpha = <v, w> / <w, w> (the least-squares scale of the learned correction,
+// 1 when w = 0), summed over the blocks in order, into part[2 nb]
+__global__ void __launch_bounds__(256) k_alpha(int nb, double *part, const KCtl *C, int gmode)
+{
+  if(gated(C, gmode)) return;
+  double vw = 0., ww = 0.;
+  for(int b = threadIdx.x; b < nb; b += 256) {
+    vw += part[2 * b];
+    ww += part[2 * b + 1];
+  }
+  vw = block_sum<double, 256>(vw);
+  __syncthreads();
+  ww = block_sum<double, 256>(ww);
+  if(threadIdx.x == 0) part[2 * nb] = (ww > 0. && vw == vw && ww < 1e300) ? vw / ww : 1.;
+}
+
+// x *= alpha; t = v - alpha w (t holds w)
+template <typename TV>
+__global__ void __launch_bounds__(256) k_scale_resid(int n, const double *alpha_p, TV *__restrict__ x,
+                                                     const TV *__restrict__ v, TV *__restrict__ t,
+                                                     const KCtl *C, int gmode)
+{
+  if(gated(C, gmode)) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if(i >= n) return;
+  const TV a = (TV)*alpha_p;
+  x[i] = a * x[i];
+  t[i] = v[i] - a * t[i];
 }
 
 template <typename TA, typename TV>
@@ -1271,16 +1314,25 @@ int prec_apply(const PrecOp<TV> &M, const TV *in, TV *out, KCtl *C, int gmode, i
   return GDP_OK;
 }
 
-// Two-stage flexible preconditioner: the learned map, then an ILU(0)
-// correction of the residual it leaves, z = N(v) + (LU)^{-1} (v - A N(v)).
+// Two-stage flexible preconditioner: the learned map, scaled to the least-
+// squares fit of v, then an ILU(0) correction of the residual it leaves,
+// z = a N(v) + (LU)^{-1} (v - a A N(v)), a = <v, A N(v)> / |A N(v)|^2.  The
+// network is trained on the direction of the error only (its output scale is
+// arbitrary), and the correction stage needs the scale.
 template <typename TA, typename TV>
 int prec_apply_two_stage(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *in, TV *out, TV *t,
-                         TV *u, KCtl *C, int gmode, int n)
+                         TV *u, double *part, KCtl *C, int gmode, int n)
 {
   GDP_TRY(user_prec_apply(M.user, in, out, C, gmode));
-  if(n)
-    hipLaunchKernelGGL((k_op_resid<TA, TV>), dim3((n + 255) / 256), dim3(256), 0, ctx().stream, A, out,
-                       in, t, C, gmode);
+  if(n) {
+    // partial sums and the scale in the cycle's partial-sum buffer (at least
+    // 66 ceil(n / 256) doubles), free while the step preconditions
+    const int nb = (n + 255) / 256;
+    hipLaunchKernelGGL((k_op_dots<TA, TV>), dim3(nb), dim3(256), 0, ctx().stream, A, out, in, t, part, C, gmode);
+    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(256), 0, ctx().stream, nb, part, C, gmode);
+    hipLaunchKernelGGL((k_scale_resid<TV>), dim3(nb), dim3(256), 0, ctx().stream, n, part + 2 * nb, out, in, t, C,
+                       gmode);
+  }
   // u = (LU)^{-1} t, and out += u in the backward sweep
   GDP_TRY(M.ilu->template apply<TV>(t, (TV)1, u, (const int *)C, gmode, out));
   GDP_HIP(hipGetLastError());
@@ -1337,7 +1389,7 @@ template <typename TA, typename TV> int gm_cycle(const GmArgs<TA, TV> &ga)
     if(two_stage) {
       TV *zk = Z + (size_t)(k - 1) * n;
       TV *t1 = Z + (size_t)m * n, *t2 = t1 + n;
-      GDP_TRY((prec_apply_two_stage<TA, TV>(A, M, vk1, zk, t1, t2, C, 2, n)));
+      GDP_TRY((prec_apply_two_stage<TA, TV>(A, M, vk1, zk, t1, t2, ga.part, C, 2, n)));
       src = zk;
     }
     else if(flexible) {

@@ -68,7 +68,7 @@ def random_multiscale(n: int, rng) -> np.ndarray:
     k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kr[None, None, :] ** 2
     alpha = rng.uniform(0.0, 3.0)
     k0 = rng.uniform(1.0, 6.0)
-    f = np.fft.irfftn(np.fft.rfftn(w) * (1.0 + k2 / k0 ** 2) ** (-alpha / 2), s=(n, n, n))
+    f = np.fft.irfftn(np.fft.rfftn(w) * (1.0 + k2 / k0 ** 2) ** (-alpha / 2), s=(n, n, n), axes=(0, 1, 2))
     return f / max(f.std(), 1e-30)
 
 
