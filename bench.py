@@ -127,6 +127,33 @@ def learned_section(n: int = 256, seed: int = 90000):
     }
 
 
+def learned3d_section(n: int = 64, seed: int = 90002):
+    """The learned preconditioner on the 3D problem (poisson3d_n: Kuhn
+    tetrahedra on an n^3 grid, unseen checkerboard coefficient field), shipped
+    weights: two-stage FGMRES in FP64 and in mixed precision (FP32 inner
+    FGMRES, FP64 refinement) against the plain GPU ILU(0)-GMRES(30) in the
+    reference's RCMK ordering (and in multicolour ordering), all to 1e-10."""
+    from getdp_amd import learned3d as L3
+    from pathlib import Path
+    w = Path(ROOT) / "getdp_amd" / "weights" / "unet_poisson3d.npz"
+    with np.load(w, allow_pickle=False) as z:
+        params = z["params"]
+    row = L3.evaluate(params, n, [seed], log=log, precisions=(0, 1))[0]
+    pick = lambda keys: min(keys, key=lambda k: row[k]["ms"] if row[k]["rel_res"] <= 1.5e-10 else 1e30)
+    b64 = pick(["learned_rcmk", "learned_multicolour"])
+    bmx = pick(["learned_rcmk_mixed", "learned_multicolour_mixed"])
+    ref = row["plain_rcmk"]["ms"]
+    return {
+        "workload": f"poisson3d_{n} checkerboard seed {seed} (unseen): {row['dofs']} DOFs, 1e-10",
+        "plain_rcmk": row["plain_rcmk"], "plain_multicolour": row["plain_multicolour"],
+        "learned_fp64": dict(row[b64], variant=b64), "learned_mixed": dict(row[bmx], variant=bmx),
+        "speedup_vs_plain_rcmk": round(ref / row[b64]["ms"], 2),
+        "speedup_mixed_vs_plain_rcmk": round(ref / row[bmx]["ms"], 2),
+        "speedup_vs_best_plain": row["speedup_vs_best_plain"],
+        "target_speedup": 5.0,
+    }
+
+
 def c2_section(n: int = 48, its64: int = 1500):
     """Curl-curl with Whitney edges (1e-6 mass, 1:1000 reluctivity): device
     assembly, then ILU(0)-GMRES(30) (multicolour ordering) in FP64 for a fixed
@@ -419,6 +446,7 @@ def main():
     }
     if not args.no_learned:
         out["learned"] = learned_section()
+        out["learned_3d"] = learned3d_section()
     if not args.no_batch:
         out["batch"] = batch_section()
     if not args.no_c2:

@@ -485,13 +485,220 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs w)
   }
 }
 
-__global__ void k_sum_chunks(int nchunks, long n, const float *part, float *out)
+// ---- weight gradient on channel-major copies (grids whose rows are multiples of 8)
+//
+// The MFMA reduction of the weight gradient runs over voxels, so its operands
+// need 8 consecutive voxels of one channel per lane: the backward pass keeps
+// channel-major copies GT[co][p] of the layer's output gradient and XT[ci][p]
+// of its logical input (pooled / upsampled-and-concatenated as the forward
+// loader forms it), written through an LDS transpose.  A voxel octet then is
+// one aligned 16-byte load of GT and, for tap (dz, dy, dx), the octet of XT at
+// p + (dz*H + dy)*W + dx: with W a multiple of 8 this is aligned for dx = 0
+// and one element off for dx = +-1 (an aligned 16-byte load plus the
+// neighbouring dword, realigned by byte shifts).  Octets whose neighbour row
+// or plane lies outside the grid are zero; in-row x edges zero one element.
+constexpr int kTrPix = 64;        // voxels per transpose block
+constexpr int kTrStride = 72;     // LDS row stride (elements, 16-byte aligned rows)
+constexpr int kXtPad = 16;        // elements before and after each XT copy
+
+// G = BF16(dY * (Y > 0)) in both layouts: g[p][c] and gt[c][P] (P % 8 == 0)
+__global__ void __launch_bounds__(256) k_relu_grad_t(long P, int C, const float *dy, const u16 *y, u16 *g,
+                                                     u16 *gt)
 {
-  long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if(i >= n) return;
+  __shared__ u16 tile[128 * kTrStride];
+  const long p0 = (long)blockIdx.x * kTrPix;
+  for(int i = threadIdx.x; i < kTrPix * C; i += 256) {
+    const int pl = i / C, c = i - pl * C;
+    const long p = p0 + pl;
+    u16 v = 0;
+    if(p < P) {
+      const long k = p * C + c;
+      v = f2bf(bf2f(y[k]) > 0.f ? dy[k] : 0.f);
+      g[k] = v;
+    }
+    tile[c * kTrStride + pl] = v;
+  }
+  __syncthreads();
+  for(int i = threadIdx.x; i < C * (kTrPix / 8); i += 256) {
+    const int c = i >> 3, seg = i & 7;
+    const long p = p0 + 8 * seg;
+    if(p < P) *(uint4 *)(gt + (long)c * P + p) = *(const uint4 *)(tile + c * kTrStride + 8 * seg);
+  }
+}
+
+// XT[ci][p] = logical input of the convolution (the forward loader's octets)
+template <int MODE>
+__global__ void __launch_bounds__(256) k_input_t(ConvArgs a, u16 *xt)
+{
+  __shared__ u16 tile[192 * kTrStride];
+  const long P = (long)a.B * a.D * a.H * a.W;
+  const long p0 = (long)blockIdx.x * kTrPix;
+  const int cin = a.cin_pad, noct = cin >> 3;
+  for(int i = threadIdx.x; i < kTrPix * noct; i += 256) {
+    const int pl = i % kTrPix, c8 = i / kTrPix;
+    const long p = p0 + pl;
+    uint4 v = {0u, 0u, 0u, 0u};
+    if(p < P) {
+      long q = p;
+      const int px = (int)(q % a.W);
+      q /= a.W;
+      const int py = (int)(q % a.H);
+      q /= a.H;
+      const int pz = (int)(q % a.D);
+      const int b = (int)(q / a.D);
+      const long pix = MODE == M_POOL ? (a.d3 ? (((long)b * 2 * a.D + 2 * pz) * (2 * a.H) + 2 * py) * (2 * a.W) + 2 * px
+                                              : ((long)b * 2 * a.H + 2 * py) * (2 * a.W) + 2 * px)
+                                      : p;
+      v = conv_load_a<MODE>(a, pix, pz, py, px, a.d3 ? b * (a.D >> 1) : b, 0, 0, 0, 8 * c8, true);
+    }
+    const u16 *e = (const u16 *)&v;
+#pragma unroll
+    for(int j = 0; j < 8; j++) tile[(8 * c8 + j) * kTrStride + pl] = e[j];
+  }
+  __syncthreads();
+  for(int i = threadIdx.x; i < cin * (kTrPix / 8); i += 256) {
+    const int c = i >> 3, seg = i & 7;
+    const long p = p0 + 8 * seg;
+    if(p < P) *(uint4 *)(xt + (long)c * P + p) = *(const uint4 *)(tile + c * kTrStride + 8 * seg);
+  }
+}
+
+struct WgradTArgs {
+  int D, H, W, d3, cin, cout, K_pad;
+  long P;
+  const u16 *gt;  // [cout][P]
+  const u16 *xt;  // [cin][P], kXtPad readable elements before and after
+  float *part;    // [nchunks][cout][K_pad]
+  int chunk;      // voxels per chunk (multiple of 128)
+  int nchunks, nco, nk;  // chunks, 16-row tiles of co, 64-column tiles of k
+};
+
+// XT octet of channel row xrow at voxel pb shifted by the tap (dz, dy, dx);
+// (x0, y, z): coordinates of pb.  Zero outside the grid.
+__device__ inline uint4 wgrad_b_octet(const u16 *xrow, long pb, long off, int dz, int dy, int dx, int x0, int y,
+                                      int z, const WgradTArgs &w)
+{
+  uint4 bv = {0u, 0u, 0u, 0u};
+  if((unsigned)(y + dy) >= (unsigned)w.H || (unsigned)(z + dz) >= (unsigned)w.D) return bv;
+  const u16 *src = xrow + pb + off;
+  if(dx == 0) return *(const uint4 *)src;
+  if(dx > 0) {  // elements 1..8 of the aligned octet at src - 1
+    const uint4 v = *(const uint4 *)(src - 1);
+    const unsigned e = *(const unsigned *)(src + 7);
+    bv.x = __builtin_amdgcn_alignbyte(v.y, v.x, 2);
+    bv.y = __builtin_amdgcn_alignbyte(v.z, v.y, 2);
+    bv.z = __builtin_amdgcn_alignbyte(v.w, v.z, 2);
+    bv.w = __builtin_amdgcn_alignbyte(e, v.w, 2);
+    if(x0 + 8 == w.W) bv.w &= 0xffffu;  // x + 1 leaves the row
+    return bv;
+  }
+  // elements -1..6 of the aligned octet at src + 1
+  const uint4 v = *(const uint4 *)(src + 1);
+  const unsigned e = *(const unsigned *)(src - 1);
+  bv.x = __builtin_amdgcn_alignbyte(v.x, e, 2);
+  bv.y = __builtin_amdgcn_alignbyte(v.y, v.x, 2);
+  bv.z = __builtin_amdgcn_alignbyte(v.z, v.y, 2);
+  bv.w = __builtin_amdgcn_alignbyte(v.w, v.z, 2);
+  if(x0 == 0) bv.x &= 0xffff0000u;  // x - 1 leaves the row
+  return bv;
+}
+
+__device__ inline void wgrad_step(int &x, int &y, int &z, int n, const WgradTArgs &w)
+{
+  x += n;
+  while(x >= w.W) {
+    x -= w.W;
+    if(++y == w.H) {
+      y = 0;
+      if(++z == w.D) z = 0;
+    }
+  }
+}
+
+// dW[co][k] over one chunk of voxels: wave tile 16 co x 16 columns k = (tap, ci).
+// The MFMA sum over voxels is order-free, so lane group q takes the 32
+// consecutive voxels p0 + 32 q .. + 31 as four octets of four MFMA steps (64
+// contiguous bytes of each operand row per lane).  Workgroups are ordered so
+// that all tiles of one chunk run on one XCD (workgroup i runs on XCD i % 8),
+// which keeps the chunk's GT and XT rows in that XCD's L2.
+__global__ void __launch_bounds__(256) k_wgrad_t(WgradTArgs w)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, q = lane >> 4;
+  const int T = w.nco * w.nk;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int chunk_id = xcd + 8 * (slot / T);
+  if(chunk_id >= w.nchunks) return;
+  const int tile = slot % T;
+  const int ty = tile / w.nk, tz = tile - ty * w.nk;
+  const long p_begin = (long)chunk_id * w.chunk;
+  const long p_end = p_begin + w.chunk < w.P ? p_begin + w.chunk : w.P;
+  const int co = ty * 16 + r16;
+  const int kcol = (tz * 4 + wave) * 16 + r16;
+  const int tap = kcol / w.cin;
+  const int ci = kcol - tap * w.cin;
+  const bool kvalid = tap < (w.d3 ? 27 : 9);
+  const int t3 = w.d3 ? tap : tap + 9;
+  const int dz = t3 / 9 - 1, dy = (t3 % 9) / 3 - 1, dx = t3 % 3 - 1;
+  const long off = ((long)dz * w.H + dy) * w.W + dx;
+  const u16 *arow = w.gt + (long)(co < w.cout ? co : 0) * w.P;
+  const u16 *xrow = w.xt + (long)(kvalid ? ci : 0) * w.P;
+  const bool aok = co < w.cout;
+  long pb = p_begin + 32 * q;
+  int x0, y, z;
+  {
+    long t = pb;
+    x0 = (int)(t % w.W);
+    t /= w.W;
+    y = (int)(t % w.H);
+    t /= w.H;
+    z = (int)(t % w.D);
+  }
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  const uint4 zero4 = {0u, 0u, 0u, 0u};
+  for(long p0 = p_begin; p0 < p_end; p0 += 128, pb += 128) {
+    uint4 av[4], bv[4];
+    int xj = x0, yj = y, zj = z;
+#pragma unroll
+    for(int j = 0; j < 4; j++) {
+      const long pj = pb + 8 * j;
+      const bool in = pj < p_end;
+      av[j] = (in && aok) ? *(const uint4 *)(arow + pj) : zero4;
+      bv[j] = (in && kvalid) ? wgrad_b_octet(xrow, pj, off, dz, dy, dx, xj, yj, zj, w) : zero4;
+      wgrad_step(xj, yj, zj, 8, w);
+    }
+#pragma unroll
+    for(int j = 0; j < 4; j++)
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av[j]), as_bf8(bv[j]), acc, 0, 0, 0);
+    wgrad_step(x0, y, z, 128, w);
+  }
+#pragma unroll
+  for(int i = 0; i < 4; i++) {
+    const int orow = ty * 16 + 4 * q + i;
+    const int ocol = (tz * 4 + wave) * 16 + r16;
+    if(orow < w.cout && ocol < w.K_pad) w.part[((long)chunk_id * w.cout + orow) * w.K_pad + ocol] = acc[i];
+  }
+}
+
+// out[i] = sum over chunks of part[c][i]: a block takes 32 elements, its 8
+// thread groups the chunks g, g + 8, ..., then the 8 group sums are added in
+// group order
+__global__ void __launch_bounds__(256) k_sum_chunks(int nchunks, long n, const float *part, float *out)
+{
+  __shared__ float sh[8][32];
+  const int e = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const long i = (long)blockIdx.x * 32 + e;
   float s = 0.f;
-  for(int c = 0; c < nchunks; c++) s += part[(long)c * n + i];
-  out[i] = s;
+  if(i < n)
+    for(int c = g; c < nchunks; c += 8) s += part[(long)c * n + i];
+  sh[g][e] = s;
+  __syncthreads();
+  if(g == 0 && i < n) {
+    float t = 0.f;
+#pragma unroll
+    for(int k = 0; k < 8; k++) t += sh[k][e];
+    out[i] = t;
+  }
 }
 
 // G = BF16(dY * (Y > 0)), with Y the layer's (ReLU) output
@@ -533,13 +740,15 @@ __global__ void __launch_bounds__(256) k_colsum_part(long n, int C, const T *x, 
   }
 }
 
+// one block per column: thread t adds blocks t, t + 256, ... in order, then a
+// fixed-order tree over the threads
 __global__ void __launch_bounds__(256) k_colsum_fin(int nblocks, int C, const float *part, float *out)
 {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if(c >= C) return;
+  const int c = blockIdx.x;
   float s = 0.f;
-  for(int b = 0; b < nblocks; b++) s += part[(long)b * C + c];
-  out[c] = s;
+  for(int b = threadIdx.x; b < nblocks; b += 256) s += part[(long)b * C + c];
+  s = block_sum<float, 256>(s);
+  if(threadIdx.x == 0) out[c] = s;
 }
 
 // adjoints of the fused loaders, accumulating into FP32 gradient tensors
@@ -816,6 +1025,8 @@ struct UserPrec {
   float *z = nullptr, *dz = nullptr, *target = nullptr, *mask = nullptr, *loss = nullptr;
   float *dx = nullptr;  // scratch for the adjoint of one convolution's input
   u16 *g = nullptr;     // ReLU-masked gradient (BF16) of one layer
+  u16 *gt = nullptr;    // the same, channel-major [cout][P]
+  u16 *xt = nullptr;    // logical input of one layer, channel-major [cin][P] (padded)
   float *part = nullptr;
   size_t part_size = 0;
   float *cpart = nullptr;  // column-sum partials (bias and output-layer gradients)
@@ -1012,7 +1223,7 @@ template <typename T> int colsum(UserPrec &U, const T *x, long P, int C, const f
     U.cpart_size = need;
   }
   hipLaunchKernelGGL(k_colsum_part<T>, dim3(nb), dim3(256), 0, ctx().stream, n, C, x, dz, U.cpart);
-  hipLaunchKernelGGL(k_colsum_fin, dim3((C + 255) / 256), dim3(256), 0, ctx().stream, nb, C, U.cpart, out);
+  hipLaunchKernelGGL(k_colsum_fin, dim3(C), dim3(256), 0, ctx().stream, nb, C, U.cpart, out);
   GDP_HIP(hipGetLastError());
   return GDP_OK;
 }
@@ -1033,30 +1244,52 @@ int backward(UserPrec &U, int B)
     const ConvLayer &l = U.L[i];
     const int H = U.H >> l.level, W = U.W >> l.level, Dl = U.d3 ? U.D >> l.level : 1;
     const long P = (long)B * Dl * H * W;
-    // G = relu'(y) * dY (BF16)
-    hipLaunchKernelGGL(k_relu_grad, g1(P * l.cout), dim3(256), 0, s, P * l.cout, U.gact[i + 1],
-                       U.act[i + 1], U.g);
+    // rows of whole voxel octets: channel-major operands for the weight gradient
+    const bool tr = W % 8 == 0;
+    // G = relu'(y) * dY (BF16), and G^T
+    if(tr)
+      hipLaunchKernelGGL(k_relu_grad_t, dim3((unsigned)((P + kTrPix - 1) / kTrPix)), dim3(256), 0, s, P, l.cout,
+                         U.gact[i + 1], U.act[i + 1], U.g, U.gt);
+    else
+      hipLaunchKernelGGL(k_relu_grad, g1(P * l.cout), dim3(256), 0, s, P * l.cout, U.gact[i + 1],
+                         U.act[i + 1], U.g);
     GDP_TRY(colsum(U, U.g, P, l.cout, (const float *)nullptr, l.gb));
-    // weight gradient
+    // weight gradient: chunks of voxels small enough for ~2048 workgroups
     ConvArgs x = conv_args(U, l, B);
-    WgradArgs wa{};
-    wa.x = x;
-    wa.g = U.g;
-    wa.chunk = 2048;
-    int nchunks = (int)((P + wa.chunk - 1) / wa.chunk);
+    const long tiles = (long)((l.cout + 15) / 16) * ((l.K_pad + 63) / 64);
+    int chunk = 2048;
+    while(chunk > 256 && (P + chunk - 1) / chunk * tiles < 2048) chunk >>= 1;
+    int nchunks = (int)((P + chunk - 1) / chunk);
     size_t need = (size_t)nchunks * l.cout * l.K_pad;
     if(need > U.part_size) {
       dfree(U.part);
       GDP_TRY(dalloc(&U.part, need));
       U.part_size = need;
     }
-    wa.part = U.part;
     dim3 gw((unsigned)nchunks, (unsigned)((l.cout + 15) / 16), (unsigned)((l.K_pad + 63) / 64));
-    if(l.mode == M_POOL) hipLaunchKernelGGL(k_wgrad<M_POOL>, gw, dim3(256), 0, s, wa);
-    else if(l.mode == M_UPCAT) hipLaunchKernelGGL(k_wgrad<M_UPCAT>, gw, dim3(256), 0, s, wa);
-    else hipLaunchKernelGGL(k_wgrad<M_PLAIN>, gw, dim3(256), 0, s, wa);
+    if(tr) {
+      u16 *xt = U.xt + kXtPad;
+      const dim3 gt((unsigned)((P + kTrPix - 1) / kTrPix));
+      if(l.mode == M_POOL) hipLaunchKernelGGL(k_input_t<M_POOL>, gt, dim3(256), 0, s, x, xt);
+      else if(l.mode == M_UPCAT) hipLaunchKernelGGL(k_input_t<M_UPCAT>, gt, dim3(256), 0, s, x, xt);
+      else hipLaunchKernelGGL(k_input_t<M_PLAIN>, gt, dim3(256), 0, s, x, xt);
+      WgradTArgs wt{Dl, H, W, U.d3, l.cin_pad, l.cout, l.K_pad, P, U.gt, xt, U.part, chunk,
+                    nchunks, (int)gw.y, (int)gw.z};
+      const unsigned nb = (unsigned)(8 * ((nchunks + 7) / 8) * gw.y * gw.z);
+      hipLaunchKernelGGL(k_wgrad_t, dim3(nb), dim3(256), 0, s, wt);
+    }
+    else {
+      WgradArgs wa{};
+      wa.x = x;
+      wa.g = U.g;
+      wa.chunk = chunk;
+      wa.part = U.part;
+      if(l.mode == M_POOL) hipLaunchKernelGGL(k_wgrad<M_POOL>, gw, dim3(256), 0, s, wa);
+      else if(l.mode == M_UPCAT) hipLaunchKernelGGL(k_wgrad<M_UPCAT>, gw, dim3(256), 0, s, wa);
+      else hipLaunchKernelGGL(k_wgrad<M_PLAIN>, gw, dim3(256), 0, s, wa);
+    }
     long nw = (long)l.cout * l.K_pad;
-    hipLaunchKernelGGL(k_sum_chunks, g1(nw), dim3(256), 0, s, nchunks, nw, U.part, l.gw);
+    hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((nw + 31) / 32)), dim3(256), 0, s, nchunks, nw, U.part, l.gw);
     if(i == 0) continue;  // no gradient needed for the network input
     // input gradient: adjoint convolution (mirrored taps) on G
     ConvArgs d{};
@@ -1134,6 +1367,8 @@ UserPrec::~UserPrec()
   zfree(loss);
   zfree(dx);
   zfree(g);
+  zfree(gt);
+  zfree(xt);
   zfree(part);
   zfree(split);
   zfree(pix2dof);
@@ -1306,6 +1541,12 @@ static int unet_create(int D, int H, int W, int batch, unsigned seed, gdp_unet *
   for(auto &l : U.L) maxin = std::max(maxin, (size_t)batch * U.vox(l.level) * l.cin_pad);
   GDP_TRY(dalloc(&U.dx, std::max(maxx, maxin)));
   GDP_TRY(dalloc(&U.g, maxx));
+  // channel-major copies for the weight gradient (transpose tiles: cout <= 128, cin <= 192)
+  for(auto &l : U.L)
+    if(l.cout > 128 || l.cin_pad > 192) return set_error(GDP_E_UNSUPPORTED, "U-Net channel counts exceed the transpose tiles");
+  GDP_TRY(dalloc(&U.gt, maxx));
+  GDP_TRY(dalloc(&U.xt, maxin + 2 * kXtPad));
+  GDP_HIP(hipMemsetAsync(U.xt, 0, (maxin + 2 * kXtPad) * sizeof(u16), ctx().stream));
   size_t P = (size_t)batch * U.vox(0);
   GDP_TRY(dalloc(&U.z, P));
   GDP_TRY(dalloc(&U.dz, P));
