@@ -55,9 +55,11 @@ def assembly_bytes(pr, N: int, nnz: int) -> int:
 
 
 def cpu_baseline(pr, its_gpu: int, cg_sample: int):
-    """Oracle (single-threaded C restatement) on a bounded sample of the same
-    workload: full assembly of the same mesh, then `cg_sample` CG iterations on
-    the same system; the solve time is scaled to the GPU's iteration count."""
+    """Oracle (the C restatement; 1 thread: the reference's order, more
+    threads: threaded element loop, amux and vector loops) on a bounded sample
+    of the same workload: full assembly of the same mesh, then `cg_sample` CG
+    iterations on the same system; the solve time is scaled to the GPU's
+    iteration count."""
     from oracle import pyoracle as O
     t0 = time.perf_counter()
     ia, ja, a, b, tab = O.assemble(pr)
@@ -68,10 +70,64 @@ def cpu_baseline(pr, its_gpu: int, cg_sample: int):
     t_cg = time.perf_counter() - t0
     per_it = t_cg / max(info["its"], 1)
     value = t_asm + per_it * its_gpu
-    return {"value": round(value, 3), "unit": "s", "cores": 1, "kind": "port",
+    return {"value": round(value, 3), "unit": "s", "kind": "port",
             "sample": (f"oracle Generate of the full mesh ({t_asm:.2f} s) + {info['its']} "
                        f"CG+Jacobi iterations ({per_it * 1e3:.1f} ms/it), scaled to the "
                        f"device's {its_gpu} iterations")}
+
+
+def cpu_threads() -> int:
+    """Host threads available to this process (affinity mask, capped by
+    OMP_NUM_THREADS when set)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_pair(fn):
+    """Run fn() with the oracle on 1 thread and on every available thread;
+    returns [result_1, result_n], each with "cores" and "kind" set."""
+    from oracle import pyoracle as O
+    out = []
+    for T in sorted({1, cpu_threads()}):
+        O.set_threads(T)
+        try:
+            r = fn()
+        finally:
+            O.set_threads(1)
+        r.update(cores=T, kind="port")
+        out.append(r)
+    return out
+
+
+def cpu_gmres(pr, its_gpu: int = 0, sample: int = 0, algorithm: int = 8, precond: int = 6, krylov: int = 30):
+    """The oracle's solve_matrix (RCMK, the reference's ordering) on problem
+    pr: the whole solve, or with `sample` > 0 that many iterations after the
+    set-up, the iteration time scaled to the device's `its_gpu` iterations."""
+    from oracle import pyoracle as O
+
+    def run():
+        ia, ja, a, b, tab = O.assemble(pr)
+        t0 = time.perf_counter()
+        x, info, _ = O.solve(ia, ja, a, b, algorithm=algorithm, preconditioner=precond, krylov=krylov,
+                             maxits=sample if sample else 100000, renumbering=1,
+                             scaling=1 if precond == 0 else 0)
+        t = time.perf_counter() - t0
+        if sample:
+            # set-up alone (renumbering, factorisation): one iteration
+            t1 = time.perf_counter()
+            O.solve(ia, ja, a, b, algorithm=algorithm, preconditioner=precond, krylov=krylov, maxits=1,
+                    renumbering=1, scaling=1 if precond == 0 else 0)
+            setup = time.perf_counter() - t1
+            per = max(t - setup, 0.0) / max(info["its"] - 1, 1)
+            return {"value_ms": round(1e3 * (setup + per * its_gpu), 1), "its": its_gpu,
+                    "sample": f"{info['its']} iterations timed ({1e3 * per:.2f} ms/it) + set-up "
+                              f"{1e3 * setup:.0f} ms, scaled to {its_gpu} iterations"}
+        return {"value_ms": round(1e3 * t, 1), "its": info["its"], "status": info["status"],
+                "sample": "whole solve"}
+    return cpu_pair(run)
 
 
 def unet_flops_per_pixel() -> float:
@@ -231,20 +287,60 @@ def default_solver_section(n: int = 512):
 
 
 def default_solver_cpu(pr):
-    """The oracle's solve_matrix with the same defaults on the same system
-    (single thread): the reference's CPU path for this solver."""
+    """The oracle's solve_matrix with the same defaults on the same system:
+    the reference's CPU path for this solver (the ILUTP factorisation and the
+    triangular solves are sequential at any thread count)."""
     from oracle import pyoracle as O
     ia, ja, a, b, tab = O.assemble(pr)
-    t0 = time.perf_counter()
-    x, info, _ = O.solve(ia, ja, a, b, algorithm=8, preconditioner=2, krylov=40, maxits=1000,
-                         renumbering=1)
-    return {"value_ms": round(1e3 * (time.perf_counter() - t0), 1), "its": info["its"],
-            "status": info["status"], "cores": 1, "kind": "port"}
+
+    def run():
+        t0 = time.perf_counter()
+        x, info, _ = O.solve(ia, ja, a, b, algorithm=8, preconditioner=2, krylov=40, maxits=1000,
+                             renumbering=1)
+        return {"value_ms": round(1e3 * (time.perf_counter() - t0), 1), "its": info["its"],
+                "status": info["status"]}
+    return cpu_pair(run)
 
 
 def batch_section(count: int = 512):
     from getdp_amd import batch as BT
     return BT.bench(count, reps=1, log=log)
+
+
+def batch_cpu(count: int = 512, stride: int = 32):
+    """The oracle on every `stride`-th of the same instances (assembly and
+    ILU(0)-GMRES(30) to 1e-10 each): on 1 thread in sequence (time scaled by
+    `stride`), and on a pool of one worker per host thread (each solve
+    sequential), where the estimate is the sum of the instance times measured
+    in the pool, scaled by `stride`, over the thread count -- a perfectly
+    balanced pool over all instances, which favours the CPU."""
+    from concurrent.futures import ThreadPoolExecutor
+    from getdp_amd import batch as BT
+    from oracle import pyoracle as O
+    probs = BT.make_instances(count, seed=0)[::stride]
+
+    def one(p):
+        t0 = time.perf_counter()
+        ia, ja, a, b, tab = O.assemble(p)
+        x, info, _ = O.solve(ia, ja, a, b, algorithm=8, preconditioner=6, krylov=30, maxits=100000,
+                             renumbering=1)
+        return info["status"], time.perf_counter() - t0
+
+    out = []
+    O.set_threads(1)
+    for T in sorted({1, cpu_threads()}):
+        if T == 1:
+            res = [one(p) for p in probs]
+        else:
+            with ThreadPoolExecutor(T) as ex:
+                res = list(ex.map(one, probs))
+        t = sum(r[1] for r in res) * stride / T
+        out.append({"value_ms": round(1e3 * t, 1), "solves_per_s": round(count / t, 2), "cores": T,
+                    "kind": "port", "converged_in_sample": int(sum(1 for r in res if r[0] == 0)),
+                    "sample": (f"{len(probs)} of the {count} instances (every {stride}th); "
+                               + ("sequential, scaled" if T == 1 else
+                                  f"instance times measured on a {T}-worker pool, scaled, over {T} workers"))})
+    return out
 
 
 def roofline_only(n: int, reps: int):
@@ -456,9 +552,27 @@ def main():
         out["default_solver"], dpr = default_solver_section()
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(pr, out["cg_its"], args.cg_sample)
+            pair = cpu_pair(lambda: cpu_baseline(pr, out["cg_its"], args.cg_sample))
+            out["cpu_baseline"] = pair[-1]
+            out["cpu_baselines"] = pair
             if dpr is not None:
                 out["default_solver"]["cpu"] = default_solver_cpu(dpr)
+            if "learned" in out:
+                from getdp_amd import learned as L
+                out["learned"]["cpu_plain_rcmk"] = cpu_gmres(L.random_field_problem(256, 90000))
+            if "learned_3d" in out:
+                from getdp_amd import learned3d as L3
+                out["learned_3d"]["cpu_plain_rcmk"] = cpu_gmres(
+                    L3.random_field_problem(64, 90002), its_gpu=out["learned_3d"]["plain_rcmk"]["its"], sample=40)
+            if "c2_mixed_precision" in out:
+                from getdp_amd import problems as P
+                c2 = out["c2_mixed_precision"]
+                cpr = P.curlcurl3d(48)
+                c2["cpu_gmres_ilu0"] = cpu_gmres(cpr, its_gpu=c2["fp64"]["its"], sample=40)
+                c2["cpu_cg_jacobi"] = cpu_gmres(cpr, its_gpu=c2["fp64_cg_jacobi"]["its"], sample=200,
+                                                algorithm=1, precond=0)
+            if "batch" in out:
+                out["batch"]["cpu"] = batch_cpu()
         except Exception as e:  # the oracle is a checker; report, do not fail
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out))

@@ -12,6 +12,7 @@
 #include "gdp_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -31,10 +32,34 @@ static int g_dot_blocks = 0;
  * the reference's order. */
 void orc_set_dot_blocks(int nblocks) { g_dot_blocks = nblocks > 0 ? nblocks : 0; }
 
+/* CPU baselines only: with nthreads > 1 the element loop of orc_assemble
+ * (bit-identical result), amux (row-parallel, bit-identical) and the vector
+ * loops of CG and GMRES run on that many OpenMP threads, and dot products are
+ * summed in 64 contiguous blocks added in order (deterministic, a different
+ * rounding order from the reference's).  ILU factorisation and triangular
+ * solves stay sequential, as in the reference.  1 (the default) is the
+ * reference's single-threaded order everywhere. */
+static int g_threads = 1;
+void orc_set_threads(int nthreads) { g_threads = nthreads > 1 ? nthreads : 1; }
+#define PFOR _Pragma("omp parallel for schedule(static) if(g_threads > 1 && n >= PAR_N) num_threads(g_threads)")
+#define PAR_N 16384 /* vectors shorter than this stay on one thread */
+
 double orc_ddot(int n, const double *x, const double *y)
 {
   double t = 0.0;
   if(n <= 0) return 0.0;
+  if(g_threads > 1 && g_dot_blocks == 0 && n >= PAR_N) {
+    double part[64];
+    const int per = (n + 63) / 64;
+    PFOR
+    for(int b = 0; b < 64; b++) {
+      double s = 0.0;
+      for(int i = b * per; i < n && i < (b + 1) * per; i++) s = s + x[i] * y[i];
+      part[b] = s;
+    }
+    for(int b = 0; b < 64; b++) t = t + part[b];
+    return t;
+  }
   if(g_dot_blocks > 0) {
     const int nb = g_dot_blocks, per = (n + nb - 1) / nb;
     for(int b = 0; b < nb; b++) {
@@ -57,6 +82,7 @@ double orc_ddot(int n, const double *x, const double *y)
 void orc_amux(int n, const double *x, double *y, const int *ia, const int *ja,
               const double *a)
 {
+  PFOR
   for(int i = 0; i < n; i++) {
     double t = 0.0;
     for(int k = ia[i]; k < ia[i + 1]; k++) t = t + a[k] * x[ja[k]];
@@ -424,10 +450,12 @@ static int mgsro(int n, double *V, int ind, double *hh)
     const double *vi = V + (size_t)i * n;
     double fct = orc_ddot(n, v, vi);
     hh[i] = fct;
+    PFOR
     for(int k = 0; k < n; k++) v[k] = v[k] - fct * vi[k];
     if(fct * fct > thr) {
       fct = orc_ddot(n, v, vi);
       hh[i] = hh[i] + fct;
+      PFOR
       for(int k = 0; k < n; k++) v[k] = v[k] - fct * vi[k];
     }
     nrm0 = nrm0 - hh[i] * hh[i];
@@ -438,6 +466,7 @@ static int mgsro(int n, double *V, int ind, double *hh)
   hh[ind] = nrm1;
   if(nrm1 <= 0.0) return -3;
   double fct = 1.0 / nrm1;
+  PFOR
   for(int k = 0; k < n; k++) v[k] = v[k] * fct;
   return 0;
 }
@@ -494,6 +523,7 @@ static int gmres_body(int flexible, int n, const double *rhs, double *sol,
   for(;;) { /* label 100: (re)start */
     matvec(ctx, sol, V);
     its++;
+    PFOR
     for(int i = 0; i < n; i++) V[i] = rhs[i] - V[i];
     double alpha = sqrt(orc_ddot(n, V, V));
     if(its == 1) {
@@ -508,6 +538,7 @@ static int gmres_body(int flexible, int n, const double *rhs, double *sol,
       goto done;
     }
     alpha = 1.0 / alpha;
+    PFOR
     for(int i = 0; i < n; i++) V[i] = alpha * V[i];
 
     int k = 0;
@@ -561,21 +592,27 @@ static int gmres_body(int flexible, int n, const double *rhs, double *sol,
     if(flexible) {
       for(int i = 0; i < k; i++) {
         const double *zi = Z + (size_t)i * N;
+        PFOR
         for(int ii = 0; ii < n; ii++) sol[ii] = sol[ii] + g[i] * zi[ii];
       }
     }
     else {
+      PFOR
       for(int ii = 0; ii < n; ii++) V[ii] = V[ii] * g[0];
       for(int i = 1; i < k; i++) {
         const double *vi = V + (size_t)i * N;
+        PFOR
         for(int ii = 0; ii < n; ii++) V[ii] = V[ii] + g[i] * vi[ii];
       }
       if(rp) {
         precond(ctx, V, Z);
+        PFOR
         for(int i = 0; i < n; i++) sol[i] = sol[i] + Z[i];
       }
-      else
+      else {
+        PFOR
         for(int i = 0; i < n; i++) sol[i] = sol[i] + V[i];
+      }
     }
     fpar6 = fpar5;
     if(ierr12 != 0) {
@@ -664,6 +701,7 @@ int orc_cg(int n, const double *rhs, double *sol, int maxits, double rtol,
       break;
     }
     alpha = fpar7 / alpha;
+    PFOR
     for(int i = 0; i < n; i++) {
       dx[i] = dx[i] + alpha * p[i];
       r[i] = r[i] - alpha * ap[i];
@@ -686,6 +724,7 @@ int orc_cg(int n, const double *rhs, double *sol, int maxits, double rtol,
     if(stop) break;
     alpha = fpar5 * fpar5 / fpar7;
     fpar7 = fpar5 * fpar5;
+    PFOR
     for(int i = 0; i < n; i++) p[i] = r[i] + alpha * p[i];
   }
   const double *delta = dx;
@@ -1855,10 +1894,132 @@ static void row_add(row_list *r, int col, double v)
   r->n++;
 }
 
+/* CSR of the row lists (csr_format, Sparskit.cpp:2186-2208): rows by
+ * equation, columns sorted; frees the lists.  Returns nnz. */
+static long rows_to_csr(row_list *rows, int N, int *ia, int *ja, double *a, long max_nnz, int threads)
+{
+  #pragma omp parallel for schedule(dynamic, 1024) if(threads > 1) num_threads(threads)
+  for(int i = 0; i < N; i++) {
+    row_list *r = &rows[i];
+    for(int p = 1; p < r->n; p++) { /* insertion sort by column */
+      int c = r->col[p];
+      double v = r->val[p];
+      int q = p - 1;
+      while(q >= 0 && r->col[q] > c) {
+        r->col[q + 1] = r->col[q];
+        r->val[q + 1] = r->val[q];
+        q--;
+      }
+      r->col[q + 1] = c;
+      r->val[q + 1] = v;
+    }
+  }
+  long nnz = 0;
+  ia[0] = 0;
+  for(int i = 0; i < N; i++) {
+    nnz += rows[i].n;
+    ia[i + 1] = (int)nnz;
+  }
+  #pragma omp parallel for schedule(dynamic, 1024) if(threads > 1) num_threads(threads)
+  for(int i = 0; i < N; i++) {
+    row_list *r = &rows[i];
+    for(int p = 0; p < r->n; p++) {
+      long k = (long)ia[i] + p;
+      if(k < max_nnz) {
+        ja[k] = r->col[p];
+        a[k] = r->val[p];
+      }
+    }
+    free(r->col);
+    free(r->val);
+  }
+  free(rows);
+  return nnz;
+}
+
+/* The element loop on g_threads threads, same result as the sequential one
+ * bit for bit: the element matrices of a block of elements are formed in
+ * parallel, then each thread adds, in element order, the contributions to
+ * its own contiguous range of equations (matrix rows and right-hand side). */
+static long assemble_threaded(const orc_mesh *m, const orc_space *sp, int nterms, const orc_term *t,
+                              int nmax, const int *dofnum, int N, int *ia, int *ja, double *a,
+                              long max_nnz, double *b)
+{
+  const int T = g_threads;
+  const long blk = 32768;
+  const size_t per = (size_t)MAXK * MAXK;
+  const long npair = blk * nterms;
+  double *Ek = (double *)malloc(sizeof(double) * per * (size_t)npair);
+  int *kk = (int *)malloc(sizeof(int) * 2 * (size_t)npair);
+  row_list *rows = (row_list *)calloc((size_t)N, sizeof(row_list));
+  int err = 0;
+  for(int i = 0; i < N; i++) b[i] = 0.;
+  for(long e0 = 0; e0 < m->nelem && !err; e0 += blk) {
+    const long ne = m->nelem - e0 < blk ? m->nelem - e0 : blk;
+    #pragma omp parallel for schedule(dynamic, 256) num_threads(T)
+    for(long q = 0; q < ne * nterms; q++) {
+      const int e = (int)(e0 + q / nterms);
+      const orc_term *tt = &t[q % nterms];
+      kk[2 * q] = -1;
+      if(!term_active(m, sp, tt, e)) continue;
+      if(elem_matrix_impl(m, tt, sp[tt->equ].bf == ORC_BF_PERP_EDGE, e, (double(*)[MAXK])(Ek + per * (size_t)q),
+                          &kk[2 * q], &kk[2 * q + 1])) {
+        #pragma omp atomic write
+        err = 1;
+      }
+    }
+    if(err) break;
+    #pragma omp parallel num_threads(T)
+    {
+      const int tid = omp_get_thread_num(), nt = omp_get_num_threads();
+      const int r0 = (int)((long)N * tid / nt), r1 = (int)((long)N * (tid + 1) / nt);
+      for(long q = 0; q < ne * nterms; q++) {
+        const int ke = kk[2 * q], kd = kk[2 * q + 1];
+        if(ke < 0) continue;
+        const int e = (int)(e0 + q / nterms);
+        const orc_term *tt = &t[q % nterms];
+        const double(*E)[MAXK] = (const double(*)[MAXK])(Ek + per * (size_t)q);
+        for(int i = 0; i < ke; i++) {
+          int sg;
+          int eq_ent = space_entity(m, &sp[tt->equ], e, i, &sg);
+          int eq = dofnum[(size_t)tt->equ * (nmax + 1) + eq_ent];
+          if(eq <= 0 || eq - 1 < r0 || eq - 1 >= r1) continue;
+          for(int j = 0; j < kd; j++) {
+            double val = E[i][j];
+            if(tt->dof < 0) {
+              if(val) b[eq - 1] += (1. * 1.) * (-val);
+              continue;
+            }
+            int dent = space_entity(m, &sp[tt->dof], e, j, &sg);
+            int dn = dofnum[(size_t)tt->dof * (nmax + 1) + dent];
+            if(dn > 0) row_add(&rows[eq - 1], dn - 1, val);
+            else if(dn < 0 && val) {
+              double fv = sp[tt->dof].fixed_value[-dn - 1];
+              b[eq - 1] += (fv * 1.) * (-val);
+            }
+          }
+        }
+      }
+    }
+  }
+  free(Ek);
+  free(kk);
+  if(err) {
+    for(int i = 0; i < N; i++) {
+      free(rows[i].col);
+      free(rows[i].val);
+    }
+    free(rows);
+    return -1;
+  }
+  return rows_to_csr(rows, N, ia, ja, a, max_nnz, T);
+}
+
 long orc_assemble(const orc_mesh *m, int nspaces, const orc_space *sp,
                   int nterms, const orc_term *t, int nmax, const int *dofnum,
                   int N, int *ia, int *ja, double *a, long max_nnz, double *b)
 {
+  if(g_threads > 1) return assemble_threaded(m, sp, nterms, t, nmax, dofnum, N, ia, ja, a, max_nnz, b);
   row_list *rows = (row_list *)calloc((size_t)N, sizeof(row_list));
   for(int i = 0; i < N; i++) b[i] = 0.;
   (void)nspaces;

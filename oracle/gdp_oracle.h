@@ -24,6 +24,7 @@ extern "C" {
 /* ---- Sparskit kernels --------------------------------------------------- */
 double orc_ddot(int n, const double *x, const double *y);
 void orc_set_dot_blocks(int nblocks);
+void orc_set_threads(int nthreads); /* CPU baselines: OpenMP threads (1: sequential) */
 void orc_amux(int n, const double *x, double *y, const int *ia, const int *ja,
               const double *a);
 int orc_ilu0_f32(int n, const int *ia, const int *ja, const double *a,

@@ -101,6 +101,13 @@ def set_dot_blocks(nblocks: int = 0):
     lib().orc_set_dot_blocks(int(nblocks))
 
 
+def set_threads(nthreads: int = 1):
+    """CPU baselines only: run the element loop, amux and the CG/GMRES vector
+    loops on ``nthreads`` OpenMP threads (dot products then summed in 64
+    ordered blocks); 1 restores the reference's sequential order."""
+    lib().orc_set_threads(int(nthreads))
+
+
 def ref(f64_ilu: bool = False):
     """The reference's Fortran Sparskit kernels (None if not built here)."""
     key = "f64" if f64_ilu else "f32"
