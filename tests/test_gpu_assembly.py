@@ -186,3 +186,51 @@ def test_solve_with_nonzero_dirichlet(gpu, oracle, name):
     assert abs(info["its"] - io["its"]) <= max(2, io["its"] // 50), (info["its"], io["its"])
     tol = 1e-8 if name.startswith("curlcurl") else 1e-9
     assert np.abs(x.get() - xo).max() <= tol * np.abs(xo).max()
+
+
+def fan(m: int, seed: int = 3):
+    """A disc of 2D triangles around one centre node: m triangles meet at the
+    centre (its matrix row holds m + 1 entries), a second ring of 2 m
+    triangles, the outer ring fixed to non-zero values; a mass term and a
+    source on part of the disc."""
+    rng = np.random.default_rng(seed)
+    ang = 2 * np.pi * (np.arange(m) + rng.uniform(-0.2, 0.2, m)) / m
+    r1 = 0.5 + rng.uniform(-0.05, 0.05, m)
+    xyz = np.zeros((1 + 2 * m, 3))
+    xyz[1:m + 1, 0], xyz[1:m + 1, 1] = r1 * np.cos(ang), r1 * np.sin(ang)
+    xyz[m + 1:, 0], xyz[m + 1:, 1] = np.cos(ang), np.sin(ang)
+    inner = 2 + np.arange(m)           # node numbers (1-based)
+    outer = m + 2 + np.arange(m)
+    nxt = np.roll(np.arange(m), -1)
+    tris = [np.stack([np.ones(m, int), inner, inner[nxt]], 1),
+            np.stack([inner, outer, outer[nxt]], 1), np.stack([inner, outer[nxt], inner[nxt]], 1)]
+    tri = np.concatenate(tris).astype(np.int32)
+    nt = tri.shape[0]
+    conn = np.zeros((nt, 4), np.int32)
+    conn[:, :3] = tri
+    region = np.where(rng.uniform(size=nt) < 0.5, 1, 2).astype(np.int32)
+    nu = np.array([[0.0], [1.0], [3.5]])
+    ms = np.array([[0.0], [0.2], [0.7]])
+    f = np.array([[0.0], [0.0], [-2.0]])
+    space = P.Space(P.BF_NODE, 1, [1, 2], outer.astype(np.int32), rng.uniform(-1, 1, m))
+    terms = [P.Term(P.T_GRAD_GRAD, 0, 0, [1, 2], 3, nu, dim=2),
+             P.Term(P.T_NODE_MASS, 0, 0, [1, 2], 3, ms, dim=2),
+             P.Term(P.T_SRC_NODE, 0, -1, [2], 3, f, dim=2)]
+    return P.Problem(f"fan_{m}", xyz, np.full(nt, P.TRI, np.int32), region, conn, [space], terms)
+
+
+@pytest.mark.parametrize("m", [150, 420])
+def test_long_rows_bit_exact(gpu, oracle, m):
+    """Rows longer than the device PRE pass takes (the centre of a fan of m
+    triangles: m + 1 entries): the host PRE pass builds the pattern and the
+    contribution lists; assembly bit-exact against the oracle."""
+    pr = fan(m)
+    print(f"case fan_{m}", flush=True)
+    ia, ja, a, b, tab = oracle.assemble(pr)
+    assert ia[1] - ia[0] == m + 1
+    fe, A, bv = gpu.assemble(pr)
+    gia, gja, ga = A.get_csr()
+    assert np.array_equal(gia, ia) and np.array_equal(gja, ja)
+    assert np.array_equal(fe.dof_table(0), tab[0])
+    assert np.array_equal(ga, a)
+    assert np.array_equal(bv.get(), b)
