@@ -194,7 +194,8 @@ def learned3d_section(n: int = 64, seed: int = 90002):
     w = Path(ROOT) / "getdp_amd" / "weights" / "unet_poisson3d.npz"
     with np.load(w, allow_pickle=False) as z:
         params = z["params"]
-    row = L3.evaluate(params, n, [seed], log=log, precisions=(0, 1))[0]
+        lsq = bool(z["lsq"]) if "lsq" in z.files else False
+    row = L3.evaluate(params, n, [seed], log=log, precisions=(0, 1), lsq=lsq)[0]
     pick = lambda keys: min(keys, key=lambda k: row[k]["ms"] if row[k]["rel_res"] <= 1.5e-10 else 1e30)
     b64 = pick(["learned_rcmk", "learned_multicolour"])
     bmx = pick(["learned_rcmk_mixed", "learned_multicolour_mixed"])

@@ -49,6 +49,11 @@ struct BatchArgs {
   const int *b_row, *b_cnt, *b_col;
   const float *b_val, *b_dinv;
   int zcap;  // instances with at most zcap rows keep the sweep vector in LDS
+  // larger instances with ring[inst] keep the last rmask + 1 sweep results in
+  // an LDS ring indexed by stream position (their record columns are stream
+  // positions); the others read the sweep vector from global memory
+  const int *ring;
+  int rmask;
   const long *pool_off;
   double *pool;
   const double *rhs;
@@ -88,7 +93,7 @@ __device__ inline void b_spmv(const BatchArgs &a, int r0, int n, const double *x
 // One row of a sweep: what a thread needs besides the solution entries it
 // depends on (loaded a level ahead, off the dependency chain).
 template <int W> struct SweepRec {
-  int row, cnt;
+  int row, cnt, pos;  // pos: the record's position in the instance's sweep stream
   int col[W];
   float val[W];
   float dinv;
@@ -96,13 +101,14 @@ template <int W> struct SweepRec {
 };
 
 template <int W, bool FWD>
-__device__ inline void ld_rec(const BatchArgs &a, int slot, const double *src, SweepRec<W> &r)
+__device__ inline void ld_rec(const BatchArgs &a, int slot, int s_first, const double *src, SweepRec<W> &r)
 {
   const int *rowp = FWD ? a.f_row : a.b_row, *cntp = FWD ? a.f_cnt : a.b_cnt;
   const int *colp = FWD ? a.f_col : a.b_col;
   const float *valp = FWD ? a.f_val : a.b_val;
   r.row = rowp[slot];
   r.cnt = cntp[slot];
+  r.pos = slot - s_first;
 #pragma unroll
   for(int w = 0; w < W; w++) {
     r.col[w] = colp[(long)slot * W + w];
@@ -114,13 +120,18 @@ __device__ inline void ld_rec(const BatchArgs &a, int slot, const double *src, S
 
 // lusol (ilut.F:2297-2347) on one row, same operation order:
 // forward x(i) = y(i) - sum alu(k) x(jlu(k)); backward x(i) = alu(i) (x(i) - sum ...)
-template <int W, bool FWD> __device__ inline void apply_rec(const SweepRec<W> &r, double *z)
+// RING: the solution entries come from the LDS ring (column = stream
+// position), and the result goes to both the ring and z
+template <int W, bool FWD, bool RING>
+__device__ inline void apply_rec(const SweepRec<W> &r, double *z, double *ring, int rmask)
 {
   double acc = r.pre;
 #pragma unroll
   for(int w = 0; w < W; w++)
-    if(w < r.cnt) acc = acc - (double)r.val[w] * z[r.col[w]];
-  z[r.row] = FWD ? acc : (double)r.dinv * acc;
+    if(w < r.cnt) acc = acc - (double)r.val[w] * (RING ? ring[r.col[w] & rmask] : z[r.col[w]]);
+  const double x = FWD ? acc : (double)r.dinv * acc;
+  z[r.row] = x;
+  if constexpr(RING) ring[r.pos & rmask] = x;
 }
 
 // Barrier between sweep levels.  With the sweep vector in LDS only LDS
@@ -140,33 +151,58 @@ template <bool LDSZ> __device__ inline void level_barrier()
 
 // One triangular sweep, level by level with a barrier between levels; the
 // next level's records are loaded before the current level is computed.
-template <int T, int W, bool FWD, bool LDSZ>
-__device__ inline void sweep(const BatchArgs &a, int inst, const double *src, double *z)
+template <int T, int W, bool FWD, bool LDSZ, bool RING = false>
+__device__ inline void sweep(const BatchArgs &a, int inst, const double *src, double *z, double *ring = nullptr)
 {
+  // software pipeline over levels: a level's records are loaded kAhead levels
+  // before it is computed, and its slot range kAhead levels before that, so
+  // that no global load sits on the level-to-level chain
+  constexpr int kAhead = 3;
   const int *ptr = FWD ? a.fl_ptr : a.bl_ptr;
   const int base = FWD ? a.fl_base[inst] : a.bl_base[inst];
   const int nl = FWD ? a.fl_n[inst] : a.bl_n[inst];
   if(nl <= 0) return;
   const int tid = threadIdx.x;
-  int s0 = ptr[base], s1 = ptr[base + 1];
-  SweepRec<W> cur, nxt;
-  bool has = s0 + tid < s1;
-  if(has) ld_rec<W, FWD>(a, s0 + tid, src, cur);
-  for(int l = 0; l < nl; l++) {
-    const int n0 = s1, n1 = (l + 1 < nl) ? ptr[base + l + 2] : s1;
-    const bool nh = (l + 1 < nl) && (n0 + tid < n1);
-    if(nh) ld_rec<W, FWD>(a, n0 + tid, src, nxt);
-    if(has) apply_rec<W, FWD>(cur, z);
-    for(int q = s0 + tid + T; q < s1; q += T) {
-      SweepRec<W> r;
-      ld_rec<W, FWD>(a, q, src, r);
-      apply_rec<W, FWD>(r, z);
+  const int sf = ptr[base];
+  SweepRec<W> buf[kAhead];
+  int bs0[kAhead], bs1[kAhead], ns0[kAhead], ns1[kAhead];
+  bool bh[kAhead];
+#pragma unroll
+  for(int d = 0; d < kAhead; d++) {
+    bs0[d] = d < nl ? ptr[base + d] : 0;
+    bs1[d] = d < nl ? ptr[base + d + 1] : 0;
+    ns0[d] = d + kAhead < nl ? ptr[base + d + kAhead] : 0;
+    ns1[d] = d + kAhead < nl ? ptr[base + d + kAhead + 1] : 0;
+  }
+#pragma unroll
+  for(int d = 0; d < kAhead; d++) {
+    bh[d] = bs0[d] + tid < bs1[d];
+    if(bh[d]) ld_rec<W, FWD>(a, bs0[d] + tid, sf, src, buf[d]);
+  }
+  for(int l0 = 0; l0 < nl; l0 += kAhead) {
+#pragma unroll
+    for(int d = 0; d < kAhead; d++) {
+      const int l = l0 + d;
+      if(l >= nl) break;
+      const SweepRec<W> cur = buf[d];
+      const bool has = bh[d];
+      const int s0 = bs0[d], s1 = bs1[d];
+      // level l + kAhead: its records; level l + 2 kAhead: its slot range
+      bs0[d] = ns0[d];
+      bs1[d] = ns1[d];
+      bh[d] = l + kAhead < nl && bs0[d] + tid < bs1[d];
+      if(bh[d]) ld_rec<W, FWD>(a, bs0[d] + tid, sf, src, buf[d]);
+      const int l2 = l + 2 * kAhead;
+      ns0[d] = l2 < nl ? ptr[base + l2] : 0;
+      ns1[d] = l2 < nl ? ptr[base + l2 + 1] : 0;
+      if(has) apply_rec<W, FWD, RING>(cur, z, ring, a.rmask);
+      for(int q = s0 + tid + T; q < s1; q += T) {
+        SweepRec<W> r;
+        ld_rec<W, FWD>(a, q, sf, src, r);
+        apply_rec<W, FWD, RING>(r, z, ring, a.rmask);
+      }
+      level_barrier<LDSZ>();
     }
-    level_barrier<LDSZ>();
-    cur = nxt;
-    has = nh;
-    s0 = n0;
-    s1 = n1;
   }
 }
 
@@ -178,6 +214,14 @@ __device__ inline void b_ilu(const BatchArgs &a, int inst, int n, const double *
     sweep<T, W, true, true>(a, inst, v, zlds);
     sweep<T, W, false, true>(a, inst, zlds, zlds);
     __syncthreads();  // the consumers of z also read global memory
+  }
+  else if(a.ring[inst]) {
+    // dependencies through the LDS ring (LDS fences between levels); the
+    // forward results reach the backward sweep's records through z
+    sweep<T, W, true, true, true>(a, inst, v, z, zlds);
+    __syncthreads();
+    sweep<T, W, false, true, true>(a, inst, z, z, zlds);
+    __syncthreads();
   }
   else {
     sweep<T, W, true, false>(a, inst, v, z);
@@ -419,6 +463,7 @@ struct gdp_batch {
   int W = 4;  // record width: most factor entries on one side of a row
   int *d_f_row = nullptr, *d_f_cnt = nullptr, *d_f_col = nullptr, *d_f_k = nullptr;
   int *d_b_row = nullptr, *d_b_cnt = nullptr, *d_b_col = nullptr, *d_b_k = nullptr, *d_b_dk = nullptr;
+  int *d_ring = nullptr;  // per instance: sweeps through the LDS ring
   float *d_f_val = nullptr, *d_b_val = nullptr, *d_b_dinv = nullptr;
   long *d_pool_off = nullptr;
   double *pool = nullptr, *rhs = nullptr, *sol = nullptr;
@@ -460,7 +505,7 @@ struct gdp_batch {
     for(auto &g : groups) dfree(g.d_order);
     for(int **p : {&d_perm, &d_rperm, &d_map, &d_off, &d_fl_base, &d_fl_n, &d_fl_ptr, &d_bl_base, &d_bl_n,
                    &d_bl_ptr, &d_queue, &d_its, &d_status, &d_f_row, &d_f_cnt, &d_f_col, &d_f_k, &d_b_row,
-                   &d_b_cnt, &d_b_col, &d_b_k, &d_b_dk})
+                   &d_b_cnt, &d_b_col, &d_b_k, &d_b_dk, &d_ring})
       dfree(*p);
     dfree(d_f_val);
     dfree(d_b_val);
@@ -482,6 +527,14 @@ namespace {
 // rows of the sweep vector kept in LDS, per workgroup size (1, 2 or 3
 // workgroups of 1024, 512, 256 threads per CU with 160 KB of LDS)
 int zcap_rows(int threads) { return threads == 1024 ? 18000 : (threads == 512 ? 8192 : 4096); }
+// entries of the LDS ring of larger instances: the largest power of two
+// within the same budget
+int ring_rows(int threads)
+{
+  int r = 1;
+  while(2 * r <= zcap_rows(threads)) r *= 2;
+  return r;
+}
 
 template <int T, int W> void launch_group(const BatchArgs &a, int blocks, hipStream_t s)
 {
@@ -605,6 +658,41 @@ int batch_prepare(gdp_batch *B, const gdp_matrix *A)
   if(wmax > 8) return set_error(GDP_E_UNSUPPORTED, "batched sweeps: %d factor entries on one side of a row (max 8)", wmax);
   B->W = wmax <= 4 ? 4 : 8;
   const int W = B->W;
+  // stream positions (per instance and sweep) and the LDS ring: an instance
+  // too large for the LDS sweep vector reads its dependencies from a ring of
+  // the last R results when no dependency lies R or more positions behind the
+  // end of the level that needs it (R: a power of two within the LDS budget)
+  std::vector<int> fpos(n, 0), bpos(n, 0), ringf(B->ninst, 0);
+  auto positions = [&](const std::vector<int> &base, const std::vector<int> &cnt, const std::vector<int> &ptr,
+                       const std::vector<int> &rows, std::vector<int> &pos) {
+    for(int i = 0; i < B->ninst; i++)
+      if(cnt[i] > 0)
+        for(int q = ptr[base[i]]; q < ptr[base[i] + cnt[i]]; q++) pos[rows[q]] = q - ptr[base[i]];
+  };
+  positions(fb, fn, fp, fr, fpos);
+  positions(bb, bn, bp, br, bpos);
+  auto ring_ok = [&](int i, int R, const std::vector<int> &base, const std::vector<int> &cnt,
+                     const std::vector<int> &ptr, const std::vector<int> &rows, bool lower,
+                     const std::vector<int> &pos) {
+    if(cnt[i] <= 0) return true;
+    const int s0 = ptr[base[i]];
+    for(int l = 0; l < cnt[i]; l++) {
+      const int last = ptr[base[i] + l + 1] - s0 - 1;
+      for(int q = ptr[base[i] + l]; q < ptr[base[i] + l + 1]; q++) {
+        const int r = rows[q];
+        const int k0 = lower ? B->ib[r] : dpos[r] + 1, k1 = lower ? dpos[r] : B->ib[r + 1];
+        for(int k = k0; k < k1; k++)
+          if(last - pos[B->jb[k]] >= R) return false;
+      }
+    }
+    return true;
+  };
+  for(int i = 0; i < B->ninst; i++) {
+    const int ni = B->off[i + 1] - B->off[i], zc = zcap_rows(B->cfg_threads[i]);
+    if(ni <= zc) continue;
+    const int R = ring_rows(B->cfg_threads[i]);
+    ringf[i] = ring_ok(i, R, fb, fn, fp, fr, true, fpos) && ring_ok(i, R, bb, bn, bp, br, false, bpos);
+  }
   auto records = [&](const std::vector<int> &rows, bool lower, std::vector<int> &rrow, std::vector<int> &rcnt,
                      std::vector<int> &rcol, std::vector<int> &rk, std::vector<int> *dk) {
     const size_t S = rows.size();
@@ -618,8 +706,9 @@ int batch_prepare(gdp_batch *B, const gdp_matrix *A)
       const int k0 = lower ? B->ib[r] : dpos[r] + 1, k1 = lower ? dpos[r] : B->ib[r + 1];
       rrow[q] = r - r0;
       rcnt[q] = k1 - k0;
+      const bool rg = ringf[inst_of[r]] != 0;
       for(int k = k0; k < k1; k++) {
-        rcol[q * W + (k - k0)] = B->jb[k] - r0;
+        rcol[q * W + (k - k0)] = rg ? (lower ? fpos : bpos)[B->jb[k]] : B->jb[k] - r0;
         rk[q * W + (k - k0)] = k;
       }
       if(dk) (*dk)[q] = dpos[r];
@@ -632,7 +721,7 @@ int batch_prepare(gdp_batch *B, const gdp_matrix *A)
         {&B->d_fl_base, &fb}, {&B->d_fl_n, &fn}, {&B->d_fl_ptr, &fp}, {&B->d_bl_base, &bb},
         {&B->d_bl_n, &bn}, {&B->d_bl_ptr, &bp}, {&B->d_f_row, &frow}, {&B->d_f_cnt, &fcnt},
         {&B->d_f_col, &fcol}, {&B->d_f_k, &fk}, {&B->d_b_row, &brow}, {&B->d_b_cnt, &bcnt},
-        {&B->d_b_col, &bcol}, {&B->d_b_k, &bk}, {&B->d_b_dk, &bdk}}) {
+        {&B->d_b_col, &bcol}, {&B->d_b_k, &bk}, {&B->d_b_dk, &bdk}, {&B->d_ring, &ringf}}) {
     GDP_TRY(dalloc(pr.first, std::max<size_t>(1, pr.second->size())));
     GDP_TRY(h2d(*pr.first, pr.second->data(), pr.second->size()));
   }
@@ -758,6 +847,8 @@ int batch_enqueue(gdp_batch *B, gdp_matrix *A, gdp_vector *b, gdp_vector *x)
     a.b_val = B->d_b_val;
     a.b_dinv = B->d_b_dinv;
     a.zcap = zcap_rows(g.threads);
+    a.ring = B->d_ring;
+    a.rmask = ring_rows(g.threads) - 1;
     a.pool_off = B->d_pool_off;
     a.pool = B->pool;
     a.rhs = B->rhs;

@@ -38,6 +38,7 @@ struct UserPrec;  // learned preconditioner (unet.hip)
 // says so (gmode as for the other kernels of a cycle).
 int user_prec_apply(UserPrec *u, const double *in, double *out, KCtl *C, int gmode);
 int user_prec_apply(UserPrec *u, const float *in, float *out, KCtl *C, int gmode);
+int user_prec_lsq(const UserPrec *u);  // two-stage with least-squares scaling
 // the solver's renumbering (original -> internal, internal -> original) and
 // diagonal scaling (by original index, may be null) for the next applications
 void user_prec_bind(UserPrec *u, const int *perm, const int *rperm, const double *rs);

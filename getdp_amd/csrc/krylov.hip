@@ -174,6 +174,20 @@ __global__ void __launch_bounds__(256) k_op_dot(SellOp<TA> A, const TV *__restri
   if(threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// t = v - A x (gated): the residual the learned stage leaves, one pass.
+template <typename TA, typename TV>
+__global__ void __launch_bounds__(256) k_op_resid(SellOp<TA> A, const TV *__restrict__ x,
+                                                  const TV *__restrict__ v, TV *__restrict__ t,
+                                                  const KCtl *C, int gmode)
+{
+  if(gated(C, gmode)) return;
+  const int row = blockIdx.x * 256 + threadIdx.x;
+  if(row >= A.n) return;
+  double sum;
+  sell_row<TA, TV>(row, A, x, sum);
+  t[row] = v[row] - (TV)sum;
+}
+
 // Scale of the learned stage (two-stage preconditioner): w = A x into t, and
 // per-block partial sums of <v, w> and <w, w> into part[2 b], part[2 b + 1].
 template <typename TA, typename TV>
@@ -1314,17 +1328,20 @@ int prec_apply(const PrecOp<TV> &M, const TV *in, TV *out, KCtl *C, int gmode, i
   return GDP_OK;
 }
 
-// Two-stage flexible preconditioner: the learned map, scaled to the least-
-// squares fit of v, then an ILU(0) correction of the residual it leaves,
-// z = a N(v) + (LU)^{-1} (v - a A N(v)), a = <v, A N(v)> / |A N(v)|^2.  The
-// network is trained on the direction of the error only (its output scale is
-// arbitrary), and the correction stage needs the scale.
+// Two-stage flexible preconditioner: the learned map, then an ILU(0)
+// correction of the residual it leaves, z = a N(v) + (LU)^{-1} (v - a A N(v)),
+// with a = 1, or (network mode bit 1) the least-squares scale
+// a = <v, A N(v)> / |A N(v)|^2 for networks whose output scale is not fixed
+// by their training (trained on the direction of the error only).
 template <typename TA, typename TV>
 int prec_apply_two_stage(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *in, TV *out, TV *t,
                          TV *u, double *part, KCtl *C, int gmode, int n)
 {
   GDP_TRY(user_prec_apply(M.user, in, out, C, gmode));
-  if(n) {
+  if(n && !user_prec_lsq(M.user))
+    hipLaunchKernelGGL((k_op_resid<TA, TV>), dim3((n + 255) / 256), dim3(256), 0, ctx().stream, A, out, in, t, C,
+                       gmode);
+  else if(n) {
     // partial sums and the scale in the cycle's partial-sum buffer (at least
     // 66 ceil(n / 256) doubles), free while the step preconditions
     const int nb = (n + 255) / 256;

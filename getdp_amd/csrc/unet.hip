@@ -1043,6 +1043,7 @@ struct UserPrec {
   double *pc_part = nullptr;
   int pc_nparts = 0;
   int odd = 0;  // apply the odd part of the network (needs batch >= 2)
+  int lsq = 0;  // two-stage: scale the network output by its least-squares factor
   // the solver's renumbering and scaling of the current solve
   const int *perm = nullptr, *rperm = nullptr;
   const double *rs = nullptr;
@@ -1419,6 +1420,8 @@ int user_prec_apply(UserPrec *u, const float *in, float *out, KCtl *C, int gmode
 {
   return user_prec_apply_t<float>(u, in, out, C, gmode);
 }
+
+int user_prec_lsq(const UserPrec *u) { return u ? u->lsq : 0; }
 
 UserPrec *solver_user_prec(gdp_solver *s);
 void solver_set_user_prec(gdp_solver *s, UserPrec *u);
@@ -1813,11 +1816,13 @@ extern "C" int gdp_unet_get_activation(gdp_unet *n, int t, int batch, float *hos
 
 // Preconditioner options: bit 0 = apply the odd part (N(r) - N(-r))/2 of the
 // network (one batch-2 pass; the network must have been created with
-// batch >= 2).
+// batch >= 2); bit 1 = in the two-stage preconditioner, scale the network's
+// output z by alpha = <v, Az> / <Az, Az> before the ILU(0) correction.
 extern "C" int gdp_unet_set_mode(gdp_unet *n, int flags)
 {
   if(!n) return set_error(GDP_E_ARG, "null network");
   if((flags & 1) && n->u->Bmax < 2) return set_error(GDP_E_ARG, "odd mode needs a network created with batch >= 2");
   n->u->odd = flags & 1;
+  n->u->lsq = (flags >> 1) & 1;
   return GDP_OK;
 }

@@ -110,19 +110,25 @@ class UNet:
         check(lib().gdp_unet_attach(self.h, solver.h, int(maps.ndof), iptr(p2d), fptr(feat),
                                     None if ds is None else dptr(ds)), "gdp_unet_attach")
 
-    def set_mode(self, odd: bool = False):
-        check(lib().gdp_unet_set_mode(self.h, 1 if odd else 0), "gdp_unet_set_mode")
+    def set_mode(self, odd: bool = False, lsq: bool = False):
+        """odd: apply the odd part of the network; lsq: least-squares scaling
+        of its output in the two-stage preconditioner."""
+        self.lsq = bool(lsq)
+        check(lib().gdp_unet_set_mode(self.h, (1 if odd else 0) | (2 if lsq else 0)), "gdp_unet_set_mode")
 
     @staticmethod
     def detach(solver):
         check(lib().gdp_unet_detach(solver.h), "gdp_unet_detach")
 
     def save(self, path):
-        np.savez_compressed(path, params=self.get_params(), H=self.H, W=self.W, D=self.D)
+        np.savez_compressed(path, params=self.get_params(), H=self.H, W=self.W, D=self.D,
+                            lsq=int(getattr(self, "lsq", False)))
 
     def load(self, path):
+        """Parameters, and the output-scaling mode the weights were evaluated with."""
         with np.load(path, allow_pickle=False) as z:
             self.set_params(z["params"])
+            self.set_mode(lsq=bool(z["lsq"]) if "lsq" in z.files else False)
 
 
 # ---------------------------------------------------------------------------

@@ -139,7 +139,7 @@ def train(net: UNet, r, e, feat, mask, steps: int, batch: int, lr: float = 1e-3,
     return np.asarray(hist)
 
 
-def evaluate(params: np.ndarray, n: int, seeds, log=None, precisions=(0, 1)):
+def evaluate(params: np.ndarray, n: int, seeds, log=None, precisions=(0, 1), lsq: bool = True):
     """Learned two-stage FGMRES (FP64, and mixed precision: FP32 inner FGMRES
     with FP64 refinement) against plain ILU(0)-GMRES(30) in the reference's
     RCMK ordering and in multicolour ordering, on unseen fields, all to 1e-10
@@ -153,6 +153,7 @@ def evaluate(params: np.ndarray, n: int, seeds, log=None, precisions=(0, 1)):
         maps = grid_maps(pr, fe.dof_table(0), A)
         net = UNet(n, n, batch=1, seed=0, D=n)
         net.set_params(params)
+        net.set_mode(lsq=lsq)
         row = {"n": n, "seed": seed, "field": "checker" if seed % 2 == 0 else "lognormal", "dofs": fe.N}
         for ordering, key in ((0, "plain_rcmk"), (1, "plain_multicolour")):
             L2.plain_solve(A, b, fe.N, ordering)
@@ -219,6 +220,7 @@ def main(argv=None):
                      log_every=250)
         del r, e, f
         out = a.weights or "unet_poisson3d.npz"
+        net.set_mode(lsq=True)  # trained on the error direction only: the scale is fitted at use
         net.save(out)
         log(f"saved {out}; final loss {hist[-200:].mean():.4f}")
         params = net.get_params()
@@ -227,7 +229,7 @@ def main(argv=None):
             params = z["params"]
     rows = []
     for n in a.eval_n:
-        rows += evaluate(params, n, a.eval_seeds, log=log)
+        rows += evaluate(params, n, a.eval_seeds, log=log, lsq=True)
     if a.json:
         with open(a.json, "w") as fh:
             json.dump(rows, fh, indent=1)

@@ -461,7 +461,8 @@ int gdp_unet_attach(gdp_unet *net, gdp_solver *solver, int n_dof, const int *pix
 int gdp_unet_detach(gdp_solver *solver);
 /* Preconditioner options, bit 0: apply the odd part (N(r) - N(-r))/2 of the
  * network (batch >= 2 at creation).  With Preconditioner 6 on the solver the
- * preconditioner is two-stage: z = N(v) + (LU)^{-1}(v - A N(v)). */
+ * preconditioner is two-stage: z = a N(v) + (LU)^{-1}(v - a A N(v)), a = 1,
+ * or with bit 1 set the least-squares scale a = <v, A N(v)> / |A N(v)|^2. */
 int gdp_unet_set_mode(gdp_unet *net, int flags);
 
 #ifdef __cplusplus

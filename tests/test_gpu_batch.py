@@ -104,3 +104,26 @@ def test_batch_rejects_unsupported_parameters(gpu, probs):
     h = C.c_void_p()
     rc = lib().gdp_batch_create(bad.h, 2, iptr(bt.off), None, C.byref(h))
     assert rc == -4  # GDP_E_UNSUPPORTED
+
+
+@pytest.mark.parametrize("threads", [256, 512, 1024])
+def test_large_instances_lds_ring(gpu, oracle, threads):
+    """Instances larger than the LDS sweep vector (4096, 8192, 18000 rows at
+    256, 512, 1024 threads) sweep through the LDS ring of recent results: the
+    solutions match the oracle's, with the reference's iteration counts (same
+    ordering and factors), next to small instances on the LDS path."""
+    sizes = [70, 150, 12, 96, 135]  # 71^2 = 5041 ... 151^2 = 22801 rows
+    probs = [P.poisson2d(n, seed=70 + i, nblocks=8, field_kind="checker") for i, n in enumerate(sizes)]
+    print(f"case batch ring threads {threads}", flush=True)
+    bt = BT.Batch(probs, config=[(threads, 0, 0)] * len(probs))
+    bt.run()
+    inf = bt.info()
+    assert (inf["status"] == 0).all(), inf["status"]
+    assert (bt.true_rel_residuals() <= 2e-10).all()
+    for i, p in enumerate(probs):
+        ia, ja, a, b, tab = oracle.assemble(p)
+        xo, io, _ = oracle.solve(ia, ja, a, b, algorithm=8, preconditioner=6, krylov=30, renumbering=1,
+                                 maxits=20000)
+        err = np.abs(bt.solution(i) - xo).max() / np.abs(xo).max()
+        assert err <= 1e-9, (i, err)
+        assert abs(int(inf["its"][i]) - io["its"]) <= max(3, io["its"] // 10), (i, inf["its"][i], io["its"])
