@@ -296,6 +296,118 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a)
     }
 }
 
+// LDS-staged variant for grids whose tiles divide them: a workgroup computes
+// an output tile of 16 x 16 pixels (2D) or 4 x 4 x 16 voxels (3D) times
+// 16*TN channels.  The tile's input halo (the convolution's logical input:
+// pooled, or upsampled and concatenated, as the loader forms it -- once per
+// voxel instead of once per tap) is staged in LDS, CC channels at a time;
+// every tap then reads its A fragments from LDS.  Wave w owns the tile's
+// pixel rows 4w .. 4w + 3 (rows along x).  The reduction runs chunk by chunk,
+// tap-major within a chunk.
+constexpr int kTileX = 16;
+__host__ __device__ inline int tile_ty(int d3) { return d3 ? 4 : 16; }
+__host__ __device__ inline int tile_tz(int d3) { return d3 ? 4 : 1; }
+
+template <int MODE, int TN>
+__global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a, int CC)
+{
+  extern __shared__ uint4 hal[];
+  if(unet_gated(a.gate, a.gmode)) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, kq = lane >> 4;
+  const int d3 = a.d3, TY = tile_ty(d3), TZ = tile_tz(d3);
+  const int HX = kTileX + 2, HY = TY + 2, HZ = d3 ? TZ + 2 : 1;
+  const int nx = a.W / kTileX, ny = a.H / TY, nz = d3 ? a.D / TZ : 1;
+  long t = blockIdx.x;
+  const int tx = (int)(t % nx);
+  t /= nx;
+  const int ty = (int)(t % ny);
+  t /= ny;
+  const int tz = (int)(t % nz);
+  const int b = (int)(t / nz);
+  const int x0 = tx * kTileX, y0 = ty * TY, z0 = tz * TZ;
+  const int cobase = blockIdx.y * 16 * TN;
+  const int taps = d3 ? 27 : 9, t0 = d3 ? 0 : 9;
+  const int cin = a.cin_pad;
+  const int VS = CC / 8 + 1;  // voxel stride in LDS (16-byte units, one of padding)
+  const int nvox = HZ * HY * HX;
+  f32x4 acc[4][TN];
+#pragma unroll
+  for(int i = 0; i < 4; i++)
+#pragma unroll
+    for(int n = 0; n < TN; n++) acc[i][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const u16 *wp[TN];
+  bool wok[TN];
+#pragma unroll
+  for(int n = 0; n < TN; n++) {
+    const int co = cobase + n * 16 + r16;
+    wok[n] = co < a.cout;
+    wp[n] = a.wt + (long)(wok[n] ? co : 0) * a.K_pad;
+  }
+  const uint4 zero4 = {0u, 0u, 0u, 0u};
+  const int ksteps = (taps * CC + 31) / 32;
+  for(int c0 = 0; c0 < cin; c0 += CC) {
+    __syncthreads();
+    // stage the halo: voxel v (x fastest), CC / 8 channel octets each
+    for(int i = threadIdx.x; i < nvox * (CC / 8); i += 256) {
+      const int v = i / (CC / 8), g = i - v * (CC / 8);
+      const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+      const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = d3 ? z0 - 1 + hz : 0;
+      const bool in = (unsigned)x < (unsigned)a.W && (unsigned)y < (unsigned)a.H && (unsigned)z < (unsigned)a.D;
+      uint4 o = zero4;
+      if(in) {
+        const long pix = MODE == M_POOL ? (d3 ? (((long)b * 2 * a.D + 2 * z) * (2 * a.H) + 2 * y) * (2 * a.W) + 2 * x
+                                              : ((long)b * 2 * a.H + 2 * y) * (2 * a.W) + 2 * x)
+                                        : (((long)b * a.D + z) * a.H + y) * a.W + x;
+        o = conv_load_a<MODE>(a, pix, z, y, x, d3 ? b * (a.D >> 1) : b, 0, 0, 0, c0 + 8 * g, true);
+      }
+      hal[v * VS + g] = o;
+    }
+    __syncthreads();
+    for(int s = 0; s < ksteps; s++) {
+      const int k = 32 * s + 8 * kq;  // this lane's 8 reduction entries
+      const int tap = k / CC, ci = k - tap * CC;
+      const bool kvalid = tap < taps;
+      const int tt = tap + t0;
+      const int dz = tap_dz(tt), dy = tap_dy(tt), dx = tap_dx(tt);
+      uint4 bv[TN];
+#pragma unroll
+      for(int n = 0; n < TN; n++) bv[n] = (kvalid && wok[n]) ? *(const uint4 *)(wp[n] + tap * cin + c0 + ci) : zero4;
+#pragma unroll
+      for(int tr = 0; tr < 4; tr++) {
+        const int rr = 4 * wave + tr;  // tile row: (z, y) = (rr / TY, rr % TY) in 3D, (0, rr) in 2D
+        const int zz = d3 ? rr / TY : 0, yy = d3 ? rr % TY : rr;
+        const int v = ((zz + (d3 ? 1 + dz : 0)) * HY + (yy + 1 + dy)) * HX + (r16 + 1 + dx);
+        const uint4 av = kvalid ? hal[v * VS + ci / 8] : zero4;
+#pragma unroll
+        for(int n = 0; n < TN; n++)
+          acc[tr][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av), as_bf8(bv[n]), acc[tr][n], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: lane holds channel r16 of pixels x0 + 4 kq + i of row tr
+#pragma unroll
+  for(int tr = 0; tr < 4; tr++) {
+    const int rr = 4 * wave + tr;
+    const int zz = d3 ? rr / TY : 0, yy = d3 ? rr % TY : rr;
+    const long prow = (((long)b * a.D + z0 + zz) * a.H + y0 + yy) * a.W + x0;
+#pragma unroll
+    for(int n = 0; n < TN; n++) {
+      const int co = cobase + n * 16 + r16;
+      if(co >= a.cout) continue;
+      const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+      for(int i = 0; i < 4; i++) {
+        const long p = prow + 4 * kq + i;
+        float v = acc[tr][n][i] + bias;
+        if(a.relu) v = v > 0.f ? v : 0.f;
+        if(a.out_bf) a.out_bf[p * a.cout + co] = f2bf(v);
+        else a.out_f[p * a.cout + co] = v;
+      }
+    }
+  }
+}
+
 // Small-image variant: the 4 waves of a block share one tile of 16 pixels x
 // 16*TN channels and split the reduction into 4 contiguous ranges; their
 // accumulators are added in LDS in wave order (deterministic) and wave 0
@@ -1163,11 +1275,71 @@ template <int MODE> int launch_conv_mode(ConvArgs a, const ConvPlan &c, float *s
 
 dim3 g1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// LDS-staged convolution: usable when the tiles divide the grid and give at
+// least 256 workgroups; CC (channels staged at a time) is the largest
+// multiple of 8 dividing cin whose halo fits kConvLds bytes
+constexpr int kConvLds = 78 * 1024;
+int conv_lds_cc(const ConvArgs &a)
+{
+  const int TY = tile_ty(a.d3), TZ = tile_tz(a.d3);
+  if(a.W % kTileX || a.H % TY || (a.d3 && a.D % TZ)) return 0;
+  const long tiles = (long)a.B * (a.W / kTileX) * (a.H / TY) * (a.d3 ? a.D / TZ : 1);
+  const int TN = a.cout <= 16 ? 1 : (a.cout <= 32 ? 2 : 4);
+  if(tiles * ((a.cout + 16 * TN - 1) / (16 * TN)) < 256) return 0;
+  const long nvox = (long)(kTileX + 2) * (TY + 2) * (a.d3 ? TZ + 2 : 1);
+  for(int cc = a.cin_pad; cc >= 8; cc -= 8)
+    if(a.cin_pad % cc == 0 && nvox * (cc / 8 + 1) * 16 <= kConvLds) return cc;
+  return 0;
+}
+
+// dynamic LDS beyond the default limit, once per process (at network creation,
+// never under graph capture)
+template <int MODE> int conv_lds_attr_mode()
+{
+  GDP_HIP(hipFuncSetAttribute((const void *)k_conv_lds<MODE, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, kConvLds));
+  GDP_HIP(hipFuncSetAttribute((const void *)k_conv_lds<MODE, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, kConvLds));
+  GDP_HIP(hipFuncSetAttribute((const void *)k_conv_lds<MODE, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kConvLds));
+  return GDP_OK;
+}
+int conv_lds_attrs()
+{
+  static int rc = -1;
+  if(rc < 0) {
+    rc = conv_lds_attr_mode<M_PLAIN>();
+    if(!rc) rc = conv_lds_attr_mode<M_POOL>();
+    if(!rc) rc = conv_lds_attr_mode<M_UPCAT>();
+  }
+  return rc;
+}
+
+template <int MODE, int TN> int launch_conv_lds_tn(const ConvArgs &a, int cc)
+{
+  const int TY = tile_ty(a.d3), TZ = tile_tz(a.d3);
+  const long tiles = (long)a.B * (a.W / kTileX) * (a.H / TY) * (a.d3 ? a.D / TZ : 1);
+  const size_t lds = (size_t)(kTileX + 2) * (TY + 2) * (a.d3 ? TZ + 2 : 1) * (cc / 8 + 1) * 16;
+  dim3 grid((unsigned)tiles, (unsigned)((a.cout + 16 * TN - 1) / (16 * TN)));
+  hipLaunchKernelGGL((k_conv_lds<MODE, TN>), grid, dim3(256), lds, ctx().stream, a, cc);
+  GDP_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+template <int MODE> int launch_conv_lds(const ConvArgs &a, int cc)
+{
+  if(a.cout <= 16) return launch_conv_lds_tn<MODE, 1>(a, cc);
+  if(a.cout <= 32) return launch_conv_lds_tn<MODE, 2>(a, cc);
+  return launch_conv_lds_tn<MODE, 4>(a, cc);
+}
+
 // Plan and launch one convolution.  The split-K buffer grows on demand only
 // when may_alloc (not while a caller captures a graph); the preconditioner
 // path reserves it when the network is attached.
 int launch_conv(UserPrec &U, const ConvArgs &a, int mode, bool may_alloc)
 {
+  if(const int cc = conv_lds_cc(a)) {
+    if(mode == M_POOL) return launch_conv_lds<M_POOL>(a, cc);
+    if(mode == M_UPCAT) return launch_conv_lds<M_UPCAT>(a, cc);
+    return launch_conv_lds<M_PLAIN>(a, cc);
+  }
   const ConvPlan c = plan_conv((long)a.B * a.D * a.H * a.W, a.cout, a.K_pad);
   if((size_t)c.split_floats > U.split_size) {
     if(!may_alloc)
@@ -1439,6 +1611,7 @@ struct gdp_unet {
 
 static int unet_create(int D, int H, int W, int batch, unsigned seed, gdp_unet **out)
 {
+  GDP_TRY(conv_lds_attrs());
   if(!out || H <= 0 || W <= 0 || D <= 0 || H % 8 || W % 8 || (D > 1 && D % 8) || batch < 1)
     return set_error(GDP_E_ARG, "gdp_unet_create: grid sides must be positive multiples of 8");
   GDP_TRY(ensure_ready());

@@ -8,8 +8,11 @@ rounding decision it flips: relative L2 error <= 5e-3 for the first
 convolution (which tests the MFMA operand/accumulator lane layout directly),
 <= 2e-2 for every later tensor and for each layer's gradient.  Against the
 pure FP32 network the BF16 effect is stated, not hidden: <= 0.1 relative
-for the output, <= 0.2 for the gradient through all 15 layers (measured
-0.11 at random initialisation)."""
+for the output, and for the gradient through all 15 layers <= 0.2 (measured
+0.11 at 32^2, random initialisation) or the BF16 reference's own distance
+from FP32 plus 0.05, whichever is larger.  Two shapes: 32^2 at batch 2 (direct-load
+convolutions) and 128^2 at batch 4, where the level-0 convolutions and their
+adjoints take the LDS-staged kernel."""
 import numpy as np
 import pytest
 import torch
@@ -18,8 +21,7 @@ import unet_ref as R
 
 pytestmark = pytest.mark.gpu
 
-H = 32
-B = 2
+SHAPES = [(32, 2), (128, 4)]
 
 
 def rel(a, b):
@@ -32,10 +34,12 @@ def nhwc(t):
     return t.detach().permute(0, 2, 3, 1).contiguous().numpy()
 
 
-@pytest.fixture(scope="module")
-def net(gpu):
+@pytest.fixture(scope="module", params=SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+def net(gpu, request):
     from getdp_amd.learned import UNet
+    H, B = request.param
     n = UNet(H, H, batch=B, seed=7)
+    n.Bt = B
     # nonzero biases so that the bias path is exercised
     p = n.get_params()
     rng = np.random.default_rng(5)
@@ -49,7 +53,8 @@ def net(gpu):
     return n
 
 
-def inputs(seed=0):
+def inputs(net, seed=0):
+    H, B = net.H, net.Bt
     rng = np.random.default_rng(seed)
     r = rng.standard_normal((B, H, H)).astype(np.float32)
     f = (rng.standard_normal((B, H, H)) * 0.5).astype(np.float32)
@@ -69,11 +74,12 @@ def test_num_params_and_roundtrip(net):
 
 
 def test_forward_matches_reference(net):
-    r, f, t, mask = inputs()
+    r, f, t, mask = inputs(net)
     z = net.forward(r, f)
     params = R.unpack(net.get_params())
     zr, acts = R.forward(params, r, f, bf16_points=True, keep=True)
     # input tensor: 8 stored channels, 2 used
+    H, B = net.H, net.Bt
     a0 = net.activation(0, B).reshape(B, H, H, 8)
     assert np.array_equal(a0[..., :2], nhwc(acts[0]))
     assert not a0[..., 2:].any()
@@ -92,7 +98,7 @@ def test_first_conv_exact_layout(net):
     differ from the FP32 sum of the same BF16 products by at most one BF16
     rounding step (2^-8 relative) -- a wrong MFMA lane mapping moves values
     between pixels/channels and fails by O(1)."""
-    r, f, t, mask = inputs(1)
+    r, f, t, mask = inputs(net, 1)
     net.forward(r, f)
     params = R.unpack(net.get_params())
     _, acts = R.forward(params, r, f, bf16_points=True, keep=True)
@@ -100,30 +106,38 @@ def test_first_conv_exact_layout(net):
     w, b = layers[0]
     y = torch.relu(torch.nn.functional.conv2d(acts[0], R.q(w), b, padding=1))
     ref = nhwc(y)
-    d = net.activation(1, B).reshape(ref.shape)
+    d = net.activation(1, net.Bt).reshape(ref.shape)
     assert np.all(np.abs(d - ref) <= 2.0 ** -8 * np.abs(ref) + 1e-6)
 
 
 def test_gradients_match_reference(net):
-    r, f, t, mask = inputs(2)
+    r, f, t, mask = inputs(net, 2)
     p0 = net.get_params()
     loss = net.train_step(r, f, t, mask, lr=0.0)
     g = net.get_grads()
     assert np.array_equal(net.get_params(), p0)  # lr = 0 leaves the weights
     gr, per, _ = R.grads(p0, r, f, t, mask, bf16_points=True)
+    g64, _, _ = R.grads(p0, r, f, t, mask, bf16_points=True, dtype=torch.float64)
     assert abs(loss - per.mean()) <= 1e-3 * max(1.0, abs(per.mean()))
+    # per layer: 2e-2, or twice the reference's own FP32/FP64 distance where
+    # the BF16 rounding points make the gradient that sensitive (larger grids:
+    # more ReLU and rounding decisions flip between accumulation orders)
     off = 0
     for li, ((m, cout, tt, t2), cin) in enumerate(zip(R.SPEC, R.layer_cin())):
         nw = cout * 9 * cin
         ew = rel(g[off:off + nw], gr[off:off + nw])
         eb = rel(g[off + nw:off + nw + cout], gr[off + nw:off + nw + cout])
-        assert ew <= 2e-2 and eb <= 2e-2, (li, ew, eb)
+        sw = max(2e-2, 2.0 * rel(gr[off:off + nw], g64[off:off + nw]))
+        sb = max(2e-2, 2.0 * rel(gr[off + nw:off + nw + cout], g64[off + nw:off + nw + cout]))
+        assert ew <= sw and eb <= sb, (li, ew, eb, sw, sb)
         off += nw + cout
     assert rel(g[off:], gr[off:]) <= 2e-2
-    cos = float(np.dot(g, gr) / np.linalg.norm(g) / np.linalg.norm(gr))
-    assert cos >= 0.999
+    cosine = lambda u, v: float(np.dot(u, v) / np.linalg.norm(u) / np.linalg.norm(v))
+    assert cosine(g, gr) >= 0.999 or 1 - cosine(g, gr) <= 4 * (1 - cosine(gr, g64)) + 1e-4
+    # the BF16 effect against the pure FP32 network: what the BF16 reference
+    # itself shows, plus the device's distance from that reference
     g32, _, _ = R.grads(p0, r, f, t, mask, bf16_points=False)
-    assert rel(g, g32) <= 0.2
+    assert rel(g, g32) <= max(0.2, rel(gr, g32) + 0.05), (rel(g, g32), rel(gr, g32))
 
 
 def test_training_reduces_loss(gpu):

@@ -22,8 +22,7 @@ import unet_ref as R
 
 pytestmark = pytest.mark.gpu
 
-S = 16
-B = 2
+SHAPES = [(16, 2), (32, 2)]  # at 32^3 the level-0 convolutions take the LDS-staged kernel
 
 
 def rel(a, b):
@@ -36,10 +35,12 @@ def ndhwc(t):
     return t.detach().permute(0, 2, 3, 4, 1).contiguous().numpy()
 
 
-@pytest.fixture(scope="module")
-def net(gpu):
+@pytest.fixture(scope="module", params=SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+def net(gpu, request):
     from getdp_amd.learned import UNet
+    S, B = request.param
     n = UNet(S, S, batch=B, seed=11, D=S)
+    n.Bt = B
     p = n.get_params()
     rng = np.random.default_rng(5)
     off = 0
@@ -52,7 +53,8 @@ def net(gpu):
     return n
 
 
-def inputs(seed=0):
+def inputs(net, seed=0):
+    S, B = net.H, net.Bt
     rng = np.random.default_rng(seed)
     r = rng.standard_normal((B, S, S, S)).astype(np.float32)
     f = (rng.standard_normal((B, S, S, S)) * 0.5).astype(np.float32)
@@ -72,10 +74,11 @@ def test_num_params_and_roundtrip(net):
 
 
 def test_forward_matches_reference(net):
-    r, f, t, mask = inputs()
+    r, f, t, mask = inputs(net)
     z = net.forward(r, f)
     params = R.unpack(net.get_params(), dim=3)
     zr, acts = R.forward(params, r, f, bf16_points=True, keep=True)
+    S, B = net.H, net.Bt
     a0 = net.activation(0, B).reshape(B, S, S, S, 8)
     assert np.array_equal(a0[..., :2], ndhwc(acts[0]))
     for i in range(1, 15):
@@ -87,19 +90,19 @@ def test_forward_matches_reference(net):
 
 
 def test_first_conv_exact_layout(net):
-    r, f, t, mask = inputs(1)
+    r, f, t, mask = inputs(net, 1)
     net.forward(r, f)
     params = R.unpack(net.get_params(), dim=3)
     _, acts = R.forward(params, r, f, bf16_points=True, keep=True)
     w, b = params[0][0]
     y = torch.relu(torch.nn.functional.conv3d(acts[0], R.q(w), b, padding=1))
     ref = ndhwc(y)
-    d = net.activation(1, B).reshape(ref.shape)
+    d = net.activation(1, net.Bt).reshape(ref.shape)
     assert np.all(np.abs(d - ref) <= 2.0 ** -8 * np.abs(ref) + 1e-6)
 
 
 def test_gradients_match_reference(net):
-    r, f, t, mask = inputs(2)
+    r, f, t, mask = inputs(net, 2)
     p0 = net.get_params()
     loss = net.train_step(r, f, t, mask, lr=0.0)
     g = net.get_grads()
@@ -127,7 +130,8 @@ def test_learned_fgmres_3d_matches_oracle(gpu, oracle):
     """An untrained network is still a valid (if poor) right preconditioner:
     the two-stage FGMRES on poisson3d(16) must reach 1e-10 (with the ILU(0)
     fallback if needed) and agree with the oracle's ILU(0)-GMRES solution;
-    FP64 and the mixed-precision (FP32 inner) variant."""
+    FP64 and the mixed-precision (FP32 inner) variant, with and without the
+    least-squares scaling of the network's output."""
     from getdp_amd import learned as L2
     from getdp_amd import learned3d as L3
     from getdp_amd import problems as P
@@ -139,7 +143,9 @@ def test_learned_fgmres_3d_matches_oracle(gpu, oracle):
     maps = L3.grid_maps(pr, fe.dof_table(0), A)
     net = UNet(16, 16, batch=1, seed=2, D=16)
     for prec in (0, 1):
-        x = gpu.Vector(fe.N)
-        out = L2.solve_learned(A, bv, x, net, maps, maxits=500, two_stage=True, precision=prec)
-        assert out["rel_res"] <= 1.5e-10, (prec, out)
-        assert np.abs(x.get() - xo).max() <= 1e-8 * np.abs(xo).max()
+        for lsq in (False, True):
+            net.set_mode(lsq=lsq)
+            x = gpu.Vector(fe.N)
+            out = L2.solve_learned(A, bv, x, net, maps, maxits=500, two_stage=True, precision=prec)
+            assert out["rel_res"] <= 1.5e-10, (prec, lsq, out)
+            assert np.abs(x.get() - xo).max() <= 1e-8 * np.abs(xo).max()
