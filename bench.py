@@ -371,6 +371,52 @@ def roofline_only(n: int, reps: int):
                       "copy_bytes_each_way": src.numel() * 8, "N": fe.N, "nnz": fe.nnz}))
 
 
+def unet_profile(mode: str, reps: int = 5):
+    """For rocprofv3 runs: U-Net launches alone.  fwd3d: forward passes of the
+    3D network on a 64^3 grid (batch 1, the preconditioner's shape); train3d:
+    training steps at 32^3 x 16; train2d: training steps at 128^2 x 32 (the
+    training line of the learned section)."""
+    from getdp_amd import learned as L
+    from getdp_amd import linalg as LA
+    LA.init(0)
+    rng = np.random.default_rng(0)
+    if mode == "fwd3d":
+        n, B, D = 64, 1, 64
+    elif mode == "train3d":
+        n, B, D = 32, 16, 32
+    else:
+        n, B, D = 128, 32, 1
+    shape = (B, D, n, n) if D > 1 else (B, n, n)
+    net = L.UNet(n, n, batch=B, seed=0, D=D)
+    r = rng.standard_normal(shape).astype(np.float32)
+    f = np.zeros_like(r)
+    e = rng.standard_normal(shape).astype(np.float32)
+    mask = np.ones(shape[1:], np.float32)
+    run = (lambda: net.forward(r, f)) if mode == "fwd3d" else (lambda: net.train_step(r, f, e, mask, 1e-4))
+    for _ in range(3):
+        run()
+    LA.lib().gdp_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    LA.lib().gdp_device_synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    px = B * n * n * (D if D > 1 else 1)
+    fpp = unet_flops_per_voxel_3d() if D > 1 else unet_flops_per_pixel()
+    flops = (1 if mode == "fwd3d" else 3) * fpp * px
+    print(json.dumps({"mode": mode, "ms": round(ms, 3), "algorithmic_gflop": round(flops / 1e9, 2),
+                      "tflops": round(flops / ms / 1e9, 2)}))
+
+
+def unet_flops_per_voxel_3d() -> float:
+    """As unet_flops_per_pixel for the 3D network: 27 taps, a level-l
+    convolution on 8^-l of the voxels."""
+    spec = [(2, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2),
+            (64, 128, 3), (128, 128, 3), (192, 64, 2), (64, 64, 2), (96, 32, 1), (32, 32, 1),
+            (48, 16, 0), (16, 16, 0)]
+    return sum(2 * 27 * ci * co / 8 ** lvl for ci, co, lvl in spec) + 2 * 16
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -388,6 +434,8 @@ def main():
                     help="only GetDP's default solver (ILUTP(20)-GMRES(40)) on poisson2d_N")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the SpMV launches and a calibration copy (for PMC runs)")
+    ap.add_argument("--unet-profile", choices=["fwd3d", "train3d", "train2d"],
+                    help="only U-Net launches (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -396,6 +444,9 @@ def main():
 
     if args.roofline_only:
         roofline_only(args.n, 20)
+        return
+    if args.unet_profile:
+        unet_profile(args.unet_profile)
         return
     if args.default_solver_only:
         from getdp_amd import linalg as LA

@@ -792,6 +792,148 @@ __global__ void __launch_bounds__(256) k_wgrad_t(WgradTArgs w)
   }
 }
 
+// Weight gradient on LDS tiles (grids the convolution tiles divide): a
+// workgroup takes a run of `ntile` output tiles (16 x 16 pixels or 4 x 4 x 16
+// voxels), 16 output channels co and CC input channels ci.  Per tile, G^T
+// (the tile's voxels x 16 co) and the input halo (CC channels, channel-major,
+// halo rows padded to 32 entries with x = -1 at entry 7) are staged in LDS;
+// the MFMA reduction runs over the tile's voxels, 32 (two tile rows) per step,
+// wave w taking taps w, w + 4, ...: the G^T fragment is shared by the wave's
+// taps, the input fragment of tap (dz, dy, dx) is the halo row segment at
+// x + dx (one element off for dx = +-1: an aligned 16-byte read plus the
+// neighbouring dword, realigned by byte shifts).
+constexpr int kWgRow = 32;  // padded halo row (entries)
+struct WgradLArgs {
+  ConvArgs x;        // logical input (MODE loader) and sizes at this level
+  const u16 *gt;     // [cout][P]
+  float *part;       // [nchunks][cout][K_pad]
+  int CC, ntile, ntiles;
+};
+
+template <int MODE, int CCN>  // CCN = CC / 16 input-channel subtiles
+__global__ void __launch_bounds__(256) k_wgrad_lds(WgradLArgs w)
+{
+  extern __shared__ uint4 wl[];
+  const ConvArgs &a = w.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, kq = lane >> 4;
+  const int d3 = a.d3, TY = tile_ty(d3), TZ = tile_tz(d3);
+  const int HY = TY + 2, HZ = d3 ? TZ + 2 : 1, nrow = HZ * HY;
+  const int nx = a.W / kTileX, ny = a.H / TY, nz = d3 ? a.D / TZ : 1;
+  const int taps = d3 ? 27 : 9, t0 = d3 ? 0 : 9;
+  const int CC = 16 * CCN, cin = a.cin_pad, cout = a.cout;
+  const int co0 = blockIdx.y * 16, c0 = blockIdx.z * CC;
+  const long P = (long)a.B * a.D * a.H * a.W;
+  u16 *gs = (u16 *)wl;                 // [16 co][256 voxels]
+  u16 *xs = gs + 16 * 256;             // [CC][nrow][kWgRow]
+  f32x4 acc[7][CCN];
+#pragma unroll
+  for(int i = 0; i < 7; i++)
+#pragma unroll
+    for(int c = 0; c < CCN; c++) acc[i][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const uint4 zero4 = {0u, 0u, 0u, 0u};
+  const int tile0 = blockIdx.x * w.ntile;
+  for(int ti = 0; ti < w.ntile; ti++) {
+    const long tile = tile0 + ti;
+    if(tile >= w.ntiles) break;
+    long t = tile;
+    const int tx = (int)(t % nx);
+    t /= nx;
+    const int ty = (int)(t % ny);
+    t /= ny;
+    const int tz = (int)(t % nz);
+    const int b = (int)(t / nz);
+    const int x0 = tx * kTileX, y0 = ty * TY, z0 = tz * TZ;
+    __syncthreads();
+    // G^T of the tile: 16 co x 16 rows x 16 x, one 16-byte octet per thread pair
+    for(int i = threadIdx.x; i < 16 * 16 * 2; i += 256) {
+      const int co = i >> 5, rr = (i >> 1) & 15, h = i & 1;
+      const int zz = d3 ? rr / TY : 0, yy = d3 ? rr % TY : rr;
+      const long p = (((long)b * a.D + z0 + zz) * a.H + y0 + yy) * a.W + x0 + 8 * h;
+      uint4 v = zero4;
+      if(co0 + co < cout) v = *(const uint4 *)(w.gt + (long)(co0 + co) * P + p);
+      *(uint4 *)(gs + co * 256 + rr * 16 + 8 * h) = v;
+    }
+    // input halo, channel-major: halo voxel hx (x = x0 - 1 + hx) at entry 7 + hx
+    for(int i = threadIdx.x; i < nrow * 18 * CCN * 2; i += 256) {
+      const int g = i % (CCN * 2), v = i / (CCN * 2);
+      const int hx = v % 18, hr = v / 18;
+      const int hy = hr % HY, hz = hr / HY;
+      const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = d3 ? z0 - 1 + hz : 0;
+      uint4 o = zero4;
+      if((unsigned)x < (unsigned)a.W && (unsigned)y < (unsigned)a.H && (unsigned)z < (unsigned)a.D &&
+         c0 + 8 * g < cin) {  // channels past cin (cin = 8) stay zero
+        const long pix = MODE == M_POOL ? (d3 ? (((long)b * 2 * a.D + 2 * z) * (2 * a.H) + 2 * y) * (2 * a.W) + 2 * x
+                                              : ((long)b * 2 * a.H + 2 * y) * (2 * a.W) + 2 * x)
+                                        : (((long)b * a.D + z) * a.H + y) * a.W + x;
+        o = conv_load_a<MODE>(a, pix, z, y, x, d3 ? b * (a.D >> 1) : b, 0, 0, 0, c0 + 8 * g, true);
+      }
+      const u16 *e = (const u16 *)&o;
+#pragma unroll
+      for(int j = 0; j < 8; j++) xs[((8 * g + j) * nrow + hr) * kWgRow + 7 + hx] = e[j];
+    }
+    // halo row padding entries (outside x = -1 .. 16) stay unread
+    __syncthreads();
+    for(int s = 0; s < 8; s++) {  // 32 voxels: tile rows 2s, 2s + 1
+      const int rr = 2 * s + (kq >> 1), xo = 8 * (kq & 1);
+      const uint4 av = *(const uint4 *)(gs + r16 * 256 + rr * 16 + xo);
+      const int zz = d3 ? rr / TY : 0, yy = d3 ? rr % TY : rr;
+#pragma unroll
+      for(int i = 0; i < 7; i++) {
+        const int tap = wave + 4 * i;
+        if(tap >= taps) break;
+        const int tt = tap + t0;
+        const int dz = tap_dz(tt), dy = tap_dy(tt), dx = tap_dx(tt);
+        const int hr = (zz + (d3 ? 1 + dz : 0)) * HY + (yy + 1 + dy);
+#pragma unroll
+        for(int c = 0; c < CCN; c++) {
+          // entries x = xo + dx .. + 7 of halo row hr, channel 16 c + r16 (x = -1 at entry 7)
+          const u16 *row = xs + ((16 * c + r16) * nrow + hr) * kWgRow + 8 + xo;
+          uint4 bv;
+          if(dx == 0) bv = *(const uint4 *)row;
+          else if(dx > 0) {
+            const uint4 v = *(const uint4 *)row;
+            const unsigned e = *(const unsigned *)(row + 8);
+            bv.x = __builtin_amdgcn_alignbyte(v.y, v.x, 2);
+            bv.y = __builtin_amdgcn_alignbyte(v.z, v.y, 2);
+            bv.z = __builtin_amdgcn_alignbyte(v.w, v.z, 2);
+            bv.w = __builtin_amdgcn_alignbyte(e, v.w, 2);
+          }
+          else {
+            const uint4 v = *(const uint4 *)row;
+            const unsigned e = *(const unsigned *)(row - 2);
+            bv.x = __builtin_amdgcn_alignbyte(v.x, e, 2);
+            bv.y = __builtin_amdgcn_alignbyte(v.y, v.x, 2);
+            bv.z = __builtin_amdgcn_alignbyte(v.z, v.y, 2);
+            bv.w = __builtin_amdgcn_alignbyte(v.w, v.z, 2);
+          }
+          acc[i][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(av), as_bf8(bv), acc[i][c], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // partials: rows co0 + 4 kq + j, columns tap * cin + c0 + 16 c + r16
+  float *out = w.part + (long)blockIdx.x * cout * a.K_pad;
+#pragma unroll
+  for(int i = 0; i < 7; i++) {
+    const int tap = wave + 4 * i;
+    if(tap >= taps) break;
+#pragma unroll
+    for(int c = 0; c < CCN; c++)
+#pragma unroll
+      for(int j = 0; j < 4; j++) {
+        const int co = co0 + 4 * kq + j, ci = c0 + 16 * c + r16;
+        if(co < cout && ci < cin) out[(long)co * a.K_pad + tap * cin + ci] = acc[i][c][j];
+      }
+  }
+  // the padding columns of the weight layout
+  if(blockIdx.z == 0)
+    for(int i = threadIdx.x; i < 16 * (a.K_pad - taps * cin); i += 256) {
+      const int co = co0 + i / (a.K_pad - taps * cin), k = taps * cin + i % (a.K_pad - taps * cin);
+      if(co < cout) out[(long)co * a.K_pad + k] = 0.f;
+    }
+}
+
 // out[i] = sum over chunks of part[c][i]: a block takes 32 elements, its 8
 // thread groups the chunks g, g + 8, ..., then the 8 group sums are added in
 // group order
@@ -1301,6 +1443,12 @@ template <int MODE> int conv_lds_attr_mode()
   GDP_HIP(hipFuncSetAttribute((const void *)k_conv_lds<MODE, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kConvLds));
   return GDP_OK;
 }
+template <int MODE> int wgrad_lds_attr_mode()
+{
+  GDP_HIP(hipFuncSetAttribute((const void *)k_wgrad_lds<MODE, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, kConvLds));
+  GDP_HIP(hipFuncSetAttribute((const void *)k_wgrad_lds<MODE, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, kConvLds));
+  return GDP_OK;
+}
 int conv_lds_attrs()
 {
   static int rc = -1;
@@ -1308,8 +1456,23 @@ int conv_lds_attrs()
     rc = conv_lds_attr_mode<M_PLAIN>();
     if(!rc) rc = conv_lds_attr_mode<M_POOL>();
     if(!rc) rc = conv_lds_attr_mode<M_UPCAT>();
+    if(!rc) rc = wgrad_lds_attr_mode<M_PLAIN>();
+    if(!rc) rc = wgrad_lds_attr_mode<M_POOL>();
+    if(!rc) rc = wgrad_lds_attr_mode<M_UPCAT>();
   }
   return rc;
+}
+
+// LDS-tiled weight gradient: channels per workgroup (16 or 32, 0: not
+// applicable -- tiles must divide the grid and CC divide cin, or cin = 8
+// with the upper half of the 16 channels zero)
+int wgrad_lds_cc(const ConvArgs &a)
+{
+  const int TY = tile_ty(a.d3), TZ = tile_tz(a.d3);
+  if(a.W % kTileX || a.H % TY || (a.d3 && a.D % TZ) || (a.cin_pad % 16 && a.cin_pad != 8)) return 0;
+  const long nrow = (long)(TY + 2) * (a.d3 ? TZ + 2 : 1);
+  if(a.cin_pad % 32 == 0 && 16 * 256 * 2 + 32 * nrow * kWgRow * 2 <= kConvLds) return 32;
+  return 16;
 }
 
 template <int MODE, int TN> int launch_conv_lds_tn(const ConvArgs &a, int cc)
@@ -1440,7 +1603,37 @@ int backward(UserPrec &U, int B)
       U.part_size = need;
     }
     dim3 gw((unsigned)nchunks, (unsigned)((l.cout + 15) / 16), (unsigned)((l.K_pad + 63) / 64));
-    if(tr) {
+    const int wcc = tr ? wgrad_lds_cc(x) : 0;
+    if(wcc) {
+      // tiles per workgroup for ~1024 workgroups (fewer partial sums to add)
+      const int TY = tile_ty(U.d3), TZ = tile_tz(U.d3);
+      const long ntiles = (long)B * (W / kTileX) * (H / TY) * (U.d3 ? Dl / TZ : 1);
+      const int nci = (l.cin_pad + wcc - 1) / wcc;
+      const long other = (long)((l.cout + 15) / 16) * nci;
+      const int ntile = (int)std::max(1L, std::min(16L, ntiles * other / 1024));
+      const int nblk = (int)((ntiles + ntile - 1) / ntile);
+      need = (size_t)nblk * l.cout * l.K_pad;
+      if(need > U.part_size) {
+        dfree(U.part);
+        GDP_TRY(dalloc(&U.part, need));
+        U.part_size = need;
+      }
+      nchunks = nblk;
+      WgradLArgs wl{x, U.gt, U.part, wcc, ntile, (int)ntiles};
+      const dim3 g((unsigned)nblk, (unsigned)((l.cout + 15) / 16), (unsigned)nci);
+      const size_t lds = (size_t)(16 * 256 + wcc * (TY + 2) * (U.d3 ? TZ + 2 : 1) * kWgRow) * 2;
+      if(wcc == 32) {
+        if(l.mode == M_POOL) hipLaunchKernelGGL((k_wgrad_lds<M_POOL, 2>), g, dim3(256), lds, s, wl);
+        else if(l.mode == M_UPCAT) hipLaunchKernelGGL((k_wgrad_lds<M_UPCAT, 2>), g, dim3(256), lds, s, wl);
+        else hipLaunchKernelGGL((k_wgrad_lds<M_PLAIN, 2>), g, dim3(256), lds, s, wl);
+      }
+      else {
+        if(l.mode == M_POOL) hipLaunchKernelGGL((k_wgrad_lds<M_POOL, 1>), g, dim3(256), lds, s, wl);
+        else if(l.mode == M_UPCAT) hipLaunchKernelGGL((k_wgrad_lds<M_UPCAT, 1>), g, dim3(256), lds, s, wl);
+        else hipLaunchKernelGGL((k_wgrad_lds<M_PLAIN, 1>), g, dim3(256), lds, s, wl);
+      }
+    }
+    else if(tr) {
       u16 *xt = U.xt + kXtPad;
       const dim3 gt((unsigned)((P + kTrPix - 1) / kTrPix));
       if(l.mode == M_POOL) hipLaunchKernelGGL(k_input_t<M_POOL>, gt, dim3(256), 0, s, x, xt);
