@@ -1426,8 +1426,7 @@ int conv_lds_cc(const ConvArgs &a)
   const int TY = tile_ty(a.d3), TZ = tile_tz(a.d3);
   if(a.W % kTileX || a.H % TY || (a.d3 && a.D % TZ)) return 0;
   const long tiles = (long)a.B * (a.W / kTileX) * (a.H / TY) * (a.d3 ? a.D / TZ : 1);
-  const int TN = a.cout <= 16 ? 1 : (a.cout <= 32 ? 2 : 4);
-  if(tiles * ((a.cout + 16 * TN - 1) / (16 * TN)) < 256) return 0;
+  if(tiles * ((a.cout + 15) / 16) < 128) return 0;
   const long nvox = (long)(kTileX + 2) * (TY + 2) * (a.d3 ? TZ + 2 : 1);
   for(int cc = a.cin_pad; cc >= 8; cc -= 8)
     if(a.cin_pad % cc == 0 && nvox * (cc / 8 + 1) * 16 <= kConvLds) return cc;
@@ -1486,11 +1485,15 @@ template <int MODE, int TN> int launch_conv_lds_tn(const ConvArgs &a, int cc)
   return GDP_OK;
 }
 
+// channel tile: the widest (most reuse of each staged input fragment) that
+// still gives 256 workgroups
 template <int MODE> int launch_conv_lds(const ConvArgs &a, int cc)
 {
-  if(a.cout <= 16) return launch_conv_lds_tn<MODE, 1>(a, cc);
-  if(a.cout <= 32) return launch_conv_lds_tn<MODE, 2>(a, cc);
-  return launch_conv_lds_tn<MODE, 4>(a, cc);
+  const long tiles = (long)a.B * (a.W / kTileX) * (a.H / tile_ty(a.d3)) * (a.d3 ? a.D / tile_tz(a.d3) : 1);
+  auto wgs = [&](int tn) { return tiles * ((a.cout + 16 * tn - 1) / (16 * tn)); };
+  if(a.cout > 32 && wgs(4) >= 256) return launch_conv_lds_tn<MODE, 4>(a, cc);
+  if(a.cout > 16 && wgs(2) >= 256) return launch_conv_lds_tn<MODE, 2>(a, cc);
+  return launch_conv_lds_tn<MODE, 1>(a, cc);
 }
 
 // Plan and launch one convolution.  The split-K buffer grows on demand only
