@@ -47,6 +47,13 @@ def spmv_bytes(N: int, nnz: int, s_val=8, s_x=8) -> int:
     return nnz * (s_val + 4) + (N + 1) * 4 + N * (s_x + s_x)
 
 
+def ilu_apply_bytes(N: int, nnz: int, s_val: int = 4) -> int:
+    """Algorithmic bytes of one ILU(0) application (both sweeps): factor
+    entries (value + 4-byte column) once, row pointers/pivots, the input
+    read once, the result written once per sweep."""
+    return (nnz - N) * (s_val + 4) + N * (4 + 8) + 2 * N * 8
+
+
 def assembly_bytes(pr, N: int, nnz: int) -> int:
     """Compulsory bytes of Generate[A]: connectivity + region per element,
     node coordinates once, CSR values and the RHS written once."""
@@ -150,6 +157,15 @@ def learned_section(n: int = 256, seed: int = 90000):
     with np.load(w, allow_pickle=False) as z:
         params = z["params"]
     row = L.evaluate(params, n, [seed], log=log)[0]
+    # C0's plain method: one ILU(0) application in the reference's RCMK order
+    pr = L.random_field_problem(n, seed)
+    fe, A, b = LA.assemble(pr)
+    s0 = LA.Solver(Algorithm=8, Preconditioner=6, Krylov_Size=30, Nb_Iter_Max=1, Stopping_Test=1e-10,
+                   Renumbering_Technique=1)
+    i0 = LA.solve(A, b, s0, LA.Vector(fe.N))
+    c0_apply = LA.solver_prec_apply(s0, b, LA.Vector(fe.N), reps=20)
+    c0_bytes = ilu_apply_bytes(fe.N, fe.nnz)
+    s0.close()
     # training-step throughput at batch 32, 128^2 (forward + backward + Adam)
     net = L.UNet(128, 128, batch=32, seed=0)
     rng = np.random.default_rng(0)
@@ -177,6 +193,8 @@ def learned_section(n: int = 256, seed: int = 90000):
         "speedup_vs_plain_rcmk": row["speedup_vs_plain_rcmk"],
         "speedup_vs_best_plain": row["speedup_vs_best_plain"],
         "rel_res": max(row["learned_rcmk"]["rel_res"], row["learned_multicolour"]["rel_res"]),
+        "c0_ilu_apply_ms": round(c0_apply, 4), "c0_ilu_levels": i0["ilu_levels"],
+        "c0_ilu_apply_gbs": round(c0_bytes / (c0_apply * 1e-3) / 1e9, 1),
         "train_step_ms_b32_128": round(step * 1e3, 2),
         "train_tflops": round(flops / step / 1e12, 2),
         "train_mfma_frac_bf16": round(flops / step / 2.5e15, 4),
@@ -238,25 +256,54 @@ def c2_section(n: int = 48, its64: int = 1500):
     x32 = LA.Vector(fe.N)
     i32 = LA.solve(A, b, s32, x32)
     a, c = x64.get(), x32.get()
+    # the reference's own FP64 solution (CG/Jacobi to 1e-10 in the reference's
+    # Fortran, sampled every 97th entry: tests/golden/c2_curlcurl48_cg.npz)
+    vs_oracle = None
+    gold = os.path.join(ROOT, "tests", "golden", "c2_curlcurl48_cg.npz")
+    if n == 48 and os.path.exists(gold):
+        with np.load(gold, allow_pickle=False) as z:
+            st, xr, sc = int(z["stride"]), z["x_sampled"], float(z["norm_inf"])
+        vs_oracle = lambda v: float(np.abs(v[::st] - xr).max() / sc)
     # the SPD operator also takes Sparskit's CG on the diagonally scaled system,
     # which does reach 1e-10 (Stopping_Test on the scaled residual, true residual reported)
     scg = LA.Solver(Algorithm=1, Preconditioner=0, Scaling=1, Renumbering_Technique=1,
                     Nb_Iter_Max=200000, Stopping_Test=1e-10)
     xcg = LA.Vector(fe.N)
     icg = LA.solve(A, b, scg, xcg)
+    # where the GMRES and CG solutions differ: the curl-curl operator alone
+    # (mass 0) applied to their difference, relative to it applied to the CG
+    # solution; near zero means the difference is a discrete gradient, which
+    # only the 1e-6 mass term sees
+    pr0 = P.curlcurl3d(n, mass=0.0)
+    fe0 = LA.FESystem(pr0)
+    K0, b0 = LA.Matrix(fe0.N), LA.Vector(fe0.N)
+    fe0.generate(K0, b0)
+    xc = xcg.get()
+    d, y = LA.Vector(fe.N, a - xc), LA.Vector(fe.N)
+    LA.prod_matrix_vector(K0, d, y)
+    kd = np.linalg.norm(y.get())
+    LA.prod_matrix_vector(K0, LA.Vector(fe.N, xc), y)
+    curl_frac = float(kd / max(np.linalg.norm(y.get()), 1e-300))
     return {
         "workload": f"curlcurl3d_{n}: {E} tets, {fe.N} edge DOFs, nnz {fe.nnz}",
         "assembly_ms": round(asm_ms, 3), "assembly_elems_per_s": round(E / (asm_ms * 1e-3), 1),
-        "fp64": {"its": i64["its"], "rel_res": i64["rel_res"], "ms": round(i64["t_solve_ms"] + i64["t_setup_ms"], 1)},
+        "fp64": {"its": i64["its"], "rel_res": i64["rel_res"], "ms": round(i64["t_solve_ms"] + i64["t_setup_ms"], 1),
+                 "error_vs_oracle": vs_oracle(a) if vs_oracle else None},
         "mixed_nested": {"outer_its": i32["its"], "inner_fp32_steps": 20 * i32["its"],
-                         "rel_res": i32["rel_res"], "ms": round(i32["t_solve_ms"] + i32["t_setup_ms"], 1)},
+                         "rel_res": i32["rel_res"], "ms": round(i32["t_solve_ms"] + i32["t_setup_ms"], 1),
+                         "error_vs_oracle": vs_oracle(c) if vs_oracle else None},
         "solution_rel_diff": float(np.abs(a - c).max() / max(np.abs(a).max(), 1e-300)),
         "fp64_cg_jacobi": {"its": icg["its"], "status": icg["status"], "rel_res": icg["rel_res"],
                            "ms": round(icg["t_solve_ms"] + icg["t_setup_ms"], 1),
-                           "gmres_solution_rel_diff": float(np.abs(a - xcg.get()).max()
-                                                            / max(np.abs(xcg.get()).max(), 1e-300))},
+                           "error_vs_oracle": vs_oracle(xcg.get()) if vs_oracle else None,
+                           "gmres_solution_rel_diff": float(np.abs(a - xc).max() / max(np.abs(xc).max(), 1e-300)),
+                           "gmres_diff_curl_fraction": curl_frac},
         "note": "ILU(0)-GMRES(30) stagnates near 6e-7 on this operator in FP64 as well; both GMRES runs are "
-                "compared at the residual the FP64 run reaches; CG on the scaled system reaches 1e-10",
+                "compared at the residual the FP64 run reaches; CG on the scaled system reaches 1e-10; "
+                "error_vs_oracle = max |x - x_ref| / max |x_ref| over every 97th entry of the reference's "
+                "FP64 CG/Jacobi solution; gmres_diff_curl_fraction = |K_curl (x_gmres - x_cg)| / |K_curl x_cg| "
+                "with K_curl the operator without its 1e-6 mass term: the GMRES error sits in the gradient "
+                "fields the curl does not see",
     }
 
 
@@ -278,6 +325,8 @@ def default_solver_section(n: int = 512):
     res = {"workload": f"poisson2d_{n}: {fe.N} DOFs, nnz {fe.nnz}", "its": info["its"],
            "status": info["status"], "rel_res": info["rel_res"],
            "setup_ms": round(info["t_setup_ms"], 1), "solve_ms": round(info["t_solve_ms"], 1),
+           "setup_factor_ms": round(info["t_factor_ms"], 1), "setup_load_ms": round(info["t_load_ms"], 1),
+           "setup_other_ms": round(info["t_setup_ms"] - info["t_factor_ms"] - info["t_load_ms"], 1),
            "solve_again_ms": round(again["t_solve_ms"], 1), "ilu_levels": info["ilu_levels"],
            "ilu_apply_ms": round(apply_ms, 3),
            "total_ms": round(info["t_setup_ms"] + info["t_solve_ms"], 1)}
@@ -371,6 +420,25 @@ def roofline_only(n: int, reps: int):
                       "copy_bytes_each_way": src.numel() * 8, "N": fe.N, "nnz": fe.nnz}))
 
 
+def ilu_profile(n: int, reps: int = 20):
+    """For rocprofv3 runs: C1's CG/ILU(0) set-up on elasticity3d_n (Scaling 1,
+    RCMK), then `reps` ILU(0) applications alone."""
+    from getdp_amd import linalg as LA
+    from getdp_amd import problems as P
+    LA.init(0)
+    pr = P.elasticity3d(n)
+    fe, A, b = LA.assemble(pr)
+    s = LA.Solver(Algorithm=1, Preconditioner=6, Scaling=1, Nb_Iter_Max=1, Stopping_Test=1e-10,
+                  Renumbering_Technique=1)
+    info = LA.solve(A, b, s, LA.Vector(fe.N))
+    ms = LA.solver_prec_apply(s, b, LA.Vector(fe.N), reps=reps)
+    nb = ilu_apply_bytes(fe.N, fe.nnz)
+    print(json.dumps({"n": n, "N": fe.N, "nnz": fe.nnz, "ilu_levels": info["ilu_levels"],
+                      "ilu_sweep": info["ilu_sweep"], "apply_ms": round(ms, 4), "algorithmic_bytes": nb,
+                      "gbs": round(nb / (ms * 1e-3) / 1e9, 1),
+                      "us_per_level": round(1e3 * ms / (2 * info["ilu_levels"]), 3)}))
+
+
 def unet_profile(mode: str, reps: int = 5):
     """For rocprofv3 runs: U-Net launches alone.  fwd3d: forward passes of the
     3D network on a 64^3 grid (batch 1, the preconditioner's shape); train3d:
@@ -436,6 +504,8 @@ def main():
                     help="only the SpMV launches and a calibration copy (for PMC runs)")
     ap.add_argument("--unet-profile", choices=["fwd3d", "train3d", "train2d"],
                     help="only U-Net launches (for rocprofv3 runs)")
+    ap.add_argument("--ilu-profile", type=int, default=0, metavar="N",
+                    help="only ILU(0) applications of CG/ILU(0) on elasticity3d_N (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -447,6 +517,9 @@ def main():
         return
     if args.unet_profile:
         unet_profile(args.unet_profile)
+        return
+    if args.ilu_profile:
+        ilu_profile(args.ilu_profile)
         return
     if args.default_solver_only:
         from getdp_amd import linalg as LA
@@ -537,6 +610,7 @@ def main():
                "setup_ms": round(io["t_setup_ms"], 1), "solve_ms": round(io["t_solve_ms"], 1),
                "ilu_levels": io["ilu_levels"],
                "ilu_apply_ms": round(LA.solver_prec_apply(so, b, LA.Vector(fe.N), reps=5), 3)}
+    cg_ilu0["ilu_apply_gbs"] = round(ilu_apply_bytes(fe.N, fe.nnz) / (cg_ilu0["ilu_apply_ms"] * 1e-3) / 1e9, 1)
     log(f"cg_ilu0: its={io['its']} status={io['status']} rel_res={io['rel_res']:.2e} "
         f"{io['t_solve_ms']:.0f} ms ({io['ilu_levels']} ILU levels)")
     so.close()

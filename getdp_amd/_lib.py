@@ -47,7 +47,8 @@ class SolveInfo(C.Structure):
     _fields_ = [("its", C.c_int), ("status", C.c_int), ("ilu_row", C.c_int),
                 ("ilu_levels", C.c_int), ("outer_its", C.c_int), ("res0", C.c_double),
                 ("res_est", C.c_double), ("true_res", C.c_double), ("rel_res", C.c_double),
-                ("t_setup_ms", C.c_double), ("t_solve_ms", C.c_double), ("ilu_sweep", C.c_int)]
+                ("t_setup_ms", C.c_double), ("t_solve_ms", C.c_double), ("ilu_sweep", C.c_int),
+                ("t_factor_ms", C.c_double), ("t_load_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -339,8 +339,15 @@ int prepare(gdp_solver *S, const gdp_matrix *A, bool reuse_factor)
   // factorisation
   c.precond = prec;
   if(prec == 6) {
+    GDP_TRY(sync());
+    const double ta = now_ms();
     if(!c.ilu.diag) GDP_TRY(c.ilu.analyse(c.Bm, c.ib, c.jb));  // first ILU(0) on this pattern
+    GDP_TRY(sync());
+    const double tb = now_ms();
     GDP_TRY(c.ilu.factor(c.Bm.a, p.ILU_Storage == 1, &c.ilu_row));
+    GDP_TRY(sync());
+    S->info.t_load_ms = tb - ta;
+    S->info.t_factor_ms = now_ms() - tb;
     if(c.ilu_row) {
       c.have_factor = false;
       return set_error(GDP_E_ZERO_PIVOT, "Zero pivot on line %d in ILU", c.ilu_row);
@@ -359,18 +366,25 @@ int prepare(gdp_solver *S, const gdp_matrix *A, bool reuse_factor)
     std::vector<double> hv(nnz);
     GDP_TRY(d2h(hv.data(), c.Bm.a, (size_t)nnz));
     const bool pivot = prec == 2;
+    const double ta = now_ms();
+    double tb = ta;
     if(p.ILU_Storage == 1) {
       IlutRows<double> f;
       GDP_TRY(ilut_factor(n, c.ib.data(), c.jb.data(), hv.data(), p.Nb_Fill, p.Dropping_Tolerance,
                           p.Permutation_Tolerance, pivot, f));
+      tb = now_ms();
       GDP_TRY(c.ilut.load(f, true));
     }
     else {
       IlutRows<float> f;
       GDP_TRY(ilut_factor(n, c.ib.data(), c.jb.data(), hv.data(), p.Nb_Fill, p.Dropping_Tolerance,
                           p.Permutation_Tolerance, pivot, f));
+      tb = now_ms();
       GDP_TRY(c.ilut.load(f, false));
     }
+    GDP_TRY(sync());
+    S->info.t_factor_ms = tb - ta;
+    S->info.t_load_ms = now_ms() - tb;
     c.ilut_fill = p.Nb_Fill;
     c.ilut_drop = p.Dropping_Tolerance;
     c.ilut_perm = p.Permutation_Tolerance;

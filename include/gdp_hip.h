@@ -87,6 +87,10 @@ typedef struct {
   int ilu_sweep;      /* kernels applying the ILU factors: 0 none,
                          GDP_SWEEP_FREE sync-free multi-wave sweeps,
                          GDP_SWEEP_WAVE one-wave level walk            */
+  double t_factor_ms; /* part of t_setup_ms: the factorisation (ILUT/ILUTP
+                         on the host, ILU(0) on the device)            */
+  double t_load_ms;   /* part of t_setup_ms: ILUT/ILUTP factors to the device,
+                         sweep analysis and packing                     */
 } gdp_solve_info;
 enum { GDP_SWEEP_FREE = 1, GDP_SWEEP_WAVE = 2 };
 int gdp_solver_info(const gdp_solver *s, gdp_solve_info *info);
