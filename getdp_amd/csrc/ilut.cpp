@@ -66,83 +66,84 @@ int ilut_factor(int n, const int *ia, const int *ja, const double *a, int lfil, 
   f.col.reserve((size_t)n * (2 * (size_t)lfil + 1));
   f.val.reserve(f.col.capacity());
   // oldcol[p]: input column now at position p; newpos[c]: position of input column c
-  std::vector<int> oldcol(n), newpos(n), at(n, 0);
+  std::vector<int> oldcol(n), newpos(n);
   for(int c = 0; c < n; c++) oldcol[c] = newpos[c] = c;
-  // working row: lower part (wl, jl), upper part (wu, ju) with the diagonal
-  // first; at[c] = 1 + index of column position c in its part, 0 if absent
-  std::vector<double> wl(n), wu(n + 1);
+  bool ident = true;  // no column swapped yet: positions are the input columns
+  // working row dense by column position (w, present where at[c] = 1); the
+  // lower columns pending elimination are bits of lbits, the upper columns
+  // are listed in order of appearance in ju (the diagonal first), and the
+  // kept multipliers go to (wl, jl) in elimination order.
+  //
+  // ilut.F picks the next row to eliminate by scanning the pending lower
+  // columns for the smallest, O(lenl^2) per row.  Here they are visited in
+  // ascending order of their bits: a row jrow's upper entries sit at
+  // positions above jrow (ILUTP's later swaps exchange positions at or above
+  // the row being formed), so every fill-in lands ahead of the scan and the
+  // columns are eliminated in the order the reference eliminates them, each
+  // entry receiving the same updates in the same order.  The rows of A hold
+  // distinct columns (an assembled CSR pattern).
+  std::vector<double> w(n), wl(n), wu(n + 1);
   std::vector<int> jl(n), ju(n + 1);
+  std::vector<unsigned char> at(n, 0);
+  std::vector<unsigned long long> lbits((size_t)n / 64 + 1, 0ull);
   for(int i = 0; i < n; i++) {
     const int j1 = ia[i], j2 = ia[i + 1];
     double tnorm = 0.0;
     for(int k = j1; k < j2; k++) tnorm = tnorm + std::fabs(a[k]);
     if(tnorm == 0.0) return set_error(GDP_E_ZERO_ROW, "Zero row encountered in ILU (row %d)", i + 1);
     tnorm = tnorm / (double)(j2 - j1);
-    int lenl = 0, lenu = 1;
+    int lenu = 1, lo = i;
     ju[0] = i;
-    wu[0] = 0.0;
+    w[i] = 0.0;
     at[i] = 1;
     for(int k = j1; k < j2; k++) {
       const int c = newpos[ja[k]];
-      const double t = a[k];
+      w[c] = a[k];
+      at[c] = 1;
       if(c < i) {
-        jl[lenl] = c;
-        wl[lenl] = t;
-        at[c] = ++lenl;
+        lbits[(size_t)c >> 6] |= 1ull << (c & 63);
+        lo = std::min(lo, c);
       }
-      else if(c == i)
-        wu[0] = t;
-      else {
-        ju[lenu] = c;
-        wu[lenu] = t;
-        at[c] = ++lenu;
-      }
+      else if(c > i)
+        ju[lenu++] = c;
     }
     // elimination, smallest remaining lower column first
     int kept = 0;
-    for(int jj = 0; jj < lenl; jj++) {
-      int k = jj, jrow = jl[jj];
-      for(int j = jj + 1; j < lenl; j++)
-        if(jl[j] < jrow) {
-          jrow = jl[j];
-          k = j;
-        }
-      if(k != jj) {
-        std::swap(jl[jj], jl[k]);
-        std::swap(wl[jj], wl[k]);
-        at[jl[k]] = k + 1;
+    const size_t wend = (size_t)i >> 6;
+    for(size_t wd = (size_t)lo >> 6; wd <= wend;) {
+      const unsigned long long bits = lbits[wd];
+      if(!bits) {
+        wd++;
+        continue;
       }
+      const int jrow = (int)(wd << 6) + __builtin_ctzll(bits);
+      lbits[wd] = bits & (bits - 1);
       at[jrow] = 0;
-      const double fact = wl[jj] * (double)f.val[f.dpos[jrow]];
+      const double fact = w[jrow] * (double)f.val[f.dpos[jrow]];
       if(std::fabs(fact) <= droptol) continue;
-      for(int kk = f.dpos[jrow] + 1; kk < f.ptr[jrow + 1]; kk++) {
-        const double sv = fact * (double)f.val[kk];
-        const int c = newpos[f.col[kk]];
-        const int p = at[c];
-        if(c >= i) {
-          if(!p) {
-            ju[lenu] = c;
-            wu[lenu] = -sv;
-            at[c] = ++lenu;
-          }
-          else
-            wu[p - 1] = wu[p - 1] - sv;
-        }
+      const int k1 = f.dpos[jrow] + 1, k2 = f.ptr[jrow + 1];
+      const int *fc = f.col.data();
+      const T *fv = f.val.data();
+      for(int kk = k1; kk < k2; kk++) {
+        const double sv = fact * (double)fv[kk];
+        const int c = ident ? fc[kk] : newpos[fc[kk]];
+        if(at[c])
+          w[c] = w[c] - sv;
         else {
-          if(!p) {
-            jl[lenl] = c;
-            wl[lenl] = -sv;
-            at[c] = ++lenl;
-          }
-          else
-            wl[p - 1] = wl[p - 1] - sv;
+          w[c] = -sv;
+          at[c] = 1;
+          if(c < i) lbits[(size_t)c >> 6] |= 1ull << (c & 63);
+          else ju[lenu++] = c;
         }
       }
       wl[kept] = fact;
       jl[kept] = jrow;
       kept++;
     }
-    for(int k = 0; k < lenu; k++) at[ju[k]] = 0;
+    for(int k = 0; k < lenu; k++) {
+      wu[k] = w[ju[k]];
+      at[ju[k]] = 0;
+    }
     // row i of L: the lfil largest multipliers
     int len = std::min(kept, lfil);
     qsplit(wl.data(), jl.data(), kept, len);
@@ -177,6 +178,7 @@ int ilut_factor(int n, const int *ia, const int *ja, const double *a, int lfil, 
       }
       std::swap(wu[0], wu[imax]);
       const int j = ju[imax];
+      if(j != i) ident = false;
       std::swap(oldcol[i], oldcol[j]);
       newpos[oldcol[i]] = i;
       newpos[oldcol[j]] = j;
