@@ -41,6 +41,7 @@ struct IluFactor {
     int *kpos = nullptr;  // position of the entry in the factors (-1: padding)
     int *row = nullptr;   // row of the lane's group (-1: none)
     int *aux = nullptr;   // forward: the row's position in the backward stream; backward: its pivot's position
+    int *slot = nullptr;  // LDS ring slot of the row's result (its index in level order mod the ring size)
     float *val32 = nullptr, *dinv32 = nullptr;  // entries (and inverted pivots) packed from the factors
     double *val64 = nullptr, *dinv64 = nullptr;
     double *pre64 = nullptr;  // forward: scale * r; backward: the forward result; in stream order

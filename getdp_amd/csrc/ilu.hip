@@ -181,9 +181,12 @@ __device__ inline bool is_gated(const int *gate, int gmode)
 
 // Bounded wait: a thread may spend at most tlimit wall-clock ticks waiting
 // in one sweep; once any thread has given up, every other waiter gives up
-// too, so a sweep always ends.
-__device__ inline bool wait_expired(unsigned long long &t0, unsigned long long tlimit, int *err)
+// too, so a sweep always ends.  The clock and the error flag are read every
+// 64th failed poll only: each is a round trip of its own (a scalar memory
+// read, a device-scope load) that would otherwise lengthen every poll.
+__device__ inline bool wait_expired(unsigned long long &t0, unsigned &spins, unsigned long long tlimit, int *err)
 {
+  if(++spins & 63) return false;
   unsigned long long now = wall_clock64();
   if(!t0) {
     t0 = now;
@@ -283,6 +286,7 @@ __device__ inline void sweep_rows(int base, int n, const int *__restrict__ ia, c
       done = true;
     }
     unsigned long long t0 = 0;
+    unsigned spins = 0;
     for(;;) {
       if(!done && pend) {
         bool any_wait = false;
@@ -299,7 +303,7 @@ __device__ inline void sweep_rows(int base, int n, const int *__restrict__ ia, c
           }
         }
         if(any_wait) {
-          if(wait_expired(t0, tlimit, err)) pend = 0;
+          if(wait_expired(t0, spins, tlimit, err)) pend = 0;
           else __builtin_amdgcn_s_sleep(1);
         }
       }
@@ -357,27 +361,31 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
 // the sweeps above pay one publication round trip through the memory
 // hierarchy per level.  Here one workgroup walks the levels in order instead.
 // A chunk holds up to 64/G rows of one level, G lanes per row, lane q of a
-// row holding its entries q, q + G, ...; row t of chunk c has the stream
-// position c * 64/G + t, and the solution entries of the last kRing stream
+// row holding its entries q, q + G, ...; a row's stream position is its index
+// in level order, and the solution entries of the last kRing stream
 // positions stay in an LDS ring indexed by position mod kRing (the host
 // checks that every dependency is still there when it is read: in level
-// order a dependency is a few levels back, 255 positions at most on ILU(0)
-// of Poisson 256^2 in RCMK order, about 6,000 on elasticity 32^3, although
-// the row numbers of one level span the whole matrix); the chunks are loaded
-// a few chunks ahead.  A level then costs LDS reads, the row's subtraction
+// order a dependency is a few levels back, although the row numbers of one
+// level span the whole matrix); the chunks are loaded a few chunks ahead.  A level then costs LDS reads, the row's subtraction
 // chain and an LDS write, with no memory round trip.  Every row subtracts its
 // products in ascending position (lusol's order; padding subtracts +0 last).
 constexpr int kRing = 16384;
 // Chunks in flight: a chunk is 2 * WG + 4 loads per lane, and the loads in
 // flight must stay within what the wave's memory counter tracks (63), or the
-// compiler waits for all of them at the top of every round.  The widest
+// compiler waits for all of them at the top of every round.  The 8-lane
 // shape keeps two chunks in flight, so that a team of 8 waves (two per SIMD)
-// holds both rounds in registers.
-__host__ __device__ constexpr int wave_depth(int wg) { return wg >= 8 ? 2 : 60 / (2 * wg + 4); }
+// holds both rounds in registers; the widest one-lane shapes keep one.
+__host__ __device__ constexpr int wave_depth(int g, int wg)
+{
+  return g == 8 ? 2 : (2 * wg + 4 > 60 ? 1 : 60 / (2 * wg + 4));
+}
 // (lanes per row, entry slots per lane) of the kernel instances, by
-// increasing capacity; a sweep takes the first that holds its longest row
-constexpr int kWaveShapes[][2] = {{1, 4}, {2, 4}, {4, 3}, {4, 6}, {8, 8}};
-constexpr int kNumWaveShapes = 5;
+// increasing capacity; a sweep takes the first that holds its longest row.
+// One lane per row up to 32 entries: the row's products and its subtraction
+// chain stay in one lane, with no cross-lane reads (with 4 lanes a row, a
+// 24-entry row took 48 LDS permutes per chunk, about half a chunk's time).
+constexpr int kWaveShapes[][2] = {{1, 4}, {1, 8}, {1, 16}, {1, 24}, {1, 32}, {8, 8}};
+constexpr int kNumWaveShapes = 6;
 
 // Several waves of one workgroup may share the walk (a team): wave w takes
 // chunks w, w + T, w + 2T, ... (T waves).  Chunk c may start once every chunk
@@ -392,7 +400,7 @@ constexpr int kMaxTeam = 8;
 template <typename TF, typename TV, int WG> struct WaveRec {
   int c[WG];
   TF v[WG];
-  int r, a, nd;
+  int r, a, nd, w;
   TF d;
   TV p;
 };
@@ -402,12 +410,12 @@ template <typename TF, typename TV, int WG> struct WaveRec {
 template <typename TF, typename TV, int G, int WG, bool FWD>
 __global__ void __launch_bounds__(kWave *kMaxTeam) k_wave_sweep(int nch, const int *__restrict__ col, const TF *__restrict__ val,
                                                    const int *__restrict__ row, const int *__restrict__ aux,
-                                                   const int *__restrict__ need,
+                                                   const int *__restrict__ slot, const int *__restrict__ need,
                                                    const TF *__restrict__ dinv, const TV *__restrict__ pre, TV *out,
                                                    int *err, unsigned long long tlimit, const int *gate, int gmode)
 {
   if(is_gated(gate, gmode)) return;
-  constexpr int kDepth = wave_depth(WG);
+  constexpr int kDepth = wave_depth(G, WG);
   __shared__ TV ring[kRing + 1];
   __shared__ int prog[kMaxTeam];
   const int lane = threadIdx.x & (kWave - 1), q = lane & (G - 1), gbase = lane - q;
@@ -430,6 +438,7 @@ __global__ void __launch_bounds__(kWave *kMaxTeam) k_wave_sweep(int nch, const i
     }
     const size_t li = (size_t)c * kWave + lane;
     b.r = row[li];
+    b.w = slot[li];
     if constexpr(FWD) b.a = aux[li];
     else b.d = dinv[li];
     b.p = pre[li];
@@ -450,20 +459,38 @@ __global__ void __launch_bounds__(kWave *kMaxTeam) k_wave_sweep(int nch, const i
         // wait until every chunk below rc.nd is done (bounded)
         const int s = rc.nd;
         const int want = lane < team && s > lane ? (s - lane + team - 1) / team : 0;
-        const unsigned long long t0 = wall_clock64();
+        // the clock is read once 256 polls have failed, and every 256th
+        // poll after that (a read of it is a scalar memory round trip)
+        unsigned long long t0 = 0;
+        unsigned spins = 0;
         while(true) {
           const int have =
             lane < team ? __hip_atomic_load(prog + lane, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
           if(__all(have >= want) || expired) break;
-          if(wall_clock64() - t0 > tlimit) {
-            expired = true;
-            if(lane == 0) atomicOr(err, 1);
+          if((++spins & 255) == 0) {
+            const unsigned long long now = wall_clock64();
+            if(!t0) t0 = now;
+            else if(now - t0 > tlimit) {
+              expired = true;
+              if(lane == 0) atomicOr(err, 1);
+            }
           }
         }
       }
-      TV prod[WG];
+      // every ring read is issued before the first product: left to itself
+      // the scheduler interleaves them in groups of four, each waiting out a
+      // full LDS round trip on the row's chain
+      TV rv[WG], prod[WG];
 #pragma unroll
-      for(int w = 0; w < WG; w++) prod[w] = (TV)rc.v[w] * ring[rc.c[w]];
+      for(int w = 0; w < WG; w++) rv[w] = ring[rc.c[w]];
+      __builtin_amdgcn_sched_group_barrier(0x100, WG, 0);  // the DS reads first
+      __builtin_amdgcn_sched_group_barrier(0x002, 4 * WG, 0);  // then the arithmetic
+#pragma unroll
+      for(int w = 0; w < WG; w++) prod[w] = (TV)rc.v[w] * rv[w];
+      // and every product before the subtraction chain, which then waits on
+      // one subtraction per entry instead of a product and a subtraction
+#pragma unroll
+      for(int w = 0; w < WG; w++) __asm__ volatile("" : "+v"(prod[w]));
       TV acc = rc.p;
 #pragma unroll
       for(int w = 0; w < WG; w++) {
@@ -475,7 +502,7 @@ __global__ void __launch_bounds__(kWave *kMaxTeam) k_wave_sweep(int nch, const i
       }
       if(q == 0 && rc.r >= 0) {
         const TV xv = FWD ? acc : (TV)rc.d * acc;
-        ring[(chunk(k0 + d) * (kWave / G) + lane / G) & (kRing - 1)] = xv;  // its stream position
+        ring[rc.w] = xv;  // the slot of its stream position
         if constexpr(FWD) out[rc.a] = xv;  // the backward sweep's input, in its order
         else out[rc.r] = xv;
       }
@@ -591,6 +618,7 @@ void IluFactor::wave_release()
     dfree(w.kpos);
     dfree(w.row);
     dfree(w.aux);
+    dfree(w.slot);
     dfree(w.val32);
     dfree(w.dinv32);
     dfree(w.val64);
@@ -644,7 +672,7 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
   struct Side {
     int G, wg, rc;
     std::vector<int> cstart, ccount;  // chunk -> rows (index into the ordered rows)
-    std::vector<int> spos;            // row -> stream position c * rc + t (its ring slot mod kRing)
+    std::vector<int> spos;            // row -> stream position: its index in level order (ring slot mod kRing)
   } sd[2];
   const std::vector<int> *rows[2] = {&lrows, &urows};
   const std::vector<int> *ptrs[2] = {&lptr, &uptr};
@@ -671,20 +699,21 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
         sd[s].ccount.push_back(std::min(sd[s].rc, p[l + 1] - a));
       }
     // the ring holds the solved entries of the last kRing stream positions
-    // (slot = position mod kRing, so it empties in solve order); every
-    // dependency must still be there when its chunk reads it, i.e. solved
-    // fewer than kRing positions earlier
+    // (a row's position is its index in level order, its slot the position
+    // mod kRing, so the ring empties in solve order and padding lanes take
+    // no slots); every dependency must still be there when its chunk reads
+    // it, i.e. solved fewer than kRing positions before the chunk starts
     const std::vector<int> &ord = *rows[s];
     sd[s].spos.assign(n, -1);
     for(size_t c = 0; c < sd[s].cstart.size(); c++)
-      for(int t = 0; t < sd[s].ccount[c]; t++) sd[s].spos[ord[sd[s].cstart[c] + t]] = (int)c * sd[s].rc + t;
+      for(int t = 0; t < sd[s].ccount[c]; t++) sd[s].spos[ord[sd[s].cstart[c] + t]] = sd[s].cstart[c] + t;
     for(size_t c = 0; c < sd[s].cstart.size(); c++) {
       for(int t = 0; t < sd[s].ccount[c]; t++) {
         const int i = ord[sd[s].cstart[c] + t];
         int b, e;
         range(s, i, b, e);
         for(int k = b; k < e; k++)
-          if((long)c * sd[s].rc - sd[s].spos[hja[k]] >= kRing - kWave)
+          if((long)sd[s].cstart[c] - sd[s].spos[hja[k]] >= kRing - kWave)
             return force ? set_error(GDP_E_UNSUPPORTED,
                                      "one-wave sweeps: row %d depends on row %d, which has left the %d-entry ring", i,
                                      hja[k], kRing)
@@ -758,11 +787,11 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
     const long nch = (long)S.cstart.size();
     // steps rounded up to whole prefetch rounds of every wave, then one round
     // of padding chunks per wave
-    const int depth = wave_depth(S.wg);
+    const int depth = wave_depth(S.G, S.wg);
     const long span = (long)team * depth;
     const long nsteps = (nch + span - 1) / span * span, nalloc = nsteps + span;
     const size_t ne = (size_t)nalloc * S.wg * kWave, nl = (size_t)nalloc * kWave;
-    std::vector<int> col(ne, kRing), kpos(ne, -1), row(nl, -1), aux(nl, -1);
+    std::vector<int> col(ne, kRing), kpos(ne, -1), row(nl, -1), aux(nl, -1), slot(nl, -1);
     const std::vector<int> &ord = *rows[s];
     for(long c = 0; c < nch; c++)
       for(int t = 0; t < S.ccount[c]; t++) {
@@ -777,6 +806,7 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
         }
         row[(size_t)c * kWave + gl] = i;
         aux[(size_t)c * kWave + gl] = s == 0 ? bpos[i] : dpos[i];
+        slot[(size_t)c * kWave + gl] = S.spos[i] & (kRing - 1);
       }
     WaveSweep &W = wave[s];
     W.G = S.G;
@@ -791,19 +821,24 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
     GDP_TRY(dalloc(&W.kpos, ne));
     GDP_TRY(dalloc(&W.row, nl));
     GDP_TRY(dalloc(&W.aux, nl));
+    GDP_TRY(dalloc(&W.slot, nl));
     GDP_TRY(dalloc(&W.pre64, nl));
     GDP_TRY(dalloc(&W.pre32, nl));
     GDP_TRY(h2d(W.col, col.data(), ne));
     GDP_TRY(h2d(W.kpos, kpos.data(), ne));
     GDP_TRY(h2d(W.row, row.data(), nl));
     GDP_TRY(h2d(W.aux, aux.data(), nl));
+    GDP_TRY(h2d(W.slot, slot.data(), nl));
   }
   wave_mode = true;
   return GDP_OK;
 }
 
 // Chunks stored per stream: the walk's chunks and one prefetch round per wave.
-long IluFactor::wave_chunks_alloc(const WaveSweep &W) const { return (long)W.nch + (long)W.team * wave_depth(W.wg); }
+long IluFactor::wave_chunks_alloc(const WaveSweep &W) const
+{
+  return (long)W.nch + (long)W.team * wave_depth(W.G, W.wg);
+}
 
 // Storage for the packed factor entries, in the factors' precision.
 int IluFactor::wave_alloc_values()
@@ -1109,18 +1144,19 @@ void wave_apply(int n, const IluFactor::WaveSweep *W, const TV *r, TV scale, TV 
 {
   hipStream_t s = ctx().stream;
   const IluFactor::WaveSweep &F = W[0], &B = W[1];
-  const long nl = ((long)F.nch + (long)F.team * wave_depth(F.wg)) * kWave;
+  const long nl = ((long)F.nch + (long)F.team * wave_depth(F.G, F.wg)) * kWave;
   hipLaunchKernelGGL(k_wave_gather<TV>, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, s, nl, F.row, r, scale,
                      wave_pre<TV>(F), gate, gmode);
 #define GDP_WAVE(GG, WW, FWD, S, OUT, ADD)                                                                \
   hipLaunchKernelGGL((k_wave_sweep<TF, TV, GG, WW, FWD>), dim3(1), dim3(kWave * S.team), 0, s, S.nch, S.col, \
-                     wave_val<TF>(S), S.row, S.aux, S.need, wave_dinv<TF>(S), wave_pre<TV>(S), OUT, err, tl, gate, \
-                     gmode)
+                     wave_val<TF>(S), S.row, S.aux, S.slot, S.need, wave_dinv<TF>(S), wave_pre<TV>(S), OUT, err, tl, \
+                     gate, gmode)
 #define GDP_WAVE_SHAPE(FWD, S, OUT, ADD)                                        \
-  if(S.G == 1) GDP_WAVE(1, 4, FWD, S, OUT, ADD);                                \
-  else if(S.G == 2) GDP_WAVE(2, 4, FWD, S, OUT, ADD);                           \
-  else if(S.G == 4 && S.wg == 3) GDP_WAVE(4, 3, FWD, S, OUT, ADD);              \
-  else if(S.G == 4) GDP_WAVE(4, 6, FWD, S, OUT, ADD);                           \
+  if(S.G == 1 && S.wg == 4) GDP_WAVE(1, 4, FWD, S, OUT, ADD);                   \
+  else if(S.G == 1 && S.wg == 8) GDP_WAVE(1, 8, FWD, S, OUT, ADD);              \
+  else if(S.G == 1 && S.wg == 16) GDP_WAVE(1, 16, FWD, S, OUT, ADD);            \
+  else if(S.G == 1 && S.wg == 24) GDP_WAVE(1, 24, FWD, S, OUT, ADD);            \
+  else if(S.G == 1) GDP_WAVE(1, 32, FWD, S, OUT, ADD);                          \
   else GDP_WAVE(8, 8, FWD, S, OUT, ADD)
   GDP_WAVE_SHAPE(true, F, wave_pre<TV>(B), (TV *)nullptr);
   GDP_WAVE_SHAPE(false, B, x, addto);
