@@ -30,20 +30,20 @@ struct IluFactor {
   float *pbuf32 = nullptr;
   // One-wave sweeps, used when the dependency chain is long and the levels
   // are narrow (threshold ILU with fill): each sweep's rows in level order,
-  // in chunks of 64/G rows of one level with G lanes per row.  Entry slot w
-  // of lane l in chunk c sits at (c * wg + w) * 64 + l; per-lane fields at
-  // c * 64 + l, set on the first lane of each row's group only.
+  // in chunks of up to 64 rows of one level, one lane per row.  Entry slot w
+  // of lane l in chunk c sits at (c * wg + w) * 64 + l in col/kpos; per-lane
+  // fields at c * 64 + l.
   struct WaveSweep {
-    int G = 1, wg = 0, nch = 0;
+    int G = 1, wg = 0, nch = 0;  // one lane per row; wg entry slots per row
     int team = 1;         // waves sharing the walk (one workgroup)
-    int *need = nullptr;  // per chunk: every chunk below it must be done first (team > 1)
-    int *col = nullptr;   // LDS ring slot of the entry's column (the zero slot: padding)
+    int *col = nullptr;   // LDS ring slot of each entry slot's column (the zero slot: padding)
     int *kpos = nullptr;  // position of the entry in the factors (-1: padding)
-    int *row = nullptr;   // row of the lane's group (-1: none)
+    int *row = nullptr;   // row of the lane (-1: none)
     int *aux = nullptr;   // forward: the row's position in the backward stream; backward: its pivot's position
-    int *slot = nullptr;  // LDS ring slot of the row's result (its index in level order mod the ring size)
-    float *val32 = nullptr, *dinv32 = nullptr;  // entries (and inverted pivots) packed from the factors
-    double *val64 = nullptr, *dinv64 = nullptr;
+    int4 *meta = nullptr; // per lane: row, forward output position, ring slot of the result, the chunk's need
+    unsigned *ent = nullptr;  // 16-byte entry records (column, value bits) packed after each factorisation
+    float *dinv32 = nullptr;  // inverted pivots (backward), packed from the factors
+    double *dinv64 = nullptr;
     double *pre64 = nullptr;  // forward: scale * r; backward: the forward result; in stream order
     float *pre32 = nullptr;
   } wave[2];  // [0] forward (L), [1] backward (U)

@@ -370,22 +370,27 @@ __global__ void __launch_bounds__(256) k_sweep(int n, const int *__restrict__ ia
 // chain and an LDS write, with no memory round trip.  Every row subtracts its
 // products in ascending position (lusol's order; padding subtracts +0 last).
 constexpr int kRing = 16384;
-// Chunks in flight: a chunk is 2 * WG + 4 loads per lane, and the loads in
-// flight must stay within what the wave's memory counter tracks (63), or the
-// compiler waits for all of them at the top of every round.  The 8-lane
-// shape keeps two chunks in flight, so that a team of 8 waves (two per SIMD)
-// holds both rounds in registers; the widest one-lane shapes keep one.
-__host__ __device__ constexpr int wave_depth(int g, int wg)
-{
-  return g == 8 ? 2 : (2 * wg + 4 > 60 ? 1 : 60 / (2 * wg + 4));
-}
-// (lanes per row, entry slots per lane) of the kernel instances, by
-// increasing capacity; a sweep takes the first that holds its longest row.
-// One lane per row up to 32 entries: the row's products and its subtraction
-// chain stay in one lane, with no cross-lane reads (with 4 lanes a row, a
-// 24-entry row took 48 LDS permutes per chunk, about half a chunk's time).
-constexpr int kWaveShapes[][2] = {{1, 4}, {1, 8}, {1, 16}, {1, 24}, {1, 32}, {8, 8}};
-constexpr int kNumWaveShapes = 6;
+// One lane per row, up to 32 entries: the row's products and its
+// subtraction chain stay in one lane, with no cross-lane reads.  The entry
+// slots of a sweep are the first of these that holds its longest row.
+constexpr int kWaveSlots[] = {4, 8, 16, 24, 32};
+constexpr int kNumWaveShapes = 5;
+
+// Chunk records.  Entry slots are stored as 16-byte records, lane-interleaved
+// (record j of lane l in chunk c at (c * NL + j) * 64 + l): two (column,
+// value) pairs per record with FP32 factors, one (column, -, value) with FP64
+// ones, so a chunk is NL = slots / 2 (or slots) coalesced 16-byte loads per
+// lane, plus one 16-byte load of the row's fields (row, forward output
+// position, ring slot, need), the right-hand side and, backward, the
+// inverted pivot.  A wave keeps NB chunks in registers and rotates through
+// them without copies: the loads for its chunk k + NB - 1 are issued before
+// chunk k is computed, so a chunk's loads have NB - 1 chunks of the wave's
+// own work (and the team's in between) to arrive.  NB is 4 while the loads
+// in flight stay within the 63 the wave's memory counter tracks, else 2.
+__host__ __device__ constexpr int walk_nl(bool f64, int wg) { return f64 ? wg : wg / 2; }
+__host__ __device__ constexpr int walk_nb(bool f64, int wg) { return walk_nl(f64, wg) + 4 <= 20 ? 4 : 2; }
+// prefetch padding per wave (chunks) after the last step: the largest NB
+constexpr int kWalkPad = 4;
 
 // Several waves of one workgroup may share the walk (a team): wave w takes
 // chunks w, w + T, w + 2T, ... (T waves).  Chunk c may start once every chunk
@@ -394,123 +399,114 @@ constexpr int kNumWaveShapes = 6;
 // last chunk that reads the value it holds (the host computes need from the
 // sequential order, in which the ring check holds).  Each wave publishes how
 // many of its chunks are done in LDS; chunks below s are all done when every
-// wave v has finished its ceil((s - v) / T) chunks below s.
-constexpr int kMaxTeam = 8;
+// wave v has finished its ceil((s - v) / T) chunks below s.  At most four
+// waves (one per SIMD), so that each may hold 512 registers.
+constexpr int kMaxTeam = 4;
 
-template <typename TF, typename TV, int WG> struct WaveRec {
-  int c[WG];
-  TF v[WG];
-  int r, a, nd, w;
-  TF d;
-  TV p;
-};
-
-// nch chunks (a multiple of team * depth), followed by team * depth padding
-// chunks that the last prefetches read; padding rows are -1 and never stored.
-template <typename TF, typename TV, int G, int WG, bool FWD>
-__global__ void __launch_bounds__(kWave *kMaxTeam) k_wave_sweep(int nch, const int *__restrict__ col, const TF *__restrict__ val,
-                                                   const int *__restrict__ row, const int *__restrict__ aux,
-                                                   const int *__restrict__ slot, const int *__restrict__ need,
-                                                   const TF *__restrict__ dinv, const TV *__restrict__ pre, TV *out,
-                                                   int *err, unsigned long long tlimit, const int *gate, int gmode)
+// nch chunks (a multiple of team * kWalkPad), followed by team * kWalkPad
+// padding chunks that the last prefetches read; padding rows are -1 and never
+// stored.
+template <typename TF, typename TV, int WG, bool FWD>
+__global__ void __launch_bounds__(kWave *kMaxTeam) k_wave_sweep(int nch, const uint4 *__restrict__ ent,
+                                                               const int4 *__restrict__ meta,
+                                                               const TF *__restrict__ dinv, const TV *__restrict__ pre,
+                                                               TV *out, int *err, unsigned long long tlimit,
+                                                               const int *gate, int gmode)
 {
   if(is_gated(gate, gmode)) return;
-  constexpr int kDepth = wave_depth(G, WG);
+  constexpr bool F64 = sizeof(TF) == 8;
+  constexpr int NL = walk_nl(F64, WG), NB = walk_nb(F64, WG);
+  struct Rec {
+    uint4 e[NL];
+    int4 m;  // row, forward output position, ring slot, need
+    TV p;
+    TF d;
+  };
   __shared__ TV ring[kRing + 1];
   __shared__ int prog[kMaxTeam];
-  const int lane = threadIdx.x & (kWave - 1), q = lane & (G - 1), gbase = lane - q;
+  const int lane = threadIdx.x & (kWave - 1);
   const int team = blockDim.x / kWave, wv = threadIdx.x / kWave;
   if(threadIdx.x == 0) ring[kRing] = 0;  // the padding slot
   if(threadIdx.x < kMaxTeam) prog[threadIdx.x] = 0;
   __syncthreads();
   bool expired = false;
-  // whole rounds double-buffered: the next round's loads are issued before
-  // the current round is computed, so the one wait per round (the output
-  // stores share the loads' counter, which makes the compiler wait for all
-  // loads at a round's end) comes a round of work after the loads
-  WaveRec<TF, TV, WG> cur[kDepth], nxt[kDepth];
-  auto load = [&](WaveRec<TF, TV, WG> &b, int c) {
+  auto load = [&](Rec &b, int c) {
 #pragma unroll
-    for(int w = 0; w < WG; w++) {
-      const size_t i = ((size_t)c * WG + w) * kWave + lane;
-      b.c[w] = col[i];
-      b.v[w] = val[i];
-    }
+    for(int j = 0; j < NL; j++) b.e[j] = ent[((size_t)c * NL + j) * kWave + lane];
     const size_t li = (size_t)c * kWave + lane;
-    b.r = row[li];
-    b.w = slot[li];
-    if constexpr(FWD) b.a = aux[li];
-    else b.d = dinv[li];
+    b.m = meta[li];
     b.p = pre[li];
-    b.nd = team > 1 ? need[c] : 0;
+    if constexpr(!FWD) b.d = dinv[li];
+  };
+  auto col = [&](const Rec &b, int w) -> int {
+    if constexpr(F64) return (int)b.e[w].x;
+    else return (int)((w & 1) ? b.e[w >> 1].z : b.e[w >> 1].x);
+  };
+  auto val = [&](const Rec &b, int w) -> TF {
+    if constexpr(F64) return __hiloint2double((int)b.e[w].w, (int)b.e[w].z);
+    else return __uint_as_float((w & 1) ? b.e[w >> 1].w : b.e[w >> 1].y);
   };
   // this wave's k-th chunk
   auto chunk = [&](int k) { return wv + team * k; };
-  const int nk = nch / team;
-#pragma unroll
-  for(int d = 0; d < kDepth; d++) load(cur[d], chunk(d));
-  for(int k0 = 0; k0 < nk; k0 += kDepth) {
-#pragma unroll
-    for(int d = 0; d < kDepth; d++) load(nxt[d], chunk(k0 + kDepth + d));
-#pragma unroll
-    for(int d = 0; d < kDepth; d++) {
-      const WaveRec<TF, TV, WG> &rc = cur[d];
-      if(team > 1 && rc.nd > 0) {
-        // wait until every chunk below rc.nd is done (bounded)
-        const int s = rc.nd;
-        const int want = lane < team && s > lane ? (s - lane + team - 1) / team : 0;
-        // the clock is read once 256 polls have failed, and every 256th
-        // poll after that (a read of it is a scalar memory round trip)
-        unsigned long long t0 = 0;
-        unsigned spins = 0;
-        while(true) {
-          const int have =
-            lane < team ? __hip_atomic_load(prog + lane, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
-          if(__all(have >= want) || expired) break;
-          if((++spins & 255) == 0) {
-            const unsigned long long now = wall_clock64();
-            if(!t0) t0 = now;
-            else if(now - t0 > tlimit) {
-              expired = true;
-              if(lane == 0) atomicOr(err, 1);
-            }
+  auto process = [&](const Rec &rc, int k) {
+    if(team > 1 && rc.m.w > 0) {
+      // wait until every chunk below rc.m.w is done (bounded); the clock is
+      // read once 256 polls have failed, and every 256th poll after that (a
+      // read of it is a scalar memory round trip)
+      const int s = rc.m.w;
+      const int want = lane < team && s > lane ? (s - lane + team - 1) / team : 0;
+      unsigned long long t0 = 0;
+      unsigned spins = 0;
+      while(true) {
+        const int have =
+          lane < team ? __hip_atomic_load(prog + lane, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) : 0;
+        if(__all(have >= want) || expired) break;
+        if((++spins & 255) == 0) {
+          const unsigned long long now = wall_clock64();
+          if(!t0) t0 = now;
+          else if(now - t0 > tlimit) {
+            expired = true;
+            if(lane == 0) atomicOr(err, 1);
           }
         }
       }
-      // every ring read is issued before the first product: left to itself
-      // the scheduler interleaves them in groups of four, each waiting out a
-      // full LDS round trip on the row's chain
-      TV rv[WG], prod[WG];
-#pragma unroll
-      for(int w = 0; w < WG; w++) rv[w] = ring[rc.c[w]];
-      __builtin_amdgcn_sched_group_barrier(0x100, WG, 0);  // the DS reads first
-      __builtin_amdgcn_sched_group_barrier(0x002, 4 * WG, 0);  // then the arithmetic
-#pragma unroll
-      for(int w = 0; w < WG; w++) prod[w] = (TV)rc.v[w] * rv[w];
-      // and every product before the subtraction chain, which then waits on
-      // one subtraction per entry instead of a product and a subtraction
-#pragma unroll
-      for(int w = 0; w < WG; w++) __asm__ volatile("" : "+v"(prod[w]));
-      TV acc = rc.p;
-#pragma unroll
-      for(int w = 0; w < WG; w++) {
-        if constexpr(G == 1) acc = acc - prod[w];
-        else {
-#pragma unroll
-          for(int j = 0; j < G; j++) acc = acc - lane_read(prod[w], gbase + j);
-        }
-      }
-      if(q == 0 && rc.r >= 0) {
-        const TV xv = FWD ? acc : (TV)rc.d * acc;
-        ring[rc.w] = xv;  // the slot of its stream position
-        if constexpr(FWD) out[rc.a] = xv;  // the backward sweep's input, in its order
-        else out[rc.r] = xv;
-      }
-      if(team > 1 && lane == 0)  // this wave's chunks up to k0 + d are done
-        __hip_atomic_store(prog + wv, k0 + d + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+    // every ring read is issued before the first product: left to itself
+    // the scheduler interleaves them in groups of four, each waiting out a
+    // full LDS round trip on the row's chain
+    TV rv[WG], prod[WG];
 #pragma unroll
-    for(int d = 0; d < kDepth; d++) cur[d] = nxt[d];
+    for(int w = 0; w < WG; w++) rv[w] = ring[col(rc, w)];
+    __builtin_amdgcn_sched_group_barrier(0x100, WG, 0);      // the DS reads first
+    __builtin_amdgcn_sched_group_barrier(0x002, 4 * WG, 0);  // then the arithmetic
+#pragma unroll
+    for(int w = 0; w < WG; w++) prod[w] = (TV)val(rc, w) * rv[w];
+    // and every product before the subtraction chain, which then waits on
+    // one subtraction per entry instead of a product and a subtraction
+#pragma unroll
+    for(int w = 0; w < WG; w++) __asm__ volatile("" : "+v"(prod[w]));
+    TV acc = rc.p;
+#pragma unroll
+    for(int w = 0; w < WG; w++) acc = acc - prod[w];  // lusol's order; padding subtracts +0 last
+    if(rc.m.x >= 0) {
+      const TV xv = FWD ? acc : (TV)rc.d * acc;
+      ring[rc.m.z] = xv;  // the slot of its stream position
+      if constexpr(FWD) out[rc.m.y] = xv;  // the backward sweep's input, in its order
+      else out[rc.m.x] = xv;
+    }
+    if(team > 1 && lane == 0)  // this wave's chunks up to k are done
+      __hip_atomic_store(prog + wv, k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  const int nk = nch / team;
+  Rec b[NB];
+#pragma unroll
+  for(int d = 0; d + 1 < NB; d++) load(b[d], chunk(d));
+  for(int k0 = 0; k0 < nk; k0 += NB) {
+#pragma unroll
+    for(int d = 0; d < NB; d++) {
+      load(b[(d + NB - 1) % NB], chunk(k0 + d + NB - 1));
+      process(b[d], k0 + d);
+    }
   }
 }
 
@@ -547,6 +543,35 @@ __global__ void __launch_bounds__(256) k_wave_pack(long cnt, const int *__restri
   if(p < cnt) {
     const int k = kpos[p];
     val[p] = k >= 0 ? lu[k] : (TF)0;
+  }
+}
+
+// The walk's 16-byte entry records from the slot arrays (slot w of lane l in
+// chunk c at ((c * wg + w) * 64 + l)): column and value bits, after each
+// factorisation (enqueued: usable under capture).
+template <typename TF>
+__global__ void __launch_bounds__(256) k_wave_pack_ent(long cnt, int wg, const int *__restrict__ col,
+                                                       const int *__restrict__ kpos, const TF *__restrict__ lu,
+                                                       unsigned *__restrict__ ent)
+{
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if(p >= cnt) return;
+  const long c = p / ((long)wg * kWave);
+  const int w = (int)(p / kWave % wg), l = (int)(p % kWave);
+  const int k = kpos[p];
+  const TF v = k >= 0 ? lu[k] : (TF)0;
+  if constexpr(sizeof(TF) == 4) {
+    const size_t word = (((size_t)c * (wg / 2) + w / 2) * kWave + l) * 4 + (w & 1) * 2;
+    ent[word] = (unsigned)col[p];
+    ent[word + 1] = __float_as_uint(v);
+  }
+  else {
+    const size_t word = (((size_t)c * wg + w) * kWave + l) * 4;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    ent[word] = (unsigned)col[p];
+    ent[word + 1] = 0;
+    ent[word + 2] = (unsigned)b;
+    ent[word + 3] = (unsigned)(b >> 32);
   }
 }
 
@@ -618,14 +643,12 @@ void IluFactor::wave_release()
     dfree(w.kpos);
     dfree(w.row);
     dfree(w.aux);
-    dfree(w.slot);
-    dfree(w.val32);
+    dfree(w.meta);
+    dfree(w.ent);
     dfree(w.dinv32);
-    dfree(w.val64);
     dfree(w.dinv64);
     dfree(w.pre64);
     dfree(w.pre32);
-    dfree(w.need);
     w = WaveSweep();
   }
   wave_mode = false;
@@ -684,13 +707,13 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
       maxc = std::max(maxc, e - b);
     }
     int shape = 0;
-    while(shape < kNumWaveShapes && kWaveShapes[shape][0] * kWaveShapes[shape][1] < maxc) shape++;
+    while(shape < kNumWaveShapes && kWaveSlots[shape] < maxc) shape++;
     if(shape == kNumWaveShapes)
       return force ? set_error(GDP_E_UNSUPPORTED, "one-wave sweeps: a row holds %d entries on one side (at most %d)",
-                               maxc, kWaveShapes[kNumWaveShapes - 1][0] * kWaveShapes[kNumWaveShapes - 1][1])
+                               maxc, kWaveSlots[kNumWaveShapes - 1])
                    : GDP_OK;
-    sd[s].G = kWaveShapes[shape][0];
-    sd[s].wg = kWaveShapes[shape][1];
+    sd[s].G = 1;
+    sd[s].wg = kWaveSlots[shape];
     sd[s].rc = kWave / sd[s].G;
     const std::vector<int> &p = *ptrs[s];
     for(size_t l = 0; l + 1 < p.size(); l++)
@@ -721,20 +744,23 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
       }
     }
   }
-  // waves per walk: GDP_ILU_TEAM overrides.  Measured per application
-  // (1, 2, 4, 8 waves): ILU(0) of Poisson 256^2 in RCMK order 0.62, 0.59,
-  // 0.48, 0.49 ms; ILUTP(20) of Poisson 512^2 28.0, 23.6, 21.3, 22.7 ms.
-  int team = 4;
+  // waves per walk: one when a level is about one chunk (the chunks depend on
+  // each other in turn, and a team only adds its hand-over), four when levels
+  // span several chunks that the team takes in parallel; GDP_ILU_TEAM
+  // overrides.  Measured per application (1, 2, 4 waves): ILUTP(20) of
+  // Poisson 512^2 (1.0 chunk a level) 18.6, 20.6, 20.8 ms; ILU(0) of Poisson
+  // 256^2 in RCMK order (2 chunks a level) 0.49, 0.50, 0.43 ms.
+  const double per_level = (double)(sd[0].cstart.size() + sd[1].cstart.size()) / std::max(1, nlevels + nul);
+  int team = per_level <= 1.25 ? 1 : 4;
   if(const char *e = std::getenv("GDP_ILU_TEAM")) team = std::max(1, std::min(kMaxTeam, atoi(e)));
-  // Estimated time per application.  A chunk of the walk costs its row
-  // chain (about 0.2 us with 4 entry slots a row, 0.55 us with 12-24, 0.9 us
-  // with 64), and the chunks of one level go about two at a time (elasticity
-  // 32^3, 9 chunks a level: 6.3 ms against 3.4 ms for the sync-free sweeps,
-  // which cost about 2 us per level).
+  // Estimated time per application.  A chunk of the walk costs its reads
+  // and its row chain (about 0.25 us with 4 entry slots a row, 0.46 us with
+  // 24), and the chunks of one level go about two at a time; the sync-free
+  // sweeps cost 1-2 us per level (one publication round trip).
   double t_wave = 0.;
   for(int s = 0; s < 2; s++) {
     const int cap = sd[s].G * sd[s].wg;
-    const double per = cap <= 4 ? 0.2 : cap <= 8 ? 0.3 : cap <= 24 ? 0.55 : 0.9;
+    const double per = cap <= 4 ? 0.25 : cap <= 8 ? 0.3 : cap <= 16 ? 0.4 : cap <= 24 ? 0.46 : 0.55;
     const std::vector<int> &p = *ptrs[s];
     for(size_t l = 0; l + 1 < p.size(); l++) {
       const int ch = (p[l + 1] - p[l] + sd[s].rc - 1) / sd[s].rc;
@@ -785,13 +811,13 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
   for(int s = 0; s < 2; s++) {
     const Side &S = sd[s];
     const long nch = (long)S.cstart.size();
-    // steps rounded up to whole prefetch rounds of every wave, then one round
-    // of padding chunks per wave
-    const int depth = wave_depth(S.G, S.wg);
-    const long span = (long)team * depth;
+    // steps rounded up to whole register rotations of every wave, then the
+    // padding chunks the last prefetches read
+    const long span = (long)team * kWalkPad;
     const long nsteps = (nch + span - 1) / span * span, nalloc = nsteps + span;
     const size_t ne = (size_t)nalloc * S.wg * kWave, nl = (size_t)nalloc * kWave;
-    std::vector<int> col(ne, kRing), kpos(ne, -1), row(nl, -1), aux(nl, -1), slot(nl, -1);
+    std::vector<int> col(ne, kRing), kpos(ne, -1), row(nl, -1), aux(nl, -1);
+    std::vector<int4> meta(nl, make_int4(-1, -1, kRing, 0));
     const std::vector<int> &ord = *rows[s];
     for(long c = 0; c < nch; c++)
       for(int t = 0; t < S.ccount[c]; t++) {
@@ -806,39 +832,34 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
         }
         row[(size_t)c * kWave + gl] = i;
         aux[(size_t)c * kWave + gl] = s == 0 ? bpos[i] : dpos[i];
-        slot[(size_t)c * kWave + gl] = S.spos[i] & (kRing - 1);
+        meta[(size_t)c * kWave + gl] = make_int4(i, s == 0 ? bpos[i] : -1, S.spos[i] & (kRing - 1), 0);
       }
+    for(long c = 0; c < nch; c++)
+      for(int l = 0; l < kWave; l++) meta[(size_t)c * kWave + l].w = team > 1 ? need[s][c] : 0;
     WaveSweep &W = wave[s];
     W.G = S.G;
     W.wg = S.wg;
     W.nch = (int)nsteps;
     W.team = team;
-    std::vector<int> nd((size_t)nalloc, 0);
-    std::copy(need[s].begin(), need[s].end(), nd.begin());
-    GDP_TRY(dalloc(&W.need, (size_t)nalloc));
-    GDP_TRY(h2d(W.need, nd.data(), (size_t)nalloc));
     GDP_TRY(dalloc(&W.col, ne));
     GDP_TRY(dalloc(&W.kpos, ne));
     GDP_TRY(dalloc(&W.row, nl));
     GDP_TRY(dalloc(&W.aux, nl));
-    GDP_TRY(dalloc(&W.slot, nl));
+    GDP_TRY(dalloc(&W.meta, nl));
     GDP_TRY(dalloc(&W.pre64, nl));
     GDP_TRY(dalloc(&W.pre32, nl));
     GDP_TRY(h2d(W.col, col.data(), ne));
     GDP_TRY(h2d(W.kpos, kpos.data(), ne));
     GDP_TRY(h2d(W.row, row.data(), nl));
     GDP_TRY(h2d(W.aux, aux.data(), nl));
-    GDP_TRY(h2d(W.slot, slot.data(), nl));
+    GDP_TRY(h2d(W.meta, meta.data(), nl));
   }
   wave_mode = true;
   return GDP_OK;
 }
 
 // Chunks stored per stream: the walk's chunks and one prefetch round per wave.
-long IluFactor::wave_chunks_alloc(const WaveSweep &W) const
-{
-  return (long)W.nch + (long)W.team * wave_depth(W.G, W.wg);
-}
+long IluFactor::wave_chunks_alloc(const WaveSweep &W) const { return (long)W.nch + (long)W.team * kWalkPad; }
 
 // Storage for the packed factor entries, in the factors' precision.
 int IluFactor::wave_alloc_values()
@@ -846,18 +867,14 @@ int IluFactor::wave_alloc_values()
   if(!wave_mode) return GDP_OK;
   for(int s = 0; s < 2; s++) {
     WaveSweep &W = wave[s];
-    const size_t nalloc = (size_t)wave_chunks_alloc(W), ne = nalloc * W.wg * kWave, nl = nalloc * kWave;
-    dfree(W.val32);
+    const size_t nalloc = (size_t)wave_chunks_alloc(W), nl = nalloc * kWave;
+    dfree(W.ent);
     dfree(W.dinv32);
-    dfree(W.val64);
     dfree(W.dinv64);
-    if(is_f64) {
-      GDP_TRY(dalloc(&W.val64, ne));
-      if(s == 1) GDP_TRY(dalloc(&W.dinv64, nl));
-    }
-    else {
-      GDP_TRY(dalloc(&W.val32, ne));
-      if(s == 1) GDP_TRY(dalloc(&W.dinv32, nl));
+    GDP_TRY(dalloc(&W.ent, nl * walk_nl(is_f64, W.wg) * 4));  // 16-byte records
+    if(s == 1) {
+      if(is_f64) GDP_TRY(dalloc(&W.dinv64, nl));
+      else GDP_TRY(dalloc(&W.dinv32, nl));
     }
   }
   return GDP_OK;
@@ -873,11 +890,11 @@ void IluFactor::wave_pack()
     const long nalloc = wave_chunks_alloc(W), ne = nalloc * W.wg * kWave, nl = nalloc * kWave;
     const dim3 ge((unsigned)((ne + 255) / 256)), gl((unsigned)((nl + 255) / 256));
     if(is_f64) {
-      hipLaunchKernelGGL(k_wave_pack<double>, ge, dim3(256), 0, st, ne, W.kpos, lu64, W.val64);
+      hipLaunchKernelGGL(k_wave_pack_ent<double>, ge, dim3(256), 0, st, ne, W.wg, W.col, W.kpos, lu64, W.ent);
       if(s == 1) hipLaunchKernelGGL(k_wave_pack<double>, gl, dim3(256), 0, st, nl, W.aux, lu64, W.dinv64);
     }
     else {
-      hipLaunchKernelGGL(k_wave_pack<float>, ge, dim3(256), 0, st, ne, W.kpos, lu32, W.val32);
+      hipLaunchKernelGGL(k_wave_pack_ent<float>, ge, dim3(256), 0, st, ne, W.wg, W.col, W.kpos, lu32, W.ent);
       if(s == 1) hipLaunchKernelGGL(k_wave_pack<float>, gl, dim3(256), 0, st, nl, W.aux, lu32, W.dinv32);
     }
   }
@@ -1125,9 +1142,6 @@ void sweep(int G, int n, const int *ia, const int *ja, const int *diag, const TF
 #undef GDP_SWEEP
 }
 
-template <typename TF> const TF *wave_val(const IluFactor::WaveSweep &W);
-template <> const float *wave_val<float>(const IluFactor::WaveSweep &W) { return W.val32; }
-template <> const double *wave_val<double>(const IluFactor::WaveSweep &W) { return W.val64; }
 template <typename TF> const TF *wave_dinv(const IluFactor::WaveSweep &W);
 template <> const float *wave_dinv<float>(const IluFactor::WaveSweep &W) { return W.dinv32; }
 template <> const double *wave_dinv<double>(const IluFactor::WaveSweep &W) { return W.dinv64; }
@@ -1144,20 +1158,18 @@ void wave_apply(int n, const IluFactor::WaveSweep *W, const TV *r, TV scale, TV 
 {
   hipStream_t s = ctx().stream;
   const IluFactor::WaveSweep &F = W[0], &B = W[1];
-  const long nl = ((long)F.nch + (long)F.team * wave_depth(F.G, F.wg)) * kWave;
+  const long nl = ((long)F.nch + (long)F.team * kWalkPad) * kWave;
   hipLaunchKernelGGL(k_wave_gather<TV>, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, s, nl, F.row, r, scale,
                      wave_pre<TV>(F), gate, gmode);
 #define GDP_WAVE(GG, WW, FWD, S, OUT, ADD)                                                                \
-  hipLaunchKernelGGL((k_wave_sweep<TF, TV, GG, WW, FWD>), dim3(1), dim3(kWave * S.team), 0, s, S.nch, S.col, \
-                     wave_val<TF>(S), S.row, S.aux, S.slot, S.need, wave_dinv<TF>(S), wave_pre<TV>(S), OUT, err, tl, \
-                     gate, gmode)
+  hipLaunchKernelGGL((k_wave_sweep<TF, TV, WW, FWD>), dim3(1), dim3(kWave * S.team), 0, s, S.nch, \
+                     (const uint4 *)S.ent, S.meta, wave_dinv<TF>(S), wave_pre<TV>(S), OUT, err, tl, gate, gmode)
 #define GDP_WAVE_SHAPE(FWD, S, OUT, ADD)                                        \
-  if(S.G == 1 && S.wg == 4) GDP_WAVE(1, 4, FWD, S, OUT, ADD);                   \
-  else if(S.G == 1 && S.wg == 8) GDP_WAVE(1, 8, FWD, S, OUT, ADD);              \
-  else if(S.G == 1 && S.wg == 16) GDP_WAVE(1, 16, FWD, S, OUT, ADD);            \
-  else if(S.G == 1 && S.wg == 24) GDP_WAVE(1, 24, FWD, S, OUT, ADD);            \
-  else if(S.G == 1) GDP_WAVE(1, 32, FWD, S, OUT, ADD);                          \
-  else GDP_WAVE(8, 8, FWD, S, OUT, ADD)
+  if(S.wg == 4) GDP_WAVE(1, 4, FWD, S, OUT, ADD);                               \
+  else if(S.wg == 8) GDP_WAVE(1, 8, FWD, S, OUT, ADD);                          \
+  else if(S.wg == 16) GDP_WAVE(1, 16, FWD, S, OUT, ADD);                        \
+  else if(S.wg == 24) GDP_WAVE(1, 24, FWD, S, OUT, ADD);                        \
+  else GDP_WAVE(1, 32, FWD, S, OUT, ADD)
   GDP_WAVE_SHAPE(true, F, wave_pre<TV>(B), (TV *)nullptr);
   GDP_WAVE_SHAPE(false, B, x, addto);
 #undef GDP_WAVE_SHAPE

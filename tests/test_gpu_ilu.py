@@ -39,6 +39,15 @@ def _device_apply(gpu, ia, ja, a, v, f64, reps=1):
 SWEEP_KIND = {"free": 1, "wave": 2}
 
 
+def _longest_side(ia, ja):
+    """Most strictly-lower or strictly-upper entries in one row."""
+    n = ia.shape[0] - 1
+    rows = np.repeat(np.arange(n), np.diff(ia))
+    lower = np.bincount(rows[ja < rows], minlength=n)
+    upper = np.bincount(rows[ja > rows], minlength=n)
+    return int(max(lower.max(), upper.max()))
+
+
 @pytest.mark.parametrize("sweep", ["free", "wave", "wave-solo", "wave-team2"])
 @pytest.mark.parametrize("f64", [False, True])
 @pytest.mark.parametrize("name", list(SYSTEMS))
@@ -46,7 +55,7 @@ def test_ilu_apply_bit_exact(gpu, oracle, name, f64, sweep, monkeypatch):
     """Both sweep kinds: the sync-free sweeps and the level walk in one
     workgroup (solution entries in an LDS ring of 16384 rows; the 192^2
     system has 36,481 rows, so the ring wraps), walked by a team of waves
-    (the default 8, and 2) or by one wave."""
+    (the default 4, and 2) or by one wave."""
     monkeypatch.setenv("GDP_ILU_SWEEP", sweep.split("-")[0])
     if sweep == "wave-solo":
         monkeypatch.setenv("GDP_ILU_TEAM", "1")
@@ -59,6 +68,13 @@ def test_ilu_apply_bit_exact(gpu, oracle, name, f64, sweep, monkeypatch):
     assert ierr == 0
     v = np.random.default_rng(5).standard_normal(ib.shape[0] - 1)
     want = oracle.lusol(alu, jlu, ju, v)
+    if sweep.startswith("wave") and _longest_side(ib, jb) > 32:
+        # the level walk takes rows of up to 32 entries a side; forced on
+        # longer ones it refuses instead of falling back
+        from getdp_amd._lib import GdpError
+        with pytest.raises(GdpError, match="entries on one side"):
+            _device_apply(gpu, ia, ja, a, v, f64)
+        return
     got, kind = _device_apply(gpu, ia, ja, a, v, f64, reps=3)  # repeated: sweep state reset each time
     assert kind == SWEEP_KIND[sweep.split("-")[0]], kind  # the forced kernel really ran
     assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), np.abs(got - want).max()
