@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 
 #include "gdp_device.h"
@@ -54,6 +55,7 @@ struct BatchArgs {
   // positions); the others read the sweep vector from global memory
   const int *ring;
   int rmask;
+  int walk;  // sweeps of instances with the sweep vector in LDS walked by one wave (else level by level)
   const long *pool_off;
   double *pool;
   const double *rhs;
@@ -118,6 +120,50 @@ __device__ inline void ld_rec(const BatchArgs &a, int slot, int s_first, const d
   r.pre = src[r.row];
 }
 
+// The same record with its right-hand side from P, the sweep's inputs staged
+// in slot order (no gather behind the load of the row).
+template <int W, bool FWD>
+__device__ inline void ld_rec_staged(const BatchArgs &a, int slot, int s_first, const double *P, SweepRec<W> &r)
+{
+  const int *rowp = FWD ? a.f_row : a.b_row, *cntp = FWD ? a.f_cnt : a.b_cnt;
+  const int *colp = FWD ? a.f_col : a.b_col;
+  const float *valp = FWD ? a.f_val : a.b_val;
+  r.row = rowp[slot];
+  r.cnt = cntp[slot];
+  r.pos = slot - s_first;
+#pragma unroll
+  for(int w = 0; w < W; w++) {
+    r.col[w] = colp[(long)slot * W + w];
+    r.val[w] = valp[(long)slot * W + w];
+  }
+  r.dinv = FWD ? 1.f : a.b_dinv[slot];
+  r.pre = P[slot - s_first];
+}
+
+// P[q] = src[row of slot q] over an instance's slots of one sweep (all threads)
+template <int T, bool FWD>
+__device__ inline void stage_inputs(const BatchArgs &a, int inst, const double *src, double *P)
+{
+  const int *ptr = FWD ? a.fl_ptr : a.bl_ptr;
+  const int base = FWD ? a.fl_base[inst] : a.bl_base[inst];
+  const int nl = FWD ? a.fl_n[inst] : a.bl_n[inst];
+  if(nl <= 0) return;
+  const int *rowp = FWD ? a.f_row : a.b_row;
+  const int sf = ptr[base], se = ptr[base + nl];
+  // eight slots a thread at a time: their row loads, then their gathers
+  for(int q0 = sf + (int)threadIdx.x; q0 < se; q0 += 8 * T) {
+    int row[8];
+#pragma unroll
+    for(int u = 0; u < 8; u++) row[u] = q0 + u * T < se ? rowp[q0 + u * T] : -1;
+    double v[8];
+#pragma unroll
+    for(int u = 0; u < 8; u++) v[u] = row[u] >= 0 ? src[row[u]] : 0.;
+#pragma unroll
+    for(int u = 0; u < 8; u++)
+      if(row[u] >= 0) P[q0 + u * T - sf] = v[u];
+  }
+}
+
 // lusol (ilut.F:2297-2347) on one row, same operation order:
 // forward x(i) = y(i) - sum alu(k) x(jlu(k)); backward x(i) = alu(i) (x(i) - sum ...)
 // RING: the solution entries come from the LDS ring (column = stream
@@ -125,10 +171,16 @@ __device__ inline void ld_rec(const BatchArgs &a, int slot, int s_first, const d
 template <int W, bool FWD, bool RING>
 __device__ inline void apply_rec(const SweepRec<W> &r, double *z, double *ring, int rmask)
 {
+  // every dependency is read before the first subtraction (padding columns
+  // are 0, in range, and their values are not used): one read latency per
+  // row instead of one per entry
+  double dv[W];
+#pragma unroll
+  for(int w = 0; w < W; w++) dv[w] = RING ? ring[r.col[w] & rmask] : z[r.col[w]];
   double acc = r.pre;
 #pragma unroll
   for(int w = 0; w < W; w++)
-    if(w < r.cnt) acc = acc - (double)r.val[w] * (RING ? ring[r.col[w] & rmask] : z[r.col[w]]);
+    if(w < r.cnt) acc = acc - (double)r.val[w] * dv[w];
   const double x = FWD ? acc : (double)r.dinv * acc;
   z[r.row] = x;
   if constexpr(RING) ring[r.pos & rmask] = x;
@@ -206,11 +258,109 @@ __device__ inline void sweep(const BatchArgs &a, int inst, const double *src, do
   }
 }
 
+// One triangular sweep walked by one wave (the caller's wave 0; the other
+// waves wait at the caller's barrier).  With the sweep vector in global
+// memory (instances that read their dependencies from the LDS ring), the
+// inputs are first staged in slot order (STAGED), so that a record's
+// right-hand side is not a gather behind the load of its row: three windows
+// ahead then cover one memory round trip instead of two.  The wave takes the instance's slots 64
+// at a time in stream (level) order; inside such a window it solves the rows
+// of one level after the other, so a row that depends on a row of the same
+// window finds it solved: the wave's LDS accesses take effect in program
+// order, and every dependency is in LDS (the sweep vector, or the ring).  No
+// barrier separates the levels; the records of the next D windows are in
+// flight while a window is solved.  Same operations per row as apply_rec.
+template <int W, bool FWD, bool RING, bool STAGED = false>
+__device__ inline void walk_sweep(const BatchArgs &a, int inst, const double *src, double *z, double *ring)
+{
+  auto load = [&](int q, int sf, SweepRec<W> &r) {
+    if constexpr(STAGED) ld_rec_staged<W, FWD>(a, q, sf, src, r);
+    else ld_rec<W, FWD>(a, q, sf, src, r);
+  };
+  constexpr int D = 3;
+  const int *ptr = FWD ? a.fl_ptr : a.bl_ptr;
+  const int base = FWD ? a.fl_base[inst] : a.bl_base[inst];
+  const int nl = FWD ? a.fl_n[inst] : a.bl_n[inst];
+  if(nl <= 0) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int sf = ptr[base], se = ptr[base + nl];
+  SweepRec<W> buf[D];
+  bool has[D];
+  // loads are unconditional (a lane past the end reloads the last slot):
+  // a load under a branch made the compiler wait for every load in flight
+  // where the branches join, once per window
+#pragma unroll
+  for(int d = 0; d < D; d++) {
+    const int q = sf + kWave * d + lane;
+    has[d] = q < se;
+    load(has[d] ? q : se - 1, sf, buf[d]);
+  }
+  // Level ends: lane j holds the end slot of level lb + j, and of level
+  // lb + 64 + j in the next block, loaded half a block ahead; a level's end
+  // is read from its lane (no memory access between two levels).
+  int l = 0, lb = 0;
+  int lcur = lane < nl ? ptr[base + 1 + lane] : 0x7fffffff;
+  int lnext = lane + kWave < nl ? ptr[base + 1 + kWave + lane] : 0x7fffffff;
+  auto level_end = [&](int lv) {
+    if(lv - lb >= kWave) {
+      lb += kWave;
+      lcur = lnext;
+      lnext = lb + kWave + lane < nl ? ptr[base + 1 + lb + kWave + lane] : 0x7fffffff;
+    }
+    return __builtin_amdgcn_readlane(lcur, lv - lb);
+  };
+  int le = level_end(0);  // the current level's end slot
+  for(int w0 = sf; w0 < se; w0 += kWave * D) {
+#pragma unroll
+    for(int d = 0; d < D; d++) {
+      const int w = w0 + kWave * d;
+      if(w >= se) break;
+      const SweepRec<W> cur = buf[d];
+      bool done = !has[d];
+      {
+        const int q = w + kWave * D + lane;  // this buffer's next window
+        has[d] = q < se;
+        load(has[d] ? q : se - 1, sf, buf[d]);
+      }
+      const int wend = min(w + kWave, se), slot = w + lane;
+      while(le <= w) le = level_end(++l);
+      for(;;) {
+        if(!done && slot < le) {
+          apply_rec<W, FWD, RING>(cur, z, ring, a.rmask);
+          done = true;
+        }
+        if(le >= wend) break;
+        le = level_end(++l);
+      }
+    }
+  }
+}
+
 // z = (LU)^{-1} v (v may alias z); z in LDS (zlds) when the instance fits
 template <int T, int W>
 __device__ inline void b_ilu(const BatchArgs &a, int inst, int n, const double *v, double *z, double *zlds)
 {
-  if(n <= a.zcap) {
+  if(n <= a.zcap && a.walk) {
+    if(threadIdx.x < kWave) {
+      walk_sweep<W, true, false>(a, inst, v, zlds, nullptr);
+      walk_sweep<W, false, false>(a, inst, zlds, zlds, nullptr);
+    }
+    __syncthreads();
+  }
+  else if(a.ring[inst] && a.walk) {
+    // inputs staged in slot order in the instance's spare workspace vector;
+    // the forward results reach the backward inputs through z
+    double *P = a.pool + a.pool_off[inst] + (size_t)(a.m + 2) * n;
+    stage_inputs<T, true>(a, inst, v, P);
+    __syncthreads();
+    if(threadIdx.x < kWave) walk_sweep<W, true, true, true>(a, inst, P, z, zlds);
+    __syncthreads();
+    stage_inputs<T, false>(a, inst, z, P);
+    __syncthreads();
+    if(threadIdx.x < kWave) walk_sweep<W, false, true, true>(a, inst, P, z, zlds);
+    __syncthreads();
+  }
+  else if(n <= a.zcap) {
     sweep<T, W, true, true>(a, inst, v, zlds);
     sweep<T, W, false, true>(a, inst, zlds, zlds);
     __syncthreads();  // the consumers of z also read global memory
@@ -728,14 +878,15 @@ int batch_prepare(gdp_batch *B, const gdp_matrix *A)
   GDP_TRY(dalloc(&B->d_f_val, std::max<size_t>(1, fk.size())));
   GDP_TRY(dalloc(&B->d_b_val, std::max<size_t>(1, bk.size())));
   GDP_TRY(dalloc(&B->d_b_dinv, std::max<size_t>(1, bdk.size())));
-  // Krylov workspace: (m + 2) vectors per instance
+  // Krylov workspace: (m + 3) vectors per instance (basis, preconditioned
+  // vector, the sweeps' staged inputs)
   const int m = std::max(2, std::min(B->p.Krylov_Size <= 1 ? 15 : B->p.Krylov_Size, kBM));
   B->p.Krylov_Size = m;
   std::vector<long> po(B->ninst);
   long tot = 0;
   for(int i = 0; i < B->ninst; i++) {
     po[i] = tot;
-    tot += (long)(m + 2) * (B->off[i + 1] - B->off[i]);
+    tot += (long)(m + 3) * (B->off[i + 1] - B->off[i]);
   }
   GDP_TRY(dalloc(&B->d_pool_off, (size_t)B->ninst));
   GDP_TRY(h2d(B->d_pool_off, po.data(), po.size()));
@@ -849,6 +1000,9 @@ int batch_enqueue(gdp_batch *B, gdp_matrix *A, gdp_vector *b, gdp_vector *x)
     a.zcap = zcap_rows(g.threads);
     a.ring = B->d_ring;
     a.rmask = ring_rows(g.threads) - 1;
+    // GDP_BATCH_WALK=0: the level-by-level sweeps for every instance
+    const char *ew = std::getenv("GDP_BATCH_WALK");
+    a.walk = ew ? atoi(ew) != 0 : 1;
     a.pool_off = B->d_pool_off;
     a.pool = B->pool;
     a.rhs = B->rhs;

@@ -106,15 +106,18 @@ def test_batch_rejects_unsupported_parameters(gpu, probs):
     assert rc == -4  # GDP_E_UNSUPPORTED
 
 
+@pytest.mark.parametrize("walk", ["1", "0"])
 @pytest.mark.parametrize("threads", [256, 512, 1024])
-def test_large_instances_lds_ring(gpu, oracle, threads):
+def test_large_instances_lds_ring(gpu, oracle, threads, walk, monkeypatch):
     """Instances larger than the LDS sweep vector (4096, 8192, 18000 rows at
     256, 512, 1024 threads) sweep through the LDS ring of recent results: the
     solutions match the oracle's, with the reference's iteration counts (same
-    ordering and factors), next to small instances on the LDS path."""
+    ordering and factors), next to small instances on the LDS path; sweeps
+    walked by one wave (the default) or level by level (GDP_BATCH_WALK=0)."""
+    monkeypatch.setenv("GDP_BATCH_WALK", walk)
     sizes = [70, 150, 12, 96, 135]  # 71^2 = 5041 ... 151^2 = 22801 rows
     probs = [P.poisson2d(n, seed=70 + i, nblocks=8, field_kind="checker") for i, n in enumerate(sizes)]
-    print(f"case batch ring threads {threads}", flush=True)
+    print(f"case batch ring threads {threads} walk {walk}", flush=True)
     bt = BT.Batch(probs, config=[(threads, 0, 0)] * len(probs))
     bt.run()
     inf = bt.info()
@@ -127,3 +130,21 @@ def test_large_instances_lds_ring(gpu, oracle, threads):
         err = np.abs(bt.solution(i) - xo).max() / np.abs(xo).max()
         assert err <= 1e-9, (i, err)
         assert abs(int(inf["its"][i]) - io["its"]) <= max(3, io["its"] // 10), (i, inf["its"][i], io["its"])
+
+
+def test_walked_and_level_sweeps_identical(gpu, monkeypatch):
+    """The one-wave walk and the level-by-level sweeps apply the factors with
+    the same operations per row: the batch gives the same iteration counts and
+    the same solution bits either way, for instances on the LDS path and on
+    the LDS-ring path."""
+    sizes = [40, 140, 20, 100]
+    probs = [P.poisson2d(n, seed=90 + i, nblocks=8, field_kind="checker") for i, n in enumerate(sizes)]
+    out = {}
+    for walk in ("1", "0"):
+        monkeypatch.setenv("GDP_BATCH_WALK", walk)
+        bt = BT.Batch(probs, config=[(1024, 0, 0)] * len(probs))
+        bt.run()
+        out[walk] = (bt.info()["its"].copy(), bt.x.get().copy())
+        bt.close()
+    assert np.array_equal(out["1"][0], out["0"][0])
+    assert np.array_equal(out["1"][1].view(np.uint64), out["0"][1].view(np.uint64))
