@@ -30,14 +30,13 @@ struct IluFactor {
   float *pbuf32 = nullptr;
   // One-wave sweeps, used when the dependency chain is long and the levels
   // are narrow (threshold ILU with fill): each sweep's rows in level order,
-  // in chunks of up to 64 rows of one level, one lane per row.  Entry slot w
-  // of lane l in chunk c sits at (c * wg + w) * 64 + l in col/kpos; per-lane
+  // in chunks of up to 64 rows of one level, one lane per row; per-lane
   // fields at c * 64 + l.
   struct WaveSweep {
     int G = 1, wg = 0, nch = 0;  // one lane per row; wg entry slots per row
     int team = 1;         // waves sharing the walk (one workgroup)
-    int *col = nullptr;   // LDS ring slot of each entry slot's column (the zero slot: padding)
-    int *kpos = nullptr;  // position of the entry in the factors (-1: padding)
+    int *rk0 = nullptr;   // per lane: position of the row's first entry on this side in the factors
+    int *rcnt = nullptr;  // per lane: its number of entries (0: no row)
     int *row = nullptr;   // row of the lane (-1: none)
     int *aux = nullptr;   // forward: the row's position in the backward stream; backward: its pivot's position
     int4 *meta = nullptr; // per lane: row, forward output position, ring slot of the result, the chunk's need
@@ -48,6 +47,7 @@ struct IluFactor {
     float *pre32 = nullptr;
   } wave[2];  // [0] forward (L), [1] backward (U)
   bool wave_mode = false;
+  int *wave_ecol = nullptr;  // per factor entry: the LDS ring slot of its dependency (level walks)
   // which sweep kernels apply the factors (gdp_solve_info.ilu_sweep)
   int sweep_kind() const { return wave_mode ? GDP_SWEEP_WAVE : GDP_SWEEP_FREE; }
 

@@ -379,7 +379,7 @@ constexpr int kNumWaveShapes = 5;
 // Chunk records.  Entry slots are stored as 16-byte records, lane-interleaved
 // (record j of lane l in chunk c at (c * NL + j) * 64 + l): two (column,
 // value) pairs per record with FP32 factors, one (column, -, value) with FP64
-// ones, so a chunk is NL = slots / 2 (or slots) coalesced 16-byte loads per
+// ones, the column as a byte offset into an FP64 ring (no shift in the walk), so a chunk is NL = slots / 2 (or slots) coalesced 16-byte loads per
 // lane, plus one 16-byte load of the row's fields (row, forward output
 // position, ring slot, need), the right-hand side and, backward, the
 // inverted pivot.  A wave keeps NB chunks in registers and rotates through
@@ -476,7 +476,8 @@ __global__ void __launch_bounds__(kWave *kMaxTeam) k_wave_sweep(int nch, const u
     // full LDS round trip on the row's chain
     TV rv[WG], prod[WG];
 #pragma unroll
-    for(int w = 0; w < WG; w++) rv[w] = ring[col(rc, w)];
+    for(int w = 0; w < WG; w++)
+      rv[w] = sizeof(TV) == 8 ? *(const TV *)((const unsigned char *)ring + col(rc, w)) : ring[col(rc, w) >> 3];
     __builtin_amdgcn_sched_group_barrier(0x100, WG, 0);      // the DS reads first
     __builtin_amdgcn_sched_group_barrier(0x002, 4 * WG, 0);  // then the arithmetic
 #pragma unroll
@@ -550,25 +551,28 @@ __global__ void __launch_bounds__(256) k_wave_pack(long cnt, const int *__restri
 // chunk c at ((c * wg + w) * 64 + l)): column and value bits, after each
 // factorisation (enqueued: usable under capture).
 template <typename TF>
-__global__ void __launch_bounds__(256) k_wave_pack_ent(long cnt, int wg, const int *__restrict__ col,
-                                                       const int *__restrict__ kpos, const TF *__restrict__ lu,
-                                                       unsigned *__restrict__ ent)
+__global__ void __launch_bounds__(256) k_wave_pack_ent(long cnt, int wg, const int *__restrict__ rk0,
+                                                       const int *__restrict__ rcnt, const int *__restrict__ ecol,
+                                                       const TF *__restrict__ lu, unsigned *__restrict__ ent)
 {
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   if(p >= cnt) return;
   const long c = p / ((long)wg * kWave);
   const int w = (int)(p / kWave % wg), l = (int)(p % kWave);
-  const int k = kpos[p];
-  const TF v = k >= 0 ? lu[k] : (TF)0;
+  const size_t q = (size_t)c * kWave + l;
+  const bool used = w < rcnt[q];  // padding: the zero ring slot and a zero value
+  const int k = used ? rk0[q] + w : -1;
+  const TF v = used ? lu[k] : (TF)0;
+  const unsigned off = (unsigned)(used ? ecol[k] : kRing) * 8u;  // byte offset into an FP64 ring
   if constexpr(sizeof(TF) == 4) {
     const size_t word = (((size_t)c * (wg / 2) + w / 2) * kWave + l) * 4 + (w & 1) * 2;
-    ent[word] = (unsigned)col[p];
+    ent[word] = off;
     ent[word + 1] = __float_as_uint(v);
   }
   else {
     const size_t word = (((size_t)c * wg + w) * kWave + l) * 4;
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    ent[word] = (unsigned)col[p];
+    ent[word] = off;
     ent[word + 1] = 0;
     ent[word + 2] = (unsigned)b;
     ent[word + 3] = (unsigned)(b >> 32);
@@ -639,8 +643,8 @@ void IluFactor::release()
 void IluFactor::wave_release()
 {
   for(auto &w : wave) {
-    dfree(w.col);
-    dfree(w.kpos);
+    dfree(w.rk0);
+    dfree(w.rcnt);
     dfree(w.row);
     dfree(w.aux);
     dfree(w.meta);
@@ -651,6 +655,7 @@ void IluFactor::wave_release()
     dfree(w.pre32);
     w = WaveSweep();
   }
+  dfree(wave_ecol);
   wave_mode = false;
 }
 
@@ -744,14 +749,12 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
       }
     }
   }
-  // waves per walk: one when a level is about one chunk (the chunks depend on
-  // each other in turn, and a team only adds its hand-over), four when levels
-  // span several chunks that the team takes in parallel; GDP_ILU_TEAM
-  // overrides.  Measured per application (1, 2, 4 waves): ILUTP(20) of
-  // Poisson 512^2 (1.0 chunk a level) 18.6, 20.6, 20.8 ms; ILU(0) of Poisson
-  // 256^2 in RCMK order (2 chunks a level) 0.49, 0.50, 0.43 ms.
-  const double per_level = (double)(sd[0].cstart.size() + sd[1].cstart.size()) / std::max(1, nlevels + nul);
-  int team = per_level <= 1.25 ? 1 : 4;
+  // waves per walk: four (one per SIMD), which share the instruction issue of
+  // the chunks' reads and products; GDP_ILU_TEAM overrides.  Measured per
+  // application (1 and 4 waves): ILUTP(20) of Poisson 512^2 (one chunk a
+  // level) 18.3 and 17.5 ms, 256^2 10.0 and 9.5 ms; ILU(0) of Poisson 256^2
+  // in RCMK order (two chunks a level) 0.45 and 0.40 ms.
+  int team = 4;
   if(const char *e = std::getenv("GDP_ILU_TEAM")) team = std::max(1, std::min(kMaxTeam, atoi(e)));
   // Estimated time per application.  A chunk of the walk costs its reads
   // and its row chain (about 0.25 us with 4 entry slots a row, 0.46 us with
@@ -808,6 +811,7 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
   std::vector<int> bpos(n);
   for(size_t c = 0; c < sd[1].cstart.size(); c++)
     for(int t = 0; t < sd[1].ccount[c]; t++) bpos[urows[sd[1].cstart[c] + t]] = (int)c * kWave + t * sd[1].G;
+  std::vector<int> ecol(std::max<long>(1, (long)hia[n]), kRing);
   for(int s = 0; s < 2; s++) {
     const Side &S = sd[s];
     const long nch = (long)S.cstart.size();
@@ -816,7 +820,7 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
     const long span = (long)team * kWalkPad;
     const long nsteps = (nch + span - 1) / span * span, nalloc = nsteps + span;
     const size_t ne = (size_t)nalloc * S.wg * kWave, nl = (size_t)nalloc * kWave;
-    std::vector<int> col(ne, kRing), kpos(ne, -1), row(nl, -1), aux(nl, -1);
+    std::vector<int> rk0(nl, 0), rcnt(nl, 0), row(nl, -1), aux(nl, -1);
     std::vector<int4> meta(nl, make_int4(-1, -1, kRing, 0));
     const std::vector<int> &ord = *rows[s];
     for(long c = 0; c < nch; c++)
@@ -824,12 +828,11 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
         const int i = ord[S.cstart[c] + t], gl = t * S.G;
         int b, e;
         range(s, i, b, e);
-        for(int k = b; k < e; k++) {
-          const int x = k - b, lane = gl + x % S.G, w = x / S.G;
-          const size_t idx = ((size_t)c * S.wg + w) * kWave + lane;
-          col[idx] = S.spos[hja[k]] & (kRing - 1);
-          kpos[idx] = k;
-        }
+        // the row's entries are b .. e - 1 of the factors, and entry k's
+        // dependency sits in ring slot ecol[k]
+        for(int k = b; k < e; k++) ecol[k] = S.spos[hja[k]] & (kRing - 1);
+        rk0[(size_t)c * kWave + gl] = b;
+        rcnt[(size_t)c * kWave + gl] = e - b;
         row[(size_t)c * kWave + gl] = i;
         aux[(size_t)c * kWave + gl] = s == 0 ? bpos[i] : dpos[i];
         meta[(size_t)c * kWave + gl] = make_int4(i, s == 0 ? bpos[i] : -1, S.spos[i] & (kRing - 1), 0);
@@ -841,19 +844,21 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
     W.wg = S.wg;
     W.nch = (int)nsteps;
     W.team = team;
-    GDP_TRY(dalloc(&W.col, ne));
-    GDP_TRY(dalloc(&W.kpos, ne));
+    GDP_TRY(dalloc(&W.rk0, nl));
+    GDP_TRY(dalloc(&W.rcnt, nl));
     GDP_TRY(dalloc(&W.row, nl));
     GDP_TRY(dalloc(&W.aux, nl));
     GDP_TRY(dalloc(&W.meta, nl));
     GDP_TRY(dalloc(&W.pre64, nl));
     GDP_TRY(dalloc(&W.pre32, nl));
-    GDP_TRY(h2d(W.col, col.data(), ne));
-    GDP_TRY(h2d(W.kpos, kpos.data(), ne));
+    GDP_TRY(h2d(W.rk0, rk0.data(), nl));
+    GDP_TRY(h2d(W.rcnt, rcnt.data(), nl));
     GDP_TRY(h2d(W.row, row.data(), nl));
     GDP_TRY(h2d(W.aux, aux.data(), nl));
     GDP_TRY(h2d(W.meta, meta.data(), nl));
   }
+  GDP_TRY(dalloc(&wave_ecol, ecol.size()));
+  GDP_TRY(h2d(wave_ecol, ecol.data(), ecol.size()));
   wave_mode = true;
   return GDP_OK;
 }
@@ -890,11 +895,13 @@ void IluFactor::wave_pack()
     const long nalloc = wave_chunks_alloc(W), ne = nalloc * W.wg * kWave, nl = nalloc * kWave;
     const dim3 ge((unsigned)((ne + 255) / 256)), gl((unsigned)((nl + 255) / 256));
     if(is_f64) {
-      hipLaunchKernelGGL(k_wave_pack_ent<double>, ge, dim3(256), 0, st, ne, W.wg, W.col, W.kpos, lu64, W.ent);
+      hipLaunchKernelGGL(k_wave_pack_ent<double>, ge, dim3(256), 0, st, ne, W.wg, W.rk0, W.rcnt, wave_ecol, lu64,
+                         W.ent);
       if(s == 1) hipLaunchKernelGGL(k_wave_pack<double>, gl, dim3(256), 0, st, nl, W.aux, lu64, W.dinv64);
     }
     else {
-      hipLaunchKernelGGL(k_wave_pack_ent<float>, ge, dim3(256), 0, st, ne, W.wg, W.col, W.kpos, lu32, W.ent);
+      hipLaunchKernelGGL(k_wave_pack_ent<float>, ge, dim3(256), 0, st, ne, W.wg, W.rk0, W.rcnt, wave_ecol, lu32,
+                         W.ent);
       if(s == 1) hipLaunchKernelGGL(k_wave_pack<float>, gl, dim3(256), 0, st, nl, W.aux, lu32, W.dinv32);
     }
   }
