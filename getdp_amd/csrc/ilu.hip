@@ -250,14 +250,26 @@ __device__ inline void sweep_rows(int base, int n, const int *__restrict__ ia, c
     // resets it for the next forward sweep
     if(!FWD && valid && q == 0) st_val<TV, SCOPE>(y + i, as_val<TV>(Sentinel<TV>::bits));
     // a lane holds positions cb + q + G*c, c < C, of the row's current span of
-    // S entries; their columns and factor entries are loaded when the span
-    // starts, off the dependency chain of the polls
+    // S entries; their columns and factor entries are loaded a span ahead
+    // (while the current span's dependencies are polled), so that moving on
+    // to the next span does not wait for a load from memory: with factors
+    // larger than the Infinity Cache that was an HBM latency on the chain
     const TV ldi = (!FWD && valid) ? (TV)lu[di] : (TV)0;
-    int jk[C];
-    TV lk[C], prod[C];
+    int jk[C], jn[C];
+    TV lk[C], ln[C], prod[C];
     unsigned pend = 0;  // bit c: position c still waits for its dependency
-    auto fetch = [&]() {
-      pend = 0;
+    auto load_next = [&]() {
+#pragma unroll
+      for(int c = 0; c < C; c++) {
+        const int k = cb + S + q + G * c;
+        if(k < kend) {
+          jn[c] = ja[k];
+          ln[c] = (TV)lu[k];
+        }
+      }
+    };
+    // first span: loaded in place; later spans: taken from the prefetch
+    if(valid) {
 #pragma unroll
       for(int c = 0; c < C; c++) {
         const int k = cb + q + G * c;
@@ -268,8 +280,22 @@ __device__ inline void sweep_rows(int base, int n, const int *__restrict__ ia, c
           pend |= 1u << c;
         }
       }
+      load_next();
+    }
+    auto fetch = [&]() {
+      pend = 0;
+#pragma unroll
+      for(int c = 0; c < C; c++) {
+        const int k = cb + q + G * c;
+        prod[c] = 0;
+        if(k < kend) {
+          jk[c] = jn[c];
+          lk[c] = ln[c];
+          pend |= 1u << c;
+        }
+      }
+      load_next();
     };
-    if(valid) fetch();
     auto publish = [&]() {
       if(q == 0) {
         if(FWD) st_val<TV, SCOPE>(y + i, acc);
