@@ -310,6 +310,14 @@ __global__ void __launch_bounds__(256) k_geometry(int nelem, const int *__restri
 // Pass 2, one thread per CSR entry / per row: the entry's contributions
 // summed in (element, term) order, the reference's order of additions
 // (Dof_AssembleInMat called element after element, term after term).
+// A term's coefficients for region reg (1, 3 or 9 values; the rest zero).
+__device__ inline void load_coef(const DevTerm &t, int reg, double cf[9])
+{
+  const double *c = t.coef + (long)reg * t.ncoef;
+#pragma unroll
+  for(int k = 0; k < 9; k++) cf[k] = k < t.ncoef ? c[k] : 0.;
+}
+
 // BF_PerpendicularEdge terms on a triangle in the plane (2D a_z forms).
 // Curl-curl: BF_CurlPerpendicularEdge, the reference-coordinate node gradient
 // turned to (dv, -du, dw) (BF_Perpendicular.cpp:70-78), through
@@ -372,63 +380,63 @@ __device__ double *elem_perp(const DevTerm &t, const double *cf, const double *G
   return out + 64 * 3;
 }
 
-__global__ void __launch_bounds__(128) k_elem_nodal(DevPlan P, double *__restrict__ ek)
+// One element's values, NN = 3 (triangles) or 4 (tetrahedra) nodes: the
+// node loops are unrolled, so the gradients stay in registers with fixed
+// indices.
+template <int NN>
+__device__ inline void elem_nodal(const DevPlan &P, int e, int gk, double *out)
 {
-  const int e = blockIdx.x * 128 + threadIdx.x;
-  if(e >= P.nelem) return;
-  const int gk = P.geokind[e];
-  if(gk == GEO_NONE) return;
+  constexpr int tri = NN == 3;
   const int reg = P.region[e];
-  const int tri = (gk != GEO_TET);
-  const int nn = tri ? 3 : 4;
   const double *G = P.geo + (long)e * kGeoStride;
   const double adet = fabs(G[9]);
-  double g[4][3];
+  double g[NN][3];
   if(gk != GEO_TRISUR)
-    for(int j = 0; j < nn; j++) {
+#pragma unroll
+    for(int j = 0; j < NN; j++) {
       double u[3];
       bf_gradnode(tri, j, u);
       form1(G + 10, u, g[j]);
     }
-  double *out = ek + P.ek_cb[e >> 6] + (e & 63);
   for(int ti = 0; ti < P.nterms; ti++) {
     const DevTerm &t = P.term[ti];
     if(!t.active[reg]) continue;
-    const double *cf = t.coef + (long)reg * t.ncoef;
+    // the region's coefficients in registers: the stores to the value array
+    // would otherwise force a reload of each coefficient for every value
+    double cf[9];
+    load_coef(t, reg, cf);
     if(t.perp) {
       out = elem_perp(t, cf, G, adet, out);
       continue;
     }
     if(t.kind == GDP_T_GRAD_GRAD) {
-      const int sym = (t.dof == t.equ);
-      for(int k = 0; k < nn; k++)
-        for(int j = 0; j < nn; j++) {
-          int ii = k, jj = j;
-          if(sym && j > k) {
-            ii = j;
-            jj = k;
-          }
+      const bool sym = (t.dof == t.equ);
+#pragma unroll
+      for(int k = 0; k < NN; k++)
+#pragma unroll
+        for(int j = 0; j < NN; j++) {
+          // symmetric terms: Ek[k][j] for j > k is Ek[j][k] (the mirror)
+          const double *gi = sym && j > k ? g[j] : g[k];
+          const double *gj = sym && j > k ? g[k] : g[j];
           double cg[3];
-          coef_times(t.ncoef, cf, g[jj], cg);
-          double pr = dot3(g[ii], cg);
+          coef_times(t.ncoef, cf, gj, cg);
+          double pr = dot3(gi, cg);
           double acc = 0.;
           for(int q = 0; q < t.npoints; q++) {
             double u, v, w, wt;
             gauss_pt(tri, t.npoints, q, u, v, w, wt);
             acc += (wt * adet) * pr;
           }
-          out[64 * (k * nn + j)] = acc;
+          out[64 * (k * NN + j)] = acc;
         }
-      out += 64 * nn * nn;
+      out += 64 * NN * NN;
     }
     else if(t.kind == GDP_T_NODE_MASS) {
-      for(int k = 0; k < nn; k++)
-        for(int j = 0; j < nn; j++) {
-          int ii = k, jj = j;
-          if(j > k) {
-            ii = j;
-            jj = k;
-          }
+#pragma unroll
+      for(int k = 0; k < NN; k++)
+#pragma unroll
+        for(int j = 0; j < NN; j++) {
+          const int ii = j > k ? j : k, jj = j > k ? k : j;
           double acc = 0.;
           for(int q = 0; q < t.npoints; q++) {
             double u, v, w, wt;
@@ -436,12 +444,13 @@ __global__ void __launch_bounds__(128) k_elem_nodal(DevPlan P, double *__restric
             double pi = bf_node(tri, ii, u, v, w), pj = bf_node(tri, jj, u, v, w);
             acc += (wt * adet) * (pi * (cf[0] * pj));
           }
-          out[64 * (k * nn + j)] = acc;
+          out[64 * (k * NN + j)] = acc;
         }
-      out += 64 * nn * nn;
+      out += 64 * NN * NN;
     }
     else if(t.kind == GDP_T_SRC_NODE) {
-      for(int k = 0; k < nn; k++) {
+#pragma unroll
+      for(int k = 0; k < NN; k++) {
         double acc = 0.;
         for(int q = 0; q < t.npoints; q++) {
           double u, v, w, wt;
@@ -450,9 +459,20 @@ __global__ void __launch_bounds__(128) k_elem_nodal(DevPlan P, double *__restric
         }
         out[64 * k] = acc;
       }
-      out += 64 * nn;
+      out += 64 * NN;
     }
   }
+}
+
+__global__ void __launch_bounds__(128) k_elem_nodal(DevPlan P, double *__restrict__ ek)
+{
+  const int e = blockIdx.x * 128 + threadIdx.x;
+  if(e >= P.nelem) return;
+  const int gk = P.geokind[e];
+  if(gk == GEO_NONE) return;
+  double *out = ek + P.ek_cb[e >> 6] + (e & 63);
+  if(gk == GEO_TET) elem_nodal<4>(P, e, gk, out);
+  else elem_nodal<3>(P, e, gk, out);
 }
 
 __global__ void __launch_bounds__(128) k_elem_edge(DevPlan P, double *__restrict__ ek)
@@ -468,7 +488,8 @@ __global__ void __launch_bounds__(128) k_elem_edge(DevPlan P, double *__restrict
   for(int ti = 0; ti < P.nterms; ti++) {
     const DevTerm &t = P.term[ti];
     if(!t.active[reg]) continue;
-    const double *cf = t.coef + (long)reg * t.ncoef;
+    double cf[9];
+    load_coef(t, reg, cf);
     if(t.kind == GDP_T_CURL_CURL) {
       for(int k = 0; k < 6; k++)
         for(int j = 0; j < 6; j++) {
@@ -545,7 +566,20 @@ __global__ void __launch_bounds__(256) k_slot_sum(long nnz, const int *__restric
   const long s = (long)blockIdx.x * 256 + threadIdx.x;
   if(s >= nnz) return;
   double acc = 0.;
-  for(int k = cptr[s]; k < cptr[s + 1]; k++) acc += ek[cidx[k]];
+  const int k1 = cptr[s + 1];
+  // up to 8 contributions in flight: their indices, then their values, are
+  // loaded together, and then added in order
+  for(int k = cptr[s]; k < k1; k += 8) {
+    int ix[8];
+    double v[8];
+#pragma unroll
+    for(int j = 0; j < 8; j++) ix[j] = k + j < k1 ? cidx[k + j] : -1;
+#pragma unroll
+    for(int j = 0; j < 8; j++) v[j] = ix[j] >= 0 ? ek[ix[j]] : 0.;
+#pragma unroll
+    for(int j = 0; j < 8; j++)
+      if(ix[j] >= 0) acc += v[j];
+  }
   a[s] = acc;
 }
 
