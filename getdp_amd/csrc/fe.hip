@@ -593,9 +593,21 @@ __global__ void __launch_bounds__(256) k_rhs_sum(int N, const int *__restrict__ 
   const int r = blockIdx.x * 256 + threadIdx.x;
   if(r >= N) return;
   double bacc = 0.;
-  for(int k = rptr[r]; k < rptr[r + 1]; k++) {
-    const double v = ek[ridx[k]];
-    if(v != 0.) bacc += (rfv[k] * 1.) * (-v);
+  const int k1 = rptr[r + 1];
+  // 8 contributions in flight at a time, added in order
+  for(int k = rptr[r]; k < k1; k += 8) {
+    int ix[8];
+    double v[8], f[8];
+#pragma unroll
+    for(int j = 0; j < 8; j++) {
+      ix[j] = k + j < k1 ? ridx[k + j] : -1;
+      f[j] = k + j < k1 ? rfv[k + j] : 0.;
+    }
+#pragma unroll
+    for(int j = 0; j < 8; j++) v[j] = ix[j] >= 0 ? ek[ix[j]] : 0.;
+#pragma unroll
+    for(int j = 0; j < 8; j++)
+      if(ix[j] >= 0 && v[j] != 0.) bacc += (f[j] * 1.) * (-v[j]);
   }
   b[r] = bacc;
 }
