@@ -475,6 +475,11 @@ __global__ void __launch_bounds__(128) k_elem_nodal(DevPlan P, double *__restric
   else elem_nodal<3>(P, e, gk, out);
 }
 
+// Element matrices of symmetric edge terms are formed for j <= k and
+// mirrored (value (k, j) = value (max, min)), as the reference does; the
+// mapped basis functions (curls, and per quadrature point the edge
+// functions) are computed once per element rather than once per value.  Every
+// value keeps the reference's operations, so the results are the same bits.
 __global__ void __launch_bounds__(128) k_elem_edge(DevPlan P, double *__restrict__ ek)
 {
   const int e = blockIdx.x * 128 + threadIdx.x;
@@ -484,6 +489,9 @@ __global__ void __launch_bounds__(128) k_elem_edge(DevPlan P, double *__restrict
   const int *ed = P.edges + (long)e * 6;
   const double *G = P.geo + (long)e * kGeoStride;
   const double det = G[9], adet = fabs(det);
+  int neg[6];
+#pragma unroll
+  for(int k = 0; k < 6; k++) neg[k] = ed[k] < 0;
   double *out = ek + P.ek_cb[e >> 6] + (e & 63);
   for(int ti = 0; ti < P.nterms; ti++) {
     const DevTerm &t = P.term[ti];
@@ -491,20 +499,19 @@ __global__ void __launch_bounds__(128) k_elem_edge(DevPlan P, double *__restrict
     double cf[9];
     load_coef(t, reg, cf);
     if(t.kind == GDP_T_CURL_CURL) {
+      double xc[6][3], cxc[6][3];
+#pragma unroll
+      for(int k = 0; k < 6; k++) {
+        double c[3];
+        bf_curledge(k, neg[k], c);
+        form2(G, det, c, xc[k]);
+        coef_times(t.ncoef, cf, xc[k], cxc[k]);
+      }
+#pragma unroll
       for(int k = 0; k < 6; k++)
-        for(int j = 0; j < 6; j++) {
-          int ii = k, jj = j;
-          if(j > k) {
-            ii = j;
-            jj = k;
-          }
-          double ci[3], cj[3], xi[3], xj[3], cx[3];
-          bf_curledge(ii, ed[ii] < 0, ci);
-          bf_curledge(jj, ed[jj] < 0, cj);
-          form2(G, det, ci, xi);
-          form2(G, det, cj, xj);
-          coef_times(t.ncoef, cf, xj, cx);
-          double pr = dot3(xi, cx);
+#pragma unroll
+        for(int j = 0; j <= k; j++) {
+          const double pr = dot3(xc[k], cxc[j]);
           double acc = 0.;
           for(int q = 0; q < t.npoints; q++) {
             double u, v, w, wt;
@@ -512,30 +519,36 @@ __global__ void __launch_bounds__(128) k_elem_edge(DevPlan P, double *__restrict
             acc += (wt * adet) * pr;
           }
           out[64 * (k * 6 + j)] = acc;
+          if(j < k) out[64 * (j * 6 + k)] = acc;
         }
       out += 64 * 36;
     }
     else if(t.kind == GDP_T_EDGE_MASS) {
-      for(int k = 0; k < 6; k++)
-        for(int j = 0; j < 6; j++) {
-          int ii = k, jj = j;
-          if(j > k) {
-            ii = j;
-            jj = k;
-          }
-          double acc = 0.;
-          for(int q = 0; q < t.npoints; q++) {
-            double u, v, w, wt;
-            gauss_pt(0, t.npoints, q, u, v, w, wt);
-            double si[3], sj[3], xi[3], xj[3], cx[3];
-            bf_edge(ii, u, v, w, ed[ii] < 0, si);
-            bf_edge(jj, u, v, w, ed[jj] < 0, sj);
-            form1(G + 10, si, xi);
-            form1(G + 10, sj, xj);
-            coef_times(t.ncoef, cf, xj, cx);
-            acc += (wt * adet) * dot3(xi, cx);
-          }
-          out[64 * (k * 6 + j)] = acc;
+      double acc[21];
+#pragma unroll
+      for(int p = 0; p < 21; p++) acc[p] = 0.;
+      for(int q = 0; q < t.npoints; q++) {
+        double u, v, w, wt;
+        gauss_pt(0, t.npoints, q, u, v, w, wt);
+        double x[6][3], cx[6][3];
+#pragma unroll
+        for(int k = 0; k < 6; k++) {
+          double sk[3];
+          bf_edge(k, u, v, w, neg[k], sk);
+          form1(G + 10, sk, x[k]);
+          coef_times(t.ncoef, cf, x[k], cx[k]);
+        }
+#pragma unroll
+        for(int k = 0, p = 0; k < 6; k++)
+#pragma unroll
+          for(int j = 0; j <= k; j++, p++) acc[p] += (wt * adet) * dot3(x[k], cx[j]);
+      }
+#pragma unroll
+      for(int k = 0, p = 0; k < 6; k++)
+#pragma unroll
+        for(int j = 0; j <= k; j++, p++) {
+          out[64 * (k * 6 + j)] = acc[p];
+          if(j < k) out[64 * (j * 6 + k)] = acc[p];
         }
       out += 64 * 36;
     }
