@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <type_traits>
 
 #include "gdp_device.h"
 #include "ilu.h"
@@ -49,6 +50,9 @@ struct BatchArgs {
   const float *f_val;
   const int *b_row, *b_cnt, *b_col;
   const float *b_val, *b_dinv;
+  // per slot {row, entry count, level within the instance, 0}: the fields the
+  // 1024-thread walk streams into LDS
+  const int4 *f_meta, *b_meta;
   int zcap;  // instances with at most zcap rows keep the sweep vector in LDS
   // larger instances with ring[inst] keep the last rmask + 1 sweep results in
   // an LDS ring indexed by stream position (their record columns are stream
@@ -336,10 +340,190 @@ __device__ inline void walk_sweep(const BatchArgs &a, int inst, const double *sr
   }
 }
 
-// z = (LU)^{-1} v (v may alias z); z in LDS (zlds) when the instance fits
-template <int T, int W>
-__device__ inline void b_ilu(const BatchArgs &a, int inst, int n, const double *v, double *z, double *zlds)
+// ---- the 1024-thread walk: records streamed into LDS --------------------------
+//
+// The walk above keeps D = 3 windows of records in registers, and at about a
+// window per 0.7 us it waits on their loads: three windows of work do not
+// cover a memory round trip, and deeper register buffers do not fit the
+// 1024-thread kernel.  Here the walking wave streams the records of the next
+// F - 1 windows straight into an LDS FIFO with LDS-DMA loads
+// (global_load_lds: no registers held while in flight), waits for the oldest
+// window with a vector-memory count (loads complete in order, so with
+// (F - 1) * L loads issued after a window's L loads, vmcnt <= (F - 1) * L means
+// the window has landed), and reads it with LDS reads.  The FIFO is its own
+// __shared__ object: the compiler then knows the sweep vector and the ring
+// cannot alias it and puts no wait for the DMA before their accesses.  The
+// FIFO reads are inline asm for the same reason (the compiler would wait for
+// every DMA in flight before a read of the FIFO); they wait for themselves in
+// the same asm block.  Same operations per row as apply_rec; levels come
+// from each slot's record, so the walk touches no other memory.
+template <int W> struct WalkFifo {
+  static constexpr int F = W == 4 ? 8 : 4;  // windows in the FIFO
+  int4 meta[F][kWave];
+  int4 col[F][W / 4][kWave];
+  uint4 val[F][W / 4][kWave];
+  unsigned dinv[F][kWave];
+  unsigned plo[F][kWave], phi[F][kWave];
+};
+
+#define GDP_LDS __attribute__((address_space(3)))
+__device__ inline void dma16(const void *g, void *l)
 {
+  __builtin_amdgcn_global_load_lds(g, (GDP_LDS void *)l, 16, 0, 0);
+}
+__device__ inline void dma4(const void *g, void *l) { __builtin_amdgcn_global_load_lds(g, (GDP_LDS void *)l, 4, 0, 0); }
+__device__ inline unsigned lds_addr(const void *p) { return (unsigned)(size_t)(const GDP_LDS void *)p; }
+// s_waitcnt vmcnt(N) (gfx9 encoding: vmcnt bits 3:0 and 15:14), other counters untouched
+template <int N> __device__ inline void wait_vmcnt()
+{
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int W, bool FWD, bool RING>
+__device__ inline void walk_fifo(const BatchArgs &a, int inst, const double *P, double *z, double *ring,
+                                 WalkFifo<W> &fifo)
+{
+  constexpr int F = WalkFifo<W>::F;
+  constexpr int L = 1 + 2 * (W / 4) + (FWD ? 0 : 1) + 2;  // DMA loads per window
+  const int *ptr = FWD ? a.fl_ptr : a.bl_ptr;
+  const int base = FWD ? a.fl_base[inst] : a.bl_base[inst];
+  const int nl = FWD ? a.fl_n[inst] : a.bl_n[inst];
+  if(nl <= 0) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int sf = ptr[base], se = ptr[base + nl];
+  const int nwin = (se - sf + kWave - 1) / kWave;
+  const int4 *metap = FWD ? a.f_meta : a.b_meta;
+  const int *colp = FWD ? a.f_col : a.b_col;
+  const float *valp = FWD ? a.f_val : a.b_val;
+  const unsigned *pw = (const unsigned *)P;
+  // window k into FIFO slot k % F (past the end: the last window again, so
+  // that every step issues the same L loads; lanes past the end: the last slot)
+  auto issue = [&](int k) {
+    const int fs = k & (F - 1);
+    const int q = min(sf + min(k, nwin - 1) * kWave + lane, se - 1);
+    dma16(metap + q, &fifo.meta[fs][0]);
+#pragma unroll
+    for(int j = 0; j < W / 4; j++) dma16(colp + (size_t)q * W + 4 * j, &fifo.col[fs][j][0]);
+#pragma unroll
+    for(int j = 0; j < W / 4; j++) dma16(valp + (size_t)q * W + 4 * j, &fifo.val[fs][j][0]);
+    if constexpr(!FWD) dma4(a.b_dinv + q, &fifo.dinv[fs][0]);
+    dma4(pw + 2 * (size_t)(q - sf), &fifo.plo[fs][0]);
+    dma4(pw + 2 * (size_t)(q - sf) + 1, &fifo.phi[fs][0]);
+  };
+#pragma unroll
+  for(int d = 0; d + 1 < F; d++) issue(d);
+  for(int k = 0; k < nwin; k++) {
+    issue(k + F - 1);
+    wait_vmcnt<(F - 1) * L>();
+    const int fs = k & (F - 1);
+    int4 m;
+    int cl[W];
+    unsigned vl[W], di, lo, hi;
+    if constexpr(W == 4) {
+      int4 c;
+      uint4 v;
+      asm volatile("ds_read_b128 %0, %6\n\t"
+                   "ds_read_b128 %1, %7\n\t"
+                   "ds_read_b128 %2, %8\n\t"
+                   "ds_read_b32 %3, %9\n\t"
+                   "ds_read_b32 %4, %10\n\t"
+                   "ds_read_b32 %5, %11\n\t"
+                   "s_waitcnt lgkmcnt(0)"
+                   : "=&v"(m), "=&v"(c), "=&v"(v), "=&v"(di), "=&v"(lo), "=&v"(hi)
+                   : "v"(lds_addr(&fifo.meta[fs][lane])), "v"(lds_addr(&fifo.col[fs][0][lane])),
+                     "v"(lds_addr(&fifo.val[fs][0][lane])), "v"(lds_addr(&fifo.dinv[fs][lane])),
+                     "v"(lds_addr(&fifo.plo[fs][lane])), "v"(lds_addr(&fifo.phi[fs][lane]))
+                   : "memory");
+      cl[0] = c.x, cl[1] = c.y, cl[2] = c.z, cl[3] = c.w;
+      vl[0] = v.x, vl[1] = v.y, vl[2] = v.z, vl[3] = v.w;
+    }
+    else {
+      int4 c0, c1;
+      uint4 v0, v1;
+      asm volatile("ds_read_b128 %0, %8\n\t"
+                   "ds_read_b128 %1, %9\n\t"
+                   "ds_read_b128 %2, %10\n\t"
+                   "ds_read_b128 %3, %11\n\t"
+                   "ds_read_b128 %4, %12\n\t"
+                   "ds_read_b32 %5, %13\n\t"
+                   "ds_read_b32 %6, %14\n\t"
+                   "ds_read_b32 %7, %15\n\t"
+                   "s_waitcnt lgkmcnt(0)"
+                   : "=&v"(m), "=&v"(c0), "=&v"(c1), "=&v"(v0), "=&v"(v1), "=&v"(di), "=&v"(lo), "=&v"(hi)
+                   : "v"(lds_addr(&fifo.meta[fs][lane])), "v"(lds_addr(&fifo.col[fs][0][lane])),
+                     "v"(lds_addr(&fifo.col[fs][W / 4 - 1][lane])), "v"(lds_addr(&fifo.val[fs][0][lane])),
+                     "v"(lds_addr(&fifo.val[fs][W / 4 - 1][lane])), "v"(lds_addr(&fifo.dinv[fs][lane])),
+                     "v"(lds_addr(&fifo.plo[fs][lane])), "v"(lds_addr(&fifo.phi[fs][lane]))
+                   : "memory");
+      cl[0] = c0.x, cl[1] = c0.y, cl[2] = c0.z, cl[3] = c0.w;
+      cl[W - 4] = c1.x, cl[W - 3] = c1.y, cl[W - 2] = c1.z, cl[W - 1] = c1.w;
+      vl[0] = v0.x, vl[1] = v0.y, vl[2] = v0.z, vl[3] = v0.w;
+      vl[W - 4] = v1.x, vl[W - 3] = v1.y, vl[W - 2] = v1.z, vl[W - 1] = v1.w;
+    }
+    const int slot = sf + k * kWave + lane;
+    const int row = m.x, cnt = m.y, lvl = m.z;
+    const double pre = __hiloint2double((int)hi, (int)lo);
+    const bool valid = slot < se;
+    bool done = !valid;
+    double x = 0.;
+    // the window's levels in order (levels are ascending in slot order): the
+    // rows of one level, then the first lane still pending names the next
+    int cur = __builtin_amdgcn_readfirstlane(lvl);
+    for(;;) {
+      if(!done && lvl == cur) {
+        double dv[W];
+#pragma unroll
+        for(int w = 0; w < W; w++) dv[w] = RING ? ring[cl[w] & a.rmask] : z[cl[w]];
+        double acc = pre;
+#pragma unroll
+        for(int w = 0; w < W; w++)
+          if(w < cnt) acc = acc - (double)__uint_as_float(vl[w]) * dv[w];
+        x = FWD ? acc : (double)__uint_as_float(di) * acc;
+        if constexpr(RING) ring[(slot - sf) & a.rmask] = x;
+        else z[row] = x;
+        done = true;
+      }
+      const unsigned long long pend = __ballot(!done);
+      if(!pend) break;
+      cur = __builtin_amdgcn_readlane(lvl, __builtin_ctzll(pend));
+    }
+    // ring sweeps read their dependencies from the ring only: the results go
+    // to the sweep vector in memory once per window (one store instruction)
+    if constexpr(RING)
+      if(valid) z[row] = x;
+  }
+  wait_vmcnt<0>();  // the padding windows' loads, before the FIFO is reused
+}
+
+// z = (LU)^{-1} v (v may alias z); z in LDS (zlds) when the instance fits
+template <int T, int W, typename FIFO>
+__device__ inline void b_ilu(const BatchArgs &a, int inst, int n, const double *v, double *z, double *zlds,
+                             FIFO &fifo)
+{
+  if constexpr(T == 1024) {
+    if(a.walk && (n <= a.zcap || a.ring[inst])) {
+      // inputs staged in slot order in the instance's spare workspace vector,
+      // then walked by wave 0 with the records streamed through the FIFO
+      const bool lz = n <= a.zcap;
+      double *P = a.pool + a.pool_off[inst] + (size_t)(a.m + 2) * n;
+      stage_inputs<T, true>(a, inst, v, P);
+      __syncthreads();
+      if(threadIdx.x < kWave) {
+        if(lz) walk_fifo<W, true, false>(a, inst, P, zlds, nullptr, fifo);
+        else walk_fifo<W, true, true>(a, inst, P, z, zlds, fifo);
+      }
+      __syncthreads();
+      stage_inputs<T, false>(a, inst, lz ? zlds : z, P);
+      __syncthreads();
+      if(threadIdx.x < kWave) {
+        if(lz) walk_fifo<W, false, false>(a, inst, P, zlds, nullptr, fifo);
+        else walk_fifo<W, false, true>(a, inst, P, z, zlds, fifo);
+      }
+      __syncthreads();
+      return;
+    }
+  }
   if(n <= a.zcap && a.walk) {
     if(threadIdx.x < kWave) {
       walk_sweep<W, true, false>(a, inst, v, zlds, nullptr);
@@ -383,6 +567,8 @@ template <int T, int W, int ORTH>
 __global__ void __launch_bounds__(T) k_batch_gmres(BatchArgs a)
 {
   extern __shared__ double zlds[];
+  // the record FIFO of the 1024-thread walk (a byte elsewhere)
+  __shared__ std::conditional_t<T == 1024, WalkFifo<W>, char> fifo;
   __shared__ double H[(kBM + 1) * kBM];
   __shared__ double cs[kBM], sn[kBM], g[kBM + 1], yv[kBM];
   __shared__ double red[T / 64];
@@ -428,7 +614,7 @@ __global__ void __launch_bounds__(T) k_batch_gmres(BatchArgs a)
       bool conv = false;
       for(int j = 0; j < m; j++) {
         double *vj = V + (size_t)j * n, *w = V + (size_t)(j + 1) * n;
-        b_ilu<T, W>(a, inst, n, vj, Z, zlds);
+        b_ilu<T, W>(a, inst, n, vj, Z, zlds, fifo);
         b_spmv(a, r0, n, Z, w, T);
         its++;
         __syncthreads();
@@ -528,7 +714,7 @@ __global__ void __launch_bounds__(T) k_batch_gmres(BatchArgs a)
         Z[r] = s;
       }
       __syncthreads();
-      b_ilu<T, W>(a, inst, n, Z, Z, zlds);
+      b_ilu<T, W>(a, inst, n, Z, Z, zlds, fifo);
       for(int r = tid; r < n; r += T) x[r] += Z[r];
       __syncthreads();
       if(conv) {
@@ -614,6 +800,7 @@ struct gdp_batch {
   int *d_f_row = nullptr, *d_f_cnt = nullptr, *d_f_col = nullptr, *d_f_k = nullptr;
   int *d_b_row = nullptr, *d_b_cnt = nullptr, *d_b_col = nullptr, *d_b_k = nullptr, *d_b_dk = nullptr;
   int *d_ring = nullptr;  // per instance: sweeps through the LDS ring
+  int4 *d_f_meta = nullptr, *d_b_meta = nullptr;
   float *d_f_val = nullptr, *d_b_val = nullptr, *d_b_dinv = nullptr;
   long *d_pool_off = nullptr;
   double *pool = nullptr, *rhs = nullptr, *sol = nullptr;
@@ -657,6 +844,8 @@ struct gdp_batch {
                    &d_bl_ptr, &d_queue, &d_its, &d_status, &d_f_row, &d_f_cnt, &d_f_col, &d_f_k, &d_b_row,
                    &d_b_cnt, &d_b_col, &d_b_k, &d_b_dk, &d_ring})
       dfree(*p);
+    dfree(d_f_meta);
+    dfree(d_b_meta);
     dfree(d_f_val);
     dfree(d_b_val);
     dfree(d_b_dinv);
@@ -676,7 +865,7 @@ namespace {
 
 // rows of the sweep vector kept in LDS, per workgroup size (1, 2 or 3
 // workgroups of 1024, 512, 256 threads per CU with 160 KB of LDS)
-int zcap_rows(int threads) { return threads == 1024 ? 18000 : (threads == 512 ? 8192 : 4096); }
+int zcap_rows(int threads) { return threads == 1024 ? 14336 : (threads == 512 ? 8192 : 4096); }
 // entries of the LDS ring of larger instances: the largest power of two
 // within the same budget
 int ring_rows(int threads)
@@ -867,6 +1056,19 @@ int batch_prepare(gdp_batch *B, const gdp_matrix *A)
   std::vector<int> frow, fcnt, fcol, fk, brow, bcnt, bcol, bk, bdk;
   records(fr, true, frow, fcnt, fcol, fk, nullptr);
   records(br, false, brow, bcnt, bcol, bk, &bdk);
+  // {row, count, level within the instance, 0} per slot
+  auto metas = [&](const std::vector<int> &base, const std::vector<int> &cnt, const std::vector<int> &ptr,
+                   const std::vector<int> &rrow, const std::vector<int> &rcnt, int4 **dst) {
+    std::vector<int4> mt(std::max<size_t>(1, rrow.size()), make_int4(0, 0, 0, 0));
+    for(int i = 0; i < B->ninst; i++)
+      for(int l = 0; l < cnt[i]; l++)
+        for(int q = ptr[base[i] + l]; q < ptr[base[i] + l + 1]; q++) mt[q] = make_int4(rrow[q], rcnt[q], l, 0);
+    int rc = dalloc(dst, mt.size());
+    if(!rc) rc = h2d(*dst, mt.data(), mt.size());
+    return rc;
+  };
+  GDP_TRY(metas(fb, fn, fp, frow, fcnt, &B->d_f_meta));
+  GDP_TRY(metas(bb, bn, bp, brow, bcnt, &B->d_b_meta));
   for(auto &pr : std::vector<std::pair<int **, std::vector<int> *>>{
         {&B->d_fl_base, &fb}, {&B->d_fl_n, &fn}, {&B->d_fl_ptr, &fp}, {&B->d_bl_base, &bb},
         {&B->d_bl_n, &bn}, {&B->d_bl_ptr, &bp}, {&B->d_f_row, &frow}, {&B->d_f_cnt, &fcnt},
@@ -997,6 +1199,8 @@ int batch_enqueue(gdp_batch *B, gdp_matrix *A, gdp_vector *b, gdp_vector *x)
     a.b_col = B->d_b_col;
     a.b_val = B->d_b_val;
     a.b_dinv = B->d_b_dinv;
+    a.f_meta = B->d_f_meta;
+    a.b_meta = B->d_b_meta;
     a.zcap = zcap_rows(g.threads);
     a.ring = B->d_ring;
     a.rmask = ring_rows(g.threads) - 1;

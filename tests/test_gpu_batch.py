@@ -109,7 +109,7 @@ def test_batch_rejects_unsupported_parameters(gpu, probs):
 @pytest.mark.parametrize("walk", ["1", "0"])
 @pytest.mark.parametrize("threads", [256, 512, 1024])
 def test_large_instances_lds_ring(gpu, oracle, threads, walk, monkeypatch):
-    """Instances larger than the LDS sweep vector (4096, 8192, 18000 rows at
+    """Instances larger than the LDS sweep vector (4096, 8192, 14336 rows at
     256, 512, 1024 threads) sweep through the LDS ring of recent results: the
     solutions match the oracle's, with the reference's iteration counts (same
     ordering and factors), next to small instances on the LDS path; sweeps
