@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "gdp_internal.h"
+#include "host_par.h"
 
 namespace gdp {
 
@@ -204,19 +205,33 @@ void permute_pattern(int n, const int *ia, const int *ja, const int *perm,
   size_t nnz = (size_t)ia[n];
   jb.resize(nnz);
   map.resize(nnz);
-  std::vector<std::pair<int, int>> tmp;
-  for(int i = 0; i < n; i++) {
-    int r = perm[i];
-    tmp.clear();
-    for(int k = ia[i]; k < ia[i + 1]; k++) tmp.emplace_back(perm[ja[k]], k);
-    std::sort(tmp.begin(), tmp.end());
-    int o = ib[r];
-    for(auto &t : tmp) {
-      jb[o] = t.first;
-      map[o] = t.second;
-      o++;
+  // rows are independent (each fills its own range of jb and map): over
+  // threads, each row's (new column, old position) pairs sorted by column
+  // (insertion sort: rows are short)
+  int *jbp = jb.data(), *mp = map.data();
+  const int *ibp = ib.data();
+  parallel_ranges((size_t)n, [=](size_t a, size_t b) {
+    std::vector<std::pair<int, int>> tmp;
+    for(size_t i = a; i < b; i++) {
+      tmp.clear();
+      for(int k = ia[i]; k < ia[i + 1]; k++) {
+        const std::pair<int, int> t(perm[ja[k]], k);
+        size_t j = tmp.size();
+        tmp.push_back(t);
+        while(j > 0 && t < tmp[j - 1]) {
+          tmp[j] = tmp[j - 1];
+          j--;
+        }
+        tmp[j] = t;
+      }
+      int o = ibp[perm[i]];
+      for(const auto &t : tmp) {
+        jbp[o] = t.first;
+        mp[o] = t.second;
+        o++;
+      }
     }
-  }
+  });
 }
 
 }  // namespace gdp

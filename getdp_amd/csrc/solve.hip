@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <utility>
 
 #include "krylov.h"
 #include "sell.h"
@@ -22,6 +23,14 @@ namespace gdp {
 namespace {
 
 constexpr int BT = 256;
+
+// row_of[k] = the row of entry k
+__global__ void __launch_bounds__(BT) k_row_of(int n, const int *__restrict__ ia, int *__restrict__ row_of)
+{
+  const int i = blockIdx.x * BT + threadIdx.x;
+  if(i < n)
+    for(int k = ia[i]; k < ia[i + 1]; k++) row_of[k] = i;
+}
 
 __global__ void __launch_bounds__(BT) k_gather_scaled(long nnz, const int *__restrict__ map,
                                                      const int *__restrict__ row_of,
@@ -267,29 +276,27 @@ int prepare(gdp_solver *S, const gdp_matrix *A, bool reuse_factor)
       rperm.swap(rperm2);
       permute_pattern(n, A->h_ia.data(), A->h_ja.data(), perm.data(), ib, jb, map);
     }
-    std::vector<int> row_of(nnz);
-    for(int i = 0; i < n; i++)
-      for(int k = ib[i]; k < ib[i + 1]; k++) row_of[k] = i;
-    c.perm = perm;
-    c.rperm = rperm;
-    c.ib = ib;
-    c.jb = jb;
-    c.map = map;
+    c.perm = std::move(perm);
+    c.rperm = std::move(rperm);
+    c.ib = std::move(ib);
+    c.jb = std::move(jb);
+    c.map = std::move(map);
     GDP_TRY(dalloc(&c.d_perm, (size_t)std::max(1, n)));
     GDP_TRY(dalloc(&c.d_rperm, (size_t)std::max(1, n)));
     GDP_TRY(dalloc(&c.d_map, (size_t)std::max(1L, nnz)));
     GDP_TRY(dalloc(&c.d_row_of, (size_t)std::max(1L, nnz)));
-    GDP_TRY(h2d(c.d_perm, perm.data(), (size_t)n));
-    GDP_TRY(h2d(c.d_rperm, rperm.data(), (size_t)n));
-    GDP_TRY(h2d(c.d_map, map.data(), (size_t)nnz));
-    GDP_TRY(h2d(c.d_row_of, row_of.data(), (size_t)nnz));
+    GDP_TRY(h2d(c.d_perm, c.perm.data(), (size_t)n));
+    GDP_TRY(h2d(c.d_rperm, c.rperm.data(), (size_t)n));
+    GDP_TRY(h2d(c.d_map, c.map.data(), (size_t)nnz));
     c.Bm.n = n;
     c.Bm.nnz = nnz;
     GDP_TRY(dalloc(&c.Bm.ia, (size_t)n + 1));
     GDP_TRY(dalloc(&c.Bm.ja, (size_t)std::max(1L, nnz)));
     GDP_TRY(dalloc(&c.Bm.a, (size_t)std::max(1L, nnz)));
-    GDP_TRY(h2d(c.Bm.ia, ib.data(), (size_t)n + 1));
-    GDP_TRY(h2d(c.Bm.ja, jb.data(), (size_t)nnz));
+    GDP_TRY(h2d(c.Bm.ia, c.ib.data(), (size_t)n + 1));
+    GDP_TRY(h2d(c.Bm.ja, c.jb.data(), (size_t)nnz));
+    if(n) hipLaunchKernelGGL(k_row_of, dim3((n + BT - 1) / BT), dim3(BT), 0, s, n, c.Bm.ia, c.d_row_of);
+    GDP_HIP(hipGetLastError());
     GDP_TRY(dalloc(&c.rhs, (size_t)std::max(1, n)));
     GDP_TRY(dalloc(&c.sol, (size_t)std::max(1, n)));
     GDP_TRY(dalloc(&c.work, (size_t)std::max(1, n)));
