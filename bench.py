@@ -35,6 +35,10 @@ HBM_PEAK_GBS = 8000.0
 # --roofline-only` (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
 # 512 MiB device copy): 403.5 MB read + 6.5 MB written.
 PMC_SPMV_TRAFFIC = {64: 409_978_000}
+# HBM bytes per assembly of elasticity3d_n (k_geometry + k_elem_nodal +
+# k_slot_sum + k_rhs_sum), from the same two counter runs: 6.29 GB read +
+# 2.36 GB written.
+PMC_ASSEMBLY_TRAFFIC = {64: 8_645_000_000}
 
 
 def log(*a):
@@ -658,6 +662,17 @@ def main():
         "assembly_ms": round(asm_s * 1e3, 3),
         "assembly_elems_per_s": round(E / asm_s, 1),
         "assembly_gbs": round(assembly_bytes(pr, fe.N, fe.nnz) / asm_s / 1e9, 1),
+        # compulsory bytes (connectivity, coordinates, values and right-hand
+        # side once) against the HBM peak, and the counter-measured traffic
+        "assembly_roofline": {
+            "bound": "hbm", "achieved": round(assembly_bytes(pr, fe.N, fe.nnz) / asm_s / 1e9, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(assembly_bytes(pr, fe.N, fe.nnz) / asm_s / 1e9 / HBM_PEAK_GBS, 4),
+            "compulsory": int(assembly_bytes(pr, fe.N, fe.nnz)),
+            "traffic": PMC_ASSEMBLY_TRAFFIC.get(args.n),
+            "traffic_gbs": (round(PMC_ASSEMBLY_TRAFFIC[args.n] / asm_s / 1e9, 1)
+                            if args.n in PMC_ASSEMBLY_TRAFFIC else None),
+        },
         "spmv_hbm_frac": roofline["frac"],
         "spmv_natural_order_ms": round(spmv_natural_ms, 5),
         "roofline": roofline,
