@@ -1751,8 +1751,11 @@ int krylov_solve(const DevOp<TA> &A, const PrecOp<TV> &M, const TV *rhs, TV *sol
   GraphCache gc;
   GmArgs<TA, TV> ga{A, &M, rhs, sol, V, Z, C, part, n, m, opt.orth, flexible, two_stage};
   auto cycle = [&]() -> int { return gm_cycle<TA, TV>(ga); };
-  for(;;) {
-    GDP_TRY(run_maybe_graph(opt.use_graphs, gc, cycle));
+  for(int cyc = 0;; cyc++) {
+    // the first two restart cycles launch their kernels directly: a solve
+    // that ends within them does not pay for a capture it would replay once
+    // at most
+    GDP_TRY(run_maybe_graph(opt.use_graphs && cyc >= 2, gc, cycle));
     GDP_TRY(read_ctl(C, &done, &status, &its));
     if(done) break;
   }
