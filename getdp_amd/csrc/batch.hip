@@ -659,16 +659,32 @@ __global__ void __launch_bounds__(T) k_batch_gmres(BatchArgs a)
           }
         }
         else {
-          // modified Gram-Schmidt
+          // modified Gram-Schmidt; the update by v_i and the product with
+          // v_{i+1} share one pass over w (a thread's rows are its own, so
+          // the product reads w just updated, as after a separate pass)
+          double h;
+          {
+            double dd = 0.;
+            for(int r = tid; r < n; r += T) dd += V[r] * w[r];
+            h = block_sum_d<T>(dd, red);
+          }
           for(int i = 0; i <= j; i++) {
             const double *vi = V + (size_t)i * n;
-            double dd = 0.;
-            for(int r = tid; r < n; r += T) dd += vi[r] * w[r];
-            const double h = block_sum_d<T>(dd, red);
-            for(int r = tid; r < n; r += T) w[r] -= h * vi[r];
             if(tid == 0) H[i * kBM + j] = h;
-            __syncthreads();
+            if(i < j) {
+              const double *vn = vi + n;
+              double dd = 0.;
+              for(int r = tid; r < n; r += T) {
+                const double wr = w[r] - h * vi[r];
+                w[r] = wr;
+                dd += vn[r] * wr;
+              }
+              h = block_sum_d<T>(dd, red);
+            }
+            else
+              for(int r = tid; r < n; r += T) w[r] -= h * vi[r];
           }
+          __syncthreads();
         }
         double ww = 0.;
         for(int r = tid; r < n; r += T) ww += w[r] * w[r];
