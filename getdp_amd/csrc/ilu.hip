@@ -746,6 +746,28 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
     sd[s].G = 1;
     sd[s].wg = kWaveSlots[shape];
     sd[s].rc = kWave / sd[s].G;
+  }
+  // Estimated time per application, before the chunk and ring analysis (a
+  // pass over every entry), which a system the walk would not win skips.  A
+  // chunk of the walk costs its reads and its row chain (about 0.25 us with
+  // 4 entry slots a row, 0.46 us with 24), and the chunks of one level go
+  // about two at a time; the sync-free sweeps cost 1-2 us per level (one
+  // publication round trip).
+  {
+    double t_wave = 0.;
+    for(int s = 0; s < 2; s++) {
+      const int cap = sd[s].G * sd[s].wg;
+      const double per = cap <= 4 ? 0.25 : cap <= 8 ? 0.3 : cap <= 16 ? 0.4 : cap <= 24 ? 0.46 : 0.55;
+      const std::vector<int> &p = *ptrs[s];
+      for(size_t l = 0; l + 1 < p.size(); l++) {
+        const int ch = (p[l + 1] - p[l] + sd[s].rc - 1) / sd[s].rc;
+        t_wave += per * std::max(1, (ch + 1) / 2);
+      }
+    }
+    const double t_free = 2.0 * ((double)nlevels + upper_nlevels);
+    if(!force && t_wave >= t_free) return GDP_OK;
+  }
+  for(int s = 0; s < 2; s++) {
     const std::vector<int> &p = *ptrs[s];
     for(size_t l = 0; l + 1 < p.size(); l++)
       for(int a = p[l]; a < p[l + 1]; a += sd[s].rc) {
@@ -782,22 +804,6 @@ int IluFactor::setup_wave(const int *hia, const int *hja, const std::vector<int>
   // in RCMK order (two chunks a level) 0.45 and 0.40 ms.
   int team = 4;
   if(const char *e = std::getenv("GDP_ILU_TEAM")) team = std::max(1, std::min(kMaxTeam, atoi(e)));
-  // Estimated time per application.  A chunk of the walk costs its reads
-  // and its row chain (about 0.25 us with 4 entry slots a row, 0.46 us with
-  // 24), and the chunks of one level go about two at a time; the sync-free
-  // sweeps cost 1-2 us per level (one publication round trip).
-  double t_wave = 0.;
-  for(int s = 0; s < 2; s++) {
-    const int cap = sd[s].G * sd[s].wg;
-    const double per = cap <= 4 ? 0.25 : cap <= 8 ? 0.3 : cap <= 16 ? 0.4 : cap <= 24 ? 0.46 : 0.55;
-    const std::vector<int> &p = *ptrs[s];
-    for(size_t l = 0; l + 1 < p.size(); l++) {
-      const int ch = (p[l + 1] - p[l] + sd[s].rc - 1) / sd[s].rc;
-      t_wave += per * std::max(1, (ch + 1) / 2);
-    }
-  }
-  const double t_free = 2.0 * ((double)nlevels + upper_nlevels);
-  if(!force && t_wave >= t_free) return GDP_OK;
   // need[c] for the team walk (sequential chunk order, in which the ring check
   // above holds): the chunks of c's dependencies, and for every slot c writes
   // its previous writer and the last reader of the value it holds
